@@ -1,0 +1,163 @@
+// Shared device helpers and launch declarations of the Louvain engine.
+// gfx950 (CDNA4): 64-lane wavefronts, LDS FP64 atomics (ds_add_f64),
+// global FP64 atomics (global_atomic_add_f64, coarse-grained memory).
+#pragma once
+
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+namespace lv {
+
+// ---------------------------------------------------------------------
+// Phase-1 execution buckets.  Buckets 0..5 coincide with the reference's
+// degree groups 0..5 (modularity_optimisation.cuh:7-19); group 6 (deg > 319)
+// is split by degree into a 256-lane LDS tier, a large-LDS workgroup tier
+// and a multi-workgroup global-hash tier.  Splitting a group changes only
+// the kernel mapping: decisions inside a group are simultaneous.
+// ---------------------------------------------------------------------
+constexpr int kNumBuckets = 9;
+constexpr int kNumGroups = 7;
+constexpr int kBucketGroup[kNumBuckets] = {0, 1, 2, 3, 4, 5, 6, 6, 6};
+constexpr int kGroupBound[kNumGroups + 1] = {0, 4, 8, 16, 32, 84, 319, 0x7fffffff};
+
+struct BucketBounds {
+    int32_t hi[kNumBuckets];  // bucket b holds hi[b-1] < deg <= hi[b]
+};
+
+__host__ __device__ inline int bucket_of(int32_t deg, const BucketBounds &bb) {
+    if (deg <= 0) return -1;
+#pragma unroll
+    for (int b = 0; b < kNumBuckets; ++b)
+        if (deg <= bb.hi[b]) return b;
+    return kNumBuckets - 1;
+}
+
+// Move-kernel operands.
+struct MoveArgs {
+    const int32_t *rowptr;
+    const int32_t *col;
+    const float *w;
+    const int32_t *comm;
+    const int32_t *size;
+    const double *k;
+    const double *sigma;
+    int32_t *newc;
+    float M32;  // (float)M
+    float den;  // (2.0f * M32) * M32, fp32 as the reference evaluates it
+};
+
+// computeGain (modularity_optimisation.cu:96-102), fp32 in the same order:
+//   K/M + k * ((S_cur - k) - S_c) / (2*M*M).  No contraction into FMAs.
+__device__ __forceinline__ float gain_of(float Kc, float M32, float den, float k32, float scur,
+                                         float sc) {
+#pragma clang fp contract(off)
+    float cs = scur - k32;
+    float num = k32 * (cs - sc);
+    float a = Kc / M32;
+    float b = num / den;
+    return a + b;
+}
+
+// Candidate ordering of computeMove (modularity_optimisation.cu:157,169):
+// larger gain wins, equal gains -> lower community id.  Packed into one
+// 64-bit key so that an unsigned max is the argmax: high word = gain mapped
+// to an order-preserving unsigned, low word = ~community.
+__device__ __forceinline__ uint64_t pack_best(float g, int32_t c) {
+    uint32_t u = __float_as_uint(g + 0.0f);  // -0 -> +0 (IEEE-equal values)
+    u = (u & 0x80000000u) ? ~u : (u | 0x80000000u);
+    return ((uint64_t)u << 32) | (uint64_t)(~(uint32_t)c);
+}
+__device__ __forceinline__ float best_gain(uint64_t key) {
+    uint32_t u = (uint32_t)(key >> 32);
+    u = (u & 0x80000000u) ? (u & 0x7fffffffu) : ~u;
+    return __uint_as_float(u);
+}
+__device__ __forceinline__ int32_t best_comm(uint64_t key) {
+    return (int32_t)(~(uint32_t)(key & 0xffffffffu));
+}
+
+// Final test of computeMove (modularity_optimisation.cu:192-196).
+__device__ __forceinline__ int32_t decide_from(uint64_t best, double kcur, float M32,
+                                               int32_t cur) {
+#pragma clang fp contract(off)
+    float bg = best_gain(best);
+    float t = (float)kcur / M32;
+    return (bg - t > 0.0f) ? best_comm(best) : cur;
+}
+
+// Singleton rule (modularity_optimisation.cu:154-155).
+__device__ __forceinline__ bool may_move(int32_t c, int32_t cur, int32_t size_cur,
+                                         const int32_t *size) {
+    return c < cur || size_cur > 1 || size[c] > 1;
+}
+
+__device__ __forceinline__ uint32_t hash_u32(int32_t c) {
+    return (uint32_t)c * 2654435761u;
+}
+
+// Open-addressing insert with linear probing; keys -1 = empty.  The table
+// must hold more slots than distinct keys.  Values are FP64 sums (exact for
+// weights on a dyadic grid, hence independent of insertion order).
+__device__ __forceinline__ int lds_insert(int32_t *keys, double *vals, uint32_t shift,
+                                          uint32_t mask, int32_t c, double w) {
+    uint32_t h = hash_u32(c) >> shift;
+    for (;;) {
+        int32_t k = __hip_atomic_load(&keys[h], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
+        if (k == c) {
+            atomicAdd(&vals[h], w);
+            return (int)h;
+        }
+        if (k == -1) {
+            int32_t old = atomicCAS(&keys[h], -1, c);
+            if (old == -1 || old == c) {
+                atomicAdd(&vals[h], w);
+                return (int)h;
+            }
+        }
+        h = (h + 1) & mask;
+    }
+}
+
+__device__ __forceinline__ void global_insert(int32_t *keys, double *vals, uint32_t shift,
+                                              uint32_t mask, int32_t c, double w) {
+    uint32_t h = hash_u32(c) >> shift;
+    for (;;) {
+        int32_t k = __hip_atomic_load(&keys[h], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+        if (k == c) {
+            atomicAdd(&vals[h], w);
+            return;
+        }
+        if (k == -1) {
+            int32_t old = atomicCAS(&keys[h], -1, c);
+            if (old == -1 || old == c) {
+                atomicAdd(&vals[h], w);
+                return;
+            }
+        }
+        h = (h + 1) & mask;
+    }
+}
+
+// Wave-level reductions over a 64-lane wavefront.
+__device__ __forceinline__ uint64_t wave_max_u64(uint64_t x) {
+#pragma unroll
+    for (int off = 32; off > 0; off >>= 1) {
+        uint64_t y = __shfl_xor(x, off, 64);
+        x = y > x ? y : x;
+    }
+    return x;
+}
+__device__ __forceinline__ double wave_sum_f64(double x) {
+#pragma unroll
+    for (int off = 32; off > 0; off >>= 1) x += __shfl_xor(x, off, 64);
+    return x;
+}
+__device__ __forceinline__ int64_t wave_sum_i64(int64_t x) {
+#pragma unroll
+    for (int off = 32; off > 0; off >>= 1) x += __shfl_xor(x, off, 64);
+    return x;
+}
+
+constexpr int kWave = 64;
+
+}  // namespace lv

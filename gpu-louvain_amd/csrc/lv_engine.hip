@@ -1,0 +1,1126 @@
+// Engine and C-ABI of the Louvain library (include/louvain.h).
+//
+// Host orchestration of one graph on one GPU stream.  Per level:
+//   setup   degree buckets (stable partition, no PCIe traffic), strengths,
+//           singleton state, heavy-vertex work tables;
+//   phase   sweeps of 7 degree groups (moves of a group are simultaneous,
+//           groups run in order), each group = move kernels + apply; one
+//           fused modularity reduction per sweep; one host sync per sweep
+//           for the termination test (optimiseModularity,
+//           modularity_optimisation.cu:284-363);
+//   merge   aggregation into the other CSR buffer (aggregateCommunities,
+//           community_aggregation.cu:248-347).
+// The level-0 CSR is kept so lv_reset can restart without re-uploading.
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cmath>
+#include <cstdio>
+#include <cstring>
+#include <string>
+#include <vector>
+
+#include "../../include/louvain.h"
+#include "lv_common.h"
+#include "lv_error.h"
+#include "lv_phase1.h"
+#include "lv_phase2.h"
+#include "lv_prims.h"
+
+namespace {
+
+thread_local std::string g_last_error;
+
+lv_status fail(lv_status st, const std::string &msg) {
+    g_last_error = msg;
+    return st;
+}
+
+#define LV_HIP(expr)                                                                         \
+    do {                                                                                     \
+        hipError_t e_ = (expr);                                                              \
+        if (e_ != hipSuccess)                                                                \
+            return fail(e_ == hipErrorOutOfMemory ? LV_ERR_OOM : LV_ERR_HIP,                 \
+                        std::string(#expr) + ": " + hipGetErrorString(e_));                  \
+    } while (0)
+
+#define LV_TRY(expr)                                                                         \
+    do {                                                                                     \
+        lv_status s_ = (expr);                                                               \
+        if (s_ != LV_OK) return s_;                                                          \
+    } while (0)
+
+#define LV_LAUNCHED(what)                                                                    \
+    do {                                                                                     \
+        hipError_t e_ = hipGetLastError();                                                   \
+        if (e_ != hipSuccess)                                                                \
+            return fail(LV_ERR_HIP, std::string("launch ") + what + ": " + hipGetErrorString(e_)); \
+    } while (0)
+
+__global__ void k_zero_u64(unsigned long long *p) { *p = 0ull; }
+
+// Row extents of the heavy vertices (host builds their work tables).
+__global__ void k_heavy_rows(const int32_t *__restrict__ rowptr, const int32_t *__restrict__ list,
+                             int32_t n, int64_t *out) {
+    const int32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) {
+        const int32_t v = list[i];
+        out[2 * i] = rowptr[v];
+        out[2 * i + 1] = rowptr[v + 1];
+    }
+}
+
+// Edge counts per bucket (algorithmic-byte accounting).
+__global__ void k_bucket_edges(const int32_t *__restrict__ rowptr, const int32_t *__restrict__ order,
+                               const int32_t *__restrict__ off, int nb,
+                               unsigned long long *edges) {
+    const int32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= off[nb]) return;
+    int b = 0;
+    while (b + 1 < nb && i >= off[b + 1]) ++b;
+    const int32_t v = order[i];
+    atomicAdd(&edges[b], (unsigned long long)(rowptr[v + 1] - rowptr[v]));
+}
+
+// Sizes and totals of an externally supplied state.
+__global__ void k_state_from_comm(int32_t V, const int32_t *__restrict__ comm,
+                                  const double *__restrict__ k, int32_t *size, double *sigma) {
+    const int32_t v = blockIdx.x * blockDim.x + threadIdx.x;
+    if (v < V) {
+        const int32_t c = comm[v];
+        atomicAdd(&size[c], 1);
+        atomicAdd(&sigma[c], k[v]);
+    }
+}
+
+// Summed member degree of each global-tier community (host builds its chunks).
+__global__ void k_agg_extent(const int32_t *__restrict__ list, int32_t n,
+                             const int32_t *__restrict__ size, const int32_t *__restrict__ mstart,
+                             const int32_t *__restrict__ mscan, int32_t *D) {
+    const int32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) {
+        const int32_t c = list[i];
+        D[i] = mscan[mstart[c] + size[c]] - mscan[mstart[c]];
+    }
+}
+
+template <typename T>
+lv_status dev_alloc(T **p, int64_t n) {
+    *p = nullptr;
+    if (n <= 0) n = 1;
+    hipError_t e = hipMalloc(reinterpret_cast<void **>(p), sizeof(T) * (size_t)n);
+    if (e != hipSuccess)
+        return fail(LV_ERR_OOM, "hipMalloc of " + std::to_string(sizeof(T) * (size_t)n) +
+                                    " bytes failed: " + hipGetErrorString(e));
+    return LV_OK;
+}
+
+template <typename T>
+void dev_free(T *&p) {
+    if (p) (void)hipFree(p);
+    p = nullptr;
+}
+
+// Growable device array.
+template <typename T>
+struct DevVec {
+    T *p = nullptr;
+    int64_t cap = 0;
+    lv_status reserve(int64_t n) {
+        if (n <= cap) return LV_OK;
+        dev_free(p);
+        cap = 0;
+        int64_t c = std::max<int64_t>(n, 1024);
+        LV_TRY(dev_alloc(&p, c));
+        cap = c;
+        return LV_OK;
+    }
+    void release() {
+        dev_free(p);
+        cap = 0;
+    }
+};
+
+unsigned blocks_for(int64_t n, int per) { return (unsigned)((n + per - 1) / per); }
+
+constexpr int kRowChunk = 4096;   // edges per heavy-row chunk
+constexpr int kSlotChunk = 4096;  // hash slots per evaluation chunk
+constexpr int kAggChunk = 4096;   // flat member edges per aggregation chunk
+
+int ceil_log2(int64_t x) {
+    int lg = 0;
+    while (((int64_t)1 << lg) < x) ++lg;
+    return lg;
+}
+
+}  // namespace
+
+void lv::set_error(const std::string &msg) { g_last_error = msg; }
+
+struct lv_graph {
+    int device = 0;
+    hipStream_t stream = nullptr;
+    lv_options opt{};
+    int max_lds = 65536;
+
+    // level 0 (kept) and two buffers for aggregated levels
+    int32_t V0 = 0;
+    int64_t E0 = 0;
+    int32_t *rp0 = nullptr, *col0 = nullptr;
+    float *w0 = nullptr;
+    int32_t *rpb[2] = {nullptr, nullptr}, *colb[2] = {nullptr, nullptr};
+    float *wb[2] = {nullptr, nullptr};
+    int cur_buf = -1;
+
+    int32_t V = 0;
+    int64_t E = 0;
+    double M = 0.0;
+    float M32 = 0.0f, den = 0.0f;
+    bool level_ready = false;
+
+    // community state
+    double *k = nullptr, *sigma = nullptr;
+    int32_t *comm = nullptr, *size = nullptr, *newc = nullptr, *orig2comm = nullptr;
+
+    // bucketing
+    lv::BucketBounds bb{};
+    int wg6_lg = 11, wg7_lg = 13, wg7_threads = 512;
+    lv::AggBounds agg_bounds{};
+    int8_t *keys8 = nullptr;
+    int32_t *order = nullptr, *d_off = nullptr;
+    int32_t h_off[lv::kNumBuckets + 1] = {0};
+    int64_t h_bedges[lv::kNumBuckets] = {0};
+    unsigned long long *d_bedges = nullptr;
+    int32_t *scratch = nullptr;
+    int64_t scratch_elems = 0;
+
+    // heavy vertices (bucket 8)
+    int32_t nh = 0, nins = 0, neval = 0;
+    DevVec<lv::Chunk> ins, eval;
+    DevVec<int64_t> hoff, hrows;
+    DevVec<int32_t> hlg;
+    DevVec<uint64_t> hbest;
+    DevVec<double> hkcur;
+    DevVec<int32_t> gkeys;  // global hash arena, kept all-empty between uses
+    DevVec<double> gvals;
+
+    // row-reduction segments
+    lv::SegTable st{};
+    DevVec<double> qpart;
+    double *d_q = nullptr;
+    unsigned long long *d_moved = nullptr;
+
+    // aggregation scratch
+    int32_t *a_newid = nullptr, *a_mstart = nullptr, *a_fill = nullptr, *a_members = nullptr,
+            *a_mscan = nullptr, *a_cnt = nullptr, *a_cstart = nullptr, *a_order = nullptr,
+            *a_off = nullptr, *a_scalars = nullptr;
+    int8_t *a_keys = nullptr;
+    int32_t *tmp_col = nullptr;
+    float *tmp_w = nullptr;
+    DevVec<lv::AggChunk> a_ins, a_ext;
+    DevVec<int64_t> a_toff, a_soff;
+    DevVec<int32_t> a_lg, a_D;
+    DevVec<uint64_t> sortA, sortB;
+
+    // pinned host staging
+    void *h_stage = nullptr;
+    size_t h_stage_bytes = 0;
+
+    // timing
+    struct Rec {
+        hipEvent_t a, b;
+        int kid;
+        double bytes;
+    };
+    std::vector<hipEvent_t> ev_pool;
+    std::vector<Rec> pending;
+    lv_kernel_time ktime[LV_K_COUNT]{};
+    double memory_ms = 0.0;
+
+    // ---------------------------------------------------------------- util
+    lv_status stage(size_t bytes) {
+        if (bytes <= h_stage_bytes) return LV_OK;
+        if (h_stage) (void)hipHostFree(h_stage);
+        h_stage = nullptr;
+        h_stage_bytes = 0;
+        size_t b = std::max<size_t>(bytes, 1 << 16);
+        LV_HIP(hipHostMalloc(&h_stage, b, hipHostMallocDefault));
+        h_stage_bytes = b;
+        return LV_OK;
+    }
+    lv_status sync() {
+        LV_HIP(hipStreamSynchronize(stream));
+        return flush_timers();
+    }
+    hipEvent_t take_event() {
+        if (!ev_pool.empty()) {
+            hipEvent_t e = ev_pool.back();
+            ev_pool.pop_back();
+            return e;
+        }
+        hipEvent_t e = nullptr;
+        if (hipEventCreate(&e) != hipSuccess) return nullptr;
+        return e;
+    }
+    hipEvent_t tbegin() {
+        if (!opt.timing) return nullptr;
+        hipEvent_t e = take_event();
+        if (e) (void)hipEventRecord(e, stream);
+        return e;
+    }
+    void tend(hipEvent_t a, int kid, double bytes) {
+        if (!opt.timing || !a) return;
+        hipEvent_t b = take_event();
+        if (!b) return;
+        (void)hipEventRecord(b, stream);
+        pending.push_back({a, b, kid, bytes});
+    }
+    lv_status flush_timers() {
+        for (auto &r : pending) {
+            float ms = 0.0f;
+            LV_HIP(hipEventElapsedTime(&ms, r.a, r.b));
+            ktime[r.kid].launches++;
+            ktime[r.kid].total_ms += ms;
+            ktime[r.kid].bytes += r.bytes;
+            ev_pool.push_back(r.a);
+            ev_pool.push_back(r.b);
+        }
+        pending.clear();
+        return LV_OK;
+    }
+
+    const int32_t *rowptr() const { return cur_buf < 0 ? rp0 : rpb[cur_buf]; }
+    const int32_t *col() const { return cur_buf < 0 ? col0 : colb[cur_buf]; }
+    const float *w() const { return cur_buf < 0 ? w0 : wb[cur_buf]; }
+
+    lv::MoveArgs move_args() const {
+        lv::MoveArgs a;
+        a.rowptr = rowptr();
+        a.col = col();
+        a.w = w();
+        a.comm = comm;
+        a.size = size;
+        a.k = k;
+        a.sigma = sigma;
+        a.newc = newc;
+        a.M32 = M32;
+        a.den = den;
+        return a;
+    }
+
+    lv::HeavyTables heavy_tables() {
+        lv::HeavyTables t;
+        t.keys = gkeys.p;
+        t.vals = gvals.p;
+        t.off = hoff.p;
+        t.cap_log2 = hlg.p;
+        t.best = hbest.p;
+        t.kcur = hkcur.p;
+        return t;
+    }
+
+    // Grow the global hash arena; new memory is cleared (the arena invariant).
+    lv_status arena(int64_t slots) {
+        if (slots <= gkeys.cap && slots <= gvals.cap) return LV_OK;
+        gkeys.release();
+        gvals.release();
+        LV_TRY(gkeys.reserve(slots));
+        LV_TRY(gvals.reserve(slots));
+        lv::launch_clear_tables(gkeys.p, gvals.p, gkeys.cap, stream);
+        LV_LAUNCHED("clear arena");
+        return LV_OK;
+    }
+
+    // ---------------------------------------------------------------- setup
+    lv_status create(const int32_t *rowptr_h, const int32_t *col_h, const float *w_h, int32_t nV,
+                     int64_t nE, const lv_options *o) {
+        if (o) opt = *o;
+        device = opt.device;
+        int ndev = 0;
+        if (hipGetDeviceCount(&ndev) != hipSuccess || ndev <= 0)
+            return fail(LV_ERR_NODEVICE, "no HIP device visible");
+        if (device < 0 || device >= ndev) return fail(LV_ERR_ARG, "device ordinal out of range");
+        LV_HIP(hipSetDevice(device));
+        LV_HIP(hipStreamCreateWithFlags(&stream, hipStreamNonBlocking));
+        int lds = 0;
+        if (hipDeviceGetAttribute(&lds, hipDeviceAttributeMaxSharedMemoryPerBlock, device) ==
+                hipSuccess &&
+            lds > 0)
+            max_lds = lds;
+        lv::set_wg_lds_limits(max_lds);
+        lv::set_merge_lds_limits(max_lds);
+        (void)hipGetLastError();
+        // LDS aggregation tiers the device can hold; the others stay empty.
+        for (int t = 0; t < lv::kAggTiers - 1; ++t) {
+            const bool fits = lv::merge_lds_bytes(t) <= max_lds;
+            agg_bounds.hi[t] = fits ? lv::kAggTierMax[t] : (t > 0 ? agg_bounds.hi[t - 1] : 0);
+        }
+        if (opt.agg_lds_max > 0)
+            for (int t = 0; t < lv::kAggTiers - 1; ++t)
+                agg_bounds.hi[t] = std::min(agg_bounds.hi[t], opt.agg_lds_max);
+
+        V0 = V = nV;
+        E0 = E = nE;
+        // Bucket bounds: 0..5 are the reference groups; group 6 splits into
+        // a 256-lane tier (cap 2^wg6_lg), a large-LDS tier and global tier.
+        wg6_lg = 11;
+        const int lds_lg = lv::move_wg_max_cap_log2(max_lds - 4096);
+        wg7_lg = std::min(13, lds_lg);
+        int mid_max = opt.mid_wg_max > 0 ? opt.mid_wg_max : 1536;
+        mid_max = std::max(320, std::min(mid_max, (3 << wg6_lg) / 4));
+        int heavy_max = opt.heavy_wg_max > 0 ? opt.heavy_wg_max : (3 << wg7_lg) / 4;
+        heavy_max = std::max(mid_max + 1, std::min(heavy_max, (3 << wg7_lg) / 4));
+        const int32_t his[lv::kNumBuckets] = {4, 8, 16, 32, 84, 319, mid_max, heavy_max,
+                                              0x7fffffff};
+        for (int b = 0; b < lv::kNumBuckets; ++b) bb.hi[b] = his[b];
+
+        // M = (sum of CSR weights)/2 in FP64, CSR order (utils.cu:36-46).
+        double s = 0.0;
+        for (int64_t e = 0; e < nE; ++e) s += (double)w_h[e];
+        M = s / 2.0;
+        M32 = (float)M;
+        {
+            volatile float two_m = 2.0f * M32;
+            volatile float d = two_m * M32;
+            den = d;
+        }
+
+        const int64_t Vp = (int64_t)nV + 1;
+        LV_TRY(dev_alloc(&rp0, Vp));
+        LV_TRY(dev_alloc(&col0, nE));
+        LV_TRY(dev_alloc(&w0, nE));
+        for (int i = 0; i < 2; ++i) {
+            LV_TRY(dev_alloc(&rpb[i], Vp));
+            LV_TRY(dev_alloc(&colb[i], nE));
+            LV_TRY(dev_alloc(&wb[i], nE));
+        }
+        LV_TRY(dev_alloc(&k, Vp));
+        LV_TRY(dev_alloc(&sigma, Vp));
+        LV_TRY(dev_alloc(&comm, Vp));
+        LV_TRY(dev_alloc(&size, Vp));
+        LV_TRY(dev_alloc(&newc, Vp));
+        LV_TRY(dev_alloc(&orig2comm, Vp));
+        LV_TRY(dev_alloc(&keys8, Vp));
+        LV_TRY(dev_alloc(&order, Vp));
+        LV_TRY(dev_alloc(&d_off, 64));
+        LV_TRY(dev_alloc(&d_bedges, 16));
+        scratch_elems = std::max(lv::partition_scratch_elems(Vp, lv::kNumBuckets),
+                                 lv::scan_scratch_elems(Vp) + 8) +
+                        64;
+        LV_TRY(dev_alloc(&scratch, scratch_elems));
+        LV_TRY(dev_alloc(&d_q, 8));
+        LV_TRY(dev_alloc(&d_moved, 2));
+        LV_TRY(dev_alloc(&a_newid, Vp));
+        LV_TRY(dev_alloc(&a_mstart, Vp));
+        LV_TRY(dev_alloc(&a_fill, Vp));
+        LV_TRY(dev_alloc(&a_members, Vp));
+        LV_TRY(dev_alloc(&a_mscan, Vp + 1));
+        LV_TRY(dev_alloc(&a_cnt, Vp));
+        LV_TRY(dev_alloc(&a_cstart, Vp));
+        LV_TRY(dev_alloc(&a_order, Vp));
+        LV_TRY(dev_alloc(&a_off, 16));
+        LV_TRY(dev_alloc(&a_scalars, 16));
+        LV_TRY(dev_alloc(&a_keys, Vp));
+        LV_TRY(dev_alloc(&tmp_col, nE));
+        LV_TRY(dev_alloc(&tmp_w, nE));
+        LV_TRY(stage(1 << 20));
+
+        hipEvent_t e0 = nullptr, e1 = nullptr;
+        LV_HIP(hipEventCreate(&e0));
+        LV_HIP(hipEventCreate(&e1));
+        LV_HIP(hipEventRecord(e0, stream));
+        LV_HIP(hipMemcpyAsync(rp0, rowptr_h, sizeof(int32_t) * Vp, hipMemcpyHostToDevice, stream));
+        if (nE > 0) {
+            LV_HIP(hipMemcpyAsync(col0, col_h, sizeof(int32_t) * nE, hipMemcpyHostToDevice, stream));
+            LV_HIP(hipMemcpyAsync(w0, w_h, sizeof(float) * nE, hipMemcpyHostToDevice, stream));
+        }
+        LV_HIP(hipEventRecord(e1, stream));
+        LV_HIP(hipStreamSynchronize(stream));
+        float ms = 0.0f;
+        LV_HIP(hipEventElapsedTime(&ms, e0, e1));
+        memory_ms = ms;
+        (void)hipEventDestroy(e0);
+        (void)hipEventDestroy(e1);
+        return reset();
+    }
+
+    // Back to the singleton state of level 0.  When level 0 is current and
+    // set up, only the community state is rewritten (no re-bucketing).
+    lv_status reset() {
+        lv::iota_i32(orig2comm, V0, stream);
+        if (cur_buf < 0 && level_ready) {
+            lv::launch_level_state(V, k, comm, size, sigma, stream);
+            LV_LAUNCHED("reset");
+            return LV_OK;
+        }
+        cur_buf = -1;
+        V = V0;
+        E = E0;
+        LV_LAUNCHED("reset");
+        level_ready = false;
+        return LV_OK;
+    }
+
+    lv_status setup_level() {
+        hipStream_t s = stream;
+        const int32_t *rp = rowptr();
+        lv::launch_degree_keys(rp, V, bb, keys8, s);
+        lv::stable_partition(keys8, V, lv::kNumBuckets, order, d_off, scratch, s);
+        hipLaunchKernelGGL(k_zero_u64, dim3(1), dim3(1), 0, s, d_bedges);
+        for (int b = 1; b < lv::kNumBuckets; ++b)
+            hipLaunchKernelGGL(k_zero_u64, dim3(1), dim3(1), 0, s, d_bedges + b);
+        if (V > 0)
+            hipLaunchKernelGGL(k_bucket_edges, dim3(blocks_for(V, 256)), dim3(256), 0, s, rp, order,
+                               d_off, lv::kNumBuckets, d_bedges);
+        LV_LAUNCHED("bucketing");
+        LV_TRY(stage(4096));
+        LV_HIP(hipMemcpyAsync(h_stage, d_off, sizeof(int32_t) * (lv::kNumBuckets + 1),
+                              hipMemcpyDeviceToHost, s));
+        LV_HIP(hipMemcpyAsync((char *)h_stage + 256, d_bedges,
+                              sizeof(unsigned long long) * lv::kNumBuckets, hipMemcpyDeviceToHost,
+                              s));
+        LV_TRY(sync());
+        std::memcpy(h_off, h_stage, sizeof(int32_t) * (lv::kNumBuckets + 1));
+        {
+            unsigned long long tmp[lv::kNumBuckets];
+            std::memcpy(tmp, (char *)h_stage + 256, sizeof tmp);
+            for (int b = 0; b < lv::kNumBuckets; ++b) h_bedges[b] = (int64_t)tmp[b];
+        }
+
+        // heavy vertices: row chunks (insert, strength, modularity), tables
+        const int HB = lv::kNumBuckets - 1;
+        nh = h_off[HB + 1] - h_off[HB];
+        nins = neval = 0;
+        std::vector<lv::Chunk> hin, hev;
+        if (nh > 0) {
+            LV_TRY(hrows.reserve(2 * (int64_t)nh));
+            hipLaunchKernelGGL(k_heavy_rows, dim3(blocks_for(nh, 256)), dim3(256), 0, s, rp,
+                               order + h_off[HB], nh, hrows.p);
+            LV_LAUNCHED("heavy rows");
+            std::vector<int64_t> rows(2 * (size_t)nh);
+            std::vector<int32_t> ids(nh);
+            LV_HIP(hipMemcpyAsync(rows.data(), hrows.p, sizeof(int64_t) * 2 * nh,
+                                  hipMemcpyDeviceToHost, s));
+            LV_HIP(hipMemcpyAsync(ids.data(), order + h_off[HB], sizeof(int32_t) * nh,
+                                  hipMemcpyDeviceToHost, s));
+            LV_TRY(sync());
+            std::vector<int64_t> off(nh);
+            std::vector<int32_t> lg(nh);
+            int64_t slots = 0;
+            for (int32_t i = 0; i < nh; ++i) {
+                const int64_t lo = rows[2 * i], hi = rows[2 * i + 1];
+                for (int64_t e = lo; e < hi; e += kRowChunk)
+                    hin.push_back({i, ids[i], e, std::min<int64_t>(hi, e + kRowChunk)});
+                lg[i] = std::max(6, ceil_log2(2 * (hi - lo)));
+                off[i] = slots;
+                for (int64_t q = 0; q < ((int64_t)1 << lg[i]); q += kSlotChunk)
+                    hev.push_back({i, ids[i], slots + q,
+                                   slots + std::min<int64_t>((int64_t)1 << lg[i], q + kSlotChunk)});
+                slots += (int64_t)1 << lg[i];
+            }
+            nins = (int32_t)hin.size();
+            neval = (int32_t)hev.size();
+            LV_TRY(arena(slots));
+            LV_TRY(ins.reserve(nins));
+            LV_TRY(eval.reserve(neval));
+            LV_TRY(hoff.reserve(nh));
+            LV_TRY(hlg.reserve(nh));
+            LV_TRY(hbest.reserve(nh));
+            LV_TRY(hkcur.reserve(nh));
+            LV_HIP(hipMemcpy(ins.p, hin.data(), sizeof(lv::Chunk) * nins, hipMemcpyHostToDevice));
+            LV_HIP(hipMemcpy(eval.p, hev.data(), sizeof(lv::Chunk) * neval, hipMemcpyHostToDevice));
+            LV_HIP(hipMemcpy(hoff.p, off.data(), sizeof(int64_t) * nh, hipMemcpyHostToDevice));
+            LV_HIP(hipMemcpy(hlg.p, lg.data(), sizeof(int32_t) * nh, hipMemcpyHostToDevice));
+            std::vector<uint64_t> zb(nh, 0);
+            std::vector<double> zk(nh, 0.0);
+            LV_HIP(hipMemcpy(hbest.p, zb.data(), sizeof(uint64_t) * nh, hipMemcpyHostToDevice));
+            LV_HIP(hipMemcpy(hkcur.p, zk.data(), sizeof(double) * nh, hipMemcpyHostToDevice));
+        }
+
+        // row-reduction segments: lanes per vertex by bucket
+        static const int lanes_lg[lv::kNumBuckets - 1] = {2, 3, 4, 5, 6, 6, 8, 8};
+        st = lv::SegTable{};
+        int32_t blk = 0;
+        for (int b = 0; b < HB; ++b) {
+            const int32_t cnt = h_off[b + 1] - h_off[b];
+            if (cnt <= 0) continue;
+            lv::Seg sg;
+            sg.list_off = h_off[b];
+            sg.count = cnt;
+            sg.block_start = blk;
+            sg.lanes_log2 = lanes_lg[b];
+            const int per = lanes_lg[b] >= 8 ? 1 : (lv::kRowThreads >> lanes_lg[b]);
+            blk += (cnt + per - 1) / per;
+            st.s[st.nseg++] = sg;
+        }
+        if (st.nseg == 0) {  // keep find_seg well-defined
+            st.s[0] = lv::Seg{0, 0, 0, 2};
+            st.nseg = 1;
+        }
+        st.chunk_block_start = blk;
+        st.nchunks = nins;
+        blk += nins;
+        st.sq_block_start = blk;
+        blk += (int32_t)((V + lv::kSqItems - 1) / lv::kSqItems);
+        st.total_blocks = std::max(blk, 1);
+        LV_TRY(qpart.reserve(2 * (int64_t)st.total_blocks));
+
+        // strengths and singleton state (utils.cu:84-89; computeEdgesSum)
+        lv::fill_f64(k, 0.0, V, s);
+        lv::launch_strength(rp, w(), order, st, ins.p, k, s);
+        lv::launch_level_state(V, k, comm, size, sigma, s);
+        LV_LAUNCHED("level state");
+        level_ready = true;
+        return LV_OK;
+    }
+
+    // ---------------------------------------------------------------- phase 1
+    lv_status run_group(int g) {
+        hipStream_t s = stream;
+        const lv::MoveArgs a = move_args();
+        for (int b = 0; b < lv::kNumBuckets; ++b) {
+            if (lv::kBucketGroup[b] != g) continue;
+            const int32_t n = h_off[b + 1] - h_off[b];
+            if (n <= 0) continue;
+            const int32_t *list = order + h_off[b];
+            const double bytes = 12.0 * (double)h_bedges[b] + 32.0 * (double)n;
+            hipEvent_t t0 = nullptr;
+            switch (b) {
+            case 0: case 1: case 2: case 3:
+                t0 = tbegin();
+                lv::launch_move_sub(4 << b, a, list, n, s);
+                tend(t0, LV_K_MOVE_SUB, bytes);
+                break;
+            case 4: case 5:
+                t0 = tbegin();
+                lv::launch_move_wave(b == 4 ? 256 : 512, a, list, n, s);
+                tend(t0, LV_K_MOVE_WAVE, bytes);
+                break;
+            case 6:
+                t0 = tbegin();
+                lv::launch_move_wg(256, wg6_lg, a, list, n, s);
+                tend(t0, LV_K_MOVE_WG, bytes);
+                break;
+            case 7:
+                t0 = tbegin();
+                lv::launch_move_wg(wg7_threads, wg7_lg, a, list, n, s);
+                tend(t0, LV_K_MOVE_WG, bytes);
+                break;
+            default:
+                t0 = tbegin();
+                lv::launch_move_heavy(a, list, nh, ins.p, nins, eval.p, neval, heavy_tables(), s);
+                tend(t0, LV_K_MOVE_GLOBAL, bytes);
+                break;
+            }
+        }
+        LV_LAUNCHED("move kernels");
+        return LV_OK;
+    }
+
+    void group_range(int g, int32_t &lo, int32_t &hi) const {
+        lo = -1;
+        hi = -1;
+        for (int b = 0; b < lv::kNumBuckets; ++b) {
+            if (lv::kBucketGroup[b] != g) continue;
+            if (lo < 0) lo = h_off[b];
+            hi = h_off[b + 1];
+        }
+    }
+
+    lv_status enqueue_modularity() {
+        hipEvent_t t0 = tbegin();
+        lv::launch_modularity(rowptr(), col(), w(), comm, sigma, V, order, st, ins.p, M, qpart.p, d_q,
+                              stream);
+        tend(t0, LV_K_MODULARITY, 12.0 * (double)E + 16.0 * (double)V);
+        LV_LAUNCHED("modularity");
+        return LV_OK;
+    }
+
+    lv_status modularity(double *q) {
+        if (!level_ready) LV_TRY(setup_level());
+        LV_TRY(enqueue_modularity());
+        LV_TRY(stage(64));
+        LV_HIP(hipMemcpyAsync(h_stage, d_q, sizeof(double), hipMemcpyDeviceToHost, stream));
+        LV_TRY(sync());
+        std::memcpy(q, h_stage, sizeof(double));
+        return LV_OK;
+    }
+
+    // One sweep: groups 0..6, each = moves then apply; then Q.
+    lv_status sweep(double *q_after, int64_t *moved, double *sweep_ms) {
+        if (!level_ready) LV_TRY(setup_level());
+        hipStream_t s = stream;
+        hipEvent_t e0 = take_event(), e1 = take_event();
+        if (e0) (void)hipEventRecord(e0, s);
+        hipLaunchKernelGGL(k_zero_u64, dim3(1), dim3(1), 0, s, d_moved);
+        for (int g = 0; g < lv::kNumGroups; ++g) {
+            int32_t lo, hi;
+            group_range(g, lo, hi);
+            if (lo < 0 || hi <= lo) continue;
+            LV_TRY(run_group(g));
+            hipEvent_t t0 = tbegin();
+            lv::launch_apply(order + lo, hi - lo, newc, comm, size, sigma, k, d_moved, s);
+            tend(t0, LV_K_APPLY, 16.0 * (double)(hi - lo));
+            LV_LAUNCHED("apply");
+        }
+        LV_TRY(enqueue_modularity());
+        if (e1) (void)hipEventRecord(e1, s);
+        LV_TRY(stage(64));
+        LV_HIP(hipMemcpyAsync(h_stage, d_q, sizeof(double), hipMemcpyDeviceToHost, s));
+        LV_HIP(hipMemcpyAsync((char *)h_stage + 8, d_moved, sizeof(unsigned long long),
+                              hipMemcpyDeviceToHost, s));
+        LV_TRY(sync());
+        std::memcpy(q_after, h_stage, sizeof(double));
+        unsigned long long mv = 0;
+        std::memcpy(&mv, (char *)h_stage + 8, sizeof mv);
+        *moved = (int64_t)mv;
+        if (sweep_ms) {
+            float ms = 0.0f;
+            if (e0 && e1) (void)hipEventElapsedTime(&ms, e0, e1);
+            *sweep_ms = ms;
+        }
+        if (e0) ev_pool.push_back(e0);
+        if (e1) ev_pool.push_back(e1);
+        return LV_OK;
+    }
+
+    // Phase (modularity_optimisation.cu:284-363): sweeps while the gain
+    // reaches min_gain and is positive.
+    lv_status optimise(float min_gain, int32_t *changed, int32_t *sweeps_out, double *ms_out,
+                       int64_t *edges_out) {
+        if (!level_ready) LV_TRY(setup_level());
+        double q0 = 0.0;
+        LV_TRY(modularity(&q0));
+        int32_t ch = 0, sweeps = 0;
+        double ms_total = 0.0;
+        for (;;) {
+            double q1 = 0.0, ms = 0.0;
+            int64_t mv = 0;
+            LV_TRY(sweep(&q1, &mv, &ms));
+            sweeps++;
+            ms_total += ms;
+            const double gain = q1 - q0;
+            if (gain > 0.0) ch = 1;
+            // A sweep without positive gain also ends the phase (identical
+            // for min_gain > 0; guarantees termination for min_gain <= 0).
+            if (mv == 0 || !(gain > 0.0)) break;
+            if (!(gain >= (double)min_gain)) break;
+            q0 = q1;
+        }
+        lv::launch_compose(orig2comm, V0, comm, stream);
+        LV_LAUNCHED("compose");
+        if (changed) *changed = ch;
+        if (sweeps_out) *sweeps_out = sweeps;
+        if (ms_out) *ms_out = ms_total;
+        if (edges_out) *edges_out = (int64_t)sweeps * E;
+        return LV_OK;
+    }
+
+    // ---------------------------------------------------------------- phase 2
+    lv_status aggregate() {
+        if (!level_ready) LV_TRY(setup_level());
+        hipStream_t s = stream;
+        hipEvent_t t0 = tbegin();
+        const int32_t *rp = rowptr();
+        int32_t *newV_d = a_scalars;
+        int32_t *etot_d = a_scalars + 1;
+        // renumbering (community_aggregation.cu:267-275)
+        lv::launch_present(size, V, a_newid, s);
+        lv::exclusive_scan(a_newid, a_newid, V, newV_d, scratch, s);
+        lv::exclusive_scan(size, a_mstart, V, nullptr, scratch, s);
+        lv::fill_i32(a_fill, 0, V, s);
+        lv::launch_members(comm, V, a_mstart, a_fill, a_members, s);
+        lv::launch_member_degrees(rp, a_members, V, a_mscan, s);
+        lv::exclusive_scan(a_mscan, a_mscan, V, a_mscan + V, scratch, s);
+        lv::launch_agg_keys(size, a_mstart, a_mscan, V, agg_bounds, a_keys, s);
+        lv::stable_partition(a_keys, V, lv::kAggTiers, a_order, a_off, scratch, s);
+        lv::fill_i32(a_cnt, 0, V, s);
+        LV_LAUNCHED("aggregation setup");
+        LV_TRY(stage(4096));
+        LV_HIP(hipMemcpyAsync(h_stage, a_off, sizeof(int32_t) * (lv::kAggTiers + 1),
+                              hipMemcpyDeviceToHost, s));
+        LV_HIP(hipMemcpyAsync((char *)h_stage + 64, newV_d, sizeof(int32_t),
+                              hipMemcpyDeviceToHost, s));
+        LV_TRY(sync());
+        int32_t toff[lv::kAggTiers + 1];
+        std::memcpy(toff, h_stage, sizeof toff);
+        int32_t newV = 0;
+        std::memcpy(&newV, (char *)h_stage + 64, sizeof newV);
+
+        lv::AggArgs a;
+        a.rowptr = rp;
+        a.col = col();
+        a.w = w();
+        a.comm = comm;
+        a.size = size;
+        a.newid = a_newid;
+        a.mstart = a_mstart;
+        a.members = a_members;
+        a.mscan = a_mscan;
+        a.cnt = a_cnt;
+        a.tmp_col = tmp_col;
+        a.tmp_w = tmp_w;
+        for (int t = 0; t < lv::kAggTiers - 1; ++t)
+            lv::launch_merge_lds(t, a, a_order + toff[t], toff[t + 1] - toff[t], s);
+        LV_LAUNCHED("merge (LDS tiers)");
+
+        const int GT = lv::kAggTiers - 1;
+        const int32_t ng = toff[GT + 1] - toff[GT];
+        if (ng > 0) {
+            const int32_t *glist = a_order + toff[GT];
+            LV_TRY(a_D.reserve(ng));
+            hipLaunchKernelGGL(k_agg_extent, dim3(blocks_for(ng, 256)), dim3(256), 0, s, glist, ng,
+                               size, a_mstart, a_mscan, a_D.p);
+            LV_LAUNCHED("aggregation extents");
+            std::vector<int32_t> D(ng), ids(ng);
+            LV_HIP(hipMemcpyAsync(D.data(), a_D.p, sizeof(int32_t) * ng, hipMemcpyDeviceToHost, s));
+            LV_HIP(hipMemcpyAsync(ids.data(), glist, sizeof(int32_t) * ng, hipMemcpyDeviceToHost,
+                                  s));
+            LV_TRY(sync());
+            std::vector<lv::AggChunk> hin, hex;
+            std::vector<int64_t> toffs(ng), soffs(ng);
+            std::vector<int32_t> lg(ng);
+            int64_t slots = 0, sorts = 0;
+            for (int32_t i = 0; i < ng; ++i) {
+                const int64_t d = D[i];
+                for (int64_t f = 0; f < d; f += kAggChunk)
+                    hin.push_back({ids[i], i, f, std::min<int64_t>(d, f + kAggChunk)});
+                const int64_t distinct_max = std::min<int64_t>(d, (int64_t)newV);
+                lg[i] = std::max(6, ceil_log2(2 * distinct_max));
+                toffs[i] = slots;
+                const int64_t cap = (int64_t)1 << lg[i];
+                for (int64_t q = 0; q < cap; q += kSlotChunk)
+                    hex.push_back({ids[i], i, slots + q, slots + std::min<int64_t>(cap, q + kSlotChunk)});
+                slots += cap;
+                soffs[i] = sorts;
+                sorts += d;
+            }
+            LV_TRY(arena(slots));
+            LV_TRY(a_ins.reserve((int64_t)hin.size()));
+            LV_TRY(a_ext.reserve((int64_t)hex.size()));
+            LV_TRY(a_toff.reserve(ng));
+            LV_TRY(a_soff.reserve(ng));
+            LV_TRY(a_lg.reserve(ng));
+            LV_TRY(sortA.reserve(sorts));
+            LV_TRY(sortB.reserve(sorts));
+            LV_HIP(hipMemcpy(a_ins.p, hin.data(), sizeof(lv::AggChunk) * hin.size(),
+                             hipMemcpyHostToDevice));
+            LV_HIP(hipMemcpy(a_ext.p, hex.data(), sizeof(lv::AggChunk) * hex.size(),
+                             hipMemcpyHostToDevice));
+            LV_HIP(hipMemcpy(a_toff.p, toffs.data(), sizeof(int64_t) * ng, hipMemcpyHostToDevice));
+            LV_HIP(hipMemcpy(a_soff.p, soffs.data(), sizeof(int64_t) * ng, hipMemcpyHostToDevice));
+            LV_HIP(hipMemcpy(a_lg.p, lg.data(), sizeof(int32_t) * ng, hipMemcpyHostToDevice));
+            lv::launch_agg_global(a, a_ins.p, (int32_t)hin.size(), a_ext.p, (int32_t)hex.size(),
+                                  glist, ng, a_toff.p, a_lg.p, gkeys.p, gvals.p, sortA.p, sortB.p,
+                                  a_soff.p, s);
+            LV_LAUNCHED("merge (global tier)");
+        }
+
+        // CSR offsets and compaction (compressEdges, community_aggregation.cu:196-220)
+        const int nb = cur_buf == 0 ? 1 : 0;
+        lv::exclusive_scan(a_cnt, a_cstart, V, etot_d, scratch, s);
+        lv::launch_new_rowptr(size, a_newid, a_cstart, V, newV_d, etot_d, rpb[nb], s);
+        for (int t = 0; t < lv::kAggTiers; ++t)
+            lv::launch_copy_rows(a_order + toff[t], toff[t + 1] - toff[t], t >= 2, a_mstart,
+                                 a_mscan, a_cnt, a_cstart, tmp_col, tmp_w, colb[nb], wb[nb], s);
+        // updateOriginalToCommunity (community_aggregation.cu:222-228)
+        lv::launch_compose(orig2comm, V0, a_newid, s);
+        LV_LAUNCHED("aggregation output");
+        tend(t0, LV_K_AGGREGATE, 24.0 * (double)E + 40.0 * (double)V);
+        LV_HIP(hipMemcpyAsync(h_stage, etot_d, sizeof(int32_t), hipMemcpyDeviceToHost, s));
+        LV_TRY(sync());
+        int32_t newE = 0;
+        std::memcpy(&newE, h_stage, sizeof newE);
+        cur_buf = nb;
+        V = newV;
+        E = newE;
+        level_ready = false;
+        return LV_OK;
+    }
+
+    lv_status destroy() {
+        if (stream) (void)hipStreamSynchronize(stream);
+        dev_free(rp0);
+        dev_free(col0);
+        dev_free(w0);
+        for (int i = 0; i < 2; ++i) {
+            dev_free(rpb[i]);
+            dev_free(colb[i]);
+            dev_free(wb[i]);
+        }
+        dev_free(k);
+        dev_free(sigma);
+        dev_free(comm);
+        dev_free(size);
+        dev_free(newc);
+        dev_free(orig2comm);
+        dev_free(keys8);
+        dev_free(order);
+        dev_free(d_off);
+        dev_free(d_bedges);
+        dev_free(scratch);
+        dev_free(d_q);
+        dev_free(d_moved);
+        dev_free(a_newid);
+        dev_free(a_mstart);
+        dev_free(a_fill);
+        dev_free(a_members);
+        dev_free(a_mscan);
+        dev_free(a_cnt);
+        dev_free(a_cstart);
+        dev_free(a_order);
+        dev_free(a_off);
+        dev_free(a_scalars);
+        dev_free(a_keys);
+        dev_free(tmp_col);
+        dev_free(tmp_w);
+        ins.release();
+        eval.release();
+        hoff.release();
+        hrows.release();
+        hlg.release();
+        hbest.release();
+        hkcur.release();
+        gkeys.release();
+        gvals.release();
+        qpart.release();
+        a_ins.release();
+        a_ext.release();
+        a_toff.release();
+        a_soff.release();
+        a_lg.release();
+        a_D.release();
+        sortA.release();
+        sortB.release();
+        if (h_stage) (void)hipHostFree(h_stage);
+        h_stage = nullptr;
+        for (auto &r : pending) {
+            ev_pool.push_back(r.a);
+            ev_pool.push_back(r.b);
+        }
+        pending.clear();
+        for (auto e : ev_pool) (void)hipEventDestroy(e);
+        ev_pool.clear();
+        if (stream) (void)hipStreamDestroy(stream);
+        stream = nullptr;
+        return LV_OK;
+    }
+};
+
+// ====================================================================== C-ABI
+
+extern "C" {
+
+const char *lv_last_error(void) { return g_last_error.c_str(); }
+
+lv_status lv_create(const int32_t *rowptr, const int32_t *col, const float *w, int32_t V,
+                    int64_t E, const lv_options *opt, lv_graph **out) {
+    if (!out) return fail(LV_ERR_ARG, "out is null");
+    *out = nullptr;
+    if (V < 0 || E < 0 || E > 0x7fffffffLL) return fail(LV_ERR_ARG, "V/E out of range");
+    if (!rowptr || (E > 0 && (!col || !w))) return fail(LV_ERR_ARG, "null CSR array");
+    if (rowptr[0] != 0 || (int64_t)rowptr[V] != E) return fail(LV_ERR_ARG, "rowptr[0]/rowptr[V] mismatch");
+    for (int32_t v = 0; v < V; ++v)
+        if (rowptr[v + 1] < rowptr[v]) return fail(LV_ERR_ARG, "rowptr not monotone");
+    for (int64_t e = 0; e < E; ++e)
+        if (col[e] < 0 || col[e] >= V) return fail(LV_ERR_ARG, "column index out of range");
+    lv_graph *g = new (std::nothrow) lv_graph();
+    if (!g) return fail(LV_ERR_OOM, "host allocation failed");
+    lv_status st = g->create(rowptr, col, w, V, E, opt);
+    if (st != LV_OK) {
+        std::string msg = g_last_error;
+        g->destroy();
+        delete g;
+        g_last_error = msg;
+        return st;
+    }
+    *out = g;
+    return LV_OK;
+}
+
+void lv_destroy(lv_graph *g) {
+    if (!g) return;
+    (void)hipSetDevice(g->device);
+    g->destroy();
+    delete g;
+}
+
+lv_status lv_reset(lv_graph *g) {
+    if (!g) return fail(LV_ERR_ARG, "null graph");
+    LV_HIP(hipSetDevice(g->device));
+    return g->reset();
+}
+
+lv_status lv_optimise(lv_graph *g, float min_gain, int32_t *changed) {
+    if (!g) return fail(LV_ERR_ARG, "null graph");
+    LV_HIP(hipSetDevice(g->device));
+    return g->optimise(min_gain, changed, nullptr, nullptr, nullptr);
+}
+
+lv_status lv_aggregate(lv_graph *g) {
+    if (!g) return fail(LV_ERR_ARG, "null graph");
+    LV_HIP(hipSetDevice(g->device));
+    return g->aggregate();
+}
+
+lv_status lv_modularity(lv_graph *g, double *q) {
+    if (!g || !q) return fail(LV_ERR_ARG, "null argument");
+    LV_HIP(hipSetDevice(g->device));
+    return g->modularity(q);
+}
+
+lv_status lv_sweep(lv_graph *g, double *q_after, int64_t *moved) {
+    if (!g) return fail(LV_ERR_ARG, "null graph");
+    LV_HIP(hipSetDevice(g->device));
+    double q1 = 0.0;
+    int64_t mv = 0;
+    LV_TRY(g->sweep(&q1, &mv, nullptr));
+    if (q_after) *q_after = q1;
+    if (moved) *moved = mv;
+    return LV_OK;
+}
+
+lv_status lv_run(lv_graph *g, float min_gain, lv_stats *stats) {
+    if (!g) return fail(LV_ERR_ARG, "null graph");
+    LV_HIP(hipSetDevice(g->device));
+    hipEvent_t e0, e1;
+    LV_HIP(hipEventCreate(&e0));
+    LV_HIP(hipEventCreate(&e1));
+    LV_HIP(hipEventRecord(e0, g->stream));
+    lv_stats st{};
+    double agg_ms = 0.0;
+    for (;;) {
+        int32_t changed = 0, sw = 0;
+        double ms = 0.0;
+        int64_t edges = 0;
+        LV_TRY(g->optimise(min_gain, &changed, &sw, &ms, &edges));
+        if (st.levels == 0) {
+            st.level0_sweeps = sw;
+            st.level0_sweep_ms = ms;
+        }
+        st.levels++;
+        st.sweeps += sw;
+        st.sweep_ms += ms;
+        st.edges_swept += edges;
+        if (!changed) break;
+        hipEvent_t a0, a1;
+        LV_HIP(hipEventCreate(&a0));
+        LV_HIP(hipEventCreate(&a1));
+        LV_HIP(hipEventRecord(a0, g->stream));
+        LV_TRY(g->aggregate());
+        LV_HIP(hipEventRecord(a1, g->stream));
+        LV_HIP(hipEventSynchronize(a1));
+        float ams = 0.0f;
+        LV_HIP(hipEventElapsedTime(&ams, a0, a1));
+        agg_ms += ams;
+        (void)hipEventDestroy(a0);
+        (void)hipEventDestroy(a1);
+    }
+    double q = 0.0;
+    LV_TRY(g->modularity(&q));
+    LV_HIP(hipEventRecord(e1, g->stream));
+    LV_HIP(hipEventSynchronize(e1));
+    float ms = 0.0f;
+    LV_HIP(hipEventElapsedTime(&ms, e0, e1));
+    (void)hipEventDestroy(e0);
+    (void)hipEventDestroy(e1);
+    st.modularity = q;
+    st.algo_ms = ms;
+    st.aggregate_ms = agg_ms;
+    st.memory_ms = g->memory_ms;
+    st.final_V = g->V;
+    st.final_E = g->E;
+    if (stats) *stats = st;
+    return LV_OK;
+}
+
+lv_status lv_assignment(lv_graph *g, int32_t *orig2comm, int32_t *level_V) {
+    if (!g) return fail(LV_ERR_ARG, "null graph");
+    LV_HIP(hipSetDevice(g->device));
+    if (orig2comm && g->V0 > 0)
+        LV_HIP(hipMemcpyAsync(orig2comm, g->orig2comm, sizeof(int32_t) * g->V0,
+                              hipMemcpyDeviceToHost, g->stream));
+    LV_TRY(g->sync());
+    if (level_V) *level_V = g->V;
+    return LV_OK;
+}
+
+lv_status lv_level_size(lv_graph *g, int32_t *V, int64_t *E) {
+    if (!g) return fail(LV_ERR_ARG, "null graph");
+    if (V) *V = g->V;
+    if (E) *E = g->E;
+    return LV_OK;
+}
+
+lv_status lv_level_graph(lv_graph *g, int32_t *rowptr, int32_t *col, float *w) {
+    if (!g) return fail(LV_ERR_ARG, "null graph");
+    LV_HIP(hipSetDevice(g->device));
+    if (rowptr)
+        LV_HIP(hipMemcpyAsync(rowptr, g->rowptr(), sizeof(int32_t) * ((size_t)g->V + 1),
+                              hipMemcpyDeviceToHost, g->stream));
+    if (col && g->E > 0)
+        LV_HIP(hipMemcpyAsync(col, g->col(), sizeof(int32_t) * g->E, hipMemcpyDeviceToHost,
+                              g->stream));
+    if (w && g->E > 0)
+        LV_HIP(hipMemcpyAsync(w, g->w(), sizeof(float) * g->E, hipMemcpyDeviceToHost, g->stream));
+    return g->sync();
+}
+
+lv_status lv_get_communities(lv_graph *g, int32_t *comm) {
+    if (!g || !comm) return fail(LV_ERR_ARG, "null argument");
+    LV_HIP(hipSetDevice(g->device));
+    if (!g->level_ready) LV_TRY(g->setup_level());
+    if (g->V > 0)
+        LV_HIP(hipMemcpyAsync(comm, g->comm, sizeof(int32_t) * g->V, hipMemcpyDeviceToHost,
+                              g->stream));
+    return g->sync();
+}
+
+lv_status lv_set_communities(lv_graph *g, const int32_t *comm) {
+    if (!g || !comm) return fail(LV_ERR_ARG, "null argument");
+    for (int32_t v = 0; v < g->V; ++v)
+        if (comm[v] < 0 || comm[v] >= g->V) return fail(LV_ERR_ARG, "community id out of range");
+    LV_HIP(hipSetDevice(g->device));
+    if (!g->level_ready) LV_TRY(g->setup_level());
+    if (g->V > 0) {
+        LV_HIP(hipMemcpyAsync(g->comm, comm, sizeof(int32_t) * g->V, hipMemcpyHostToDevice,
+                              g->stream));
+        lv::fill_i32(g->size, 0, g->V, g->stream);
+        lv::fill_f64(g->sigma, 0.0, g->V, g->stream);
+        hipLaunchKernelGGL(k_state_from_comm, dim3(blocks_for(g->V, 256)), dim3(256), 0, g->stream,
+                           g->V, g->comm, g->k, g->size, g->sigma);
+        LV_LAUNCHED("state from communities");
+    }
+    return g->sync();
+}
+
+lv_status lv_decide(lv_graph *g, int32_t group, int32_t *new_comm) {
+    if (!g || !new_comm) return fail(LV_ERR_ARG, "null argument");
+    if (group < 0 || group >= lv::kNumGroups) return fail(LV_ERR_ARG, "group out of range");
+    LV_HIP(hipSetDevice(g->device));
+    if (!g->level_ready) LV_TRY(g->setup_level());
+    if (g->V > 0) {
+        LV_HIP(hipMemcpyAsync(g->newc, g->comm, sizeof(int32_t) * g->V, hipMemcpyDeviceToDevice,
+                              g->stream));
+        LV_TRY(g->run_group(group));
+        LV_HIP(hipMemcpyAsync(new_comm, g->newc, sizeof(int32_t) * g->V, hipMemcpyDeviceToHost,
+                              g->stream));
+    }
+    return g->sync();
+}
+
+lv_status lv_kernel_timing(lv_graph *g, int32_t kernel, lv_kernel_time *out) {
+    if (!g || !out || kernel < 0 || kernel >= LV_K_COUNT) return fail(LV_ERR_ARG, "bad argument");
+    LV_TRY(g->sync());
+    *out = g->ktime[kernel];
+    return LV_OK;
+}
+
+lv_status lv_kernel_timing_reset(lv_graph *g) {
+    if (!g) return fail(LV_ERR_ARG, "null graph");
+    LV_TRY(g->sync());
+    for (auto &t : g->ktime) t = lv_kernel_time{};
+    return LV_OK;
+}
+
+}  // extern "C"

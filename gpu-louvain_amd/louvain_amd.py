@@ -1,0 +1,328 @@
+"""Python binding of the MI355X Louvain library (ctypes over include/louvain.h).
+
+Host-side mirror of the reference's operations, for tests and benchmarks:
+
+    Graph(csr)            copyStructures + initM      (utils.cu:64-106, modularity_optimisation.cu:420-422)
+    Graph.optimise(g)     optimiseModularity           (modularity_optimisation.cu:284-363)
+    Graph.aggregate()     aggregateCommunities         (community_aggregation.cu:248-347)
+    Graph.modularity()    calculateModularity          (modularity_optimisation.cu:395-401)
+    Graph.run(g)          the level loop of main       (gpulouvain.cu:30-44)
+    Graph.assignment()    data of printOriginalToCommunity (modularity_optimisation.cu:403-418)
+    read_mtx(path)        readInputData                (utils.cu:9-62)
+
+Every call goes to the native library; there is no Python fallback.  A
+missing library raises LouvainError at load time.
+"""
+from __future__ import annotations
+
+import ctypes
+import os
+from dataclasses import dataclass
+
+import numpy as np
+
+_HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(_HERE, "lib", "libgpulouvain.so")
+HEADER_PATH = os.path.join(os.path.dirname(_HERE), "include", "louvain.h")
+
+LV_OK = 0
+LV_ERR_ARG = -1
+LV_ERR_OOM = -2
+LV_ERR_HIP = -3
+LV_ERR_IO = -4
+LV_ERR_STATE = -5
+LV_ERR_NODEVICE = -6
+
+K_MOVE_SUB, K_MOVE_WAVE, K_MOVE_WG, K_MOVE_GLOBAL, K_APPLY, K_MODULARITY, K_AGGREGATE = range(7)
+KERNEL_NAMES = ["move_sub", "move_wave", "move_wg", "move_global", "apply", "modularity", "aggregate"]
+
+I32P = ctypes.POINTER(ctypes.c_int32)
+I64P = ctypes.POINTER(ctypes.c_int64)
+F32P = ctypes.POINTER(ctypes.c_float)
+F64P = ctypes.POINTER(ctypes.c_double)
+
+
+class LouvainError(RuntimeError):
+    def __init__(self, status: int, message: str):
+        super().__init__(f"louvain status {status}: {message}")
+        self.status = status
+
+
+class Options(ctypes.Structure):
+    _fields_ = [
+        ("device", ctypes.c_int32),
+        ("timing", ctypes.c_int32),
+        ("heavy_wg_max", ctypes.c_int32),
+        ("mid_wg_max", ctypes.c_int32),
+        ("agg_lds_max", ctypes.c_int32),
+        ("reserved", ctypes.c_int32 * 11),
+    ]
+
+
+class Stats(ctypes.Structure):
+    _fields_ = [
+        ("levels", ctypes.c_int32),
+        ("sweeps", ctypes.c_int32),
+        ("final_V", ctypes.c_int32),
+        ("level0_sweeps", ctypes.c_int32),
+        ("final_E", ctypes.c_int64),
+        ("edges_swept", ctypes.c_int64),
+        ("modularity", ctypes.c_double),
+        ("algo_ms", ctypes.c_double),
+        ("sweep_ms", ctypes.c_double),
+        ("level0_sweep_ms", ctypes.c_double),
+        ("aggregate_ms", ctypes.c_double),
+        ("memory_ms", ctypes.c_double),
+        ("reserved", ctypes.c_double * 6),
+    ]
+
+    def as_dict(self) -> dict:
+        return {name: getattr(self, name) for name, _ in self._fields_ if name != "reserved"}
+
+
+class KernelTime(ctypes.Structure):
+    _fields_ = [("launches", ctypes.c_int64), ("total_ms", ctypes.c_double), ("bytes", ctypes.c_double)]
+
+
+_lib = None
+
+
+def load() -> ctypes.CDLL:
+    """Load libgpulouvain.so (built by `make -C gpu-louvain_amd`)."""
+    global _lib
+    if _lib is not None:
+        return _lib
+    if not os.path.exists(LIB_PATH):
+        raise LouvainError(LV_ERR_STATE, f"native library missing: {LIB_PATH} (build it first)")
+    lib = ctypes.CDLL(LIB_PATH)
+    sig = {
+        "lv_create": (ctypes.c_int32, [I32P, I32P, F32P, ctypes.c_int32, ctypes.c_int64,
+                                       ctypes.POINTER(Options), ctypes.POINTER(ctypes.c_void_p)]),
+        "lv_destroy": (None, [ctypes.c_void_p]),
+        "lv_reset": (ctypes.c_int32, [ctypes.c_void_p]),
+        "lv_optimise": (ctypes.c_int32, [ctypes.c_void_p, ctypes.c_float, I32P]),
+        "lv_aggregate": (ctypes.c_int32, [ctypes.c_void_p]),
+        "lv_modularity": (ctypes.c_int32, [ctypes.c_void_p, F64P]),
+        "lv_run": (ctypes.c_int32, [ctypes.c_void_p, ctypes.c_float, ctypes.POINTER(Stats)]),
+        "lv_assignment": (ctypes.c_int32, [ctypes.c_void_p, I32P, I32P]),
+        "lv_level_size": (ctypes.c_int32, [ctypes.c_void_p, I32P, I64P]),
+        "lv_level_graph": (ctypes.c_int32, [ctypes.c_void_p, I32P, I32P, F32P]),
+        "lv_get_communities": (ctypes.c_int32, [ctypes.c_void_p, I32P]),
+        "lv_set_communities": (ctypes.c_int32, [ctypes.c_void_p, I32P]),
+        "lv_decide": (ctypes.c_int32, [ctypes.c_void_p, ctypes.c_int32, I32P]),
+        "lv_sweep": (ctypes.c_int32, [ctypes.c_void_p, F64P, I64P]),
+        "lv_kernel_timing": (ctypes.c_int32, [ctypes.c_void_p, ctypes.c_int32, ctypes.POINTER(KernelTime)]),
+        "lv_kernel_timing_reset": (ctypes.c_int32, [ctypes.c_void_p]),
+        "lv_last_error": (ctypes.c_char_p, []),
+        "lv_read_mtx": (ctypes.c_int32, [ctypes.c_char_p, ctypes.POINTER(I32P), ctypes.POINTER(I32P),
+                                         ctypes.POINTER(F32P), I32P, I64P]),
+        "lv_write_mtx": (ctypes.c_int32, [ctypes.c_char_p, I32P, I32P, F32P, ctypes.c_int32]),
+        "lv_gen_rmat": (ctypes.c_int32, [ctypes.c_int32, ctypes.c_int32, ctypes.c_double, ctypes.c_double,
+                                         ctypes.c_double, ctypes.c_uint64, ctypes.c_int32,
+                                         ctypes.POINTER(I32P), ctypes.POINTER(I32P), ctypes.POINTER(F32P),
+                                         I32P, I64P]),
+        "lv_gen_planted": (ctypes.c_int32, [ctypes.c_int32, ctypes.c_int32, ctypes.c_double, ctypes.c_double,
+                                            ctypes.c_uint64, ctypes.c_int32, ctypes.POINTER(I32P),
+                                            ctypes.POINTER(I32P), ctypes.POINTER(F32P), I32P, I64P]),
+        "lv_free_host": (None, [ctypes.c_void_p]),
+    }
+    for name, (res, args) in sig.items():
+        fn = getattr(lib, name)
+        fn.restype = res
+        fn.argtypes = args
+    _lib = lib
+    return lib
+
+
+def _check(status: int) -> None:
+    if status != LV_OK:
+        raise LouvainError(status, load().lv_last_error().decode(errors="replace"))
+
+
+def _p(a: np.ndarray, ctype):
+    return a.ctypes.data_as(ctypes.POINTER(ctype))
+
+
+@dataclass
+class CSR:
+    """Symmetric CSR graph: rowptr[V+1] int32, col[E] int32, w[E] float32."""
+
+    rowptr: np.ndarray
+    col: np.ndarray
+    w: np.ndarray
+
+    @property
+    def V(self) -> int:
+        return int(self.rowptr.shape[0] - 1)
+
+    @property
+    def E(self) -> int:
+        return int(self.col.shape[0])
+
+    def degrees(self) -> np.ndarray:
+        return np.diff(self.rowptr)
+
+
+def _take_host_csr(rp, cl, wt, V: ctypes.c_int32, E: ctypes.c_int64) -> CSR:
+    lib = load()
+    n, m = V.value, E.value
+    rowptr = np.ctypeslib.as_array(rp, shape=(n + 1,)).copy()
+    col = np.ctypeslib.as_array(cl, shape=(max(m, 1),))[:m].copy()
+    w = np.ctypeslib.as_array(wt, shape=(max(m, 1),))[:m].copy()
+    for ptr in (rp, cl, wt):
+        lib.lv_free_host(ctypes.cast(ptr, ctypes.c_void_p))
+    return CSR(rowptr, col, w)
+
+
+def read_mtx(path: str) -> CSR:
+    lib = load()
+    rp, cl, wt = I32P(), I32P(), F32P()
+    V, E = ctypes.c_int32(), ctypes.c_int64()
+    _check(lib.lv_read_mtx(path.encode(), ctypes.byref(rp), ctypes.byref(cl), ctypes.byref(wt),
+                           ctypes.byref(V), ctypes.byref(E)))
+    return _take_host_csr(rp, cl, wt, V, E)
+
+
+def write_mtx(path: str, g: CSR) -> None:
+    lib = load()
+    rowptr = np.ascontiguousarray(g.rowptr, dtype=np.int32)
+    col = np.ascontiguousarray(g.col, dtype=np.int32)
+    w = np.ascontiguousarray(g.w, dtype=np.float32)
+    _check(lib.lv_write_mtx(path.encode(), _p(rowptr, ctypes.c_int32), _p(col, ctypes.c_int32),
+                            _p(w, ctypes.c_float), g.V))
+
+
+def gen_rmat(scale: int, edge_factor: int = 16, a: float = 0.57, b: float = 0.19, c: float = 0.19,
+             seed: int = 1, unit_weights: bool = False) -> CSR:
+    lib = load()
+    rp, cl, wt = I32P(), I32P(), F32P()
+    V, E = ctypes.c_int32(), ctypes.c_int64()
+    _check(lib.lv_gen_rmat(scale, edge_factor, a, b, c, seed, int(unit_weights), ctypes.byref(rp),
+                           ctypes.byref(cl), ctypes.byref(wt), ctypes.byref(V), ctypes.byref(E)))
+    return _take_host_csr(rp, cl, wt, V, E)
+
+
+def gen_planted(n: int, blocks: int, p_in: float, p_out: float, seed: int = 1,
+                unit_weights: bool = True) -> CSR:
+    lib = load()
+    rp, cl, wt = I32P(), I32P(), F32P()
+    V, E = ctypes.c_int32(), ctypes.c_int64()
+    _check(lib.lv_gen_planted(n, blocks, p_in, p_out, seed, int(unit_weights), ctypes.byref(rp),
+                              ctypes.byref(cl), ctypes.byref(wt), ctypes.byref(V), ctypes.byref(E)))
+    return _take_host_csr(rp, cl, wt, V, E)
+
+
+class Graph:
+    """A graph resident on one GPU with its community state (lv_graph)."""
+
+    def __init__(self, g: CSR, device: int = 0, timing: bool = False, mid_wg_max: int = 0,
+                 heavy_wg_max: int = 0):
+        lib = load()
+        self._lib = lib
+        self._rowptr = np.ascontiguousarray(g.rowptr, dtype=np.int32)
+        self._col = np.ascontiguousarray(g.col, dtype=np.int32)
+        self._w = np.ascontiguousarray(g.w, dtype=np.float32)
+        self.V0 = int(self._rowptr.shape[0] - 1)
+        opt = Options()
+        opt.device = device
+        opt.timing = int(timing)
+        opt.mid_wg_max = mid_wg_max
+        opt.heavy_wg_max = heavy_wg_max
+        h = ctypes.c_void_p()
+        col = self._col if self._col.size else np.zeros(1, np.int32)
+        w = self._w if self._w.size else np.zeros(1, np.float32)
+        _check(lib.lv_create(_p(self._rowptr, ctypes.c_int32), _p(col, ctypes.c_int32),
+                             _p(w, ctypes.c_float), self.V0, int(self._col.size), ctypes.byref(opt),
+                             ctypes.byref(h)))
+        self._h = h
+
+    # --- lifecycle
+    def close(self) -> None:
+        if getattr(self, "_h", None):
+            self._lib.lv_destroy(self._h)
+            self._h = None
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *exc):
+        self.close()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def reset(self) -> None:
+        _check(self._lib.lv_reset(self._h))
+
+    # --- phases
+    def optimise(self, min_gain: float) -> bool:
+        changed = ctypes.c_int32()
+        _check(self._lib.lv_optimise(self._h, min_gain, ctypes.byref(changed)))
+        return bool(changed.value)
+
+    def aggregate(self) -> None:
+        _check(self._lib.lv_aggregate(self._h))
+
+    def modularity(self) -> float:
+        q = ctypes.c_double()
+        _check(self._lib.lv_modularity(self._h, ctypes.byref(q)))
+        return q.value
+
+    def run(self, min_gain: float) -> dict:
+        st = Stats()
+        _check(self._lib.lv_run(self._h, min_gain, ctypes.byref(st)))
+        return st.as_dict()
+
+    def sweep(self) -> tuple[float, int]:
+        """One sweep; returns (modularity after the sweep, vertices moved)."""
+        q, moved = ctypes.c_double(), ctypes.c_int64()
+        _check(self._lib.lv_sweep(self._h, ctypes.byref(q), ctypes.byref(moved)))
+        return q.value, moved.value
+
+    # --- state
+    def level_size(self) -> tuple[int, int]:
+        V, E = ctypes.c_int32(), ctypes.c_int64()
+        _check(self._lib.lv_level_size(self._h, ctypes.byref(V), ctypes.byref(E)))
+        return V.value, E.value
+
+    def level_graph(self) -> CSR:
+        V, E = self.level_size()
+        rowptr = np.zeros(V + 1, np.int32)
+        col = np.zeros(max(E, 1), np.int32)
+        w = np.zeros(max(E, 1), np.float32)
+        _check(self._lib.lv_level_graph(self._h, _p(rowptr, ctypes.c_int32), _p(col, ctypes.c_int32),
+                                        _p(w, ctypes.c_float)))
+        return CSR(rowptr, col[:E], w[:E])
+
+    def assignment(self) -> tuple[np.ndarray, int]:
+        o2c = np.zeros(max(self.V0, 1), np.int32)
+        lv = ctypes.c_int32()
+        _check(self._lib.lv_assignment(self._h, _p(o2c, ctypes.c_int32), ctypes.byref(lv)))
+        return o2c[: self.V0], lv.value
+
+    def communities(self) -> np.ndarray:
+        V, _ = self.level_size()
+        comm = np.zeros(max(V, 1), np.int32)
+        _check(self._lib.lv_get_communities(self._h, _p(comm, ctypes.c_int32)))
+        return comm[:V]
+
+    def set_communities(self, comm: np.ndarray) -> None:
+        comm = np.ascontiguousarray(comm, dtype=np.int32)
+        _check(self._lib.lv_set_communities(self._h, _p(comm, ctypes.c_int32)))
+
+    def decide(self, group: int) -> np.ndarray:
+        V, _ = self.level_size()
+        out = np.zeros(max(V, 1), np.int32)
+        _check(self._lib.lv_decide(self._h, group, _p(out, ctypes.c_int32)))
+        return out[:V]
+
+    def kernel_timing(self, kernel: int) -> dict:
+        kt = KernelTime()
+        _check(self._lib.lv_kernel_timing(self._h, kernel, ctypes.byref(kt)))
+        return {"launches": kt.launches, "total_ms": kt.total_ms, "bytes": kt.bytes}
+
+    def kernel_timing_reset(self) -> None:
+        _check(self._lib.lv_kernel_timing_reset(self._h))

@@ -1,0 +1,158 @@
+/*
+ * louvain.h — C-ABI of the MI355X-native Louvain engine (gfx950, HIP).
+ *
+ * The reference (olearczuk/gpu-louvain) has no plugin or FFI layer: its
+ * phases are C++ functions over structs of raw device pointers, linked into
+ * one executable.  Each entry point below replaces one of them; the comment
+ * names the reference interface (file:line) it stands in for.  Plain
+ * pointers and sizes only; every call returns an lv_status and never exits
+ * the process (the reference's HANDLE_ERROR exits, utils.cuh:92-104).
+ *
+ * Threading: one lv_graph owns one HIP stream and is not thread-safe;
+ * separate handles may be used from separate threads.
+ *
+ * Numerics: M, vertex strengths, community totals, per-candidate weight
+ * sums, merged aggregation weights and modularity are FP64; gains are fp32
+ * in the reference's operation order (modularity_optimisation.cu:96-102);
+ * aggregated weights are stored fp32 (the reference's CSR type).
+ */
+#ifndef LOUVAIN_H
+#define LOUVAIN_H
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+typedef int32_t lv_status;
+#define LV_OK 0
+#define LV_ERR_ARG (-1)          /* bad argument / malformed input        */
+#define LV_ERR_OOM (-2)          /* device or host allocation failed      */
+#define LV_ERR_HIP (-3)          /* HIP runtime / kernel launch error     */
+#define LV_ERR_IO (-4)           /* file cannot be read or written        */
+#define LV_ERR_STATE (-5)        /* call not valid in the handle's state  */
+#define LV_ERR_NODEVICE (-6)     /* no HIP device visible                 */
+
+typedef struct lv_graph lv_graph;
+
+/* Creation options.  Zero-initialise, then set what you need. */
+typedef struct {
+    int32_t device;          /* HIP device ordinal (default 0)                       */
+    int32_t timing;          /* 1: record per-kernel HIP events (lv_kernel_time)      */
+    int32_t heavy_wg_max;    /* degree bound of the workgroup/LDS tier (0 = auto)     */
+    int32_t mid_wg_max;      /* degree bound of the 256-lane LDS tier (0 = 1536)      */
+    int32_t agg_lds_max;     /* summed-degree bound of LDS aggregation (0 = auto)     */
+    int32_t reserved[11];
+} lv_options;
+
+/* Run statistics (lv_run).  Times are device times from HIP events. */
+typedef struct {
+    int32_t levels;          /* phases run                                          */
+    int32_t sweeps;          /* sweeps over all levels                              */
+    int32_t final_V;         /* vertices of the last level                          */
+    int32_t level0_sweeps;   /* sweeps of the first phase                           */
+    int64_t final_E;         /* CSR entries of the last level                       */
+    int64_t edges_swept;     /* sum over sweeps of the level's CSR entries          */
+    double modularity;       /* final Q, FP64                                       */
+    double algo_ms;          /* device time of the whole run (after upload)         */
+    double sweep_ms;         /* device time inside sweeps (moves+apply+Q)           */
+    double level0_sweep_ms;  /* device time of the first phase's sweeps             */
+    double aggregate_ms;     /* device time inside aggregation                      */
+    double memory_ms;        /* upload time at lv_create                            */
+    double reserved[6];
+} lv_stats;
+
+/* Per-kernel timing (options.timing = 1): kernel ids below. */
+typedef struct {
+    int64_t launches;
+    double total_ms;         /* summed HIP-event durations                          */
+    double bytes;            /* algorithmic bytes moved by those launches           */
+} lv_kernel_time;
+
+#define LV_K_MOVE_SUB 0      /* sub-wavefront move kernels (degree <= 32)           */
+#define LV_K_MOVE_WAVE 1     /* one-wavefront move kernels (33..319)                */
+#define LV_K_MOVE_WG 2       /* workgroup + LDS hash move kernels                   */
+#define LV_K_MOVE_GLOBAL 3   /* heavy-vertex global-hash move kernels               */
+#define LV_K_APPLY 4         /* apply moves + incremental totals                    */
+#define LV_K_MODULARITY 5    /* fused modularity reduction                          */
+#define LV_K_AGGREGATE 6     /* whole aggregation (phase 2)                         */
+#define LV_K_COUNT 7
+
+/* ---- graph handle (utils.cu:64-106 copyStructures + initM mo.cu:420-422) ---- */
+/* Uploads the CSR (rowptr[V+1], col[E], w[E]; host pointers) and sets up the
+ * singleton state comm = iota, size = 1 (utils.cu:84-89).  M = (sum w)/2. */
+lv_status lv_create(const int32_t *rowptr, const int32_t *col, const float *w, int32_t V,
+                    int64_t E, const lv_options *opt, lv_graph **out);
+/* utils.cu:108-141 deleteStructures. */
+void lv_destroy(lv_graph *g);
+/* Back to the uploaded level-0 graph with the singleton state (no re-upload;
+ * when level 0 is current only the community state is rewritten). */
+lv_status lv_reset(lv_graph *g);
+
+/* ---- phases ---- */
+/* One modularity-optimisation phase (modularity_optimisation.cu:284-363,
+ * declared modularity_optimisation.cuh:28): sweeps while gain >= min_gain
+ * and gain > 0 (the second condition only matters for min_gain <= 0, where
+ * the reference can loop forever); *changed = any sweep had gain > 0. */
+lv_status lv_optimise(lv_graph *g, float min_gain, int32_t *changed);
+/* Community aggregation (community_aggregation.cu:248-347, declared
+ * community_aggregation.cuh:11-12).  Rows come out in ascending new id. */
+lv_status lv_aggregate(lv_graph *g);
+/* Modularity of the current state, FP64 (modularity_optimisation.cu:395-401,
+ * declared modularity_optimisation.cuh:30). */
+lv_status lv_modularity(lv_graph *g, double *q);
+/* Whole multi-level run (gpulouvain.cu:30-44). */
+lv_status lv_run(lv_graph *g, float min_gain, lv_stats *stats);
+
+/* ---- state access (for -v output and parity checks) ---- */
+/* Final/current original-vertex -> community map in the current level's ids
+ * (data printed by printOriginalToCommunity, modularity_optimisation.cu:403-418). */
+lv_status lv_assignment(lv_graph *g, int32_t *orig2comm, int32_t *level_V);
+lv_status lv_level_size(lv_graph *g, int32_t *V, int64_t *E);
+lv_status lv_level_graph(lv_graph *g, int32_t *rowptr, int32_t *col, float *w);
+lv_status lv_get_communities(lv_graph *g, int32_t *comm);
+/* Replace the current level's community state (sizes and totals are rebuilt). */
+lv_status lv_set_communities(lv_graph *g, const int32_t *comm);
+/* Move decisions of degree group `group` (0..6, bounds {0,4,8,16,32,84,319,inf},
+ * modularity_optimisation.cuh:7-8) from the current state, without applying
+ * them (computeMove, modularity_optimisation.cu:111-228).  new_comm[V]
+ * receives the decision for the group's vertices and comm for the others. */
+lv_status lv_decide(lv_graph *g, int32_t group, int32_t *new_comm);
+/* One sweep over groups 0..6 with the modularity after it (the body of the
+ * loop at modularity_optimisation.cu:308-354); *moved = vertices moved. */
+lv_status lv_sweep(lv_graph *g, double *q_after, int64_t *moved);
+/* Per-kernel timing, id in [0, LV_K_COUNT). */
+lv_status lv_kernel_timing(lv_graph *g, int32_t kernel, lv_kernel_time *out);
+lv_status lv_kernel_timing_reset(lv_graph *g);
+
+const char *lv_last_error(void);
+
+/* ---- host-side input (utils.cu:9-62 readInputData) and synthetic graphs ---- */
+/* MatrixMarket coordinate reader with the reference's conventions (skip '%'
+ * lines, V := cols, each non-loop triple stored in both rows, insertion
+ * order kept).  Arrays are malloc'd; free with lv_free_host. */
+lv_status lv_read_mtx(const char *path, int32_t **rowptr, int32_t **col, float **w,
+                      int32_t *V, int64_t *E);
+/* Writes the lower triangle (i >= j) of a symmetric CSR as a symmetric real
+ * (or integer-valued real) MatrixMarket file. */
+lv_status lv_write_mtx(const char *path, const int32_t *rowptr, const int32_t *col,
+                       const float *w, int32_t V);
+/* RMAT graph: 2^scale vertices, edge_factor*2^scale samples with
+ * probabilities (a,b,c,1-a-b-c), self-loops and duplicates removed,
+ * symmetrised, seeded vertex permutation; weights 1 (unit_weights) or
+ * (1 + floor(r*2^24))*2^-24 in (0,1].  Arrays malloc'd (lv_free_host). */
+lv_status lv_gen_rmat(int32_t scale, int32_t edge_factor, double a, double b, double c,
+                      uint64_t seed, int32_t unit_weights, int32_t **rowptr, int32_t **col,
+                      float **w, int32_t *V, int64_t *E);
+/* Planted partition: n vertices in `blocks` near-equal blocks, each pair
+ * an edge with probability p_in (same block) or p_out. */
+lv_status lv_gen_planted(int32_t n, int32_t blocks, double p_in, double p_out, uint64_t seed,
+                         int32_t unit_weights, int32_t **rowptr, int32_t **col, float **w,
+                         int32_t *V, int64_t *E);
+void lv_free_host(void *p);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

@@ -1,0 +1,55 @@
+"""Small graph builders shared by the tests (plain numpy; no product code)."""
+import numpy as np
+
+
+def csr_from_edges(n, edges, weights=None):
+    """Symmetric CSR in the reference reader's layout: for each undirected
+    edge (i, j, w) in order, row i gets (j, w) and, if i != j, row j gets (i, w)."""
+    rows = [[] for _ in range(n)]
+    for t, (i, j) in enumerate(edges):
+        w = 1.0 if weights is None else weights[t]
+        rows[i].append((j, w))
+        if i != j:
+            rows[j].append((i, w))
+    rowptr = np.zeros(n + 1, np.int32)
+    for v in range(n):
+        rowptr[v + 1] = rowptr[v] + len(rows[v])
+    col = np.array([c for r in rows for c, _ in r], np.int32)
+    w = np.array([x for r in rows for _, x in r], np.float32)
+    return rowptr, col, w
+
+
+def two_triangles():
+    return csr_from_edges(6, [(0, 1), (0, 2), (1, 2), (2, 3), (3, 4), (3, 5), (4, 5)])
+
+
+def random_graph(n, m, seed, weighted=False, self_loops=0):
+    rng = np.random.default_rng(seed)
+    seen = set()
+    edges = []
+    while len(edges) < m:
+        i, j = rng.integers(0, n, 2)
+        if i == j:
+            continue
+        key = (min(i, j), max(i, j))
+        if key in seen:
+            continue
+        seen.add(key)
+        edges.append((int(i), int(j)))
+    for _ in range(self_loops):
+        v = int(rng.integers(0, n))
+        if (v, v) not in seen:
+            seen.add((v, v))
+            edges.append((v, v))
+    weights = None
+    if weighted:
+        weights = [float((1 + rng.integers(0, 1 << 24)) * 2.0**-24) for _ in edges]
+    return csr_from_edges(n, edges, weights)
+
+
+def star(leaves, extra_ring=True):
+    """Hub 0 joined to `leaves` vertices; optional ring among the leaves."""
+    edges = [(0, i) for i in range(1, leaves + 1)]
+    if extra_ring:
+        edges += [(i, i + 1) for i in range(1, leaves)]
+    return csr_from_edges(leaves + 1, edges)

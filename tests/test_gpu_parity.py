@@ -1,0 +1,205 @@
+"""GPU-vs-oracle parity of the HIP Louvain path (needs an MI355X).
+
+Every check compares the native library (called through the C-ABI) with
+the CPU oracle on the same seeded inputs.  The inputs use integer weights or
+weights on the 2^-24 grid, where all FP64 sums are exact, so decisions,
+sweeps, aggregated CSRs and whole runs must match bit for bit; modularity is
+compared to 1e-9 absolute (the bar is 1e-6 relative).
+"""
+import numpy as np
+import pytest
+
+import graphs
+
+pytestmark = pytest.mark.gpu
+
+Q_TOL = 1e-9
+
+
+def _csr(lvlib, rowptr, col, w):
+    return lvlib.CSR(np.asarray(rowptr, np.int32), np.asarray(col, np.int32), np.asarray(w, np.float32))
+
+
+def _cases(lvlib):
+    """(name, CSR, graph options) of the parity inputs."""
+    out = []
+    out.append(("two_triangles", _csr(lvlib, *graphs.two_triangles()), {}))
+    out.append(("planted_c1_unit", lvlib.gen_planted(10000, 20, 0.02, 0.0005, seed=1, unit_weights=True), {}))
+    out.append(("rmat12_real", lvlib.gen_rmat(12, 16, seed=12), {}))
+    out.append(("rmat14_real", lvlib.gen_rmat(14, 16, seed=14), {}))
+    out.append(("rmat13_unit", lvlib.gen_rmat(13, 16, seed=13, unit_weights=True), {}))
+    out.append(("random_selfloops_real", _csr(lvlib, *graphs.random_graph(400, 3000, 7, True, 40)), {}))
+    out.append(("star3000", _csr(lvlib, *graphs.star(3000)), {}))
+    out.append(("star20000", _csr(lvlib, *graphs.star(20000)), {}))
+    # force the multi-workgroup global-hash tier on a mid-size graph
+    out.append(("rmat14_global_tier", lvlib.gen_rmat(14, 16, seed=41), {"mid_wg_max": 320, "heavy_wg_max": 321}))
+    return out
+
+
+@pytest.fixture(scope="module")
+def cases(lvlib):
+    return _cases(lvlib)
+
+
+def _names():
+    return ["two_triangles", "planted_c1_unit", "rmat12_real", "rmat14_real", "rmat13_unit",
+            "random_selfloops_real", "star3000", "star20000", "rmat14_global_tier"]
+
+
+def _get(cases, name):
+    for n, g, opt in cases:
+        if n == name:
+            return g, opt
+    raise KeyError(name)
+
+
+@pytest.mark.parametrize("name", _names())
+def test_decide_singletons_matches_oracle(lvlib, oracle, cases, name):
+    g, opt = _get(cases, name)
+    print(f"case {name} V={g.V} E={g.E}", flush=True)
+    og = oracle.Graph.from_csr(g)
+    comm = np.arange(g.V, dtype=np.int32)
+    with lvlib.Graph(g, **opt) as G:
+        for grp in range(7):
+            got = G.decide(grp)
+            want = oracle.decide(og, comm, grp)
+            bad = np.nonzero(got != want)[0]
+            assert bad.size == 0, f"group {grp}: {bad.size} differ, first {bad[:5]} got {got[bad[:5]]} want {want[bad[:5]]}"
+
+
+@pytest.mark.parametrize("name", _names())
+def test_decide_mid_state_matches_oracle(lvlib, oracle, cases, name):
+    g, opt = _get(cases, name)
+    print(f"case {name}", flush=True)
+    og = oracle.Graph.from_csr(g)
+    comm = np.arange(g.V, dtype=np.int32)
+    oracle.sweep(og, comm)  # a non-trivial state reached by the oracle
+    with lvlib.Graph(g, **opt) as G:
+        G.set_communities(comm)
+        assert np.array_equal(G.communities(), comm)
+        for grp in range(7):
+            got = G.decide(grp)
+            want = oracle.decide(og, comm, grp)
+            bad = np.nonzero(got != want)[0]
+            assert bad.size == 0, f"group {grp}: {bad.size} differ, first {bad[:5]}"
+
+
+@pytest.mark.parametrize("name", _names())
+def test_sweeps_and_modularity_match_oracle(lvlib, oracle, cases, name):
+    g, opt = _get(cases, name)
+    print(f"case {name}", flush=True)
+    og = oracle.Graph.from_csr(g)
+    M = oracle.total_weight(og)
+    comm = np.arange(g.V, dtype=np.int32)
+    with lvlib.Graph(g, **opt) as G:
+        assert abs(G.modularity() - oracle.modularity(og, comm, M)) <= Q_TOL
+        for s in range(4):
+            moved_o = oracle.sweep(og, comm, M)
+            q_after, moved = G.sweep()
+            assert moved == moved_o, f"sweep {s}: moved {moved} vs {moved_o}"
+            got = G.communities()
+            bad = np.nonzero(got != comm)[0]
+            assert bad.size == 0, f"sweep {s}: {bad.size} vertices differ"
+            q1 = oracle.modularity(og, comm, M)
+            assert abs(q_after - q1) <= Q_TOL
+            assert abs(G.modularity() - q1) <= Q_TOL
+
+
+@pytest.mark.parametrize("name", _names())
+def test_phase_and_aggregation_match_oracle(lvlib, oracle, cases, name):
+    g, opt = _get(cases, name)
+    print(f"case {name}", flush=True)
+    og = oracle.Graph.from_csr(g)
+    M = oracle.total_weight(og)
+    comm = np.arange(g.V, dtype=np.int32)
+    o2c = np.arange(g.V, dtype=np.int32)
+    changed_o, _ = oracle.phase(og, comm, 1e-6, o2c, M)
+    with lvlib.Graph(g, **opt) as G:
+        changed = G.optimise(1e-6)
+        assert changed == changed_o
+        assert np.array_equal(G.communities(), comm)
+        a, lv = G.assignment()
+        assert np.array_equal(a, o2c)
+        G.aggregate()
+        nxt, nid = oracle.aggregate(og, comm, o2c)
+        h = G.level_graph()
+        assert h.V == nxt.V and h.E == nxt.E
+        rp, cl, w = nxt.csr_arrays()
+        assert np.array_equal(h.rowptr, rp)
+        assert np.array_equal(h.col, cl)
+        assert np.array_equal(h.w.view(np.uint32), w.view(np.uint32))
+        a, lv = G.assignment()
+        assert lv == nxt.V
+        assert np.array_equal(a, o2c)
+        # Q is preserved by aggregation up to the fp32 rounding of merged weights
+        q_level = G.modularity()
+        assert abs(q_level - oracle.modularity(nxt, np.arange(nxt.V, dtype=np.int32), M)) <= Q_TOL
+
+
+@pytest.mark.parametrize("name", _names())
+def test_full_run_matches_oracle(lvlib, oracle, cases, name):
+    g, opt = _get(cases, name)
+    print(f"case {name}", flush=True)
+    og = oracle.Graph.from_csr(g)
+    st_o, o2c_o, lv_o = oracle.run(og, 1e-6)
+    with lvlib.Graph(g, **opt) as G:
+        st = G.run(1e-6)
+        o2c, lv = G.assignment()
+    assert st["levels"] == st_o["levels"]
+    assert st["sweeps"] == st_o["sweeps"]
+    assert lv == lv_o
+    assert np.array_equal(o2c, o2c_o)
+    assert abs(st["modularity"] - st_o["modularity"]) <= Q_TOL
+
+
+def test_rerun_after_reset_is_identical(lvlib):
+    g = lvlib.gen_rmat(13, 16, seed=5)
+    with lvlib.Graph(g) as G:
+        a = G.run(1e-6)
+        o1, l1 = G.assignment()
+        G.reset()
+        b = G.run(1e-6)
+        o2, l2 = G.assignment()
+    assert a["modularity"] == b["modularity"] and l1 == l2
+    assert np.array_equal(o1, o2)
+
+
+def test_global_tier_aggregation_with_large_sort(lvlib, oracle):
+    """A community with > 8192 distinct neighbour communities goes through
+    the global hash and the merge-path sort."""
+    rowptr, col, w = graphs.star(20000, extra_ring=False)
+    g = _csr(lvlib, rowptr, col, w)
+    og = oracle.Graph.from_csr(g)
+    comm = np.arange(g.V, dtype=np.int32)
+    comm[1:50] = 0  # hub community holds 49 leaves; 19951 leaf singletons remain
+    o2c = np.arange(g.V, dtype=np.int32)
+    with lvlib.Graph(g) as G:
+        G.set_communities(comm)
+        G.aggregate()
+        h = G.level_graph()
+    nxt, _ = oracle.aggregate(og, comm, o2c)
+    rp, cl, ww = nxt.csr_arrays()
+    assert np.array_equal(h.rowptr, rp)
+    assert np.array_equal(h.col, cl)
+    assert np.array_equal(h.w.view(np.uint32), ww.view(np.uint32))
+
+
+@pytest.mark.parametrize("kind", ["no_edges", "self_loops_only", "single_edge", "two_components"])
+def test_edge_cases(lvlib, oracle, kind):
+    if kind == "no_edges":
+        rowptr, col, w = np.zeros(6, np.int32), np.zeros(0, np.int32), np.zeros(0, np.float32)
+    elif kind == "self_loops_only":
+        rowptr, col, w = graphs.csr_from_edges(4, [(0, 0), (1, 1), (2, 2)])
+    elif kind == "single_edge":
+        rowptr, col, w = graphs.csr_from_edges(3, [(0, 2)])
+    else:
+        rowptr, col, w = graphs.csr_from_edges(8, [(0, 1), (1, 2), (0, 2), (4, 5), (5, 6), (4, 6), (6, 7)])
+    g = _csr(lvlib, rowptr, col, w)
+    og = oracle.Graph.from_csr(g)
+    st_o, o2c_o, lv_o = oracle.run(og, 1e-6)
+    with lvlib.Graph(g) as G:
+        st = G.run(1e-6)
+        o2c, lv = G.assignment()
+    assert lv == lv_o and np.array_equal(o2c, o2c_o)
+    if g.E > 0:
+        assert abs(st["modularity"] - st_o["modularity"]) <= Q_TOL
