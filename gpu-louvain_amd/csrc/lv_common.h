@@ -11,13 +11,14 @@ namespace lv {
 // ---------------------------------------------------------------------
 // Phase-1 execution buckets.  Buckets 0..5 coincide with the reference's
 // degree groups 0..5 (modularity_optimisation.cuh:7-19); group 6 (deg > 319)
-// is split by degree into a 256-lane LDS tier, a large-LDS workgroup tier
-// and a multi-workgroup global-hash tier.  Splitting a group changes only
-// the kernel mapping: decisions inside a group are simultaneous.
+// is split by degree into a one-workgroup LDS tier and a chunked tier
+// (rows split over many workgroups, merged by hash partition in LDS).
+// Splitting a group changes only the kernel mapping: decisions inside a
+// group are simultaneous.
 // ---------------------------------------------------------------------
-constexpr int kNumBuckets = 9;
+constexpr int kNumBuckets = 8;
 constexpr int kNumGroups = 7;
-constexpr int kBucketGroup[kNumBuckets] = {0, 1, 2, 3, 4, 5, 6, 6, 6};
+constexpr int kBucketGroup[kNumBuckets] = {0, 1, 2, 3, 4, 5, 6, 6};
 constexpr int kGroupBound[kNumGroups + 1] = {0, 4, 8, 16, 32, 84, 319, 0x7fffffff};
 
 struct BucketBounds {
@@ -93,6 +94,18 @@ __device__ __forceinline__ bool may_move(int32_t c, int32_t cur, int32_t size_cu
 
 __device__ __forceinline__ uint32_t hash_u32(int32_t c) {
     return (uint32_t)c * 2654435761u;
+}
+
+// Partition of a community among P merge workgroups; mixes differently from
+// hash_u32 so that slots inside a partition stay uniformly used.
+__device__ __forceinline__ uint32_t part_of(int32_t c, uint32_t P) {
+    uint32_t x = (uint32_t)c;
+    x ^= x >> 16;
+    x *= 0x7feb352du;
+    x ^= x >> 15;
+    x *= 0x846ca68bu;
+    x ^= x >> 16;
+    return (uint32_t)(((uint64_t)x * P) >> 32);
 }
 
 // Open-addressing insert with linear probing; keys -1 = empty.  The table

@@ -59,6 +59,11 @@ lv_status fail(lv_status st, const std::string &msg) {
 
 __global__ void k_zero_u64(unsigned long long *p) { *p = 0ull; }
 
+__global__ void k_zero_u64_n(unsigned long long *p, int64_t n) {
+    const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) p[i] = 0ull;
+}
+
 // Row extents of the heavy vertices (host builds their work tables).
 __global__ void k_heavy_rows(const int32_t *__restrict__ rowptr, const int32_t *__restrict__ list,
                              int32_t n, int64_t *out) {
@@ -70,16 +75,24 @@ __global__ void k_heavy_rows(const int32_t *__restrict__ rowptr, const int32_t *
     }
 }
 
-// Edge counts per bucket (algorithmic-byte accounting).
-__global__ void k_bucket_edges(const int32_t *__restrict__ rowptr, const int32_t *__restrict__ order,
-                               const int32_t *__restrict__ off, int nb,
-                               unsigned long long *edges) {
+// Edge counts per bucket (algorithmic-byte accounting): per-block LDS
+// counters, one global atomic per block and bucket.
+__global__ __launch_bounds__(256) void k_bucket_edges(const int32_t *__restrict__ rowptr,
+                                                      const int32_t *__restrict__ order,
+                                                      const int32_t *__restrict__ off, int nb,
+                                                      unsigned long long *edges) {
+    __shared__ unsigned long long cnt[16];
+    if (threadIdx.x < 16) cnt[threadIdx.x] = 0ull;
+    __syncthreads();
     const int32_t i = blockIdx.x * blockDim.x + threadIdx.x;
-    if (i >= off[nb]) return;
-    int b = 0;
-    while (b + 1 < nb && i >= off[b + 1]) ++b;
-    const int32_t v = order[i];
-    atomicAdd(&edges[b], (unsigned long long)(rowptr[v + 1] - rowptr[v]));
+    if (i < off[nb]) {
+        int b = 0;
+        while (b + 1 < nb && i >= off[b + 1]) ++b;
+        const int32_t v = order[i];
+        atomicAdd(&cnt[b], (unsigned long long)(rowptr[v + 1] - rowptr[v]));
+    }
+    __syncthreads();
+    if (threadIdx.x < nb && cnt[threadIdx.x]) atomicAdd(&edges[threadIdx.x], cnt[threadIdx.x]);
 }
 
 // Sizes and totals of an externally supplied state.
@@ -143,7 +156,6 @@ struct DevVec {
 
 unsigned blocks_for(int64_t n, int per) { return (unsigned)((n + per - 1) / per); }
 
-constexpr int kRowChunk = 4096;   // edges per heavy-row chunk
 constexpr int kSlotChunk = 4096;  // hash slots per evaluation chunk
 constexpr int kAggChunk = 4096;   // flat member edges per aggregation chunk
 
@@ -184,7 +196,8 @@ struct lv_graph {
 
     // bucketing
     lv::BucketBounds bb{};
-    int wg6_lg = 11, wg7_lg = 13, wg7_threads = 512;
+    int wg6_lg = 12;             // LDS slots (log2) of the one-workgroup tier
+    int chunk_edges = lv::kChunkEdges, part_target = 2048, merge_limit = lv::kMergeLimit;
     lv::AggBounds agg_bounds{};
     int8_t *keys8 = nullptr;
     int32_t *order = nullptr, *d_off = nullptr;
@@ -194,11 +207,14 @@ struct lv_graph {
     int32_t *scratch = nullptr;
     int64_t scratch_elems = 0;
 
-    // heavy vertices (bucket 8)
-    int32_t nh = 0, nins = 0, neval = 0;
-    DevVec<lv::Chunk> ins, eval;
-    DevVec<int64_t> hoff, hrows;
-    DevVec<int32_t> hlg;
+    // chunked tier (last bucket)
+    int32_t nh = 0, nins = 0, nitems = 0;
+    DevVec<lv::Chunk> ins;
+    DevVec<lv::HeavyVertex> hvs;
+    DevVec<lv::MergeItem> items;
+    DevVec<int64_t> hrows;
+    DevVec<int32_t> out_key, poff, ptot;
+    DevVec<double> out_val;
     DevVec<uint64_t> hbest;
     DevVec<double> hkcur;
     DevVec<int32_t> gkeys;  // global hash arena, kept all-empty between uses
@@ -308,14 +324,20 @@ struct lv_graph {
         return a;
     }
 
-    lv::HeavyTables heavy_tables() {
-        lv::HeavyTables t;
-        t.keys = gkeys.p;
-        t.vals = gvals.p;
-        t.off = hoff.p;
-        t.cap_log2 = hlg.p;
+    lv::ChunkedTier chunked_tier() {
+        lv::ChunkedTier t;
+        t.chunks = ins.p;
+        t.hvs = hvs.p;
+        t.items = items.p;
+        t.out_key = out_key.p;
+        t.out_val = out_val.p;
+        t.poff = poff.p;
+        t.ptot = ptot.p;
+        t.gkeys = gkeys.p;
+        t.gvals = gvals.p;
         t.best = hbest.p;
         t.kcur = hkcur.p;
+        t.merge_limit = merge_limit;
         return t;
     }
 
@@ -362,17 +384,15 @@ struct lv_graph {
         V0 = V = nV;
         E0 = E = nE;
         // Bucket bounds: 0..5 are the reference groups; group 6 splits into
-        // a 256-lane tier (cap 2^wg6_lg), a large-LDS tier and global tier.
-        wg6_lg = 11;
-        const int lds_lg = lv::move_wg_max_cap_log2(max_lds - 4096);
-        wg7_lg = std::min(13, lds_lg);
-        int mid_max = opt.mid_wg_max > 0 ? opt.mid_wg_max : 1536;
-        mid_max = std::max(320, std::min(mid_max, (3 << wg6_lg) / 4));
-        int heavy_max = opt.heavy_wg_max > 0 ? opt.heavy_wg_max : (3 << wg7_lg) / 4;
-        heavy_max = std::max(mid_max + 1, std::min(heavy_max, (3 << wg7_lg) / 4));
-        const int32_t his[lv::kNumBuckets] = {4, 8, 16, 32, 84, 319, mid_max, heavy_max,
-                                              0x7fffffff};
+        // a one-workgroup tier (LDS table of 2^wg6_lg slots at load <= 1/2)
+        // and the chunked tier.
+        int wg_max = opt.heavy_wg_max > 0 ? opt.heavy_wg_max : (1 << wg6_lg) / 2;
+        wg_max = std::max(320, std::min(wg_max, (1 << wg6_lg) / 2));
+        const int32_t his[lv::kNumBuckets] = {4, 8, 16, 32, 84, 319, wg_max, 0x7fffffff};
         for (int b = 0; b < lv::kNumBuckets; ++b) bb.hi[b] = his[b];
+        if (opt.chunk_edges > 0) chunk_edges = std::max(16, std::min(opt.chunk_edges, lv::kChunkEdges));
+        if (opt.part_target > 0) part_target = std::max(16, opt.part_target);
+        if (opt.merge_limit > 0) merge_limit = std::min(opt.merge_limit, lv::kMergeLimit);
 
         // M = (sum of CSR weights)/2 in FP64, CSR order (utils.cu:36-46).
         double s = 0.0;
@@ -487,11 +507,11 @@ struct lv_graph {
             for (int b = 0; b < lv::kNumBuckets; ++b) h_bedges[b] = (int64_t)tmp[b];
         }
 
-        // heavy vertices: row chunks (insert, strength, modularity), tables
+        // chunked tier: row chunks (stage A, strength, modularity), per-vertex
+        // partitions (stage B) and overflow tables
         const int HB = lv::kNumBuckets - 1;
         nh = h_off[HB + 1] - h_off[HB];
-        nins = neval = 0;
-        std::vector<lv::Chunk> hin, hev;
+        nins = nitems = 0;
         if (nh > 0) {
             LV_TRY(hrows.reserve(2 * (int64_t)nh));
             hipLaunchKernelGGL(k_heavy_rows, dim3(blocks_for(nh, 256)), dim3(256), 0, s, rp,
@@ -504,41 +524,66 @@ struct lv_graph {
             LV_HIP(hipMemcpyAsync(ids.data(), order + h_off[HB], sizeof(int32_t) * nh,
                                   hipMemcpyDeviceToHost, s));
             LV_TRY(sync());
-            std::vector<int64_t> off(nh);
-            std::vector<int32_t> lg(nh);
-            int64_t slots = 0;
+            std::vector<lv::Chunk> hin;
+            std::vector<lv::HeavyVertex> hh(nh);
+            std::vector<lv::MergeItem> hit;
+            int64_t slots = 0, outs = 0;
+            int32_t pbase = 0, tbase = 0;
             for (int32_t i = 0; i < nh; ++i) {
-                const int64_t lo = rows[2 * i], hi = rows[2 * i + 1];
-                for (int64_t e = lo; e < hi; e += kRowChunk)
-                    hin.push_back({i, ids[i], e, std::min<int64_t>(hi, e + kRowChunk)});
-                lg[i] = std::max(6, ceil_log2(2 * (hi - lo)));
-                off[i] = slots;
-                for (int64_t q = 0; q < ((int64_t)1 << lg[i]); q += kSlotChunk)
-                    hev.push_back({i, ids[i], slots + q,
-                                   slots + std::min<int64_t>((int64_t)1 << lg[i], q + kSlotChunk)});
-                slots += (int64_t)1 << lg[i];
+                const int64_t lo = rows[2 * i], hi = rows[2 * i + 1], deg = hi - lo;
+                int32_t P = (int32_t)std::min<int64_t>(lv::kMaxParts, (deg + part_target - 1) / part_target);
+                P = std::max(P, 1);
+                lv::HeavyVertex &h = hh[i];
+                h.v = ids[i];
+                h.P = P;
+                h.chunk0 = (int32_t)hin.size();
+                h.ptot_base = tbase;
+                h.tab_lg = std::max(6, ceil_log2(2 * deg));
+                h.tab_off = slots;
+                for (int64_t e = lo; e < hi; e += chunk_edges) {
+                    lv::Chunk c{};
+                    c.hv = i;
+                    c.v = ids[i];
+                    c.lo = e;
+                    c.hi = std::min<int64_t>(hi, e + chunk_edges);
+                    c.out = outs;
+                    c.pbase = pbase;
+                    c.P = P;
+                    c.ptot_base = tbase;
+                    outs += c.hi - c.lo;
+                    pbase += P + 1;
+                    hin.push_back(c);
+                }
+                h.nchunks = (int32_t)hin.size() - h.chunk0;
+                for (int32_t p = 0; p < P; ++p) hit.push_back({i, p});
+                tbase += P;
+                slots += (int64_t)1 << h.tab_lg;
             }
             nins = (int32_t)hin.size();
-            neval = (int32_t)hev.size();
+            nitems = (int32_t)hit.size();
             LV_TRY(arena(slots));
             LV_TRY(ins.reserve(nins));
-            LV_TRY(eval.reserve(neval));
-            LV_TRY(hoff.reserve(nh));
-            LV_TRY(hlg.reserve(nh));
+            LV_TRY(hvs.reserve(nh));
+            LV_TRY(items.reserve(nitems));
+            LV_TRY(out_key.reserve(outs));
+            LV_TRY(out_val.reserve(outs));
+            LV_TRY(poff.reserve(pbase));
+            LV_TRY(ptot.reserve(tbase));
             LV_TRY(hbest.reserve(nh));
             LV_TRY(hkcur.reserve(nh));
             LV_HIP(hipMemcpy(ins.p, hin.data(), sizeof(lv::Chunk) * nins, hipMemcpyHostToDevice));
-            LV_HIP(hipMemcpy(eval.p, hev.data(), sizeof(lv::Chunk) * neval, hipMemcpyHostToDevice));
-            LV_HIP(hipMemcpy(hoff.p, off.data(), sizeof(int64_t) * nh, hipMemcpyHostToDevice));
-            LV_HIP(hipMemcpy(hlg.p, lg.data(), sizeof(int32_t) * nh, hipMemcpyHostToDevice));
-            std::vector<uint64_t> zb(nh, 0);
-            std::vector<double> zk(nh, 0.0);
-            LV_HIP(hipMemcpy(hbest.p, zb.data(), sizeof(uint64_t) * nh, hipMemcpyHostToDevice));
-            LV_HIP(hipMemcpy(hkcur.p, zk.data(), sizeof(double) * nh, hipMemcpyHostToDevice));
+            LV_HIP(hipMemcpy(hvs.p, hh.data(), sizeof(lv::HeavyVertex) * nh, hipMemcpyHostToDevice));
+            LV_HIP(hipMemcpy(items.p, hit.data(), sizeof(lv::MergeItem) * nitems,
+                             hipMemcpyHostToDevice));
+            lv::fill_i32(ptot.p, 0, tbase, s);
+            lv::fill_f64(hkcur.p, 0.0, nh, s);
+            hipLaunchKernelGGL(k_zero_u64_n, dim3(blocks_for(nh, 256)), dim3(256), 0, s,
+                               reinterpret_cast<unsigned long long *>(hbest.p), (int64_t)nh);
+            LV_LAUNCHED("chunked tier tables");
         }
 
         // row-reduction segments: lanes per vertex by bucket
-        static const int lanes_lg[lv::kNumBuckets - 1] = {2, 3, 4, 5, 6, 6, 8, 8};
+        static const int lanes_lg[lv::kNumBuckets - 1] = {2, 3, 4, 5, 6, 6, 8};
         st = lv::SegTable{};
         int32_t blk = 0;
         for (int b = 0; b < HB; ++b) {
@@ -563,7 +608,7 @@ struct lv_graph {
         st.sq_block_start = blk;
         blk += (int32_t)((V + lv::kSqItems - 1) / lv::kSqItems);
         st.total_blocks = std::max(blk, 1);
-        LV_TRY(qpart.reserve(2 * (int64_t)st.total_blocks));
+        LV_TRY(qpart.reserve(2 * (int64_t)st.total_blocks + 2 * 256));
 
         // strengths and singleton state (utils.cu:84-89; computeEdgesSum)
         lv::fill_f64(k, 0.0, V, s);
@@ -601,14 +646,9 @@ struct lv_graph {
                 lv::launch_move_wg(256, wg6_lg, a, list, n, s);
                 tend(t0, LV_K_MOVE_WG, bytes);
                 break;
-            case 7:
-                t0 = tbegin();
-                lv::launch_move_wg(wg7_threads, wg7_lg, a, list, n, s);
-                tend(t0, LV_K_MOVE_WG, bytes);
-                break;
             default:
                 t0 = tbegin();
-                lv::launch_move_heavy(a, list, nh, ins.p, nins, eval.p, neval, heavy_tables(), s);
+                lv::launch_move_chunked(a, chunked_tier(), nins, nitems, list, nh, s);
                 tend(t0, LV_K_MOVE_GLOBAL, bytes);
                 break;
             }
@@ -875,10 +915,13 @@ struct lv_graph {
         dev_free(tmp_col);
         dev_free(tmp_w);
         ins.release();
-        eval.release();
-        hoff.release();
+        hvs.release();
+        items.release();
+        out_key.release();
+        out_val.release();
+        poff.release();
+        ptot.release();
         hrows.release();
-        hlg.release();
         hbest.release();
         hkcur.release();
         gkeys.release();

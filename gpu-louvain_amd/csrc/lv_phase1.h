@@ -5,23 +5,56 @@
 
 namespace lv {
 
-// Work table for rows too long for one workgroup: one entry per chunk of a
-// heavy vertex's row (edge range) or of its hash table (slot range).
+// Chunked tier (rows too long for one workgroup).  A heavy vertex's row is
+// cut into chunks of at most kChunkEdges edges; stage A aggregates each
+// chunk in LDS and writes its distinct (community, weight) pairs bucketed
+// by partition part_of(c, P); stage B merges one partition of one vertex in
+// LDS and evaluates its gains.  Chunks also drive the heavy rows of the
+// strength and modularity reductions.
+constexpr int kChunkEdges = 2048;    // max edges per chunk (stage A LDS: 4096 slots)
+constexpr int kChunkCapLog2 = 12;
+constexpr int kMergeCapLog2 = 13;    // stage B LDS: 8192 slots
+constexpr int kMergeLimit = 6144;    // entries a stage-B table may take
+constexpr int kMaxParts = 1024;
+
 struct Chunk {
-    int32_t hv;  // index into the heavy-vertex list
-    int32_t v;   // vertex id
-    int64_t lo;  // first edge / slot (absolute)
-    int64_t hi;  // one past the last
+    int32_t hv;         // index of the heavy vertex
+    int32_t v;          // vertex id
+    int64_t lo, hi;     // edge range
+    int64_t out;        // first output pair of this chunk
+    int32_t pbase;      // offsets of its partitions in poff[pbase .. pbase+P]
+    int32_t P;          // partitions of the vertex
+    int32_t ptot_base;  // per-partition totals of the vertex in ptot[]
+    int32_t pad;
 };
 
-// Heavy-vertex global hash tables (one region per heavy vertex).
-struct HeavyTables {
-    int32_t *keys;
-    double *vals;
-    const int64_t *off;       // [nh] slot offset of each vertex's table
-    const int32_t *cap_log2;  // [nh]
-    uint64_t *best;           // [nh] packed argmax, 0 = none
-    double *kcur;             // [nh] weight to the own community
+struct HeavyVertex {
+    int32_t v;
+    int32_t P;
+    int32_t chunk0, nchunks;
+    int32_t ptot_base;
+    int32_t tab_lg;     // overflow global table: 2^tab_lg slots at tab_off
+    int64_t tab_off;
+};
+
+struct MergeItem {
+    int32_t hv;
+    int32_t p;
+};
+
+struct ChunkedTier {
+    const Chunk *chunks;
+    const HeavyVertex *hvs;
+    const MergeItem *items;
+    int32_t *out_key;   // pairs written by stage A
+    double *out_val;
+    int32_t *poff;      // per chunk: P+1 partition offsets
+    int32_t *ptot;      // per vertex and partition: pairs (zeroed by stage B)
+    int32_t *gkeys;     // overflow tables (kept all-empty between uses)
+    double *gvals;
+    uint64_t *best;     // per vertex packed argmax, 0 = none
+    double *kcur;       // per vertex weight to its own community
+    int32_t merge_limit;
 };
 
 // Segment of the vertex list processed with a fixed lane count per vertex
@@ -57,11 +90,9 @@ void launch_move_sub(int X, const MoveArgs &a, const int32_t *list, int32_t n, h
 void launch_move_wave(int cap, const MoveArgs &a, const int32_t *list, int32_t n, hipStream_t s);
 void launch_move_wg(int threads, int cap_log2, const MoveArgs &a, const int32_t *list, int32_t n,
                     hipStream_t s);
-int move_wg_max_cap_log2(int max_lds_bytes);
 void set_wg_lds_limits(int bytes);
-void launch_move_heavy(const MoveArgs &a, const int32_t *heavy_list, int32_t nh,
-                       const Chunk *ins, int32_t nins, const Chunk *eval, int32_t neval,
-                       const HeavyTables &t, hipStream_t s);
+void launch_move_chunked(const MoveArgs &a, const ChunkedTier &t, int32_t nchunks,
+                         int32_t nitems, const int32_t *heavy_list, int32_t nh, hipStream_t s);
 void launch_clear_tables(int32_t *keys, double *vals, int64_t n, hipStream_t s);
 
 // --- apply ---

@@ -16,9 +16,11 @@
 //                   sequential ones), argmax by xor-shuffle of packed keys.
 //   k_move_wave<C>  one wavefront per vertex, C-slot LDS hash per wave.
 //   k_move_wg<T>    one T-lane workgroup per vertex, LDS hash sized per launch.
-//   k_heavy_*       rows beyond the LDS tier: edge chunks from many
-//                   workgroups insert into a per-vertex global hash; slot
-//                   chunks evaluate gains into a packed 64-bit atomic max.
+//   k_chunk_*       rows beyond one workgroup: stage A aggregates 2048-edge
+//                   chunks in LDS and buckets the distinct pairs by hash
+//                   partition; stage B merges each (vertex, partition) in
+//                   LDS, evaluates gains and combines partitions through a
+//                   packed 64-bit atomic max.
 #include "lv_phase1.h"
 
 namespace lv {
@@ -213,54 +215,170 @@ __global__ __launch_bounds__(T) void k_move_wg(MoveArgs a, const int32_t *__rest
     }
 }
 
-__global__ __launch_bounds__(256) void k_heavy_insert(MoveArgs a, const Chunk *__restrict__ ins,
-                                                      HeavyTables t) {
-    const Chunk ch = ins[blockIdx.x];
-    const int64_t off = t.off[ch.hv];
-    const int lg = t.cap_log2[ch.hv];
-    int32_t *K = t.keys + off;
-    double *W = t.vals + off;
+// ---------------------------------------------------------- chunked tier
+
+// Stage A: aggregate one chunk of a heavy row in LDS; write its distinct
+// (community, weight) pairs grouped by partition.
+__global__ __launch_bounds__(256) void k_chunk_aggregate(MoveArgs a, ChunkedTier t) {
+    constexpr int C = 1 << kChunkCapLog2;
+    __shared__ int32_t keys[C];
+    __shared__ double vals[C];
+    __shared__ int32_t pcnt[kMaxParts];
+    const Chunk ch = t.chunks[blockIdx.x];
+    const int P = ch.P;
+    for (int i = threadIdx.x; i < C; i += 256) {
+        keys[i] = -1;
+        vals[i] = 0.0;
+    }
+    for (int p = threadIdx.x; p < P; p += 256) pcnt[p] = 0;
+    __syncthreads();
     const int32_t v = ch.v;
     for (int64_t e = ch.lo + threadIdx.x; e < ch.hi; e += 256) {
         const int32_t u = a.col[e];
         const float wf = a.w[e];
         if (u == v) continue;
-        global_insert(K, W, 32 - lg, (1u << lg) - 1, a.comm[u], (double)wf);
+        lds_insert(keys, vals, 32 - kChunkCapLog2, C - 1, a.comm[u], (double)wf);
+    }
+    __syncthreads();
+    for (int s = threadIdx.x; s < C; s += 256) {
+        const int32_t k = keys[s];
+        if (k >= 0) atomicAdd(&pcnt[part_of(k, (uint32_t)P)], 1);
+    }
+    __syncthreads();
+    if (threadIdx.x < 64) {  // wave 0: exclusive scan of the partition counts
+        const int lane = threadIdx.x;
+        int32_t carry = 0;
+        for (int b = 0; b < P; b += 64) {
+            const int p = b + lane;
+            const int32_t x = p < P ? pcnt[p] : 0;
+            int32_t inc = x;
+#pragma unroll
+            for (int off = 1; off < 64; off <<= 1) {
+                const int32_t y = __shfl_up(inc, off, 64);
+                if (lane >= off) inc += y;
+            }
+            if (p < P) {
+                t.poff[ch.pbase + p] = carry + inc - x;
+                if (x > 0) atomicAdd(&t.ptot[ch.ptot_base + p], x);
+                pcnt[p] = carry + inc - x;  // becomes the scatter cursor
+            }
+            carry += __shfl(inc, 63, 64);
+        }
+        if (lane == 0) t.poff[ch.pbase + P] = carry;
+    }
+    __syncthreads();
+    for (int s = threadIdx.x; s < C; s += 256) {
+        const int32_t k = keys[s];
+        if (k < 0) continue;
+        const int32_t pos = atomicAdd(&pcnt[part_of(k, (uint32_t)P)], 1);
+        t.out_key[ch.out + pos] = k;
+        t.out_val[ch.out + pos] = vals[s];
     }
 }
 
-__global__ __launch_bounds__(256) void k_heavy_eval(MoveArgs a, const Chunk *__restrict__ ev,
-                                                    HeavyTables t) {
-    const Chunk ch = ev[blockIdx.x];
-    const int32_t v = ch.v;
+// Stage B: merge partition p of a heavy vertex and evaluate its candidates.
+// A partition holding more pairs than an LDS table takes falls back to the
+// vertex's global table (always large enough), so every insert terminates.
+constexpr int kMergeThreads = 512;
+
+__global__ __launch_bounds__(kMergeThreads) void k_chunk_merge(MoveArgs a, ChunkedTier t) {
+    constexpr int C = 1 << kMergeCapLog2;
+    extern __shared__ double merge_lds[];
+    double *vals = merge_lds;
+    int32_t *keys = reinterpret_cast<int32_t *>(merge_lds + C);
+    __shared__ uint64_t red_best[kMergeThreads / 64];
+    __shared__ double red_kc[kMergeThreads / 64];
+    const MergeItem it = t.items[blockIdx.x];
+    const HeavyVertex h = t.hvs[it.hv];
+    const int32_t v = h.v;
     const int32_t cur = a.comm[v];
     const float k32 = (float)a.k[v];
     const float scur = (float)a.sigma[cur];
     const int32_t size_cur = a.size[cur];
+    const int32_t n = t.ptot[h.ptot_base + it.p];
     uint64_t best = 0;
-    for (int64_t i = ch.lo + threadIdx.x; i < ch.hi; i += 256) {
-        const int32_t c = t.keys[i];
-        if (c < 0) continue;
-        const double Kc = t.vals[i];
-        t.keys[i] = -1;  // leave the table clean for the next sweep
-        t.vals[i] = 0.0;
-        if (c == cur) {
-            t.kcur[ch.hv] = Kc;
-            continue;
+    double kc = 0.0;
+    if (n <= t.merge_limit) {
+        for (int i = threadIdx.x; i < C; i += kMergeThreads) {
+            keys[i] = -1;
+            vals[i] = 0.0;
         }
-        if (!may_move(c, cur, size_cur, a.size)) continue;
-        const float g = gain_of((float)Kc, a.M32, a.den, k32, scur, (float)a.sigma[c]);
-        const uint64_t key = pack_best(g, c);
-        best = key > best ? key : best;
+        __syncthreads();
+        for (int32_t c = h.chunk0; c < h.chunk0 + h.nchunks; ++c) {
+            const Chunk ch = t.chunks[c];
+            const int32_t lo = t.poff[ch.pbase + it.p], hi = t.poff[ch.pbase + it.p + 1];
+            for (int32_t i = lo + threadIdx.x; i < hi; i += kMergeThreads)
+                lds_insert(keys, vals, 32 - kMergeCapLog2, C - 1, t.out_key[ch.out + i],
+                           t.out_val[ch.out + i]);
+        }
+        __syncthreads();
+        for (int i = threadIdx.x; i < C; i += kMergeThreads) {
+            const int32_t c = keys[i];
+            if (c < 0) continue;
+            const double Kc = vals[i];
+            if (c == cur) {
+                kc = Kc;
+                continue;
+            }
+            if (!may_move(c, cur, size_cur, a.size)) continue;
+            const float g = gain_of((float)Kc, a.M32, a.den, k32, scur, (float)a.sigma[c]);
+            const uint64_t key = pack_best(g, c);
+            best = key > best ? key : best;
+        }
+    } else {
+        int32_t *K = t.gkeys + h.tab_off;
+        double *W = t.gvals + h.tab_off;
+        const uint32_t cap = 1u << h.tab_lg;
+        for (int32_t c = h.chunk0; c < h.chunk0 + h.nchunks; ++c) {
+            const Chunk ch = t.chunks[c];
+            const int32_t lo = t.poff[ch.pbase + it.p], hi = t.poff[ch.pbase + it.p + 1];
+            for (int32_t i = lo + threadIdx.x; i < hi; i += kMergeThreads)
+                global_insert(K, W, 32 - h.tab_lg, cap - 1, t.out_key[ch.out + i],
+                              t.out_val[ch.out + i]);
+        }
+        __threadfence();
+        __syncthreads();
+        for (uint32_t i = threadIdx.x; i < cap; i += kMergeThreads) {
+            const int32_t c = __hip_atomic_load(&K[i], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+            if (c < 0 || part_of(c, (uint32_t)h.P) != (uint32_t)it.p) continue;
+            const double Kc = __hip_atomic_load(&W[i], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+            K[i] = -1;  // leave the table clean
+            W[i] = 0.0;
+            if (c == cur) {
+                kc = Kc;
+                continue;
+            }
+            if (!may_move(c, cur, size_cur, a.size)) continue;
+            const float g = gain_of((float)Kc, a.M32, a.den, k32, scur, (float)a.sigma[c]);
+            const uint64_t key = pack_best(g, c);
+            best = key > best ? key : best;
+        }
     }
     best = wave_max_u64(best);
-    if ((threadIdx.x & 63) == 0 && best != 0)
-        atomicMax(reinterpret_cast<unsigned long long *>(&t.best[ch.hv]),
-                  (unsigned long long)best);
+    kc = wave_sum_f64(kc);
+    const int lane = threadIdx.x & 63, wid = threadIdx.x >> 6;
+    if (lane == 0) {
+        red_best[wid] = best;
+        red_kc[wid] = kc;
+    }
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        uint64_t b = red_best[0];
+        double kk = red_kc[0];
+        for (int w = 1; w < kMergeThreads / 64; ++w) {
+            b = red_best[w] > b ? red_best[w] : b;
+            kk += red_kc[w];
+        }
+        if (b != 0)
+            atomicMax(reinterpret_cast<unsigned long long *>(&t.best[it.hv]),
+                      (unsigned long long)b);
+        if (part_of(cur, (uint32_t)h.P) == (uint32_t)it.p) t.kcur[it.hv] = kk;
+        t.ptot[h.ptot_base + it.p] = 0;  // ready for the next sweep
+    }
 }
 
 __global__ void k_heavy_final(MoveArgs a, const int32_t *__restrict__ heavy, int32_t nh,
-                              HeavyTables t) {
+                              ChunkedTier t) {
     const int32_t hv = blockIdx.x * blockDim.x + threadIdx.x;
     if (hv >= nh) return;
     const int32_t v = heavy[hv];
@@ -310,6 +428,31 @@ __global__ __launch_bounds__(256) void k_apply(const int32_t *__restrict__ list,
 }
 
 // ------------------------------------------------------- row reductions
+
+// Weight from a row slice [lo, hi) (stride `st`) to community cv; four
+// entries per step so that their gathers are in flight together.
+__device__ __forceinline__ double own_weight(const int32_t *__restrict__ col,
+                                             const float *__restrict__ w,
+                                             const int32_t *__restrict__ comm, int32_t cv,
+                                             int64_t lo, int64_t hi, int st) {
+    double own = 0.0;
+    int64_t e = lo;
+    for (; e + 3 * st < hi; e += 4 * st) {
+        const int32_t u0 = col[e], u1 = col[e + st], u2 = col[e + 2 * st], u3 = col[e + 3 * st];
+        const float w0 = w[e], w1 = w[e + st], w2 = w[e + 2 * st], w3 = w[e + 3 * st];
+        const int32_t c0 = comm[u0], c1 = comm[u1], c2 = comm[u2], c3 = comm[u3];
+        own += c0 == cv ? (double)w0 : 0.0;
+        own += c1 == cv ? (double)w1 : 0.0;
+        own += c2 == cv ? (double)w2 : 0.0;
+        own += c3 == cv ? (double)w3 : 0.0;
+    }
+    for (; e < hi; e += st) {
+        const int32_t u = col[e];
+        const float wf = w[e];
+        if (comm[u] == cv) own += (double)wf;
+    }
+    return own;
+}
 
 __device__ __forceinline__ int find_seg(const SegTable &st, int b) {
     int si = 0;
@@ -397,12 +540,7 @@ __global__ __launch_bounds__(kRowThreads) void k_modularity(
         }
     } else if (b >= st.chunk_block_start) {
         const Chunk ch = chunks[b - st.chunk_block_start];
-        const int32_t cv = comm[ch.v];
-        for (int64_t e = ch.lo + threadIdx.x; e < ch.hi; e += kRowThreads) {
-            const int32_t u = col[e];
-            const float wf = w[e];
-            if (comm[u] == cv) own += (double)wf;
-        }
+        own = own_weight(col, w, comm, comm[ch.v], ch.lo + threadIdx.x, ch.hi, kRowThreads);
     } else {
         const Seg sg = st.s[find_seg(st, b)];
         const int lg = sg.lanes_log2;
@@ -412,21 +550,12 @@ __global__ __launch_bounds__(kRowThreads) void k_modularity(
             const int lane = threadIdx.x & (X - 1);
             if (idx < sg.count) {
                 const int32_t v = order[sg.list_off + idx];
-                const int32_t cv = comm[v];
-                for (int32_t e = rowptr[v] + lane; e < rowptr[v + 1]; e += X) {
-                    const int32_t u = col[e];
-                    const float wf = w[e];
-                    if (comm[u] == cv) own += (double)wf;
-                }
+                own = own_weight(col, w, comm, comm[v], rowptr[v] + lane, rowptr[v + 1], X);
             }
         } else {
             const int32_t v = order[sg.list_off + (b - sg.block_start)];
-            const int32_t cv = comm[v];
-            for (int32_t e = rowptr[v] + threadIdx.x; e < rowptr[v + 1]; e += kRowThreads) {
-                const int32_t u = col[e];
-                const float wf = w[e];
-                if (comm[u] == cv) own += (double)wf;
-            }
+            own = own_weight(col, w, comm, comm[v], rowptr[v] + threadIdx.x, rowptr[v + 1],
+                             kRowThreads);
         }
     }
     own = block_sum_f64(own, lds);
@@ -437,14 +566,33 @@ __global__ __launch_bounds__(kRowThreads) void k_modularity(
     }
 }
 
-__global__ __launch_bounds__(1024) void k_modularity_final(const double *__restrict__ partials,
-                                                           int32_t nb, double M, double *q) {
-    __shared__ double lds[16];
+// Two-level fixed-order reduction of the per-block partials: 256 blocks
+// each add a contiguous range, then one block adds the 256 results.
+constexpr int kQReduceBlocks = 256;
+
+__global__ __launch_bounds__(256) void k_modularity_reduce(const double *__restrict__ partials,
+                                                           int32_t nb, double *partials2) {
+    __shared__ double lds[4];
+    const int64_t lo = (int64_t)nb * blockIdx.x / kQReduceBlocks;
+    const int64_t hi = (int64_t)nb * (blockIdx.x + 1) / kQReduceBlocks;
     double own = 0.0, sq = 0.0;
-    for (int32_t i = threadIdx.x; i < nb; i += 1024) {
+    for (int64_t i = lo + threadIdx.x; i < hi; i += 256) {
         own += partials[2 * i];
         sq += partials[2 * i + 1];
     }
+    own = block_sum_f64(own, lds);
+    sq = block_sum_f64(sq, lds);
+    if (threadIdx.x == 0) {
+        partials2[2 * blockIdx.x] = own;
+        partials2[2 * blockIdx.x + 1] = sq;
+    }
+}
+
+__global__ __launch_bounds__(256) void k_modularity_final(const double *__restrict__ partials2,
+                                                          double M, double *q) {
+    __shared__ double lds[4];
+    double own = partials2[2 * threadIdx.x];
+    double sq = partials2[2 * threadIdx.x + 1];
     own = block_sum_f64(own, lds);
     sq = block_sum_f64(sq, lds);
     if (threadIdx.x == 0) {
@@ -504,31 +652,21 @@ void launch_move_wave(int cap, const MoveArgs &a, const int32_t *list, int32_t n
         hipLaunchKernelGGL(k_move_wave<512>, dim3(g), dim3(256), 0, s, a, list, n);
 }
 
-int move_wg_max_cap_log2(int max_lds_bytes) {
-    int lg = 4;
-    while (((size_t)12 << (lg + 1)) + 1024 <= (size_t)max_lds_bytes) ++lg;
-    return lg;
-}
-
 void launch_move_wg(int threads, int cap_log2, const MoveArgs &a, const int32_t *list, int32_t n,
                     hipStream_t s) {
+    (void)threads;
     if (n <= 0) return;
     const size_t lds = (size_t)12 << cap_log2;
     const unsigned g = (unsigned)(n < 1048576 ? n : 1048576);
-    if (threads <= 256)
-        hipLaunchKernelGGL(k_move_wg<256>, dim3(g), dim3(256), lds, s, a, list, n, cap_log2);
-    else if (threads <= 512)
-        hipLaunchKernelGGL(k_move_wg<512>, dim3(g), dim3(512), lds, s, a, list, n, cap_log2);
-    else
-        hipLaunchKernelGGL(k_move_wg<1024>, dim3(g), dim3(1024), lds, s, a, list, n, cap_log2);
+    hipLaunchKernelGGL(k_move_wg<256>, dim3(g), dim3(256), lds, s, a, list, n, cap_log2);
 }
 
-void launch_move_heavy(const MoveArgs &a, const int32_t *heavy_list, int32_t nh,
-                       const Chunk *ins, int32_t nins, const Chunk *eval, int32_t neval,
-                       const HeavyTables &t, hipStream_t s) {
+void launch_move_chunked(const MoveArgs &a, const ChunkedTier &t, int32_t nchunks,
+                         int32_t nitems, const int32_t *heavy_list, int32_t nh, hipStream_t s) {
     if (nh <= 0) return;
-    hipLaunchKernelGGL(k_heavy_insert, dim3(nins), dim3(256), 0, s, a, ins, t);
-    hipLaunchKernelGGL(k_heavy_eval, dim3(neval), dim3(256), 0, s, a, eval, t);
+    hipLaunchKernelGGL(k_chunk_aggregate, dim3(nchunks), dim3(256), 0, s, a, t);
+    hipLaunchKernelGGL(k_chunk_merge, dim3(nitems), dim3(kMergeThreads),
+                       (size_t)12 << kMergeCapLog2, s, a, t);
     hipLaunchKernelGGL(k_heavy_final, dim3(blocks_for(nh, 256)), dim3(256), 0, s, a, heavy_list,
                        nh, t);
 }
@@ -554,8 +692,10 @@ void launch_modularity(const int32_t *rowptr, const int32_t *col, const float *w
                        double *q, hipStream_t s) {
     hipLaunchKernelGGL(k_modularity, dim3(st.total_blocks), dim3(kRowThreads), 0, s, rowptr, col,
                        w, comm, sigma, V, order, st, chunks, partials);
-    hipLaunchKernelGGL(k_modularity_final, dim3(1), dim3(1024), 0, s, partials, st.total_blocks, M,
-                       q);
+    double *partials2 = partials + 2 * (int64_t)st.total_blocks;
+    hipLaunchKernelGGL(k_modularity_reduce, dim3(kQReduceBlocks), dim3(256), 0, s, partials,
+                       st.total_blocks, partials2);
+    hipLaunchKernelGGL(k_modularity_final, dim3(1), dim3(256), 0, s, partials2, M, q);
 }
 
 void launch_compose(int32_t *orig2comm, int32_t V0, const int32_t *map, hipStream_t s) {
@@ -565,11 +705,9 @@ void launch_compose(int32_t *orig2comm, int32_t V0, const int32_t *map, hipStrea
 
 void set_wg_lds_limits(int bytes) {
     (void)hipFuncSetAttribute(reinterpret_cast<const void *>(&k_move_wg<256>),
-                        hipFuncAttributeMaxDynamicSharedMemorySize, bytes);
-    (void)hipFuncSetAttribute(reinterpret_cast<const void *>(&k_move_wg<512>),
-                        hipFuncAttributeMaxDynamicSharedMemorySize, bytes);
-    (void)hipFuncSetAttribute(reinterpret_cast<const void *>(&k_move_wg<1024>),
-                        hipFuncAttributeMaxDynamicSharedMemorySize, bytes);
+                              hipFuncAttributeMaxDynamicSharedMemorySize, bytes);
+    (void)hipFuncSetAttribute(reinterpret_cast<const void *>(&k_chunk_merge),
+                              hipFuncAttributeMaxDynamicSharedMemorySize, bytes);
 }
 
 }  // namespace lv

@@ -34,7 +34,7 @@ LV_ERR_STATE = -5
 LV_ERR_NODEVICE = -6
 
 K_MOVE_SUB, K_MOVE_WAVE, K_MOVE_WG, K_MOVE_GLOBAL, K_APPLY, K_MODULARITY, K_AGGREGATE = range(7)
-KERNEL_NAMES = ["move_sub", "move_wave", "move_wg", "move_global", "apply", "modularity", "aggregate"]
+KERNEL_NAMES = ["move_sub", "move_wave", "move_wg", "move_chunked", "apply", "modularity", "aggregate"]
 
 I32P = ctypes.POINTER(ctypes.c_int32)
 I64P = ctypes.POINTER(ctypes.c_int64)
@@ -53,9 +53,11 @@ class Options(ctypes.Structure):
         ("device", ctypes.c_int32),
         ("timing", ctypes.c_int32),
         ("heavy_wg_max", ctypes.c_int32),
-        ("mid_wg_max", ctypes.c_int32),
+        ("chunk_edges", ctypes.c_int32),
         ("agg_lds_max", ctypes.c_int32),
-        ("reserved", ctypes.c_int32 * 11),
+        ("part_target", ctypes.c_int32),
+        ("merge_limit", ctypes.c_int32),
+        ("reserved", ctypes.c_int32 * 9),
     ]
 
 
@@ -215,8 +217,8 @@ def gen_planted(n: int, blocks: int, p_in: float, p_out: float, seed: int = 1,
 class Graph:
     """A graph resident on one GPU with its community state (lv_graph)."""
 
-    def __init__(self, g: CSR, device: int = 0, timing: bool = False, mid_wg_max: int = 0,
-                 heavy_wg_max: int = 0):
+    def __init__(self, g: CSR, device: int = 0, timing: bool = False, heavy_wg_max: int = 0,
+                 chunk_edges: int = 0, part_target: int = 0, merge_limit: int = 0, agg_lds_max: int = 0):
         lib = load()
         self._lib = lib
         self._rowptr = np.ascontiguousarray(g.rowptr, dtype=np.int32)
@@ -226,8 +228,11 @@ class Graph:
         opt = Options()
         opt.device = device
         opt.timing = int(timing)
-        opt.mid_wg_max = mid_wg_max
         opt.heavy_wg_max = heavy_wg_max
+        opt.chunk_edges = chunk_edges
+        opt.part_target = part_target
+        opt.merge_limit = merge_limit
+        opt.agg_lds_max = agg_lds_max
         h = ctypes.c_void_p()
         col = self._col if self._col.size else np.zeros(1, np.int32)
         w = self._w if self._w.size else np.zeros(1, np.float32)

@@ -40,10 +40,12 @@ typedef struct lv_graph lv_graph;
 typedef struct {
     int32_t device;          /* HIP device ordinal (default 0)                       */
     int32_t timing;          /* 1: record per-kernel HIP events (lv_kernel_time)      */
-    int32_t heavy_wg_max;    /* degree bound of the workgroup/LDS tier (0 = auto)     */
-    int32_t mid_wg_max;      /* degree bound of the 256-lane LDS tier (0 = 1536)      */
+    int32_t heavy_wg_max;    /* degree bound of the one-workgroup LDS tier (0 = 2048)  */
+    int32_t chunk_edges;     /* edges per chunk of the chunked tier (0 = 2048)        */
     int32_t agg_lds_max;     /* summed-degree bound of LDS aggregation (0 = auto)     */
-    int32_t reserved[11];
+    int32_t part_target;     /* pairs per merge partition of the chunked tier (0=2048) */
+    int32_t merge_limit;     /* pairs an LDS merge table may take (0 = 6144)          */
+    int32_t reserved[9];
 } lv_options;
 
 /* Run statistics (lv_run).  Times are device times from HIP events. */
@@ -72,8 +74,8 @@ typedef struct {
 
 #define LV_K_MOVE_SUB 0      /* sub-wavefront move kernels (degree <= 32)           */
 #define LV_K_MOVE_WAVE 1     /* one-wavefront move kernels (33..319)                */
-#define LV_K_MOVE_WG 2       /* workgroup + LDS hash move kernels                   */
-#define LV_K_MOVE_GLOBAL 3   /* heavy-vertex global-hash move kernels               */
+#define LV_K_MOVE_WG 2       /* one-workgroup LDS hash move kernel (320..2048)      */
+#define LV_K_MOVE_GLOBAL 3   /* chunked tier: chunk aggregation + partition merge   */
 #define LV_K_APPLY 4         /* apply moves + incremental totals                    */
 #define LV_K_MODULARITY 5    /* fused modularity reduction                          */
 #define LV_K_AGGREGATE 6     /* whole aggregation (phase 2)                         */

@@ -31,8 +31,12 @@ def _cases(lvlib):
     out.append(("random_selfloops_real", _csr(lvlib, *graphs.random_graph(400, 3000, 7, True, 40)), {}))
     out.append(("star3000", _csr(lvlib, *graphs.star(3000)), {}))
     out.append(("star20000", _csr(lvlib, *graphs.star(20000)), {}))
-    # force the multi-workgroup global-hash tier on a mid-size graph
-    out.append(("rmat14_global_tier", lvlib.gen_rmat(14, 16, seed=41), {"mid_wg_max": 320, "heavy_wg_max": 321}))
+    # force the chunked tier (many chunks and partitions) on a mid-size graph
+    out.append(("rmat14_chunked_tier", lvlib.gen_rmat(14, 16, seed=41),
+                {"heavy_wg_max": 320, "chunk_edges": 64, "part_target": 64}))
+    # force the chunked tier's global-table fallback (LDS merge limit 16 pairs)
+    out.append(("rmat13_merge_overflow", lvlib.gen_rmat(13, 16, seed=43),
+                {"heavy_wg_max": 320, "chunk_edges": 128, "part_target": 100000, "merge_limit": 16}))
     return out
 
 
@@ -43,7 +47,8 @@ def cases(lvlib):
 
 def _names():
     return ["two_triangles", "planted_c1_unit", "rmat12_real", "rmat14_real", "rmat13_unit",
-            "random_selfloops_real", "star3000", "star20000", "rmat14_global_tier"]
+            "random_selfloops_real", "star3000", "star20000", "rmat14_chunked_tier",
+            "rmat13_merge_overflow"]
 
 
 def _get(cases, name):
