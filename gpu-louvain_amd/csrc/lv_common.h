@@ -42,10 +42,23 @@ struct MoveArgs {
     const int32_t *size;
     const double *k;
     const double *sigma;
+    const uint64_t *crec;  // packed community record, see pack_rec
     int32_t *newc;
     float M32;  // (float)M
     float den;  // (2.0f * M32) * M32, fp32 as the reference evaluates it
 };
+
+// Community record read by the move kernels: the fp32 rounding of the FP64
+// community total (the value the gain uses) and the community size, in one
+// 8-byte word so that a candidate costs one gather.
+__host__ __device__ __forceinline__ uint64_t pack_rec(double sigma, int32_t size) {
+    const float s = (float)sigma;
+    uint32_t bits;
+    __builtin_memcpy(&bits, &s, 4);
+    return ((uint64_t)(uint32_t)size << 32) | bits;
+}
+__device__ __forceinline__ float rec_sigma(uint64_t r) { return __uint_as_float((uint32_t)r); }
+__device__ __forceinline__ int32_t rec_size(uint64_t r) { return (int32_t)(r >> 32); }
 
 // computeGain (modularity_optimisation.cu:96-102), fp32 in the same order:
 //   K/M + k * ((S_cur - k) - S_c) / (2*M*M).  No contraction into FMAs.
@@ -86,10 +99,16 @@ __device__ __forceinline__ int32_t decide_from(uint64_t best, double kcur, float
     return (bg - t > 0.0f) ? best_comm(best) : cur;
 }
 
-// Singleton rule (modularity_optimisation.cu:154-155).
-__device__ __forceinline__ bool may_move(int32_t c, int32_t cur, int32_t size_cur,
-                                         const int32_t *size) {
-    return c < cur || size_cur > 1 || size[c] > 1;
+// Candidate c with weight Kc from v (strength k32) in cur: singleton rule
+// (modularity_optimisation.cu:154-155), then the gain folded into the
+// packed argmax.  rc = record of c.
+__device__ __forceinline__ void consider(uint64_t &best, int32_t c, double Kc, uint64_t rc,
+                                         int32_t cur, int32_t size_cur, float k32, float scur,
+                                         float M32, float den) {
+    if (!(c < cur || size_cur > 1 || rec_size(rc) > 1)) return;
+    const float g = gain_of((float)Kc, M32, den, k32, scur, rec_sigma(rc));
+    const uint64_t key = pack_best(g, c);
+    best = key > best ? key : best;
 }
 
 __device__ __forceinline__ uint32_t hash_u32(int32_t c) {

@@ -193,11 +193,12 @@ struct lv_graph {
     // community state
     double *k = nullptr, *sigma = nullptr;
     int32_t *comm = nullptr, *size = nullptr, *newc = nullptr, *orig2comm = nullptr;
+    uint64_t *crec = nullptr;
 
     // bucketing
     lv::BucketBounds bb{};
     int wg6_lg = 12;             // LDS slots (log2) of the one-workgroup tier
-    int chunk_edges = lv::kChunkEdges, part_target = 2048, merge_limit = lv::kMergeLimit;
+    int chunk_edges = lv::kChunkEdges, part_target = lv::kPartTarget, merge_limit = lv::kMergeLimit;
     lv::AggBounds agg_bounds{};
     int8_t *keys8 = nullptr;
     int32_t *order = nullptr, *d_off = nullptr;
@@ -318,6 +319,7 @@ struct lv_graph {
         a.size = size;
         a.k = k;
         a.sigma = sigma;
+        a.crec = crec;
         a.newc = newc;
         a.M32 = M32;
         a.den = den;
@@ -420,6 +422,7 @@ struct lv_graph {
         LV_TRY(dev_alloc(&size, Vp));
         LV_TRY(dev_alloc(&newc, Vp));
         LV_TRY(dev_alloc(&orig2comm, Vp));
+        LV_TRY(dev_alloc(&crec, Vp));
         LV_TRY(dev_alloc(&keys8, Vp));
         LV_TRY(dev_alloc(&order, Vp));
         LV_TRY(dev_alloc(&d_off, 64));
@@ -469,7 +472,7 @@ struct lv_graph {
     lv_status reset() {
         lv::iota_i32(orig2comm, V0, stream);
         if (cur_buf < 0 && level_ready) {
-            lv::launch_level_state(V, k, comm, size, sigma, stream);
+            lv::launch_level_state(V, k, comm, size, sigma, crec, stream);
             LV_LAUNCHED("reset");
             return LV_OK;
         }
@@ -613,7 +616,7 @@ struct lv_graph {
         // strengths and singleton state (utils.cu:84-89; computeEdgesSum)
         lv::fill_f64(k, 0.0, V, s);
         lv::launch_strength(rp, w(), order, st, ins.p, k, s);
-        lv::launch_level_state(V, k, comm, size, sigma, s);
+        lv::launch_level_state(V, k, comm, size, sigma, crec, s);
         LV_LAUNCHED("level state");
         level_ready = true;
         return LV_OK;
@@ -699,7 +702,7 @@ struct lv_graph {
             if (lo < 0 || hi <= lo) continue;
             LV_TRY(run_group(g));
             hipEvent_t t0 = tbegin();
-            lv::launch_apply(order + lo, hi - lo, newc, comm, size, sigma, k, d_moved, s);
+            lv::launch_apply(order + lo, hi - lo, newc, comm, size, sigma, k, crec, d_moved, s);
             tend(t0, LV_K_APPLY, 16.0 * (double)(hi - lo));
             LV_LAUNCHED("apply");
         }
@@ -894,6 +897,7 @@ struct lv_graph {
         dev_free(size);
         dev_free(newc);
         dev_free(orig2comm);
+        dev_free(crec);
         dev_free(keys8);
         dev_free(order);
         dev_free(d_off);
@@ -1132,6 +1136,7 @@ lv_status lv_set_communities(lv_graph *g, const int32_t *comm) {
         lv::fill_f64(g->sigma, 0.0, g->V, g->stream);
         hipLaunchKernelGGL(k_state_from_comm, dim3(blocks_for(g->V, 256)), dim3(256), 0, g->stream,
                            g->V, g->comm, g->k, g->size, g->sigma);
+        lv::launch_records_all(g->V, g->sigma, g->size, g->crec, g->stream);
         LV_LAUNCHED("state from communities");
     }
     return g->sync();

@@ -11,11 +11,13 @@ namespace lv {
 // by partition part_of(c, P); stage B merges one partition of one vertex in
 // LDS and evaluates its gains.  Chunks also drive the heavy rows of the
 // strength and modularity reductions.
-constexpr int kChunkEdges = 2048;    // max edges per chunk (stage A LDS: 4096 slots)
-constexpr int kChunkCapLog2 = 12;
-constexpr int kMergeCapLog2 = 13;    // stage B LDS: 8192 slots
-constexpr int kMergeLimit = 6144;    // entries a stage-B table may take
+constexpr int kChunkEdges = 1024;    // max edges per chunk (stage A LDS: 2048 slots)
+constexpr int kChunkCapLog2 = 11;
+constexpr int kMergeCapLog2 = 11;    // stage B LDS: 2048 slots
+constexpr int kMergeLimit = 1536;    // pairs a stage-B table may take (else global)
+constexpr int kPartTarget = 512;     // expected pairs per partition
 constexpr int kMaxParts = 1024;
+constexpr int kMergeTile = 1024;     // chunks staged per pass in stage B
 
 struct Chunk {
     int32_t hv;         // index of the heavy vertex
@@ -83,6 +85,8 @@ void launch_degree_keys(const int32_t *rowptr, int32_t V, BucketBounds bb, int8_
 void launch_strength(const int32_t *rowptr, const float *w, const int32_t *order,
                      const SegTable &st, const Chunk *chunks, double *k, hipStream_t s);
 void launch_level_state(int32_t V, const double *k, int32_t *comm, int32_t *size, double *sigma,
+                        uint64_t *crec, hipStream_t s);
+void launch_records_all(int32_t V, const double *sigma, const int32_t *size, uint64_t *crec,
                         hipStream_t s);
 
 // --- moves ---
@@ -96,8 +100,10 @@ void launch_move_chunked(const MoveArgs &a, const ChunkedTier &t, int32_t nchunk
 void launch_clear_tables(int32_t *keys, double *vals, int64_t n, hipStream_t s);
 
 // --- apply ---
-void launch_apply(const int32_t *list, int32_t n, const int32_t *newc, int32_t *comm,
-                  int32_t *size, double *sigma, const double *k, unsigned long long *moved,
+// Applies the decisions of a group (newc[v] then holds the community v
+// left) and refreshes the records of the communities touched.
+void launch_apply(const int32_t *list, int32_t n, int32_t *newc, int32_t *comm, int32_t *size,
+                  double *sigma, const double *k, uint64_t *crec, unsigned long long *moved,
                   hipStream_t s);
 
 // --- modularity ---

@@ -81,13 +81,10 @@ __global__ __launch_bounds__(256) void k_move_sub(MoveArgs a, const int32_t *__r
 
     uint64_t best = pack_best(0.0f, cur);
     if (owner && c != cur) {
-        const int32_t size_cur = a.size[cur];
-        if (may_move(c, cur, size_cur, a.size)) {
-            const float g = gain_of((float)K, a.M32, a.den, (float)a.k[v], (float)a.sigma[cur],
-                                    (float)a.sigma[c]);
-            const uint64_t key = pack_best(g, c);
-            best = key > best ? key : best;
-        }
+        const uint64_t rcur = a.crec[cur];
+        const uint64_t rc = a.crec[c];
+        consider(best, c, K, rc, cur, rec_size(rcur), (float)a.k[v], rec_sigma(rcur), a.M32,
+                 a.den);
     }
 #pragma unroll
     for (int off = X / 2; off > 0; off >>= 1) {
@@ -101,6 +98,8 @@ template <int C>
 __global__ __launch_bounds__(256) void k_move_wave(MoveArgs a, const int32_t *__restrict__ list,
                                                    int32_t n) {
     constexpr int LOG2C = ilog2_c(C);
+    constexpr int MAXE = C == 256 ? 2 : 5;  // edges per lane: deg <= 84 / <= 319
+    constexpr int S = C / 64;               // slots per lane
     __shared__ int32_t keys_s[4 * C];
     __shared__ double vals_s[4 * C];
     const int lane = threadIdx.x & 63, wid = threadIdx.x >> 6;
@@ -108,9 +107,10 @@ __global__ __launch_bounds__(256) void k_move_wave(MoveArgs a, const int32_t *__
     const bool active = slot < n;
     int32_t *K = keys_s + wid * C;
     double *W = vals_s + wid * C;
-    for (int i = lane; i < C; i += 64) {
-        K[i] = -1;
-        W[i] = 0.0;
+#pragma unroll
+    for (int j = 0; j < S; ++j) {
+        K[lane + 64 * j] = -1;
+        W[lane + 64 * j] = 0.0;
     }
     __syncthreads();
     int32_t v = 0, cur = 0;
@@ -118,32 +118,47 @@ __global__ __launch_bounds__(256) void k_move_wave(MoveArgs a, const int32_t *__
         v = list[slot];
         const int32_t rs = a.rowptr[v], re = a.rowptr[v + 1];
         cur = a.comm[v];
-        for (int32_t e = rs + lane; e < re; e += 64) {
-            const int32_t u = a.col[e];
-            const float wf = a.w[e];
-            if (u == v) continue;
-            lds_insert(K, W, 32 - LOG2C, C - 1, a.comm[u], (double)wf);
+        int32_t u[MAXE];
+        float wv[MAXE];
+#pragma unroll
+        for (int j = 0; j < MAXE; ++j) {
+            const int32_t e = rs + lane + 64 * j;
+            u[j] = e < re ? a.col[e] : -1;
+            wv[j] = e < re ? a.w[e] : 0.0f;
         }
+        int32_t cc[MAXE];
+#pragma unroll
+        for (int j = 0; j < MAXE; ++j) cc[j] = (u[j] >= 0 && u[j] != v) ? a.comm[u[j]] : -1;
+#pragma unroll
+        for (int j = 0; j < MAXE; ++j)
+            if (cc[j] >= 0) lds_insert(K, W, 32 - LOG2C, C - 1, cc[j], (double)wv[j]);
     }
     __syncthreads();
     if (!active) return;  // only wave-level communication below
     const float k32 = (float)a.k[v];
-    const float scur = (float)a.sigma[cur];
-    const int32_t size_cur = a.size[cur];
+    const uint64_t rcur = a.crec[cur];
+    const float scur = rec_sigma(rcur);
+    const int32_t size_cur = rec_size(rcur);
+    int32_t kk[S];
+    double vv[S];
+    uint64_t rr[S];
+#pragma unroll
+    for (int j = 0; j < S; ++j) {
+        kk[j] = K[lane + 64 * j];
+        vv[j] = W[lane + 64 * j];
+    }
+#pragma unroll
+    for (int j = 0; j < S; ++j) rr[j] = (kk[j] >= 0 && kk[j] != cur) ? a.crec[kk[j]] : 0ull;
     uint64_t best = pack_best(0.0f, cur);
     double kc = 0.0;
-    for (int i = lane; i < C; i += 64) {
-        const int32_t c = K[i];
-        if (c < 0) continue;
-        const double Kc = W[i];
-        if (c == cur) {
-            kc = Kc;
+#pragma unroll
+    for (int j = 0; j < S; ++j) {
+        if (kk[j] < 0) continue;
+        if (kk[j] == cur) {
+            kc = vv[j];
             continue;
         }
-        if (!may_move(c, cur, size_cur, a.size)) continue;
-        const float g = gain_of((float)Kc, a.M32, a.den, k32, scur, (float)a.sigma[c]);
-        const uint64_t key = pack_best(g, c);
-        best = key > best ? key : best;
+        consider(best, kk[j], vv[j], rr[j], cur, size_cur, k32, scur, a.M32, a.den);
     }
     best = wave_max_u64(best);
     kc = wave_sum_f64(kc);
@@ -170,30 +185,50 @@ __global__ __launch_bounds__(T) void k_move_wg(MoveArgs a, const int32_t *__rest
         const int32_t v = list[vi];
         const int32_t rs = a.rowptr[v], re = a.rowptr[v + 1];
         const int32_t cur = a.comm[v];
-        for (int32_t e = rs + threadIdx.x; e < re; e += T) {
-            const int32_t u = a.col[e];
-            const float wf = a.w[e];
-            if (u == v) continue;
-            lds_insert(K, W, shift, mask, a.comm[u], (double)wf);
+        for (int32_t e0 = rs + threadIdx.x; e0 < re; e0 += 4 * T) {
+            int32_t u[4];
+            float wv[4];
+#pragma unroll
+            for (int j = 0; j < 4; ++j) {
+                const int32_t e = e0 + j * T;
+                u[j] = e < re ? a.col[e] : -1;
+                wv[j] = e < re ? a.w[e] : 0.0f;
+            }
+            int32_t cc[4];
+#pragma unroll
+            for (int j = 0; j < 4; ++j) cc[j] = (u[j] >= 0 && u[j] != v) ? a.comm[u[j]] : -1;
+#pragma unroll
+            for (int j = 0; j < 4; ++j)
+                if (cc[j] >= 0) lds_insert(K, W, shift, mask, cc[j], (double)wv[j]);
         }
         __syncthreads();
         const float k32 = (float)a.k[v];
-        const float scur = (float)a.sigma[cur];
-        const int32_t size_cur = a.size[cur];
+        const uint64_t rcur = a.crec[cur];
+        const float scur = rec_sigma(rcur);
+        const int32_t size_cur = rec_size(rcur);
         uint64_t best = pack_best(0.0f, cur);
         double kc = 0.0;
-        for (int i = threadIdx.x; i < C; i += T) {
-            const int32_t c = K[i];
-            if (c < 0) continue;
-            const double Kc = W[i];
-            if (c == cur) {
-                kc = Kc;
-                continue;
+        for (int i0 = threadIdx.x; i0 < C; i0 += 4 * T) {
+            int32_t kk[4];
+            double vv[4];
+            uint64_t rr[4];
+#pragma unroll
+            for (int j = 0; j < 4; ++j) {
+                const int i = i0 + j * T;
+                kk[j] = i < C ? K[i] : -1;
+                vv[j] = i < C ? W[i] : 0.0;
             }
-            if (!may_move(c, cur, size_cur, a.size)) continue;
-            const float g = gain_of((float)Kc, a.M32, a.den, k32, scur, (float)a.sigma[c]);
-            const uint64_t key = pack_best(g, c);
-            best = key > best ? key : best;
+#pragma unroll
+            for (int j = 0; j < 4; ++j) rr[j] = (kk[j] >= 0 && kk[j] != cur) ? a.crec[kk[j]] : 0ull;
+#pragma unroll
+            for (int j = 0; j < 4; ++j) {
+                if (kk[j] < 0) continue;
+                if (kk[j] == cur) {
+                    kc = vv[j];
+                    continue;
+                }
+                consider(best, kk[j], vv[j], rr[j], cur, size_cur, k32, scur, a.M32, a.den);
+            }
         }
         best = wave_max_u64(best);
         kc = wave_sum_f64(kc);
@@ -277,15 +312,73 @@ __global__ __launch_bounds__(256) void k_chunk_aggregate(MoveArgs a, ChunkedTier
 }
 
 // Stage B: merge partition p of a heavy vertex and evaluate its candidates.
-// A partition holding more pairs than an LDS table takes falls back to the
+// The partition's segments (one per chunk) are staged in LDS with an
+// exclusive scan, so lanes walk one flat range whatever the chunk count.
+// A partition holding more pairs than the LDS table takes falls back to the
 // vertex's global table (always large enough), so every insert terminates.
-constexpr int kMergeThreads = 512;
+constexpr int kMergeThreads = 256;
+
+// Visits the pairs of partition p of vertex h: f(key, value).
+template <typename F>
+__device__ __forceinline__ void for_partition_pairs(const ChunkedTier &t, const HeavyVertex &h,
+                                                    int p, int64_t *seg_base, int32_t *seg_off,
+                                                    F &&f) {
+    __shared__ int32_t wsum[kMergeThreads / 64];
+    const int lane = threadIdx.x & 63, wid = threadIdx.x >> 6;
+    for (int32_t ct0 = 0; ct0 < h.nchunks; ct0 += kMergeTile) {
+        const int32_t nct = h.nchunks - ct0 < kMergeTile ? h.nchunks - ct0 : kMergeTile;
+        // stage segment bases and lengths; block exclusive scan of lengths
+        int32_t carry = 0;
+        for (int32_t b0 = 0; b0 < nct; b0 += kMergeThreads) {
+            const int32_t i = b0 + threadIdx.x;
+            int32_t len = 0;
+            if (i < nct) {
+                const Chunk ch = t.chunks[h.chunk0 + ct0 + i];
+                const int32_t lo = t.poff[ch.pbase + p];
+                len = t.poff[ch.pbase + p + 1] - lo;
+                seg_base[i] = ch.out + lo;
+            }
+            int32_t inc = len;
+#pragma unroll
+            for (int off = 1; off < 64; off <<= 1) {
+                const int32_t y = __shfl_up(inc, off, 64);
+                if (lane >= off) inc += y;
+            }
+            if (lane == 63) wsum[wid] = inc;
+            __syncthreads();
+            int32_t woff = 0, tot = 0;
+#pragma unroll
+            for (int w2 = 0; w2 < kMergeThreads / 64; ++w2) {
+                woff += w2 < wid ? wsum[w2] : 0;
+                tot += wsum[w2];
+            }
+            if (i < nct) seg_off[i] = carry + woff + inc - len;
+            carry += tot;
+            __syncthreads();
+        }
+        if (threadIdx.x == 0) seg_off[nct] = carry;
+        __syncthreads();
+        const int32_t total = carry;
+        for (int32_t f0 = threadIdx.x; f0 < total; f0 += kMergeThreads) {
+            int32_t a = 0, b = nct - 1;  // last segment with seg_off <= f0
+            while (a < b) {
+                const int32_t mid = (a + b + 1) >> 1;
+                if (seg_off[mid] <= f0) a = mid;
+                else b = mid - 1;
+            }
+            const int64_t idx = seg_base[a] + (f0 - seg_off[a]);
+            f(t.out_key[idx], t.out_val[idx]);
+        }
+        __syncthreads();
+    }
+}
 
 __global__ __launch_bounds__(kMergeThreads) void k_chunk_merge(MoveArgs a, ChunkedTier t) {
     constexpr int C = 1 << kMergeCapLog2;
-    extern __shared__ double merge_lds[];
-    double *vals = merge_lds;
-    int32_t *keys = reinterpret_cast<int32_t *>(merge_lds + C);
+    __shared__ int32_t keys[C];
+    __shared__ double vals[C];
+    __shared__ int64_t seg_base[kMergeTile];
+    __shared__ int32_t seg_off[kMergeTile + 1];
     __shared__ uint64_t red_best[kMergeThreads / 64];
     __shared__ double red_kc[kMergeThreads / 64];
     const MergeItem it = t.items[blockIdx.x];
@@ -293,8 +386,9 @@ __global__ __launch_bounds__(kMergeThreads) void k_chunk_merge(MoveArgs a, Chunk
     const int32_t v = h.v;
     const int32_t cur = a.comm[v];
     const float k32 = (float)a.k[v];
-    const float scur = (float)a.sigma[cur];
-    const int32_t size_cur = a.size[cur];
+    const uint64_t rcur = a.crec[cur];
+    const float scur = rec_sigma(rcur);
+    const int32_t size_cur = rec_size(rcur);
     const int32_t n = t.ptot[h.ptot_base + it.p];
     uint64_t best = 0;
     double kc = 0.0;
@@ -304,38 +398,38 @@ __global__ __launch_bounds__(kMergeThreads) void k_chunk_merge(MoveArgs a, Chunk
             vals[i] = 0.0;
         }
         __syncthreads();
-        for (int32_t c = h.chunk0; c < h.chunk0 + h.nchunks; ++c) {
-            const Chunk ch = t.chunks[c];
-            const int32_t lo = t.poff[ch.pbase + it.p], hi = t.poff[ch.pbase + it.p + 1];
-            for (int32_t i = lo + threadIdx.x; i < hi; i += kMergeThreads)
-                lds_insert(keys, vals, 32 - kMergeCapLog2, C - 1, t.out_key[ch.out + i],
-                           t.out_val[ch.out + i]);
-        }
-        __syncthreads();
-        for (int i = threadIdx.x; i < C; i += kMergeThreads) {
-            const int32_t c = keys[i];
-            if (c < 0) continue;
-            const double Kc = vals[i];
-            if (c == cur) {
-                kc = Kc;
-                continue;
+        for_partition_pairs(t, h, it.p, seg_base, seg_off, [&](int32_t c, double x) {
+            lds_insert(keys, vals, 32 - kMergeCapLog2, C - 1, c, x);
+        });
+        for (int i0 = threadIdx.x; i0 < C; i0 += 4 * kMergeThreads) {
+            int32_t kk[4];
+            double vv[4];
+            uint64_t rr[4];
+#pragma unroll
+            for (int j = 0; j < 4; ++j) {
+                const int i = i0 + j * kMergeThreads;
+                kk[j] = i < C ? keys[i] : -1;
+                vv[j] = i < C ? vals[i] : 0.0;
             }
-            if (!may_move(c, cur, size_cur, a.size)) continue;
-            const float g = gain_of((float)Kc, a.M32, a.den, k32, scur, (float)a.sigma[c]);
-            const uint64_t key = pack_best(g, c);
-            best = key > best ? key : best;
+#pragma unroll
+            for (int j = 0; j < 4; ++j) rr[j] = (kk[j] >= 0 && kk[j] != cur) ? a.crec[kk[j]] : 0ull;
+#pragma unroll
+            for (int j = 0; j < 4; ++j) {
+                if (kk[j] < 0) continue;
+                if (kk[j] == cur) {
+                    kc = vv[j];
+                    continue;
+                }
+                consider(best, kk[j], vv[j], rr[j], cur, size_cur, k32, scur, a.M32, a.den);
+            }
         }
     } else {
         int32_t *K = t.gkeys + h.tab_off;
         double *W = t.gvals + h.tab_off;
         const uint32_t cap = 1u << h.tab_lg;
-        for (int32_t c = h.chunk0; c < h.chunk0 + h.nchunks; ++c) {
-            const Chunk ch = t.chunks[c];
-            const int32_t lo = t.poff[ch.pbase + it.p], hi = t.poff[ch.pbase + it.p + 1];
-            for (int32_t i = lo + threadIdx.x; i < hi; i += kMergeThreads)
-                global_insert(K, W, 32 - h.tab_lg, cap - 1, t.out_key[ch.out + i],
-                              t.out_val[ch.out + i]);
-        }
+        for_partition_pairs(t, h, it.p, seg_base, seg_off, [&](int32_t c, double x) {
+            global_insert(K, W, 32 - h.tab_lg, cap - 1, c, x);
+        });
         __threadfence();
         __syncthreads();
         for (uint32_t i = threadIdx.x; i < cap; i += kMergeThreads) {
@@ -348,10 +442,7 @@ __global__ __launch_bounds__(kMergeThreads) void k_chunk_merge(MoveArgs a, Chunk
                 kc = Kc;
                 continue;
             }
-            if (!may_move(c, cur, size_cur, a.size)) continue;
-            const float g = gain_of((float)Kc, a.M32, a.den, k32, scur, (float)a.sigma[c]);
-            const uint64_t key = pack_best(g, c);
-            best = key > best ? key : best;
+            consider(best, c, Kc, a.crec[c], cur, size_cur, k32, scur, a.M32, a.den);
         }
     }
     best = wave_max_u64(best);
@@ -402,29 +493,87 @@ __global__ void k_clear_tables(int32_t *keys, double *vals, int64_t n) {
 // ------------------------------------------------------------------ apply
 // updateVertexCommunity (modularity_optimisation.cu:237-249) fused with the
 // community-total update (replaces the full rebuild at :328-330 by exact
-// FP64 increments).
+// FP64 increments).  Many movers of a wave often join the same (hub)
+// community: the lanes sharing the wave leader's key are combined into one
+// atomic per field, a few times, before the rest update one by one.
+__device__ __forceinline__ void wave_peel_update(bool todo, int32_t key, double x, int32_t dn,
+                                                 int32_t *size, double *sigma, int peels) {
+    const int lane = threadIdx.x & 63;
+    for (int it = 0; it < peels; ++it) {
+        const uint64_t pend = __ballot(todo);
+        if (pend == 0) return;
+        const int leader = __ffsll((long long)pend) - 1;
+        const int32_t lk = __shfl(key, leader, 64);
+        const bool mine = todo && key == lk;
+        const uint64_t m = __ballot(mine);
+        const double sx = wave_sum_f64(mine ? x : 0.0);
+        if (lane == leader) {
+            atomicAdd(&sigma[lk], sx);
+            atomicAdd(&size[lk], dn * __popcll(m));
+        }
+        todo = todo && !mine;
+    }
+    if (todo) {
+        atomicAdd(&sigma[key], x);
+        atomicAdd(&size[key], dn);
+    }
+}
+
 __global__ __launch_bounds__(256) void k_apply(const int32_t *__restrict__ list, int32_t n,
-                                               const int32_t *__restrict__ newc, int32_t *comm,
-                                               int32_t *size, double *sigma,
-                                               const double *__restrict__ k,
+                                               int32_t *newc, int32_t *comm, int32_t *size,
+                                               double *sigma, const double *__restrict__ k,
                                                unsigned long long *moved) {
     const int32_t i = blockIdx.x * 256 + threadIdx.x;
-    int mv = 0;
+    bool mv = false;
+    int32_t nc = 0, oc = 0;
+    double kv = 0.0;
     if (i < n) {
         const int32_t v = list[i];
-        const int32_t nc = newc[v], oc = comm[v];
+        nc = newc[v];
+        oc = comm[v];
         if (nc != oc) {
+            mv = true;
+            kv = k[v];
             comm[v] = nc;
-            atomicSub(&size[oc], 1);
-            atomicAdd(&size[nc], 1);
-            const double kv = k[v];
-            atomicAdd(&sigma[oc], -kv);
-            atomicAdd(&sigma[nc], kv);
-            mv = 1;
+            newc[v] = oc;  // the record fix-up finds the community left
         }
     }
+    wave_peel_update(mv, nc, kv, 1, size, sigma, 3);
+    wave_peel_update(mv, oc, -kv, -1, size, sigma, 1);
     const uint64_t m = __ballot(mv);
     if ((threadIdx.x & 63) == 0 && m) atomicAdd(moved, (unsigned long long)__popcll(m));
+}
+
+// Records of the communities a group's movers left or joined.
+__global__ __launch_bounds__(256) void k_fix_records(const int32_t *__restrict__ list, int32_t n,
+                                                     const int32_t *__restrict__ newc,
+                                                     const int32_t *__restrict__ comm,
+                                                     const int32_t *__restrict__ size,
+                                                     const double *__restrict__ sigma,
+                                                     uint64_t *crec) {
+    const int32_t i = blockIdx.x * 256 + threadIdx.x;
+    bool mv = false;
+    int32_t oc = 0, nc = 0;
+    if (i < n) {
+        const int32_t v = list[i];
+        oc = newc[v];
+        nc = comm[v];
+        mv = oc != nc;
+    }
+    // one lane per distinct joined community of the wave leader's key
+    const uint64_t pend = __ballot(mv);
+    if (pend) {
+        const int leader = __ffsll((long long)pend) - 1;
+        const int32_t lk = __shfl(nc, leader, 64);
+        if (mv && (nc != lk || (threadIdx.x & 63) == leader)) crec[nc] = pack_rec(sigma[nc], size[nc]);
+    }
+    if (mv) crec[oc] = pack_rec(sigma[oc], size[oc]);
+}
+
+__global__ void k_records_all(int32_t V, const double *__restrict__ sigma,
+                              const int32_t *__restrict__ size, uint64_t *crec) {
+    const int32_t c = blockIdx.x * blockDim.x + threadIdx.x;
+    if (c < V) crec[c] = pack_rec(sigma[c], size[c]);
 }
 
 // ------------------------------------------------------- row reductions
@@ -508,12 +657,13 @@ __global__ __launch_bounds__(kRowThreads) void k_strength(const int32_t *__restr
 }
 
 __global__ void k_level_state(int32_t V, const double *__restrict__ k, int32_t *comm,
-                              int32_t *size, double *sigma) {
+                              int32_t *size, double *sigma, uint64_t *crec) {
     const int32_t v = blockIdx.x * blockDim.x + threadIdx.x;
     if (v < V) {
         comm[v] = v;
         size[v] = 1;
         sigma[v] = k[v];
+        crec[v] = pack_rec(k[v], 1);
     }
 }
 
@@ -626,10 +776,17 @@ void launch_strength(const int32_t *rowptr, const float *w, const int32_t *order
 }
 
 void launch_level_state(int32_t V, const double *k, int32_t *comm, int32_t *size, double *sigma,
-                        hipStream_t s) {
+                        uint64_t *crec, hipStream_t s) {
     if (V <= 0) return;
     hipLaunchKernelGGL(k_level_state, dim3(blocks_for(V, 256)), dim3(256), 0, s, V, k, comm, size,
-                       sigma);
+                       sigma, crec);
+}
+
+void launch_records_all(int32_t V, const double *sigma, const int32_t *size, uint64_t *crec,
+                        hipStream_t s) {
+    if (V <= 0) return;
+    hipLaunchKernelGGL(k_records_all, dim3(blocks_for(V, 256)), dim3(256), 0, s, V, sigma, size,
+                       crec);
 }
 
 void launch_move_sub(int X, const MoveArgs &a, const int32_t *list, int32_t n, hipStream_t s) {
@@ -665,8 +822,7 @@ void launch_move_chunked(const MoveArgs &a, const ChunkedTier &t, int32_t nchunk
                          int32_t nitems, const int32_t *heavy_list, int32_t nh, hipStream_t s) {
     if (nh <= 0) return;
     hipLaunchKernelGGL(k_chunk_aggregate, dim3(nchunks), dim3(256), 0, s, a, t);
-    hipLaunchKernelGGL(k_chunk_merge, dim3(nitems), dim3(kMergeThreads),
-                       (size_t)12 << kMergeCapLog2, s, a, t);
+    hipLaunchKernelGGL(k_chunk_merge, dim3(nitems), dim3(kMergeThreads), 0, s, a, t);
     hipLaunchKernelGGL(k_heavy_final, dim3(blocks_for(nh, 256)), dim3(256), 0, s, a, heavy_list,
                        nh, t);
 }
@@ -678,12 +834,14 @@ void launch_clear_tables(int32_t *keys, double *vals, int64_t n, hipStream_t s) 
     hipLaunchKernelGGL(k_clear_tables, dim3((unsigned)g), dim3(256), 0, s, keys, vals, n);
 }
 
-void launch_apply(const int32_t *list, int32_t n, const int32_t *newc, int32_t *comm,
-                  int32_t *size, double *sigma, const double *k, unsigned long long *moved,
+void launch_apply(const int32_t *list, int32_t n, int32_t *newc, int32_t *comm, int32_t *size,
+                  double *sigma, const double *k, uint64_t *crec, unsigned long long *moved,
                   hipStream_t s) {
     if (n <= 0) return;
     hipLaunchKernelGGL(k_apply, dim3(blocks_for(n, 256)), dim3(256), 0, s, list, n, newc, comm,
                        size, sigma, k, moved);
+    hipLaunchKernelGGL(k_fix_records, dim3(blocks_for(n, 256)), dim3(256), 0, s, list, n, newc,
+                       comm, size, sigma, crec);
 }
 
 void launch_modularity(const int32_t *rowptr, const int32_t *col, const float *w,
@@ -705,8 +863,6 @@ void launch_compose(int32_t *orig2comm, int32_t V0, const int32_t *map, hipStrea
 
 void set_wg_lds_limits(int bytes) {
     (void)hipFuncSetAttribute(reinterpret_cast<const void *>(&k_move_wg<256>),
-                              hipFuncAttributeMaxDynamicSharedMemorySize, bytes);
-    (void)hipFuncSetAttribute(reinterpret_cast<const void *>(&k_chunk_merge),
                               hipFuncAttributeMaxDynamicSharedMemorySize, bytes);
 }
 
