@@ -43,7 +43,8 @@ struct MoveArgs {
     const double *k;
     const double *sigma;
     const uint64_t *crec;  // packed community record, see pack_rec
-    int32_t *newc;
+    int32_t *newc;         // decision
+    double *dk;            // mover: K[new] - K[cur] (weight change with neighbours fixed)
     float M32;  // (float)M
     float den;  // (2.0f * M32) * M32, fp32 as the reference evaluates it
 };

@@ -194,6 +194,13 @@ struct lv_graph {
     double *k = nullptr, *sigma = nullptr;
     int32_t *comm = nullptr, *size = nullptr, *newc = nullptr, *orig2comm = nullptr;
     uint64_t *crec = nullptr;
+    double *dk = nullptr;        // per mover: change of weight to own community
+    uint32_t *bits = nullptr;    // two mover bitmaps (alternating group steps)
+    int32_t bit_words = 0;
+    uint64_t step = 0;           // group steps applied with marks
+    bool own_valid = false;      // d_q[1] holds the weight inside communities
+    int64_t last_moved = -1;     // movers of the previous sweep (-1: unknown)
+    int32_t incr_den = 8;        // incremental modularity when movers <= V/incr_den
 
     // bucketing
     lv::BucketBounds bb{};
@@ -216,6 +223,8 @@ struct lv_graph {
     DevVec<int64_t> hrows;
     DevVec<int32_t> out_key, poff, ptot;
     DevVec<double> out_val;
+    DevVec<uint64_t> item_best;
+    DevVec<double> item_K;
     DevVec<uint64_t> hbest;
     DevVec<double> hkcur;
     DevVec<int32_t> gkeys;  // global hash arena, kept all-empty between uses
@@ -223,6 +232,7 @@ struct lv_graph {
 
     // row-reduction segments
     lv::SegTable st{};
+    lv::SegTable gst[lv::kNumGroups]{};
     DevVec<double> qpart;
     double *d_q = nullptr;
     unsigned long long *d_moved = nullptr;
@@ -321,9 +331,27 @@ struct lv_graph {
         a.sigma = sigma;
         a.crec = crec;
         a.newc = newc;
+        a.dk = dk;
         a.M32 = M32;
         a.den = den;
         return a;
+    }
+
+    // The state changed other than by a sweep: the tracked weight inside
+    // communities and the mover bitmaps start over.
+    void state_changed() {
+        lv::fill_i32(reinterpret_cast<int32_t *>(bits), 0, 2 * (int64_t)bit_words, stream);
+        own_valid = false;
+        last_moved = -1;
+    }
+
+    lv::GroupMarks marks() const {
+        lv::GroupMarks gm;
+        const int cur = (int)(step & 1);
+        gm.movers = bits + (int64_t)cur * bit_words;
+        gm.clear_movers = bits + (int64_t)(cur ^ 1) * bit_words;
+        gm.words = bit_words;
+        return gm;
     }
 
     lv::ChunkedTier chunked_tier() {
@@ -333,6 +361,8 @@ struct lv_graph {
         t.items = items.p;
         t.out_key = out_key.p;
         t.out_val = out_val.p;
+        t.item_best = item_best.p;
+        t.item_K = item_K.p;
         t.poff = poff.p;
         t.ptot = ptot.p;
         t.gkeys = gkeys.p;
@@ -395,6 +425,7 @@ struct lv_graph {
         if (opt.chunk_edges > 0) chunk_edges = std::max(16, std::min(opt.chunk_edges, lv::kChunkEdges));
         if (opt.part_target > 0) part_target = std::max(16, opt.part_target);
         if (opt.merge_limit > 0) merge_limit = std::min(opt.merge_limit, lv::kMergeLimit);
+        if (opt.q_incremental != 0) incr_den = opt.q_incremental;
 
         // M = (sum of CSR weights)/2 in FP64, CSR order (utils.cu:36-46).
         double s = 0.0;
@@ -423,6 +454,9 @@ struct lv_graph {
         LV_TRY(dev_alloc(&newc, Vp));
         LV_TRY(dev_alloc(&orig2comm, Vp));
         LV_TRY(dev_alloc(&crec, Vp));
+        LV_TRY(dev_alloc(&dk, Vp));
+        bit_words = (int32_t)((Vp + 31) / 32);
+        LV_TRY(dev_alloc(&bits, 2 * (int64_t)bit_words));
         LV_TRY(dev_alloc(&keys8, Vp));
         LV_TRY(dev_alloc(&order, Vp));
         LV_TRY(dev_alloc(&d_off, 64));
@@ -473,6 +507,7 @@ struct lv_graph {
         lv::iota_i32(orig2comm, V0, stream);
         if (cur_buf < 0 && level_ready) {
             lv::launch_level_state(V, k, comm, size, sigma, crec, stream);
+            state_changed();
             LV_LAUNCHED("reset");
             return LV_OK;
         }
@@ -570,6 +605,8 @@ struct lv_graph {
             LV_TRY(items.reserve(nitems));
             LV_TRY(out_key.reserve(outs));
             LV_TRY(out_val.reserve(outs));
+            LV_TRY(item_best.reserve(nitems));
+            LV_TRY(item_K.reserve(nitems));
             LV_TRY(poff.reserve(pbase));
             LV_TRY(ptot.reserve(tbase));
             LV_TRY(hbest.reserve(nh));
@@ -612,11 +649,40 @@ struct lv_graph {
         blk += (int32_t)((V + lv::kSqItems - 1) / lv::kSqItems);
         st.total_blocks = std::max(blk, 1);
         LV_TRY(qpart.reserve(2 * (int64_t)st.total_blocks + 2 * 256));
+        // per-group row segments for the modularity correction of the applies
+        for (int g = 0; g < lv::kNumGroups; ++g) {
+            lv::SegTable &t = gst[g];
+            t = lv::SegTable{};
+            int32_t gb = 0;
+            for (int b = 0; b < HB; ++b) {
+                if (lv::kBucketGroup[b] != g) continue;
+                const int32_t cnt = h_off[b + 1] - h_off[b];
+                if (cnt <= 0) continue;
+                lv::Seg sg;
+                sg.list_off = h_off[b];
+                sg.count = cnt;
+                sg.block_start = gb;
+                sg.lanes_log2 = lanes_lg[b];
+                const int per = lanes_lg[b] >= 8 ? 1 : (lv::kRowThreads >> lanes_lg[b]);
+                gb += (cnt + per - 1) / per;
+                t.s[t.nseg++] = sg;
+            }
+            if (t.nseg == 0) {
+                t.s[0] = lv::Seg{0, 0, 0, 2};
+                t.nseg = 1;
+            }
+            t.chunk_block_start = gb;
+            t.nchunks = lv::kBucketGroup[HB] == g ? nins : 0;
+            gb += t.nchunks;
+            t.sq_block_start = gb;
+            t.total_blocks = gb;
+        }
 
         // strengths and singleton state (utils.cu:84-89; computeEdgesSum)
         lv::fill_f64(k, 0.0, V, s);
         lv::launch_strength(rp, w(), order, st, ins.p, k, s);
         lv::launch_level_state(V, k, comm, size, sigma, crec, s);
+        state_changed();
         LV_LAUNCHED("level state");
         level_ready = true;
         return LV_OK;
@@ -676,6 +742,7 @@ struct lv_graph {
                               stream);
         tend(t0, LV_K_MODULARITY, 12.0 * (double)E + 16.0 * (double)V);
         LV_LAUNCHED("modularity");
+        own_valid = true;
         return LV_OK;
     }
 
@@ -689,24 +756,40 @@ struct lv_graph {
         return LV_OK;
     }
 
-    // One sweep: groups 0..6, each = moves then apply; then Q.
+    // One sweep: groups 0..6, each = moves then apply; then Q.  After a sweep
+    // that moved few vertices the applies carry the weight inside
+    // communities forward (correction over the movers' rows) and Q needs
+    // only the sum of S_c^2; otherwise a full modularity pass follows.
     lv_status sweep(double *q_after, int64_t *moved, double *sweep_ms) {
         if (!level_ready) LV_TRY(setup_level());
         hipStream_t s = stream;
+        const bool incremental = incr_den > 0 && own_valid && last_moved >= 0 &&
+                                 last_moved * (int64_t)incr_den <= (int64_t)V;
         hipEvent_t e0 = take_event(), e1 = take_event();
         if (e0) (void)hipEventRecord(e0, s);
         hipLaunchKernelGGL(k_zero_u64, dim3(1), dim3(1), 0, s, d_moved);
+        const lv::MoveArgs a = move_args();
         for (int g = 0; g < lv::kNumGroups; ++g) {
             int32_t lo, hi;
             group_range(g, lo, hi);
             if (lo < 0 || hi <= lo) continue;
             LV_TRY(run_group(g));
             hipEvent_t t0 = tbegin();
-            lv::launch_apply(order + lo, hi - lo, newc, comm, size, sigma, k, crec, d_moved, s);
+            const lv::GroupMarks gm = marks();
+            lv::launch_apply(a, order, lo, hi, &gst[g], ins.p, comm, size, sigma, crec,
+                             incremental ? &gm : nullptr, d_moved, d_q + 1, s);
             tend(t0, LV_K_APPLY, 16.0 * (double)(hi - lo));
             LV_LAUNCHED("apply");
+            if (incremental) step++;
         }
-        LV_TRY(enqueue_modularity());
+        if (incremental) {
+            hipEvent_t tq = tbegin();
+            lv::launch_modularity_from_own(sigma, V, M, qpart.p, d_q, s);
+            tend(tq, LV_K_MODULARITY, 8.0 * (double)V);
+            LV_LAUNCHED("modularity");
+        } else {
+            LV_TRY(enqueue_modularity());
+        }
         if (e1) (void)hipEventRecord(e1, s);
         LV_TRY(stage(64));
         LV_HIP(hipMemcpyAsync(h_stage, d_q, sizeof(double), hipMemcpyDeviceToHost, s));
@@ -717,6 +800,7 @@ struct lv_graph {
         unsigned long long mv = 0;
         std::memcpy(&mv, (char *)h_stage + 8, sizeof mv);
         *moved = (int64_t)mv;
+        last_moved = (int64_t)mv;
         if (sweep_ms) {
             float ms = 0.0f;
             if (e0 && e1) (void)hipEventElapsedTime(&ms, e0, e1);
@@ -898,6 +982,8 @@ struct lv_graph {
         dev_free(newc);
         dev_free(orig2comm);
         dev_free(crec);
+        dev_free(dk);
+        dev_free(bits);
         dev_free(keys8);
         dev_free(order);
         dev_free(d_off);
@@ -930,6 +1016,8 @@ struct lv_graph {
         hkcur.release();
         gkeys.release();
         gvals.release();
+        item_best.release();
+        item_K.release();
         qpart.release();
         a_ins.release();
         a_ext.release();
@@ -1137,6 +1225,7 @@ lv_status lv_set_communities(lv_graph *g, const int32_t *comm) {
         hipLaunchKernelGGL(k_state_from_comm, dim3(blocks_for(g->V, 256)), dim3(256), 0, g->stream,
                            g->V, g->comm, g->k, g->size, g->sigma);
         lv::launch_records_all(g->V, g->sigma, g->size, g->crec, g->stream);
+        g->state_changed();
         LV_LAUNCHED("state from communities");
     }
     return g->sync();

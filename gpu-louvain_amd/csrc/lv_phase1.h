@@ -34,7 +34,7 @@ struct HeavyVertex {
     int32_t v;
     int32_t P;
     int32_t chunk0, nchunks;
-    int32_t ptot_base;
+    int32_t ptot_base;  // also the index of the vertex's first merge item
     int32_t tab_lg;     // overflow global table: 2^tab_lg slots at tab_off
     int64_t tab_off;
 };
@@ -56,6 +56,8 @@ struct ChunkedTier {
     double *gvals;
     uint64_t *best;     // per vertex packed argmax, 0 = none
     double *kcur;       // per vertex weight to its own community
+    uint64_t *item_best;  // per merge item: its packed argmax
+    double *item_K;       // ... and the weight to that candidate
     int32_t merge_limit;
 };
 
@@ -99,19 +101,36 @@ void launch_move_chunked(const MoveArgs &a, const ChunkedTier &t, int32_t nchunk
                          int32_t nitems, const int32_t *heavy_list, int32_t nh, hipStream_t s);
 void launch_clear_tables(int32_t *keys, double *vals, int64_t n, hipStream_t s);
 
+// Bitmaps of the movers of a group step (two, alternating by step, so that
+// one is cleared while the other is read).
+struct GroupMarks {
+    uint32_t *movers;        // this step's bitmap over vertices
+    uint32_t *clear_movers;  // the other step's, cleared by this step
+    int32_t words;
+};
+
 // --- apply ---
-// Applies the decisions of a group (newc[v] then holds the community v
-// left) and refreshes the records of the communities touched.
-void launch_apply(const int32_t *list, int32_t n, int32_t *newc, int32_t *comm, int32_t *size,
-                  double *sigma, const double *k, uint64_t *crec, unsigned long long *moved,
-                  hipStream_t s);
+// Applies the decisions of the group's vertices order[lo..hi) (newc[v] then
+// holds the community v left) and refreshes the records of the touched
+// communities.  With gm (incremental modularity) the movers are marked and
+// the group's change of the weight inside communities is added to *own
+// (gst: the group's row segments).
+void launch_apply(const MoveArgs &a, const int32_t *order, int32_t lo, int32_t hi,
+                  const SegTable *gst, const Chunk *chunks, int32_t *comm, int32_t *size,
+                  double *sigma, uint64_t *crec, const GroupMarks *gm,
+                  unsigned long long *moved, double *own, hipStream_t s);
 
 // --- modularity ---
-// Writes per-block partials (own, sq) and then q[0] = Q, q[1] = own, q[2] = sq.
+// Full pass: per-block partials (own, sq), then q[0] = Q, q[1] = own, q[2] = sq.
 void launch_modularity(const int32_t *rowptr, const int32_t *col, const float *w,
                        const int32_t *comm, const double *sigma, int32_t V, const int32_t *order,
                        const SegTable &st, const Chunk *chunks, double M, double *partials,
                        double *q, hipStream_t s);
+
+// Incremental: one pass for sum of S_c^2; q[0] = Q from q[1] (the weight
+// inside communities, kept current by the applies), q[2] = sq.
+void launch_modularity_from_own(const double *sigma, int32_t V, double M, double *partials,
+                                double *q, hipStream_t s);
 
 // --- original-vertex map ---
 void launch_compose(int32_t *orig2comm, int32_t V0, const int32_t *map, hipStream_t s);

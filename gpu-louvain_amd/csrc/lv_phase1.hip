@@ -41,6 +41,37 @@ __device__ __forceinline__ double block_sum_f64(double x, double *lds) {
     return t;
 }
 
+// Candidate evaluation that keeps the weight of the best candidate with it.
+__device__ __forceinline__ void consider_k(uint64_t &best, double &bestK, int32_t c, double Kc,
+                                           uint64_t rc, int32_t cur, int32_t size_cur,
+                                           float k32, float scur, float M32, float den) {
+    const uint64_t before = best;
+    consider(best, c, Kc, rc, cur, size_cur, k32, scur, M32, den);
+    if (best != before) bestK = Kc;
+}
+
+// Max of packed keys over WIDTH lanes (xor butterfly), carrying the weight.
+template <int WIDTH>
+__device__ __forceinline__ void lanes_max_pair(uint64_t &best, double &bestK) {
+#pragma unroll
+    for (int off = WIDTH / 2; off > 0; off >>= 1) {
+        const uint64_t ob = __shfl_xor(best, off, WIDTH);
+        const double oK = __shfl_xor(bestK, off, WIDTH);
+        if (ob > best) {
+            best = ob;
+            bestK = oK;
+        }
+    }
+}
+
+// Decision plus dk = K[new] - K[cur] for a mover (0 otherwise).
+__device__ __forceinline__ void write_decision(const MoveArgs &a, int32_t v, int32_t cur,
+                                               uint64_t best, double bestK, double kc) {
+    const int32_t nc = decide_from(best, kc, a.M32, cur);
+    a.newc[v] = nc;
+    a.dk[v] = nc != cur ? bestK - kc : 0.0;
+}
+
 // ------------------------------------------------------------------ moves
 
 template <int X>
@@ -80,18 +111,15 @@ __global__ __launch_bounds__(256) void k_move_sub(MoveArgs a, const int32_t *__r
     for (int off = X / 2; off > 0; off >>= 1) kc += __shfl_xor(kc, off, X);
 
     uint64_t best = pack_best(0.0f, cur);
+    double bestK = 0.0;
     if (owner && c != cur) {
         const uint64_t rcur = a.crec[cur];
         const uint64_t rc = a.crec[c];
-        consider(best, c, K, rc, cur, rec_size(rcur), (float)a.k[v], rec_sigma(rcur), a.M32,
-                 a.den);
+        consider_k(best, bestK, c, K, rc, cur, rec_size(rcur), (float)a.k[v], rec_sigma(rcur),
+                   a.M32, a.den);
     }
-#pragma unroll
-    for (int off = X / 2; off > 0; off >>= 1) {
-        const uint64_t o = __shfl_xor(best, off, X);
-        best = o > best ? o : best;
-    }
-    if (lane == 0) a.newc[v] = decide_from(best, kc, a.M32, cur);
+    lanes_max_pair<X>(best, bestK);
+    if (lane == 0) write_decision(a, v, cur, best, bestK, kc);
 }
 
 template <int C>
@@ -150,7 +178,7 @@ __global__ __launch_bounds__(256) void k_move_wave(MoveArgs a, const int32_t *__
 #pragma unroll
     for (int j = 0; j < S; ++j) rr[j] = (kk[j] >= 0 && kk[j] != cur) ? a.crec[kk[j]] : 0ull;
     uint64_t best = pack_best(0.0f, cur);
-    double kc = 0.0;
+    double bestK = 0.0, kc = 0.0;
 #pragma unroll
     for (int j = 0; j < S; ++j) {
         if (kk[j] < 0) continue;
@@ -158,11 +186,11 @@ __global__ __launch_bounds__(256) void k_move_wave(MoveArgs a, const int32_t *__
             kc = vv[j];
             continue;
         }
-        consider(best, kk[j], vv[j], rr[j], cur, size_cur, k32, scur, a.M32, a.den);
+        consider_k(best, bestK, kk[j], vv[j], rr[j], cur, size_cur, k32, scur, a.M32, a.den);
     }
-    best = wave_max_u64(best);
+    lanes_max_pair<64>(best, bestK);
     kc = wave_sum_f64(kc);
-    if (lane == 0) a.newc[v] = decide_from(best, kc, a.M32, cur);
+    if (lane == 0) write_decision(a, v, cur, best, bestK, kc);
 }
 
 template <int T>
@@ -173,6 +201,7 @@ __global__ __launch_bounds__(T) void k_move_wg(MoveArgs a, const int32_t *__rest
     double *W = dyn_lds;
     int32_t *K = reinterpret_cast<int32_t *>(dyn_lds + C);
     __shared__ uint64_t red_best[T / 64];
+    __shared__ double red_K[T / 64];
     __shared__ double red_kc[T / 64];
     const int lane = threadIdx.x & 63, wid = threadIdx.x >> 6;
     const uint32_t shift = 32 - cap_log2, mask = (uint32_t)C - 1;
@@ -207,7 +236,7 @@ __global__ __launch_bounds__(T) void k_move_wg(MoveArgs a, const int32_t *__rest
         const float scur = rec_sigma(rcur);
         const int32_t size_cur = rec_size(rcur);
         uint64_t best = pack_best(0.0f, cur);
-        double kc = 0.0;
+        double bestK = 0.0, kc = 0.0;
         for (int i0 = threadIdx.x; i0 < C; i0 += 4 * T) {
             int32_t kk[4];
             double vv[4];
@@ -227,24 +256,29 @@ __global__ __launch_bounds__(T) void k_move_wg(MoveArgs a, const int32_t *__rest
                     kc = vv[j];
                     continue;
                 }
-                consider(best, kk[j], vv[j], rr[j], cur, size_cur, k32, scur, a.M32, a.den);
+                consider_k(best, bestK, kk[j], vv[j], rr[j], cur, size_cur, k32, scur, a.M32,
+                           a.den);
             }
         }
-        best = wave_max_u64(best);
+        lanes_max_pair<64>(best, bestK);
         kc = wave_sum_f64(kc);
         if (lane == 0) {
             red_best[wid] = best;
+            red_K[wid] = bestK;
             red_kc[wid] = kc;
         }
         __syncthreads();
         if (threadIdx.x == 0) {
             uint64_t b = red_best[0];
-            double kk = red_kc[0];
-            for (int w = 1; w < T / 64; ++w) {
-                b = red_best[w] > b ? red_best[w] : b;
-                kk += red_kc[w];
+            double bK = red_K[0], kk = red_kc[0];
+            for (int w2 = 1; w2 < T / 64; ++w2) {
+                if (red_best[w2] > b) {
+                    b = red_best[w2];
+                    bK = red_K[w2];
+                }
+                kk += red_kc[w2];
             }
-            a.newc[v] = decide_from(b, kk, a.M32, cur);
+            write_decision(a, v, cur, b, bK, kk);
         }
         __syncthreads();
     }
@@ -380,8 +414,10 @@ __global__ __launch_bounds__(kMergeThreads) void k_chunk_merge(MoveArgs a, Chunk
     __shared__ int64_t seg_base[kMergeTile];
     __shared__ int32_t seg_off[kMergeTile + 1];
     __shared__ uint64_t red_best[kMergeThreads / 64];
+    __shared__ double red_K[kMergeThreads / 64];
     __shared__ double red_kc[kMergeThreads / 64];
-    const MergeItem it = t.items[blockIdx.x];
+    const int32_t item = blockIdx.x;
+    const MergeItem it = t.items[item];
     const HeavyVertex h = t.hvs[it.hv];
     const int32_t v = h.v;
     const int32_t cur = a.comm[v];
@@ -391,7 +427,7 @@ __global__ __launch_bounds__(kMergeThreads) void k_chunk_merge(MoveArgs a, Chunk
     const int32_t size_cur = rec_size(rcur);
     const int32_t n = t.ptot[h.ptot_base + it.p];
     uint64_t best = 0;
-    double kc = 0.0;
+    double bestK = 0.0, kc = 0.0;
     if (n <= t.merge_limit) {
         for (int i = threadIdx.x; i < C; i += kMergeThreads) {
             keys[i] = -1;
@@ -420,7 +456,8 @@ __global__ __launch_bounds__(kMergeThreads) void k_chunk_merge(MoveArgs a, Chunk
                     kc = vv[j];
                     continue;
                 }
-                consider(best, kk[j], vv[j], rr[j], cur, size_cur, k32, scur, a.M32, a.den);
+                consider_k(best, bestK, kk[j], vv[j], rr[j], cur, size_cur, k32, scur, a.M32,
+                           a.den);
             }
         }
     } else {
@@ -442,24 +479,30 @@ __global__ __launch_bounds__(kMergeThreads) void k_chunk_merge(MoveArgs a, Chunk
                 kc = Kc;
                 continue;
             }
-            consider(best, c, Kc, a.crec[c], cur, size_cur, k32, scur, a.M32, a.den);
+            consider_k(best, bestK, c, Kc, a.crec[c], cur, size_cur, k32, scur, a.M32, a.den);
         }
     }
-    best = wave_max_u64(best);
+    lanes_max_pair<64>(best, bestK);
     kc = wave_sum_f64(kc);
     const int lane = threadIdx.x & 63, wid = threadIdx.x >> 6;
     if (lane == 0) {
         red_best[wid] = best;
+        red_K[wid] = bestK;
         red_kc[wid] = kc;
     }
     __syncthreads();
     if (threadIdx.x == 0) {
         uint64_t b = red_best[0];
-        double kk = red_kc[0];
-        for (int w = 1; w < kMergeThreads / 64; ++w) {
-            b = red_best[w] > b ? red_best[w] : b;
-            kk += red_kc[w];
+        double bK = red_K[0], kk = red_kc[0];
+        for (int w2 = 1; w2 < kMergeThreads / 64; ++w2) {
+            if (red_best[w2] > b) {
+                b = red_best[w2];
+                bK = red_K[w2];
+            }
+            kk += red_kc[w2];
         }
+        t.item_best[item] = b;
+        t.item_K[item] = bK;
         if (b != 0)
             atomicMax(reinterpret_cast<unsigned long long *>(&t.best[it.hv]),
                       (unsigned long long)b);
@@ -468,16 +511,19 @@ __global__ __launch_bounds__(kMergeThreads) void k_chunk_merge(MoveArgs a, Chunk
     }
 }
 
-__global__ void k_heavy_final(MoveArgs a, const int32_t *__restrict__ heavy, int32_t nh,
-                              ChunkedTier t) {
+__global__ void k_heavy_final(MoveArgs a, int32_t nh, ChunkedTier t) {
     const int32_t hv = blockIdx.x * blockDim.x + threadIdx.x;
     if (hv >= nh) return;
-    const int32_t v = heavy[hv];
+    const HeavyVertex h = t.hvs[hv];
+    const int32_t v = h.v;
     const int32_t cur = a.comm[v];
     uint64_t best = t.best[hv];
+    double bestK = 0.0;
+    for (int32_t p = 0; p < h.P; ++p)
+        if (best != 0 && t.item_best[h.ptot_base + p] == best) bestK = t.item_K[h.ptot_base + p];
     const uint64_t init = pack_best(0.0f, cur);
-    best = best > init ? best : init;
-    a.newc[v] = decide_from(best, t.kcur[hv], a.M32, cur);
+    if (init > best) best = init;
+    write_decision(a, v, cur, best, bestK, t.kcur[hv]);
     t.best[hv] = 0;
     t.kcur[hv] = 0.0;
 }
@@ -491,11 +537,20 @@ __global__ void k_clear_tables(int32_t *keys, double *vals, int64_t n) {
 }
 
 // ------------------------------------------------------------------ apply
-// updateVertexCommunity (modularity_optimisation.cu:237-249) fused with the
-// community-total update (replaces the full rebuild at :328-330 by exact
-// FP64 increments).  Many movers of a wave often join the same (hub)
-// community: the lanes sharing the wave leader's key are combined into one
-// atomic per field, a few times, before the rest update one by one.
+// updateVertexCommunity (modularity_optimisation.cu:237-249) with the
+// community totals updated by exact FP64 increments (the reference rebuilds
+// them, :328-330).  Many movers of a wave often join the same (hub)
+// community: lanes sharing the wave leader's key are combined into one
+// atomic per field, a few times, before the rest go one by one.  A vertex
+// leaving a singleton community is that community's only writer, so that
+// update is a plain read-modify-write in a second launch, after all joins.
+// A third launch refreshes the records of the touched communities and, when
+// the sweep tracks modularity incrementally, adds the group's change of the
+// weight inside communities:
+//   2*dk[v] per mover v (its own view, neighbours fixed), plus for each
+//   entry (v,u) with u also a mover of the group
+//   w * ([c'u=c'v] - [cu=c'v] - [cv=c'u] + [cu=cv])   (seen from both ends).
+
 __device__ __forceinline__ void wave_peel_update(bool todo, int32_t key, double x, int32_t dn,
                                                  int32_t *size, double *sigma, int peels) {
     const int lane = threadIdx.x & 63;
@@ -519,55 +574,148 @@ __device__ __forceinline__ void wave_peel_update(bool todo, int32_t key, double 
     }
 }
 
-__global__ __launch_bounds__(256) void k_apply(const int32_t *__restrict__ list, int32_t n,
-                                               int32_t *newc, int32_t *comm, int32_t *size,
-                                               double *sigma, const double *__restrict__ k,
-                                               unsigned long long *moved) {
+// Sets bit `idx`; lanes of the wave sharing the leader's word are combined
+// into one atomic OR.
+__device__ __forceinline__ void wave_set_bit(bool todo, int32_t idx, uint32_t *bits) {
+    const int lane = threadIdx.x & 63;
+    const uint64_t pend = __ballot(todo);
+    if (pend == 0) return;
+    const int leader = __ffsll((long long)pend) - 1;
+    const int32_t lw = __shfl(idx >> 5, leader, 64);
+    const bool mine = todo && (idx >> 5) == lw;
+    uint32_t m = mine ? (1u << (idx & 31)) : 0u;
+#pragma unroll
+    for (int off = 32; off > 0; off >>= 1) m |= __shfl_xor(m, off, 64);
+    if (lane == leader) atomicOr(&bits[lw], m);
+    if (todo && !mine) atomicOr(&bits[idx >> 5], 1u << (idx & 31));
+}
+
+__device__ __forceinline__ bool test_bit(const uint32_t *bits, int32_t idx) {
+    return (bits[idx >> 5] >> (idx & 31)) & 1u;
+}
+
+__global__ __launch_bounds__(256) void k_apply_join(const int32_t *__restrict__ list, int32_t n,
+                                                    int32_t *newc, int32_t *comm, int32_t *size,
+                                                    double *sigma, const double *__restrict__ k,
+                                                    const uint64_t *__restrict__ crec,
+                                                    uint32_t *movers, unsigned long long *moved) {
     const int32_t i = blockIdx.x * 256 + threadIdx.x;
-    bool mv = false;
-    int32_t nc = 0, oc = 0;
+    bool mv = false, shared_old = false;
+    int32_t v = 0, nc = 0, oc = 0;
     double kv = 0.0;
     if (i < n) {
-        const int32_t v = list[i];
+        v = list[i];
         nc = newc[v];
         oc = comm[v];
         if (nc != oc) {
             mv = true;
+            shared_old = rec_size(crec[oc]) > 1;
             kv = k[v];
             comm[v] = nc;
-            newc[v] = oc;  // the record fix-up finds the community left
+            newc[v] = oc;  // from here on newc holds the community left
         }
     }
     wave_peel_update(mv, nc, kv, 1, size, sigma, 3);
-    wave_peel_update(mv, oc, -kv, -1, size, sigma, 1);
+    wave_peel_update(mv && shared_old, oc, -kv, -1, size, sigma, 1);
+    if (movers) wave_set_bit(mv, v, movers);
     const uint64_t m = __ballot(mv);
     if ((threadIdx.x & 63) == 0 && m) atomicAdd(moved, (unsigned long long)__popcll(m));
 }
 
-// Records of the communities a group's movers left or joined.
-__global__ __launch_bounds__(256) void k_fix_records(const int32_t *__restrict__ list, int32_t n,
+__global__ __launch_bounds__(256) void k_apply_leave(const int32_t *__restrict__ list, int32_t n,
                                                      const int32_t *__restrict__ newc,
                                                      const int32_t *__restrict__ comm,
-                                                     const int32_t *__restrict__ size,
-                                                     const double *__restrict__ sigma,
-                                                     uint64_t *crec) {
+                                                     int32_t *size, double *sigma,
+                                                     const double *__restrict__ k,
+                                                     const uint64_t *__restrict__ crec) {
     const int32_t i = blockIdx.x * 256 + threadIdx.x;
-    bool mv = false;
-    int32_t oc = 0, nc = 0;
-    if (i < n) {
-        const int32_t v = list[i];
-        oc = newc[v];
-        nc = comm[v];
-        mv = oc != nc;
+    if (i >= n) return;
+    const int32_t v = list[i];
+    const int32_t oc = newc[v];
+    if (oc == comm[v]) return;
+    if (rec_size(crec[oc]) > 1) return;  // shared community: updated by atomics
+    size[oc] -= 1;
+    sigma[oc] -= k[v];
+}
+
+constexpr int kClearWords = 4096;
+
+// Blocks [0, fix_start): modularity correction over the group's movers'
+// rows (incremental sweeps only); [fix_start, clear_start): records of the
+// touched communities; [clear_start, ...): clear the other step's bitmap.
+__global__ __launch_bounds__(kRowThreads) void k_apply_finish(
+    MoveArgs a, const int32_t *__restrict__ order, int32_t lo, int32_t n, SegTable st,
+    const Chunk *__restrict__ chunks, const int32_t *__restrict__ comm,
+    const int32_t *__restrict__ size, const double *__restrict__ sigma, uint64_t *crec,
+    const uint32_t *__restrict__ movers, uint32_t *clear_bits, int32_t words, double *own,
+    int32_t fix_start, int32_t clear_start) {
+    __shared__ double lds[kRowThreads / 64];
+    const int b = blockIdx.x;
+    if (b >= clear_start) {
+        const int64_t w0 = (int64_t)(b - clear_start) * kClearWords;
+        for (int64_t x = w0 + threadIdx.x; x < w0 + kClearWords && x < words; x += kRowThreads)
+            clear_bits[x] = 0u;
+        return;
     }
-    // one lane per distinct joined community of the wave leader's key
-    const uint64_t pend = __ballot(mv);
-    if (pend) {
-        const int leader = __ffsll((long long)pend) - 1;
-        const int32_t lk = __shfl(nc, leader, 64);
-        if (mv && (nc != lk || (threadIdx.x & 63) == leader)) crec[nc] = pack_rec(sigma[nc], size[nc]);
+    if (b >= fix_start) {
+        const int32_t i = (b - fix_start) * kRowThreads + threadIdx.x;
+        bool mv = false;
+        int32_t oc = 0, nc = 0;
+        if (i < n) {
+            const int32_t v = order[lo + i];
+            oc = a.newc[v];
+            nc = comm[v];
+            mv = oc != nc;
+        }
+        // one write per distinct joined community of the wave leader's key
+        const uint64_t pend = __ballot(mv);
+        if (pend) {
+            const int leader = __ffsll((long long)pend) - 1;
+            const int32_t lk = __shfl(nc, leader, 64);
+            if (mv && (nc != lk || (int)(threadIdx.x & 63) == leader))
+                crec[nc] = pack_rec(sigma[nc], size[nc]);
+        }
+        if (mv) crec[oc] = pack_rec(sigma[oc], size[oc]);
+        return;
     }
-    if (mv) crec[oc] = pack_rec(sigma[oc], size[oc]);
+    double acc = 0.0;
+    auto row = [&](int32_t v, int64_t elo, int64_t ehi, int stride, bool first) {
+        const int32_t cv_old = a.newc[v], cv_new = comm[v];
+        if (cv_old == cv_new) return;
+        if (first) acc += 2.0 * a.dk[v];
+        for (int64_t e = elo; e < ehi; e += stride) {
+            const int32_t u = a.col[e];
+            if (u == v || !test_bit(movers, u)) continue;
+            const int32_t cu_new = comm[u], cu_old = a.newc[u];
+            const int d = (int)(cu_new == cv_new) - (int)(cu_old == cv_new) -
+                          (int)(cv_old == cu_new) + (int)(cu_old == cv_old);
+            if (d != 0) acc += (double)d * (double)a.w[e];
+        }
+    };
+    if (b >= st.chunk_block_start) {
+        const Chunk ch = chunks[b - st.chunk_block_start];
+        row(ch.v, ch.lo + threadIdx.x, ch.hi, kRowThreads,
+            threadIdx.x == 0 && ch.lo == a.rowptr[ch.v]);
+    } else {
+        int si = 0;
+        while (si + 1 < st.nseg && b >= st.s[si + 1].block_start) ++si;
+        const Seg sg = st.s[si];
+        const int lg = sg.lanes_log2;
+        if (lg <= 6) {
+            const int X = 1 << lg;
+            const int32_t idx = (b - sg.block_start) * (kRowThreads >> lg) + (threadIdx.x >> lg);
+            const int lane = threadIdx.x & (X - 1);
+            if (idx < sg.count) {
+                const int32_t v = order[sg.list_off + idx];
+                row(v, a.rowptr[v] + lane, a.rowptr[v + 1], X, lane == 0);
+            }
+        } else {
+            const int32_t v = order[sg.list_off + (b - sg.block_start)];
+            row(v, a.rowptr[v] + threadIdx.x, a.rowptr[v + 1], kRowThreads, threadIdx.x == 0);
+        }
+    }
+    acc = block_sum_f64(acc, lds);
+    if (threadIdx.x == 0 && acc != 0.0) atomicAdd(own, acc);
 }
 
 __global__ void k_records_all(int32_t V, const double *__restrict__ sigma,
@@ -716,6 +864,27 @@ __global__ __launch_bounds__(kRowThreads) void k_modularity(
     }
 }
 
+// Sum of S_c^2 only (incremental sweeps keep the weight inside communities
+// current on their own).
+__global__ __launch_bounds__(kRowThreads) void k_sum_squares(const double *__restrict__ sigma,
+                                                             int32_t V, double *partials) {
+    __shared__ double lds[kRowThreads / 64];
+    const int64_t base = (int64_t)blockIdx.x * kSqItems;
+    double sq = 0.0;
+    for (int j = threadIdx.x; j < kSqItems; j += kRowThreads) {
+        const int64_t c = base + j;
+        if (c < V) {
+            const double x = sigma[c];
+            sq += x * x;
+        }
+    }
+    sq = block_sum_f64(sq, lds);
+    if (threadIdx.x == 0) {
+        partials[2 * blockIdx.x] = 0.0;
+        partials[2 * blockIdx.x + 1] = sq;
+    }
+}
+
 // Two-level fixed-order reduction of the per-block partials: 256 blocks
 // each add a contiguous range, then one block adds the 256 results.
 constexpr int kQReduceBlocks = 256;
@@ -738,14 +907,16 @@ __global__ __launch_bounds__(256) void k_modularity_reduce(const double *__restr
     }
 }
 
+// use_own: the weight inside communities comes from q[1], not the partials.
 __global__ __launch_bounds__(256) void k_modularity_final(const double *__restrict__ partials2,
-                                                          double M, double *q) {
+                                                          double M, double *q, int use_own) {
     __shared__ double lds[4];
     double own = partials2[2 * threadIdx.x];
     double sq = partials2[2 * threadIdx.x + 1];
     own = block_sum_f64(own, lds);
     sq = block_sum_f64(sq, lds);
     if (threadIdx.x == 0) {
+        if (use_own) own = q[1];
         q[0] = own / (2.0 * M) - sq / (4.0 * M * M);
         q[1] = own;
         q[2] = sq;
@@ -823,8 +994,7 @@ void launch_move_chunked(const MoveArgs &a, const ChunkedTier &t, int32_t nchunk
     if (nh <= 0) return;
     hipLaunchKernelGGL(k_chunk_aggregate, dim3(nchunks), dim3(256), 0, s, a, t);
     hipLaunchKernelGGL(k_chunk_merge, dim3(nitems), dim3(kMergeThreads), 0, s, a, t);
-    hipLaunchKernelGGL(k_heavy_final, dim3(blocks_for(nh, 256)), dim3(256), 0, s, a, heavy_list,
-                       nh, t);
+    hipLaunchKernelGGL(k_heavy_final, dim3(blocks_for(nh, 256)), dim3(256), 0, s, a, nh, t);
 }
 
 void launch_clear_tables(int32_t *keys, double *vals, int64_t n, hipStream_t s) {
@@ -834,14 +1004,25 @@ void launch_clear_tables(int32_t *keys, double *vals, int64_t n, hipStream_t s) 
     hipLaunchKernelGGL(k_clear_tables, dim3((unsigned)g), dim3(256), 0, s, keys, vals, n);
 }
 
-void launch_apply(const int32_t *list, int32_t n, int32_t *newc, int32_t *comm, int32_t *size,
-                  double *sigma, const double *k, uint64_t *crec, unsigned long long *moved,
-                  hipStream_t s) {
+void launch_apply(const MoveArgs &a, const int32_t *order, int32_t lo, int32_t hi,
+                  const SegTable *gst, const Chunk *chunks, int32_t *comm, int32_t *size,
+                  double *sigma, uint64_t *crec, const GroupMarks *gm,
+                  unsigned long long *moved, double *own, hipStream_t s) {
+    const int32_t n = hi - lo;
     if (n <= 0) return;
-    hipLaunchKernelGGL(k_apply, dim3(blocks_for(n, 256)), dim3(256), 0, s, list, n, newc, comm,
-                       size, sigma, k, moved);
-    hipLaunchKernelGGL(k_fix_records, dim3(blocks_for(n, 256)), dim3(256), 0, s, list, n, newc,
-                       comm, size, sigma, crec);
+    const int32_t *list = order + lo;
+    hipLaunchKernelGGL(k_apply_join, dim3(blocks_for(n, 256)), dim3(256), 0, s, list, n, a.newc,
+                       comm, size, sigma, a.k, crec, gm ? gm->movers : nullptr, moved);
+    hipLaunchKernelGGL(k_apply_leave, dim3(blocks_for(n, 256)), dim3(256), 0, s, list, n, a.newc,
+                       comm, size, sigma, a.k, crec);
+    const int32_t corr = gm ? gst->total_blocks : 0;
+    const int32_t fix = (int32_t)blocks_for(n, kRowThreads);
+    const int32_t clear = gm ? (int32_t)blocks_for(gm->words, kClearWords) : 0;
+    const SegTable empty{};
+    hipLaunchKernelGGL(k_apply_finish, dim3(corr + fix + clear), dim3(kRowThreads), 0, s, a, order,
+                       lo, n, gm ? *gst : empty, chunks, comm, size, sigma, crec,
+                       gm ? gm->movers : nullptr, gm ? gm->clear_movers : nullptr,
+                       gm ? gm->words : 0, own, corr, corr + fix);
 }
 
 void launch_modularity(const int32_t *rowptr, const int32_t *col, const float *w,
@@ -853,7 +1034,17 @@ void launch_modularity(const int32_t *rowptr, const int32_t *col, const float *w
     double *partials2 = partials + 2 * (int64_t)st.total_blocks;
     hipLaunchKernelGGL(k_modularity_reduce, dim3(kQReduceBlocks), dim3(256), 0, s, partials,
                        st.total_blocks, partials2);
-    hipLaunchKernelGGL(k_modularity_final, dim3(1), dim3(256), 0, s, partials2, M, q);
+    hipLaunchKernelGGL(k_modularity_final, dim3(1), dim3(256), 0, s, partials2, M, q, 0);
+}
+
+void launch_modularity_from_own(const double *sigma, int32_t V, double M, double *partials,
+                                double *q, hipStream_t s) {
+    const int32_t nb = (int32_t)(V > 0 ? blocks_for(V, kSqItems) : 1);
+    hipLaunchKernelGGL(k_sum_squares, dim3(nb), dim3(kRowThreads), 0, s, sigma, V, partials);
+    double *partials2 = partials + 2 * (int64_t)nb;
+    hipLaunchKernelGGL(k_modularity_reduce, dim3(kQReduceBlocks), dim3(256), 0, s, partials, nb,
+                       partials2);
+    hipLaunchKernelGGL(k_modularity_final, dim3(1), dim3(256), 0, s, partials2, M, q, 1);
 }
 
 void launch_compose(int32_t *orig2comm, int32_t V0, const int32_t *map, hipStream_t s) {

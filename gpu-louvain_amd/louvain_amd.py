@@ -57,7 +57,8 @@ class Options(ctypes.Structure):
         ("agg_lds_max", ctypes.c_int32),
         ("part_target", ctypes.c_int32),
         ("merge_limit", ctypes.c_int32),
-        ("reserved", ctypes.c_int32 * 9),
+        ("q_incremental", ctypes.c_int32),
+        ("reserved", ctypes.c_int32 * 8),
     ]
 
 
@@ -218,7 +219,8 @@ class Graph:
     """A graph resident on one GPU with its community state (lv_graph)."""
 
     def __init__(self, g: CSR, device: int = 0, timing: bool = False, heavy_wg_max: int = 0,
-                 chunk_edges: int = 0, part_target: int = 0, merge_limit: int = 0, agg_lds_max: int = 0):
+                 chunk_edges: int = 0, part_target: int = 0, merge_limit: int = 0, agg_lds_max: int = 0,
+                 q_incremental: int = 0):
         lib = load()
         self._lib = lib
         self._rowptr = np.ascontiguousarray(g.rowptr, dtype=np.int32)
@@ -233,6 +235,7 @@ class Graph:
         opt.part_target = part_target
         opt.merge_limit = merge_limit
         opt.agg_lds_max = agg_lds_max
+        opt.q_incremental = q_incremental
         h = ctypes.c_void_p()
         col = self._col if self._col.size else np.zeros(1, np.int32)
         w = self._w if self._w.size else np.zeros(1, np.float32)

@@ -34,6 +34,11 @@ def _cases(lvlib):
     # force the chunked tier (many chunks and partitions) on a mid-size graph
     out.append(("rmat14_chunked_tier", lvlib.gen_rmat(14, 16, seed=41),
                 {"heavy_wg_max": 320, "chunk_edges": 64, "part_target": 64}))
+    # modularity carried by the applies' corrections after every sweep
+    out.append(("rmat14_incremental_q", lvlib.gen_rmat(14, 16, seed=44),
+                {"q_incremental": 1, "heavy_wg_max": 320, "chunk_edges": 256}))
+    out.append(("planted_incremental_q", lvlib.gen_planted(5000, 25, 0.03, 0.001, seed=45),
+                {"q_incremental": 1}))
     # force the chunked tier's global-table fallback (LDS merge limit 16 pairs)
     out.append(("rmat13_merge_overflow", lvlib.gen_rmat(13, 16, seed=43),
                 {"heavy_wg_max": 320, "chunk_edges": 128, "part_target": 100000, "merge_limit": 16}))
@@ -48,7 +53,7 @@ def cases(lvlib):
 def _names():
     return ["two_triangles", "planted_c1_unit", "rmat12_real", "rmat14_real", "rmat13_unit",
             "random_selfloops_real", "star3000", "star20000", "rmat14_chunked_tier",
-            "rmat13_merge_overflow"]
+            "rmat13_merge_overflow", "rmat14_incremental_q", "planted_incremental_q"]
 
 
 def _get(cases, name):
