@@ -45,6 +45,9 @@ struct MoveArgs {
     const uint64_t *crec;  // packed community record, see pack_rec
     int32_t *newc;         // decision
     double *dk;            // mover: K[new] - K[cur] (weight change with neighbours fixed)
+    float *margin;         // evaluated vertex: bestGain - K[cur]/M (move iff > 0)
+    float *kown;           // evaluated vertex: K[cur] (fp32)
+    const uint8_t *active; // frontier flags (chunked tier; null = every vertex)
     float M32;  // (float)M
     float den;  // (2.0f * M32) * M32, fp32 as the reference evaluates it
 };

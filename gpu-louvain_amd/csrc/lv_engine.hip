@@ -23,6 +23,7 @@
 #include "../../include/louvain.h"
 #include "lv_common.h"
 #include "lv_error.h"
+#include "lv_learn.h"
 #include "lv_phase1.h"
 #include "lv_phase2.h"
 #include "lv_prims.h"
@@ -158,6 +159,9 @@ unsigned blocks_for(int64_t n, int per) { return (unsigned)((n + per - 1) / per)
 
 constexpr int kSlotChunk = 4096;  // hash slots per evaluation chunk
 constexpr int kAggChunk = 4096;   // flat member edges per aggregation chunk
+// A phase with a frontier sweeps only full after this many sweeps (bounds a
+// phase whose pruned and full sweeps keep undoing each other).
+constexpr int64_t kMaxPrunedSweeps = 4096;
 
 int ceil_log2(int64_t x) {
     int lg = 0;
@@ -195,6 +199,17 @@ struct lv_graph {
     int32_t *comm = nullptr, *size = nullptr, *newc = nullptr, *orig2comm = nullptr;
     uint64_t *crec = nullptr;
     double *dk = nullptr;        // per mover: change of weight to own community
+    float *margin = nullptr, *kown = nullptr;  // last evaluation of each vertex
+    uint8_t *hist = nullptr;     // move history (bit 0 = last sweep)
+    uint8_t *active = nullptr;   // frontier flags of a pruned sweep
+    int32_t *nmv = nullptr;      // row entries whose neighbour moved in the last sweep
+    int8_t *fkeys = nullptr;     // frontier bucket keys
+    int32_t *forder = nullptr, *fd_off = nullptr;  // frontier lists by bucket
+    int32_t h_foff[lv::kNumBuckets + 1] = {0};
+    float *mlp = nullptr;        // frontier model parameters
+    bool mlp_set = false;
+    uint64_t sweep_count = 0;
+    DevVec<float> tfeat;         // feature staging of traced sweeps
     uint32_t *bits = nullptr;    // two mover bitmaps (alternating group steps)
     int32_t bit_words = 0;
     uint64_t step = 0;           // group steps applied with marks
@@ -332,17 +347,41 @@ struct lv_graph {
         a.crec = crec;
         a.newc = newc;
         a.dk = dk;
+        a.margin = margin;
+        a.kown = kown;
+        a.active = nullptr;
         a.M32 = M32;
         a.den = den;
         return a;
     }
 
     // The state changed other than by a sweep: the tracked weight inside
-    // communities and the mover bitmaps start over.
+    // communities, the mover bitmaps and the frontier observations start over.
     void state_changed() {
         lv::fill_i32(reinterpret_cast<int32_t *>(bits), 0, 2 * (int64_t)bit_words, stream);
         own_valid = false;
         last_moved = -1;
+        const int64_t Vp = (int64_t)V + 1;
+        lv::fill_i32(reinterpret_cast<int32_t *>(margin), 0, Vp, stream);
+        lv::fill_i32(reinterpret_cast<int32_t *>(kown), 0, Vp, stream);
+        lv::fill_i32(nmv, 0, Vp, stream);
+        lv::fill_i32(reinterpret_cast<int32_t *>(hist), 0, (Vp + 3) / 4, stream);
+    }
+
+    lv::FeatureArgs feature_args() const {
+        lv::FeatureArgs f;
+        f.rowptr = rowptr();
+        f.k = k;
+        f.comm = comm;
+        f.crec = crec;
+        f.margin = margin;
+        f.kown = kown;
+        f.hist = hist;
+        f.nmv = nmv;
+        f.V = V;
+        f.M = M32;
+        f.vscale = M > 0.0 ? (float)((double)V / (2.0 * M)) : 0.0f;
+        return f;
     }
 
     lv::GroupMarks marks() const {
@@ -455,6 +494,15 @@ struct lv_graph {
         LV_TRY(dev_alloc(&orig2comm, Vp));
         LV_TRY(dev_alloc(&crec, Vp));
         LV_TRY(dev_alloc(&dk, Vp));
+        LV_TRY(dev_alloc(&margin, Vp));
+        LV_TRY(dev_alloc(&kown, Vp));
+        LV_TRY(dev_alloc(&hist, Vp + 8));
+        LV_TRY(dev_alloc(&active, Vp + 8));
+        LV_TRY(dev_alloc(&nmv, Vp));
+        LV_TRY(dev_alloc(&fkeys, Vp));
+        LV_TRY(dev_alloc(&forder, Vp));
+        LV_TRY(dev_alloc(&fd_off, 64));
+        LV_TRY(dev_alloc(&mlp, lv::kParams));
         bit_words = (int32_t)((Vp + 31) / 32);
         LV_TRY(dev_alloc(&bits, 2 * (int64_t)bit_words));
         LV_TRY(dev_alloc(&keys8, Vp));
@@ -689,15 +737,21 @@ struct lv_graph {
     }
 
     // ---------------------------------------------------------------- phase 1
-    lv_status run_group(int g) {
+    // Move kernels of group g; pruned: only the frontier lists (forder by
+    // h_foff), heavy rows by their active flags.
+    lv_status run_group(int g, bool pruned = false) {
         hipStream_t s = stream;
-        const lv::MoveArgs a = move_args();
+        lv::MoveArgs a = move_args();
+        if (pruned) a.active = active;
         for (int b = 0; b < lv::kNumBuckets; ++b) {
             if (lv::kBucketGroup[b] != g) continue;
-            const int32_t n = h_off[b + 1] - h_off[b];
+            const int32_t nfull = h_off[b + 1] - h_off[b];
+            const int32_t n = pruned ? h_foff[b + 1] - h_foff[b] : nfull;
             if (n <= 0) continue;
-            const int32_t *list = order + h_off[b];
-            const double bytes = 12.0 * (double)h_bedges[b] + 32.0 * (double)n;
+            const int32_t *list = pruned ? forder + h_foff[b] : order + h_off[b];
+            // pruned: edges estimated from the bucket's mean degree
+            const double bytes =
+                12.0 * (double)h_bedges[b] * ((double)n / (double)nfull) + 32.0 * (double)n;
             hipEvent_t t0 = nullptr;
             switch (b) {
             case 0: case 1: case 2: case 3:
@@ -717,7 +771,7 @@ struct lv_graph {
                 break;
             default:
                 t0 = tbegin();
-                lv::launch_move_chunked(a, chunked_tier(), nins, nitems, list, nh, s);
+                lv::launch_move_chunked(a, chunked_tier(), nins, nitems, order + h_off[b], nh, s);
                 tend(t0, LV_K_MOVE_GLOBAL, bytes);
                 break;
             }
@@ -756,28 +810,63 @@ struct lv_graph {
         return LV_OK;
     }
 
+    // Frontier of a pruned sweep from the state after the last sweep:
+    // active flags and bucket keys, compacted into forder by bucket; the
+    // vertices left out keep their community (newc = comm).
+    lv_status select_frontier(const lv_frontier &fr) {
+        hipStream_t s = stream;
+        const lv::FeatureArgs fa = feature_args();
+        if (fr.kind == LV_FRONTIER_MODEL) {
+            if (!mlp_set) return fail(LV_ERR_STATE, "frontier model not set (lv_set_model)");
+            lv::launch_frontier(fa, mlp, fr.threshold, bb, fkeys, active, nullptr, s);
+        } else {
+            lv::launch_frontier_rule(fr.kind, fa, fr.random_p, fr.seed + 0x9e37ull * sweep_count,
+                                     bb, fkeys, active, s);
+        }
+        lv::stable_partition(fkeys, V, lv::kNumBuckets, forder, fd_off, scratch, s);
+        if (V > 0)
+            LV_HIP(hipMemcpyAsync(newc, comm, sizeof(int32_t) * V, hipMemcpyDeviceToDevice, s));
+        LV_LAUNCHED("frontier");
+        LV_TRY(stage(4096));
+        LV_HIP(hipMemcpyAsync(h_stage, fd_off, sizeof(int32_t) * (lv::kNumBuckets + 1),
+                              hipMemcpyDeviceToHost, s));
+        LV_TRY(sync());
+        std::memcpy(h_foff, h_stage, sizeof(int32_t) * (lv::kNumBuckets + 1));
+        return LV_OK;
+    }
+
     // One sweep: groups 0..6, each = moves then apply; then Q.  After a sweep
     // that moved few vertices the applies carry the weight inside
     // communities forward (correction over the movers' rows) and Q needs
     // only the sum of S_c^2; otherwise a full modularity pass follows.
-    lv_status sweep(double *q_after, int64_t *moved, double *sweep_ms) {
+    // fr: evaluate only the frontier it selects (pruned sweep).  track: count
+    // the neighbour moves afterwards (a frontier feature).
+    lv_status sweep(double *q_after, int64_t *moved, double *sweep_ms,
+                    const lv_frontier *fr = nullptr, bool track = false,
+                    int64_t *evaluated = nullptr) {
         if (!level_ready) LV_TRY(setup_level());
         hipStream_t s = stream;
         const bool incremental = incr_den > 0 && own_valid && last_moved >= 0 &&
                                  last_moved * (int64_t)incr_den <= (int64_t)V;
         hipEvent_t e0 = take_event(), e1 = take_event();
         if (e0) (void)hipEventRecord(e0, s);
+        const bool pruned = fr != nullptr && fr->kind != LV_FRONTIER_ALL;
+        if (pruned) LV_TRY(select_frontier(*fr));
+        if (evaluated)
+            *evaluated = pruned ? h_foff[lv::kNumBuckets] : h_off[lv::kNumBuckets];
+        sweep_count++;
+        lv::launch_hist_shift(hist, V, s);
         hipLaunchKernelGGL(k_zero_u64, dim3(1), dim3(1), 0, s, d_moved);
         const lv::MoveArgs a = move_args();
         for (int g = 0; g < lv::kNumGroups; ++g) {
             int32_t lo, hi;
             group_range(g, lo, hi);
             if (lo < 0 || hi <= lo) continue;
-            LV_TRY(run_group(g));
+            LV_TRY(run_group(g, pruned));
             hipEvent_t t0 = tbegin();
             const lv::GroupMarks gm = marks();
             lv::launch_apply(a, order, lo, hi, &gst[g], ins.p, comm, size, sigma, crec,
-                             incremental ? &gm : nullptr, d_moved, d_q + 1, s);
+                             incremental ? &gm : nullptr, hist, d_moved, d_q + 1, s);
             tend(t0, LV_K_APPLY, 16.0 * (double)(hi - lo));
             LV_LAUNCHED("apply");
             if (incremental) step++;
@@ -789,6 +878,11 @@ struct lv_graph {
             LV_LAUNCHED("modularity");
         } else {
             LV_TRY(enqueue_modularity());
+        }
+        if (track) {
+            lv::fill_i32(nmv, 0, V, s);
+            lv::launch_neighbor_moves(rowptr(), col(), order, st, ins.p, hist, nmv, s);
+            LV_LAUNCHED("neighbour moves");
         }
         if (e1) (void)hipEventRecord(e1, s);
         LV_TRY(stage(64));
@@ -811,35 +905,90 @@ struct lv_graph {
         return LV_OK;
     }
 
+    // Features and move labels of traced sweeps, copied to the caller.
+    struct TraceSink {
+        float *feat;      // cap x V x 8
+        uint8_t *moved;   // cap x V
+        int32_t cap;
+        int32_t n;
+    };
+
+    lv_status trace_features(TraceSink *t) {
+        if (V <= 0) return LV_OK;
+        LV_TRY(tfeat.reserve((int64_t)V * lv::kFeat));
+        lv::launch_features(feature_args(), tfeat.p, stream);
+        LV_LAUNCHED("features");
+        LV_HIP(hipMemcpyAsync(t->feat + (int64_t)t->n * V * lv::kFeat, tfeat.p,
+                              sizeof(float) * (size_t)V * lv::kFeat, hipMemcpyDeviceToHost, stream));
+        return sync();
+    }
+
+    lv_status trace_labels(TraceSink *t) {
+        if (V <= 0) return LV_OK;
+        uint8_t *dst = t->moved + (int64_t)t->n * V;
+        LV_HIP(hipMemcpyAsync(dst, hist, (size_t)V, hipMemcpyDeviceToHost, stream));
+        LV_TRY(sync());
+        for (int32_t v = 0; v < V; ++v) dst[v] &= 1u;
+        return LV_OK;
+    }
+
     // Phase (modularity_optimisation.cu:284-363): sweeps while the gain
     // reaches min_gain and is positive.
+    //
+    // With a frontier fr (learned pruning): sweep i is full when i is a
+    // multiple of fr->full_every, pruned otherwise; a pruned sweep that would
+    // end the phase is followed by a full sweep, and only a full sweep ends
+    // it.  With a trace sink every sweep is full and its features (before)
+    // and moves (after) are recorded.
     lv_status optimise(float min_gain, int32_t *changed, int32_t *sweeps_out, double *ms_out,
-                       int64_t *edges_out) {
+                       int64_t *edges_out, const lv_frontier *fr = nullptr,
+                       int64_t *evaluated_out = nullptr, int32_t *full_out = nullptr,
+                       TraceSink *trace = nullptr) {
         if (!level_ready) LV_TRY(setup_level());
         double q0 = 0.0;
         LV_TRY(modularity(&q0));
-        int32_t ch = 0, sweeps = 0;
+        const int k_full = fr ? (fr->full_every > 0 ? fr->full_every : 4) : 1;
+        int32_t ch = 0, sweeps = 0, fulls = 0;
+        int64_t evaluated = 0, edges = 0;
         double ms_total = 0.0;
-        for (;;) {
+        bool confirm = false;
+        for (int64_t i = 0;; ++i) {
+            const bool full = fr == nullptr || confirm || i % k_full == 0 || i >= kMaxPrunedSweeps;
+            const bool track = trace != nullptr || (fr != nullptr && (i + 1) % k_full != 0);
+            if (trace && trace->n < trace->cap) LV_TRY(trace_features(trace));
             double q1 = 0.0, ms = 0.0;
-            int64_t mv = 0;
-            LV_TRY(sweep(&q1, &mv, &ms));
+            int64_t mv = 0, ev = 0;
+            LV_TRY(sweep(&q1, &mv, &ms, full ? nullptr : fr, track, &ev));
+            if (trace && trace->n < trace->cap) {
+                LV_TRY(trace_labels(trace));
+                trace->n++;
+            }
             sweeps++;
+            fulls += full ? 1 : 0;
+            evaluated += ev;
+            edges += full ? E : 0;
             ms_total += ms;
             const double gain = q1 - q0;
             if (gain > 0.0) ch = 1;
             // A sweep without positive gain also ends the phase (identical
             // for min_gain > 0; guarantees termination for min_gain <= 0).
-            if (mv == 0 || !(gain > 0.0)) break;
-            if (!(gain >= (double)min_gain)) break;
+            const bool stop = mv == 0 || !(gain > 0.0) || !(gain >= (double)min_gain);
             q0 = q1;
+            if (stop) {
+                if (full) break;
+                confirm = true;
+                continue;
+            }
+            confirm = false;
         }
         lv::launch_compose(orig2comm, V0, comm, stream);
         LV_LAUNCHED("compose");
         if (changed) *changed = ch;
         if (sweeps_out) *sweeps_out = sweeps;
         if (ms_out) *ms_out = ms_total;
-        if (edges_out) *edges_out = (int64_t)sweeps * E;
+        if (edges_out) *edges_out = edges;
+        if (evaluated_out) *evaluated_out = evaluated;
+        if (full_out) *full_out = fulls;
         return LV_OK;
     }
 
@@ -983,6 +1132,16 @@ struct lv_graph {
         dev_free(orig2comm);
         dev_free(crec);
         dev_free(dk);
+        dev_free(margin);
+        dev_free(kown);
+        dev_free(hist);
+        dev_free(active);
+        dev_free(nmv);
+        dev_free(fkeys);
+        dev_free(forder);
+        dev_free(fd_off);
+        dev_free(mlp);
+        tfeat.release();
         dev_free(bits);
         dev_free(keys8);
         dev_free(order);
@@ -1115,8 +1274,13 @@ lv_status lv_sweep(lv_graph *g, double *q_after, int64_t *moved) {
     return LV_OK;
 }
 
-lv_status lv_run(lv_graph *g, float min_gain, lv_stats *stats) {
-    if (!g) return fail(LV_ERR_ARG, "null graph");
+}  // extern "C"
+
+namespace {
+
+// Level loop (gpulouvain.cu:30-44): phase, then aggregation while the phase
+// changed the communities; fr selects pruned phases.
+lv_status run_levels(lv_graph *g, float min_gain, const lv_frontier *fr, lv_stats *stats) {
     LV_HIP(hipSetDevice(g->device));
     hipEvent_t e0, e1;
     LV_HIP(hipEventCreate(&e0));
@@ -1125,16 +1289,18 @@ lv_status lv_run(lv_graph *g, float min_gain, lv_stats *stats) {
     lv_stats st{};
     double agg_ms = 0.0;
     for (;;) {
-        int32_t changed = 0, sw = 0;
+        int32_t changed = 0, sw = 0, full = 0;
         double ms = 0.0;
-        int64_t edges = 0;
-        LV_TRY(g->optimise(min_gain, &changed, &sw, &ms, &edges));
+        int64_t edges = 0, ev = 0;
+        LV_TRY(g->optimise(min_gain, &changed, &sw, &ms, &edges, fr, &ev, &full));
         if (st.levels == 0) {
             st.level0_sweeps = sw;
             st.level0_sweep_ms = ms;
         }
         st.levels++;
         st.sweeps += sw;
+        st.full_sweeps += full;
+        st.evaluated += ev;
         st.sweep_ms += ms;
         st.edges_swept += edges;
         if (!changed) break;
@@ -1167,6 +1333,115 @@ lv_status lv_run(lv_graph *g, float min_gain, lv_stats *stats) {
     st.final_E = g->E;
     if (stats) *stats = st;
     return LV_OK;
+}
+
+lv_status check_frontier(const lv_frontier *fr) {
+    if (!fr) return fail(LV_ERR_ARG, "null frontier");
+    if (fr->kind < LV_FRONTIER_MODEL || fr->kind > LV_FRONTIER_ALL)
+        return fail(LV_ERR_ARG, "unknown frontier kind");
+    if (fr->full_every < 0) return fail(LV_ERR_ARG, "full_every must be >= 0");
+    return LV_OK;
+}
+
+}  // namespace
+
+extern "C" {
+
+lv_status lv_run(lv_graph *g, float min_gain, lv_stats *stats) {
+    if (!g) return fail(LV_ERR_ARG, "null graph");
+    return run_levels(g, min_gain, nullptr, stats);
+}
+
+lv_status lv_run_pruned(lv_graph *g, float min_gain, const lv_frontier *fr, lv_stats *stats) {
+    if (!g) return fail(LV_ERR_ARG, "null graph");
+    LV_TRY(check_frontier(fr));
+    return run_levels(g, min_gain, fr, stats);
+}
+
+lv_status lv_optimise_pruned(lv_graph *g, float min_gain, const lv_frontier *fr,
+                             int32_t *changed, int32_t *sweeps, int64_t *evaluated) {
+    if (!g) return fail(LV_ERR_ARG, "null graph");
+    LV_TRY(check_frontier(fr));
+    LV_HIP(hipSetDevice(g->device));
+    return g->optimise(min_gain, changed, sweeps, nullptr, nullptr, fr, evaluated, nullptr);
+}
+
+lv_status lv_sweep_pruned(lv_graph *g, const lv_frontier *fr, int32_t track, double *q_after,
+                          int64_t *moved, int64_t *evaluated) {
+    if (!g) return fail(LV_ERR_ARG, "null graph");
+    LV_TRY(check_frontier(fr));
+    LV_HIP(hipSetDevice(g->device));
+    double q1 = 0.0;
+    int64_t mv = 0, ev = 0;
+    LV_TRY(g->sweep(&q1, &mv, nullptr, fr, track != 0, &ev));
+    if (q_after) *q_after = q1;
+    if (moved) *moved = mv;
+    if (evaluated) *evaluated = ev;
+    return LV_OK;
+}
+
+lv_status lv_trace_phase(lv_graph *g, float min_gain, float *feat, uint8_t *moved,
+                         int32_t max_sweeps, int32_t *traced, int32_t *changed) {
+    if (!g) return fail(LV_ERR_ARG, "null graph");
+    if (max_sweeps < 0 || (max_sweeps > 0 && (!feat || !moved)))
+        return fail(LV_ERR_ARG, "trace buffers");
+    LV_HIP(hipSetDevice(g->device));
+    lv_graph::TraceSink t{feat, moved, max_sweeps, 0};
+    int32_t sw = 0;
+    LV_TRY(g->optimise(min_gain, changed, &sw, nullptr, nullptr, nullptr, nullptr, nullptr, &t));
+    if (traced) *traced = t.n;
+    return LV_OK;
+}
+
+lv_status lv_features(lv_graph *g, float *feat) {
+    if (!g || !feat) return fail(LV_ERR_ARG, "null argument");
+    LV_HIP(hipSetDevice(g->device));
+    if (!g->level_ready) LV_TRY(g->setup_level());
+    lv_graph::TraceSink t{feat, nullptr, 1, 0};
+    return g->trace_features(&t);
+}
+
+lv_status lv_move_state(lv_graph *g, float *margin, float *kown, uint8_t *hist, int32_t *nmv) {
+    if (!g) return fail(LV_ERR_ARG, "null graph");
+    LV_HIP(hipSetDevice(g->device));
+    if (!g->level_ready) LV_TRY(g->setup_level());
+    const size_t V = (size_t)g->V;
+    if (V > 0) {
+        if (margin)
+            LV_HIP(hipMemcpyAsync(margin, g->margin, 4 * V, hipMemcpyDeviceToHost, g->stream));
+        if (kown) LV_HIP(hipMemcpyAsync(kown, g->kown, 4 * V, hipMemcpyDeviceToHost, g->stream));
+        if (hist) LV_HIP(hipMemcpyAsync(hist, g->hist, V, hipMemcpyDeviceToHost, g->stream));
+        if (nmv) LV_HIP(hipMemcpyAsync(nmv, g->nmv, 4 * V, hipMemcpyDeviceToHost, g->stream));
+    }
+    return g->sync();
+}
+
+lv_status lv_set_model(lv_graph *g, const float *params) {
+    if (!g || !params) return fail(LV_ERR_ARG, "null argument");
+    LV_HIP(hipSetDevice(g->device));
+    LV_HIP(hipMemcpyAsync(g->mlp, params, sizeof(float) * lv::kParams, hipMemcpyHostToDevice,
+                          g->stream));
+    LV_TRY(g->sync());
+    g->mlp_set = true;
+    return LV_OK;
+}
+
+lv_status lv_predict(lv_graph *g, float threshold, float *prob, uint8_t *active) {
+    if (!g) return fail(LV_ERR_ARG, "null graph");
+    if (!g->mlp_set) return fail(LV_ERR_STATE, "frontier model not set (lv_set_model)");
+    LV_HIP(hipSetDevice(g->device));
+    if (!g->level_ready) LV_TRY(g->setup_level());
+    if (g->V <= 0) return LV_OK;
+    LV_TRY(g->tfeat.reserve(g->V));
+    lv::launch_frontier(g->feature_args(), g->mlp, threshold, g->bb, g->fkeys, g->active,
+                        g->tfeat.p, g->stream);
+    LV_LAUNCHED("frontier model");
+    if (prob)
+        LV_HIP(hipMemcpyAsync(prob, g->tfeat.p, sizeof(float) * g->V, hipMemcpyDeviceToHost,
+                              g->stream));
+    if (active)
+        LV_HIP(hipMemcpyAsync(active, g->active, (size_t)g->V, hipMemcpyDeviceToHost, g->stream));
+    return g->sync();
 }
 
 lv_status lv_assignment(lv_graph *g, int32_t *orig2comm, int32_t *level_V) {
@@ -1260,4 +1535,130 @@ lv_status lv_kernel_timing_reset(lv_graph *g) {
     return LV_OK;
 }
 
+// ------------------------------------------------------------ frontier model
+
 }  // extern "C"
+
+namespace {
+
+// Device buffers of one standalone model call, freed on return.
+struct ModelBuffers {
+    std::vector<void *> ptrs;
+    template <typename T>
+    lv_status get(T **p, int64_t n) {
+        LV_TRY(dev_alloc(p, n));
+        ptrs.push_back(*p);
+        return LV_OK;
+    }
+    ~ModelBuffers() {
+        for (void *p : ptrs) (void)hipFree(p);
+    }
+};
+
+lv_status model_device(int32_t device) {
+    int ndev = 0;
+    if (hipGetDeviceCount(&ndev) != hipSuccess || ndev <= 0)
+        return fail(LV_ERR_NODEVICE, "no HIP device visible");
+    if (device < 0 || device >= ndev) return fail(LV_ERR_ARG, "device ordinal out of range");
+    LV_HIP(hipSetDevice(device));
+    return LV_OK;
+}
+
+}  // namespace
+
+extern "C" {
+
+lv_status lv_mlp_forward(int32_t device, const float *params, const float *feat, int64_t n,
+                         float *prob) {
+    if (!params || n < 0 || (n > 0 && (!feat || !prob))) return fail(LV_ERR_ARG, "bad argument");
+    LV_TRY(model_device(device));
+    if (n == 0) return LV_OK;
+    ModelBuffers mb;
+    float *dP, *dX, *dY;
+    LV_TRY(mb.get(&dP, lv::kParams));
+    LV_TRY(mb.get(&dX, n * lv::kFeat));
+    LV_TRY(mb.get(&dY, n));
+    LV_HIP(hipMemcpy(dP, params, sizeof(float) * lv::kParams, hipMemcpyHostToDevice));
+    LV_HIP(hipMemcpy(dX, feat, sizeof(float) * n * lv::kFeat, hipMemcpyHostToDevice));
+    lv::launch_mlp_forward(dX, n, dP, dY, nullptr);
+    LV_LAUNCHED("mlp forward");
+    LV_HIP(hipMemcpy(prob, dY, sizeof(float) * n, hipMemcpyDeviceToHost));
+    return LV_OK;
+}
+
+lv_status lv_mlp_grad(int32_t device, const float *params, const float *feat, const float *labels,
+                      int64_t n, float pos_weight, float *grad, float *loss) {
+    if (!params || !grad || n <= 0 || !feat || !labels) return fail(LV_ERR_ARG, "bad argument");
+    LV_TRY(model_device(device));
+    ModelBuffers mb;
+    float *dP, *dX, *dL, *dpart, *dG;
+    const int64_t nb = lv::grad_blocks(n);
+    LV_TRY(mb.get(&dP, lv::kParams));
+    LV_TRY(mb.get(&dX, n * lv::kFeat));
+    LV_TRY(mb.get(&dL, n));
+    LV_TRY(mb.get(&dpart, nb * (lv::kParams + 1)));
+    LV_TRY(mb.get(&dG, lv::kParams + 1));
+    LV_HIP(hipMemcpy(dP, params, sizeof(float) * lv::kParams, hipMemcpyHostToDevice));
+    LV_HIP(hipMemcpy(dX, feat, sizeof(float) * n * lv::kFeat, hipMemcpyHostToDevice));
+    LV_HIP(hipMemcpy(dL, labels, sizeof(float) * n, hipMemcpyHostToDevice));
+    lv::launch_train_grad(dX, dL, n, dP, pos_weight, dpart, nullptr);
+    lv::launch_train_reduce(dpart, nb, n, dG, nullptr);
+    LV_LAUNCHED("mlp gradient");
+    std::vector<float> h(lv::kParams + 1);
+    LV_HIP(hipMemcpy(h.data(), dG, sizeof(float) * (lv::kParams + 1), hipMemcpyDeviceToHost));
+    std::memcpy(grad, h.data(), sizeof(float) * lv::kParams);
+    if (loss) *loss = h[lv::kParams];
+    return LV_OK;
+}
+
+lv_status lv_mlp_train(int32_t device, const float *feat, const float *labels, int64_t n,
+                       const lv_train_options *opt, float *params, float *loss_per_epoch) {
+    if (!feat || !labels || !params || !opt || n <= 0) return fail(LV_ERR_ARG, "bad argument");
+    if (opt->epochs < 0 || opt->batch <= 0) return fail(LV_ERR_ARG, "epochs/batch");
+    LV_TRY(model_device(device));
+    ModelBuffers mb;
+    const int64_t batch = std::min<int64_t>(opt->batch, n);
+    const int64_t nb = lv::grad_blocks(batch);
+    float *dP, *dX, *dL, *dpart, *dG, *dm, *dv;
+    double *dloss;
+    const int32_t epochs = opt->epochs;
+    LV_TRY(mb.get(&dP, lv::kParams));
+    LV_TRY(mb.get(&dm, lv::kParams));
+    LV_TRY(mb.get(&dv, lv::kParams));
+    LV_TRY(mb.get(&dX, n * lv::kFeat));
+    LV_TRY(mb.get(&dL, n));
+    LV_TRY(mb.get(&dpart, nb * (lv::kParams + 1)));
+    LV_TRY(mb.get(&dG, lv::kParams + 1));
+    LV_TRY(mb.get(&dloss, std::max(epochs, 1)));
+    LV_HIP(hipMemcpy(dP, params, sizeof(float) * lv::kParams, hipMemcpyHostToDevice));
+    LV_HIP(hipMemcpy(dX, feat, sizeof(float) * n * lv::kFeat, hipMemcpyHostToDevice));
+    LV_HIP(hipMemcpy(dL, labels, sizeof(float) * n, hipMemcpyHostToDevice));
+    lv::fill_i32(reinterpret_cast<int32_t *>(dm), 0, lv::kParams, nullptr);
+    lv::fill_i32(reinterpret_cast<int32_t *>(dv), 0, lv::kParams, nullptr);
+    lv::fill_f64(dloss, 0.0, std::max(epochs, 1), nullptr);
+    const float b1 = opt->beta1 > 0.0f ? opt->beta1 : 0.9f;
+    const float b2 = opt->beta2 > 0.0f ? opt->beta2 : 0.999f;
+    const float eps = opt->eps > 0.0f ? opt->eps : 1e-8f;
+    const float pw = opt->pos_weight > 0.0f ? opt->pos_weight : 1.0f;
+    int32_t t = 0;
+    for (int32_t e = 0; e < epochs; ++e) {
+        for (int64_t lo = 0; lo < n; lo += batch) {
+            const int64_t bn = std::min(batch, n - lo);
+            lv::launch_train_grad(dX + lo * lv::kFeat, dL + lo, bn, dP, pw, dpart, nullptr);
+            lv::launch_train_reduce(dpart, lv::grad_blocks(bn), bn, dG, nullptr);
+            lv::launch_accumulate_loss(dG, (double)bn / (double)n, dloss + e, nullptr);
+            lv::launch_adam(dP, dG, dm, dv, ++t, opt->lr, b1, b2, eps, nullptr);
+        }
+        LV_LAUNCHED("mlp training step");
+    }
+    LV_HIP(hipMemcpy(params, dP, sizeof(float) * lv::kParams, hipMemcpyDeviceToHost));
+    if (loss_per_epoch && epochs > 0) {
+        std::vector<double> h(epochs);
+        LV_HIP(hipMemcpy(h.data(), dloss, sizeof(double) * epochs, hipMemcpyDeviceToHost));
+        for (int32_t e = 0; e < epochs; ++e) loss_per_epoch[e] = (float)h[e];
+    }
+    return LV_OK;
+}
+
+}  // extern "C"
+

@@ -1,0 +1,78 @@
+// Learned frontier: per-vertex features, the BF16-MFMA inference of the
+// move predictor (8 -> 32 -> 32 -> 1, ReLU, sigmoid), rule-based frontiers
+// for comparison, and the training kernels (forward + backward with a
+// fixed-order gradient reduction, Adam).
+#pragma once
+
+#include "lv_common.h"
+#include "lv_phase1.h"
+
+namespace lv {
+
+constexpr int kFeat = 8;
+constexpr int kHid = 32;
+// Parameter layout (fp32 master copy): W1[32][8], b1[32], W2[32][32], b2[32], w3[32], b3.
+// W[a][j] multiplies input j of output unit a.
+constexpr int kOffW1 = 0;
+constexpr int kOffB1 = kOffW1 + kHid * kFeat;
+constexpr int kOffW2 = kOffB1 + kHid;
+constexpr int kOffB2 = kOffW2 + kHid * kHid;
+constexpr int kOffW3 = kOffB2 + kHid;
+constexpr int kOffB3 = kOffW3 + kHid;
+constexpr int kParams = kOffB3 + 1;  // 1377
+
+// Inputs of the feature function (state after the last sweep).
+struct FeatureArgs {
+    const int32_t *rowptr;
+    const double *k;
+    const int32_t *comm;
+    const uint64_t *crec;
+    const float *margin;  // last evaluation: bestGain - K[cur]/M
+    const float *kown;    // last evaluation: K[cur]
+    const uint8_t *hist;  // move history, bit 0 = most recent sweep
+    const int32_t *nmv;   // row entries whose neighbour moved in the last sweep
+    int32_t V;
+    float M;
+    float vscale;         // V / (2M)
+};
+
+// Frontier selection rules.
+enum FrontierKind { kFrontierModel = 0, kFrontierNeighbor = 1, kFrontierRandom = 2, kFrontierAll = 3 };
+
+// hist[v] <<= 1 for every vertex (start of a sweep).
+void launch_hist_shift(uint8_t *hist, int32_t V, hipStream_t s);
+// nmv[u] += 1 for every row entry (u, v) of a vertex v that moved in the last
+// sweep (hist bit 0); nmv must be zero before.  Rows are walked with the
+// segment table of the level (heavy rows by chunks).
+void launch_neighbor_moves(const int32_t *rowptr, const int32_t *col, const int32_t *order,
+                           const SegTable &st, const Chunk *chunks, const uint8_t *hist,
+                           int32_t *nmv, hipStream_t s);
+// feat[v*8 .. v*8+8) for every vertex.
+void launch_features(const FeatureArgs &f, float *feat, hipStream_t s);
+// Frontier from the model: active[v] = deg > 0 and sigmoid(MLP(features)) >
+// threshold; keys[v] = execution bucket of an active vertex, -1 otherwise;
+// prob (optional) = the predicted probability.
+void launch_frontier(const FeatureArgs &f, const float *params, float threshold, BucketBounds bb,
+                     int8_t *keys, uint8_t *active, float *prob, hipStream_t s);
+// Frontier from a rule: kFrontierNeighbor = moved in the last sweep or a
+// neighbour did; kFrontierRandom = seeded Bernoulli(p) per vertex and sweep;
+// kFrontierAll = every vertex of positive degree.
+void launch_frontier_rule(int kind, const FeatureArgs &f, float p, uint64_t seed, BucketBounds bb,
+                          int8_t *keys, uint8_t *active, hipStream_t s);
+// prob[i] = sigmoid(MLP(feat[i*8 .. i*8+8))) on MFMA (bf16 operands, fp32 accumulation).
+void launch_mlp_forward(const float *feat, int64_t n, const float *params, float *prob,
+                        hipStream_t s);
+
+// Training.  partials: grad_blocks(n) x (kParams + 1) floats (last = loss).
+int64_t grad_blocks(int64_t n);
+void launch_train_grad(const float *feat, const float *labels, int64_t n, const float *params,
+                       float pos_weight, float *partials, hipStream_t s);
+// grads[0..kParams) = mean gradient, grads[kParams] = mean loss (fixed order).
+void launch_train_reduce(const float *partials, int64_t nblocks, int64_t n, float *grads,
+                         hipStream_t s);
+// *acc += weight * grads[kParams] (loss bookkeeping of an epoch).
+void launch_accumulate_loss(const float *grads, double weight, double *acc, hipStream_t s);
+void launch_adam(float *params, const float *grads, float *m, float *v, int32_t t, float lr,
+                 float beta1, float beta2, float eps, hipStream_t s);
+
+}  // namespace lv

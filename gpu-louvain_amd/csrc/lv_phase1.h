@@ -115,9 +115,10 @@ struct GroupMarks {
 // communities.  With gm (incremental modularity) the movers are marked and
 // the group's change of the weight inside communities is added to *own
 // (gst: the group's row segments).
+// hist[v] gets bit 0 set for a mover (the move history of the frontier model).
 void launch_apply(const MoveArgs &a, const int32_t *order, int32_t lo, int32_t hi,
                   const SegTable *gst, const Chunk *chunks, int32_t *comm, int32_t *size,
-                  double *sigma, uint64_t *crec, const GroupMarks *gm,
+                  double *sigma, uint64_t *crec, const GroupMarks *gm, uint8_t *hist,
                   unsigned long long *moved, double *own, hipStream_t s);
 
 // --- modularity ---

@@ -64,12 +64,20 @@ __device__ __forceinline__ void lanes_max_pair(uint64_t &best, double &bestK) {
     }
 }
 
-// Decision plus dk = K[new] - K[cur] for a mover (0 otherwise).
+// Decision plus dk = K[new] - K[cur] for a mover (0 otherwise), and the
+// observations the frontier model reads (move margin, own-community weight).
 __device__ __forceinline__ void write_decision(const MoveArgs &a, int32_t v, int32_t cur,
                                                uint64_t best, double bestK, double kc) {
+#pragma clang fp contract(off)
     const int32_t nc = decide_from(best, kc, a.M32, cur);
     a.newc[v] = nc;
     a.dk[v] = nc != cur ? bestK - kc : 0.0;
+    // margin as computeMove sees it: bestGain starts at 0 (no candidate or
+    // only negative gains leave it there)
+    const float bg = best ? fmaxf(best_gain(best), 0.0f) : 0.0f;
+    const float t = (float)kc / a.M32;
+    a.margin[v] = bg - t;
+    a.kown[v] = (float)kc;
 }
 
 // ------------------------------------------------------------------ moves
@@ -294,6 +302,7 @@ __global__ __launch_bounds__(256) void k_chunk_aggregate(MoveArgs a, ChunkedTier
     __shared__ double vals[C];
     __shared__ int32_t pcnt[kMaxParts];
     const Chunk ch = t.chunks[blockIdx.x];
+    if (a.active && !a.active[ch.v]) return;  // not in this sweep's frontier
     const int P = ch.P;
     for (int i = threadIdx.x; i < C; i += 256) {
         keys[i] = -1;
@@ -420,6 +429,7 @@ __global__ __launch_bounds__(kMergeThreads) void k_chunk_merge(MoveArgs a, Chunk
     const MergeItem it = t.items[item];
     const HeavyVertex h = t.hvs[it.hv];
     const int32_t v = h.v;
+    if (a.active && !a.active[v]) return;  // not in this sweep's frontier
     const int32_t cur = a.comm[v];
     const float k32 = (float)a.k[v];
     const uint64_t rcur = a.crec[cur];
@@ -517,6 +527,11 @@ __global__ void k_heavy_final(MoveArgs a, int32_t nh, ChunkedTier t) {
     const HeavyVertex h = t.hvs[hv];
     const int32_t v = h.v;
     const int32_t cur = a.comm[v];
+    if (a.active && !a.active[v]) {  // not evaluated: stays
+        a.newc[v] = cur;
+        a.dk[v] = 0.0;
+        return;
+    }
     uint64_t best = t.best[hv];
     double bestK = 0.0;
     for (int32_t p = 0; p < h.P; ++p)
@@ -598,7 +613,8 @@ __global__ __launch_bounds__(256) void k_apply_join(const int32_t *__restrict__ 
                                                     int32_t *newc, int32_t *comm, int32_t *size,
                                                     double *sigma, const double *__restrict__ k,
                                                     const uint64_t *__restrict__ crec,
-                                                    uint32_t *movers, unsigned long long *moved) {
+                                                    uint32_t *movers, uint8_t *hist,
+                                                    unsigned long long *moved) {
     const int32_t i = blockIdx.x * 256 + threadIdx.x;
     bool mv = false, shared_old = false;
     int32_t v = 0, nc = 0, oc = 0;
@@ -613,6 +629,7 @@ __global__ __launch_bounds__(256) void k_apply_join(const int32_t *__restrict__ 
             kv = k[v];
             comm[v] = nc;
             newc[v] = oc;  // from here on newc holds the community left
+            hist[v] |= 1;  // move history (bit 0 = this sweep)
         }
     }
     wave_peel_update(mv, nc, kv, 1, size, sigma, 3);
@@ -1006,13 +1023,13 @@ void launch_clear_tables(int32_t *keys, double *vals, int64_t n, hipStream_t s) 
 
 void launch_apply(const MoveArgs &a, const int32_t *order, int32_t lo, int32_t hi,
                   const SegTable *gst, const Chunk *chunks, int32_t *comm, int32_t *size,
-                  double *sigma, uint64_t *crec, const GroupMarks *gm,
+                  double *sigma, uint64_t *crec, const GroupMarks *gm, uint8_t *hist,
                   unsigned long long *moved, double *own, hipStream_t s) {
     const int32_t n = hi - lo;
     if (n <= 0) return;
     const int32_t *list = order + lo;
     hipLaunchKernelGGL(k_apply_join, dim3(blocks_for(n, 256)), dim3(256), 0, s, list, n, a.newc,
-                       comm, size, sigma, a.k, crec, gm ? gm->movers : nullptr, moved);
+                       comm, size, sigma, a.k, crec, gm ? gm->movers : nullptr, hist, moved);
     hipLaunchKernelGGL(k_apply_leave, dim3(blocks_for(n, 256)), dim3(256), 0, s, list, n, a.newc,
                        comm, size, sigma, a.k, crec);
     const int32_t corr = gm ? gst->total_blocks : 0;

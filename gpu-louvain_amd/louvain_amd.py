@@ -10,6 +10,10 @@ Host-side mirror of the reference's operations, for tests and benchmarks:
     Graph.assignment()    data of printOriginalToCommunity (modularity_optimisation.cu:403-418)
     read_mtx(path)        readInputData                (utils.cu:9-62)
 
+The learned frontier (no reference counterpart) adds Graph.run_pruned,
+Graph.trace_phase, Graph.features, Graph.set_model, Graph.predict and the
+standalone model kernels mlp_forward, mlp_grad and mlp_train.
+
 Every call goes to the native library; there is no Python fallback.  A
 missing library raises LouvainError at load time.
 """
@@ -40,6 +44,7 @@ I32P = ctypes.POINTER(ctypes.c_int32)
 I64P = ctypes.POINTER(ctypes.c_int64)
 F32P = ctypes.POINTER(ctypes.c_float)
 F64P = ctypes.POINTER(ctypes.c_double)
+U8P = ctypes.POINTER(ctypes.c_uint8)
 
 
 class LouvainError(RuntimeError):
@@ -76,11 +81,51 @@ class Stats(ctypes.Structure):
         ("level0_sweep_ms", ctypes.c_double),
         ("aggregate_ms", ctypes.c_double),
         ("memory_ms", ctypes.c_double),
-        ("reserved", ctypes.c_double * 6),
+        ("evaluated", ctypes.c_int64),
+        ("full_sweeps", ctypes.c_int32),
+        ("pad_", ctypes.c_int32),
+        ("reserved", ctypes.c_double * 4),
     ]
 
     def as_dict(self) -> dict:
-        return {name: getattr(self, name) for name, _ in self._fields_ if name != "reserved"}
+        return {name: getattr(self, name) for name, _ in self._fields_
+                if name not in ("reserved", "pad_")}
+
+
+MLP_PARAMS = 1377
+FEATURES = 8
+HIDDEN = 32
+FRONTIER_MODEL, FRONTIER_NEIGHBOR, FRONTIER_RANDOM, FRONTIER_ALL = range(4)
+
+
+class Frontier(ctypes.Structure):
+    _fields_ = [
+        ("kind", ctypes.c_int32),
+        ("full_every", ctypes.c_int32),
+        ("threshold", ctypes.c_float),
+        ("random_p", ctypes.c_float),
+        ("seed", ctypes.c_uint64),
+        ("reserved", ctypes.c_int32 * 8),
+    ]
+
+
+class TrainOptions(ctypes.Structure):
+    _fields_ = [
+        ("epochs", ctypes.c_int32),
+        ("batch", ctypes.c_int32),
+        ("lr", ctypes.c_float),
+        ("pos_weight", ctypes.c_float),
+        ("beta1", ctypes.c_float),
+        ("beta2", ctypes.c_float),
+        ("eps", ctypes.c_float),
+        ("reserved", ctypes.c_int32 * 9),
+    ]
+
+
+def frontier(kind: int = FRONTIER_MODEL, full_every: int = 4, threshold: float = 0.5,
+             random_p: float = 0.25, seed: int = 1) -> Frontier:
+    return Frontier(kind=kind, full_every=full_every, threshold=threshold, random_p=random_p,
+                    seed=seed)
 
 
 class KernelTime(ctypes.Structure):
@@ -128,6 +173,23 @@ def load() -> ctypes.CDLL:
                                             ctypes.c_uint64, ctypes.c_int32, ctypes.POINTER(I32P),
                                             ctypes.POINTER(I32P), ctypes.POINTER(F32P), I32P, I64P]),
         "lv_free_host": (None, [ctypes.c_void_p]),
+        "lv_run_pruned": (ctypes.c_int32, [ctypes.c_void_p, ctypes.c_float, ctypes.POINTER(Frontier),
+                                           ctypes.POINTER(Stats)]),
+        "lv_optimise_pruned": (ctypes.c_int32, [ctypes.c_void_p, ctypes.c_float, ctypes.POINTER(Frontier),
+                                                I32P, I32P, I64P]),
+        "lv_sweep_pruned": (ctypes.c_int32, [ctypes.c_void_p, ctypes.POINTER(Frontier), ctypes.c_int32,
+                                             F64P, I64P, I64P]),
+        "lv_trace_phase": (ctypes.c_int32, [ctypes.c_void_p, ctypes.c_float, F32P, U8P, ctypes.c_int32,
+                                            I32P, I32P]),
+        "lv_features": (ctypes.c_int32, [ctypes.c_void_p, F32P]),
+        "lv_move_state": (ctypes.c_int32, [ctypes.c_void_p, F32P, F32P, U8P, I32P]),
+        "lv_set_model": (ctypes.c_int32, [ctypes.c_void_p, F32P]),
+        "lv_predict": (ctypes.c_int32, [ctypes.c_void_p, ctypes.c_float, F32P, U8P]),
+        "lv_mlp_forward": (ctypes.c_int32, [ctypes.c_int32, F32P, F32P, ctypes.c_int64, F32P]),
+        "lv_mlp_grad": (ctypes.c_int32, [ctypes.c_int32, F32P, F32P, F32P, ctypes.c_int64, ctypes.c_float,
+                                         F32P, F32P]),
+        "lv_mlp_train": (ctypes.c_int32, [ctypes.c_int32, F32P, F32P, ctypes.c_int64,
+                                          ctypes.POINTER(TrainOptions), F32P, F32P]),
     }
     for name, (res, args) in sig.items():
         fn = getattr(lib, name)
@@ -334,3 +396,113 @@ class Graph:
 
     def kernel_timing_reset(self) -> None:
         _check(self._lib.lv_kernel_timing_reset(self._h))
+
+    # --- learned frontier
+    def run_pruned(self, min_gain: float, fr: Frontier) -> dict:
+        st = Stats()
+        _check(self._lib.lv_run_pruned(self._h, min_gain, ctypes.byref(fr), ctypes.byref(st)))
+        return st.as_dict()
+
+    def optimise_pruned(self, min_gain: float, fr: Frontier) -> tuple[bool, int, int]:
+        """One pruned phase; returns (changed, sweeps, vertex evaluations)."""
+        ch, sw, ev = ctypes.c_int32(), ctypes.c_int32(), ctypes.c_int64()
+        _check(self._lib.lv_optimise_pruned(self._h, min_gain, ctypes.byref(fr), ctypes.byref(ch),
+                                            ctypes.byref(sw), ctypes.byref(ev)))
+        return bool(ch.value), sw.value, ev.value
+
+    def sweep_pruned(self, fr: Frontier, track: bool = True) -> tuple[float, int, int]:
+        """One sweep over fr's frontier; returns (Q after, moved, evaluated)."""
+        q, mv, ev = ctypes.c_double(), ctypes.c_int64(), ctypes.c_int64()
+        _check(self._lib.lv_sweep_pruned(self._h, ctypes.byref(fr), int(track), ctypes.byref(q),
+                                         ctypes.byref(mv), ctypes.byref(ev)))
+        return q.value, mv.value, ev.value
+
+    def trace_phase(self, min_gain: float, max_sweeps: int = 64) -> tuple[np.ndarray, np.ndarray, bool]:
+        """One plain phase with traces: features [s, V, 8] before each sweep and
+        moved flags [s, V] of each sweep."""
+        V, _ = self.level_size()
+        feat = np.zeros((max_sweeps, max(V, 1), FEATURES), np.float32)
+        moved = np.zeros((max_sweeps, max(V, 1)), np.uint8)
+        n, ch = ctypes.c_int32(), ctypes.c_int32()
+        _check(self._lib.lv_trace_phase(self._h, min_gain, _p(feat, ctypes.c_float), _p(moved, ctypes.c_uint8),
+                                        max_sweeps, ctypes.byref(n), ctypes.byref(ch)))
+        return feat[: n.value, :V], moved[: n.value, :V], bool(ch.value)
+
+    def features(self) -> np.ndarray:
+        V, _ = self.level_size()
+        feat = np.zeros((max(V, 1), FEATURES), np.float32)
+        _check(self._lib.lv_features(self._h, _p(feat, ctypes.c_float)))
+        return feat[:V]
+
+    def move_state(self) -> dict:
+        V, _ = self.level_size()
+        n = max(V, 1)
+        margin = np.zeros(n, np.float32)
+        kown = np.zeros(n, np.float32)
+        hist = np.zeros(n, np.uint8)
+        nmv = np.zeros(n, np.int32)
+        _check(self._lib.lv_move_state(self._h, _p(margin, ctypes.c_float), _p(kown, ctypes.c_float),
+                                       _p(hist, ctypes.c_uint8), _p(nmv, ctypes.c_int32)))
+        return {"margin": margin[:V], "kown": kown[:V], "hist": hist[:V], "nmv": nmv[:V]}
+
+    def set_model(self, params: np.ndarray) -> None:
+        params = np.ascontiguousarray(params, dtype=np.float32)
+        if params.size != MLP_PARAMS:
+            raise ValueError(f"model has {MLP_PARAMS} parameters, got {params.size}")
+        _check(self._lib.lv_set_model(self._h, _p(params, ctypes.c_float)))
+
+    def predict(self, threshold: float = 0.5) -> tuple[np.ndarray, np.ndarray]:
+        V, _ = self.level_size()
+        prob = np.zeros(max(V, 1), np.float32)
+        act = np.zeros(max(V, 1), np.uint8)
+        _check(self._lib.lv_predict(self._h, threshold, _p(prob, ctypes.c_float), _p(act, ctypes.c_uint8)))
+        return prob[:V], act[:V]
+
+
+def mlp_forward(params: np.ndarray, feat: np.ndarray, device: int = 0) -> np.ndarray:
+    """MFMA inference: sigmoid(MLP(feat)) for feat [n, 8]."""
+    lib = load()
+    params = np.ascontiguousarray(params, dtype=np.float32)
+    feat = np.ascontiguousarray(feat, dtype=np.float32).reshape(-1, FEATURES)
+    prob = np.zeros(max(feat.shape[0], 1), np.float32)
+    _check(lib.lv_mlp_forward(device, _p(params, ctypes.c_float), _p(feat, ctypes.c_float), feat.shape[0],
+                              _p(prob, ctypes.c_float)))
+    return prob[: feat.shape[0]]
+
+
+def mlp_grad(params: np.ndarray, feat: np.ndarray, labels: np.ndarray, pos_weight: float = 1.0,
+             device: int = 0) -> tuple[np.ndarray, float]:
+    """Mean gradient and mean loss of the weighted BCE over (feat, labels)."""
+    lib = load()
+    params = np.ascontiguousarray(params, dtype=np.float32)
+    feat = np.ascontiguousarray(feat, dtype=np.float32).reshape(-1, FEATURES)
+    labels = np.ascontiguousarray(labels, dtype=np.float32)
+    grad = np.zeros(MLP_PARAMS, np.float32)
+    loss = ctypes.c_float()
+    _check(lib.lv_mlp_grad(device, _p(params, ctypes.c_float), _p(feat, ctypes.c_float),
+                           _p(labels, ctypes.c_float), feat.shape[0], pos_weight, _p(grad, ctypes.c_float),
+                           ctypes.byref(loss)))
+    return grad, loss.value
+
+
+def mlp_train(feat: np.ndarray, labels: np.ndarray, params: np.ndarray, epochs: int, batch: int = 4096,
+              lr: float = 1e-3, pos_weight: float = 1.0, device: int = 0) -> tuple[np.ndarray, np.ndarray]:
+    """Adam training on the device; returns (trained params, mean loss per epoch)."""
+    lib = load()
+    params = np.array(params, dtype=np.float32, copy=True)
+    feat = np.ascontiguousarray(feat, dtype=np.float32).reshape(-1, FEATURES)
+    labels = np.ascontiguousarray(labels, dtype=np.float32)
+    losses = np.zeros(max(epochs, 1), np.float32)
+    opt = TrainOptions(epochs=epochs, batch=batch, lr=lr, pos_weight=pos_weight)
+    _check(lib.lv_mlp_train(device, _p(feat, ctypes.c_float), _p(labels, ctypes.c_float), feat.shape[0],
+                            ctypes.byref(opt), _p(params, ctypes.c_float), _p(losses, ctypes.c_float)))
+    return params, losses[:epochs]
+
+
+def mlp_init(seed: int = 0) -> np.ndarray:
+    """He-initialised parameters in the library's layout."""
+    rng = np.random.default_rng(seed)
+    w1 = rng.normal(0.0, np.sqrt(2.0 / FEATURES), (HIDDEN, FEATURES))
+    w2 = rng.normal(0.0, np.sqrt(2.0 / HIDDEN), (HIDDEN, HIDDEN))
+    w3 = rng.normal(0.0, np.sqrt(1.0 / HIDDEN), HIDDEN)
+    return np.concatenate([w1.ravel(), np.zeros(HIDDEN), w2.ravel(), np.zeros(HIDDEN), w3, [0.0]]).astype(np.float32)

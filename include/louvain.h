@@ -64,7 +64,10 @@ typedef struct {
     double level0_sweep_ms;  /* device time of the first phase's sweeps             */
     double aggregate_ms;     /* device time inside aggregation                      */
     double memory_ms;        /* upload time at lv_create                            */
-    double reserved[6];
+    int64_t evaluated;       /* vertex evaluations over all sweeps (pruned: frontier) */
+    int32_t full_sweeps;     /* sweeps that evaluated every vertex                  */
+    int32_t pad_;
+    double reserved[4];
 } lv_stats;
 
 /* Per-kernel timing (options.timing = 1): kernel ids below. */
@@ -131,6 +134,79 @@ lv_status lv_kernel_timing(lv_graph *g, int32_t kernel, lv_kernel_time *out);
 lv_status lv_kernel_timing_reset(lv_graph *g);
 
 const char *lv_last_error(void);
+
+/* ---- learned frontier (no counterpart in the reference) ----
+ * A move predictor (MLP 8 -> 32 -> 32 -> 1, ReLU, sigmoid; 1377 fp32
+ * parameters W1[32][8], b1[32], W2[32][32], b2[32], w3[32], b3) chooses
+ * which vertices a pruned sweep re-evaluates.  Its 8 features per vertex
+ * are read from the state after the last sweep: log-degree, own-community
+ * weight fraction and best-gain margin of the last evaluation, moves in
+ * the last 4 sweeps, log community size, log community total, share of
+ * neighbours that moved in the last sweep, sweeps since the last move.
+ * Inference uses bf16 MFMA with fp32 accumulation; training is fp32. */
+#define LV_MLP_PARAMS 1377
+#define LV_FEATURES 8
+
+#define LV_FRONTIER_MODEL 0      /* p(move) > threshold                         */
+#define LV_FRONTIER_NEIGHBOR 1   /* moved in the last sweep or a neighbour did  */
+#define LV_FRONTIER_RANDOM 2     /* seeded Bernoulli(random_p) per vertex/sweep  */
+#define LV_FRONTIER_ALL 3        /* every vertex (a plain sweep)                 */
+
+typedef struct {
+    int32_t kind;            /* LV_FRONTIER_*                                        */
+    int32_t full_every;      /* a full sweep every k sweeps of a phase (0 = 4)       */
+    float threshold;         /* model: evaluate when p(move) > threshold             */
+    float random_p;          /* random: evaluation probability                       */
+    uint64_t seed;           /* random: seed                                         */
+    int32_t reserved[8];
+} lv_frontier;
+
+typedef struct {
+    int32_t epochs;
+    int32_t batch;           /* samples per Adam step (contiguous ranges, caller shuffles) */
+    float lr;
+    float pos_weight;        /* BCE weight of the positive class (0 = 1)             */
+    float beta1, beta2, eps; /* Adam (0 = 0.9, 0.999, 1e-8)                           */
+    int32_t reserved[9];
+} lv_train_options;
+
+/* Levels like lv_run, with pruned phases: sweep i of a phase is full when
+ * i % full_every == 0, pruned (frontier only) otherwise; a pruned sweep that
+ * would end the phase is followed by a full sweep, and only a full sweep
+ * ends it (termination rule of modularity_optimisation.cu:284-363). */
+lv_status lv_run_pruned(lv_graph *g, float min_gain, const lv_frontier *fr, lv_stats *stats);
+/* One pruned phase (as lv_optimise). */
+lv_status lv_optimise_pruned(lv_graph *g, float min_gain, const lv_frontier *fr,
+                             int32_t *changed, int32_t *sweeps, int64_t *evaluated);
+/* One sweep over the frontier fr selects (LV_FRONTIER_ALL: a plain sweep);
+ * track = 1 also counts neighbour moves for the next sweep's features. */
+lv_status lv_sweep_pruned(lv_graph *g, const lv_frontier *fr, int32_t track, double *q_after,
+                          int64_t *moved, int64_t *evaluated);
+/* One plain phase (as lv_optimise) that records, for up to max_sweeps
+ * sweeps, the features before the sweep (feat[s][V][8]) and whether each
+ * vertex moved in it (moved[s][V], 0/1). */
+lv_status lv_trace_phase(lv_graph *g, float min_gain, float *feat, uint8_t *moved,
+                         int32_t max_sweeps, int32_t *traced, int32_t *changed);
+/* Features of the current level and state, feat[V][8]. */
+lv_status lv_features(lv_graph *g, float *feat);
+/* Raw frontier observations of the current level (any may be null):
+ * margin[V], kown[V] (last evaluation), hist[V] (bit 0 = moved in the last
+ * sweep), nmv[V] (row entries whose neighbour moved in the last sweep). */
+lv_status lv_move_state(lv_graph *g, float *margin, float *kown, uint8_t *hist, int32_t *nmv);
+/* Frontier model of the handle (LV_MLP_PARAMS floats). */
+lv_status lv_set_model(lv_graph *g, const float *params);
+/* Model output for the current level: prob[V] and active[V] (p > threshold
+ * and degree > 0); either may be null. */
+lv_status lv_predict(lv_graph *g, float threshold, float *prob, uint8_t *active);
+/* Standalone model kernels on `device`: MFMA inference prob[n] from feat[n][8];
+ * mean gradient and loss of the weighted BCE; Adam training (params in/out,
+ * loss_per_epoch[epochs] may be null). */
+lv_status lv_mlp_forward(int32_t device, const float *params, const float *feat, int64_t n,
+                         float *prob);
+lv_status lv_mlp_grad(int32_t device, const float *params, const float *feat, const float *labels,
+                      int64_t n, float pos_weight, float *grad, float *loss);
+lv_status lv_mlp_train(int32_t device, const float *feat, const float *labels, int64_t n,
+                       const lv_train_options *opt, float *params, float *loss_per_epoch);
 
 /* ---- host-side input (utils.cu:9-62 readInputData) and synthetic graphs ---- */
 /* MatrixMarket coordinate reader with the reference's conventions (skip '%'

@@ -221,7 +221,7 @@ static void scratch_free(orc_scratch *s) {
  *   move iff bestGain - K[cur]/M > 0                            (:192-196) */
 static int32_t decide_vertex(const orc_csr *g, float M32, int32_t v, const int32_t *comm,
                              const int32_t *size, const double *k, const double *sigma,
-                             orc_scratch *s) {
+                             orc_scratch *s, float *margin, float *kown) {
     const int32_t cur = comm[v];
     int32_t nt = 0;
     for (int32_t e = g->rowptr[v]; e < g->rowptr[v + 1]; ++e) {
@@ -255,6 +255,8 @@ static int32_t decide_vertex(const orc_csr *g, float M32, int32_t v, const int32
     }
     volatile float t = (float)kcur64 / M32;
     volatile float d = best_gain - t;
+    if (margin) margin[v] = d;
+    if (kown) kown[v] = (float)kcur64;
     return d > 0.0f ? best : cur;
 }
 
@@ -270,7 +272,8 @@ static int nthreads_eff(int nthreads) {
 /* Decide for a list of vertices; parallel over the list (Jacobi within a group). */
 static void decide_list(const orc_csr *g, float M32, const int32_t *list, int32_t n,
                         const int32_t *comm, const int32_t *size, const double *k,
-                        const double *sigma, int32_t *new_comm, orc_scratch *scr, int nthr) {
+                        const double *sigma, int32_t *new_comm, orc_scratch *scr, int nthr,
+                        const uint8_t *active, float *margin, float *kown) {
 #ifdef _OPENMP
     if (nthr > 1 && n > 256) {
 #pragma omp parallel num_threads(nthr)
@@ -278,7 +281,10 @@ static void decide_list(const orc_csr *g, float M32, const int32_t *list, int32_
             orc_scratch *s = &scr[omp_get_thread_num()];
 #pragma omp for schedule(dynamic, 256)
             for (int32_t i = 0; i < n; ++i)
-                new_comm[list[i]] = decide_vertex(g, M32, list[i], comm, size, k, sigma, s);
+                new_comm[list[i]] = (active && !active[list[i]])
+                                        ? comm[list[i]]
+                                        : decide_vertex(g, M32, list[i], comm, size, k, sigma, s,
+                                                        margin, kown);
         }
         return;
     }
@@ -286,7 +292,10 @@ static void decide_list(const orc_csr *g, float M32, const int32_t *list, int32_
     (void)nthr;
 #endif
     for (int32_t i = 0; i < n; ++i)
-        new_comm[list[i]] = decide_vertex(g, M32, list[i], comm, size, k, sigma, &scr[0]);
+        new_comm[list[i]] = (active && !active[list[i]])
+                                ? comm[list[i]]
+                                : decide_vertex(g, M32, list[i], comm, size, k, sigma, &scr[0],
+                                                margin, kown);
 }
 
 /* Vertex lists per degree group, ascending vertex id. */
@@ -314,6 +323,11 @@ static void build_groups(const orc_csr *g, orc_groups *gr) {
 
 void orc_decide_group(const orc_csr *g, double M, const int32_t *comm, int group,
                       int32_t *new_comm, int nthreads) {
+    orc_decide_group_obs(g, M, comm, group, new_comm, NULL, NULL, nthreads);
+}
+
+void orc_decide_group_obs(const orc_csr *g, double M, const int32_t *comm, int group,
+                          int32_t *new_comm, float *margin, float *kown, int nthreads) {
     const int32_t V = g->V;
     int nthr = nthreads_eff(nthreads);
     double *k = malloc(sizeof(double) * ((size_t)V + 1));
@@ -331,7 +345,7 @@ void orc_decide_group(const orc_csr *g, double M, const int32_t *comm, int group
     int lo = group < 0 ? 0 : group, hi = group < 0 ? ORC_NGROUPS - 1 : group;
     for (int b = lo; b <= hi; ++b)
         decide_list(g, M32, gr.ids + gr.off[b], gr.off[b + 1] - gr.off[b], comm, size, k, sigma,
-                    new_comm, scr, nthr);
+                    new_comm, scr, nthr, NULL, margin, kown);
     for (int t = 0; t < nthr; ++t) scratch_free(&scr[t]);
     free(scr);
     free(gr.ids);
@@ -366,13 +380,15 @@ double orc_modularity(const orc_csr *g, double M, const int32_t *comm) {
  * (:237-249) and rebuilds the community totals (:328-330). */
 static int64_t sweep_impl(const orc_csr *g, float M32, int32_t *comm, int32_t *size,
                           const double *k, double *sigma, int32_t *new_comm,
-                          const orc_groups *gr, orc_scratch *scr, int nthr) {
+                          const orc_groups *gr, orc_scratch *scr, int nthr,
+                          const uint8_t *active) {
     int64_t moved = 0;
     for (int b = 0; b < ORC_NGROUPS; ++b) {
         const int32_t *list = gr->ids + gr->off[b];
         const int32_t n = gr->off[b + 1] - gr->off[b];
         if (n == 0) continue;
-        decide_list(g, M32, list, n, comm, size, k, sigma, new_comm, scr, nthr);
+        decide_list(g, M32, list, n, comm, size, k, sigma, new_comm, scr, nthr, active, NULL,
+                    NULL);
         for (int32_t i = 0; i < n; ++i) {
             int32_t v = list[i];
             if (new_comm[v] != comm[v]) {
@@ -388,6 +404,11 @@ static int64_t sweep_impl(const orc_csr *g, float M32, int32_t *comm, int32_t *s
 }
 
 int64_t orc_sweep(const orc_csr *g, double M, int32_t *comm, int nthreads) {
+    return orc_sweep_masked(g, M, comm, NULL, nthreads);
+}
+
+int64_t orc_sweep_masked(const orc_csr *g, double M, int32_t *comm, const uint8_t *active,
+                         int nthreads) {
     const int32_t V = g->V;
     int nthr = nthreads_eff(nthreads);
     double *k = malloc(sizeof(double) * ((size_t)V + 1));
@@ -401,7 +422,7 @@ int64_t orc_sweep(const orc_csr *g, double M, int32_t *comm, int nthreads) {
     for (int t = 0; t < nthr; ++t) scratch_init(&scr[t], V);
     orc_groups gr;
     build_groups(g, &gr);
-    int64_t moved = sweep_impl(g, (float)M, comm, size, k, sigma, nc, &gr, scr, nthr);
+    int64_t moved = sweep_impl(g, (float)M, comm, size, k, sigma, nc, &gr, scr, nthr, active);
     for (int t = 0; t < nthr; ++t) scratch_free(&scr[t]);
     free(scr);
     free(gr.ids);
@@ -437,7 +458,7 @@ int orc_phase(const orc_csr *g, double M, int32_t *comm, float min_gain,
     int32_t sweeps = 0;
     double q0 = orc_modularity(g, M, comm);
     for (;;) {
-        int64_t moved = sweep_impl(g, M32, comm, size, k, sigma, nc, &gr, scr, nthr);
+        int64_t moved = sweep_impl(g, M32, comm, size, k, sigma, nc, &gr, scr, nthr, NULL);
         sweeps++;
         double q1 = orc_modularity(g, M, comm);
         double gain = q1 - q0;

@@ -65,6 +65,11 @@ void orc_sizes(int32_t V, const int32_t *comm, int32_t *size);
  * new_comm[v] for those vertices and comm[v] for all others. */
 void orc_decide_group(const orc_csr *g, double M, const int32_t *comm, int group,
                       int32_t *new_comm, int nthreads);
+/* As orc_decide_group, also recording for each decided vertex the move
+ * margin (bestGain - K[cur]/M in fp32, bestGain starting at 0 as in
+ * computeMove) and K[cur] as fp32 (margin/kown may be NULL). */
+void orc_decide_group_obs(const orc_csr *g, double M, const int32_t *comm, int group,
+                          int32_t *new_comm, float *margin, float *kown, int nthreads);
 
 /* Gain of moving v to community c (fp32, reference operation order). */
 float orc_gain(float K_c, float M32, float k_v, float sigma_cur, float sigma_c);
@@ -76,6 +81,10 @@ double orc_modularity(const orc_csr *g, double M, const int32_t *comm);
 /* One sweep over groups 0..6 in place; returns the number of moved
  * vertices.  size/sigma/k are maintained for the state. */
 int64_t orc_sweep(const orc_csr *g, double M, int32_t *comm, int nthreads);
+/* A sweep in which only vertices with active[v] != 0 are decided (the
+ * others keep their community); active == NULL decides every vertex. */
+int64_t orc_sweep_masked(const orc_csr *g, double M, int32_t *comm, const uint8_t *active,
+                         int nthreads);
 
 /* Runs sweeps while gain >= min_gain (and at least one vertex moved).
  * Updates orig2comm[x] = comm[orig2comm[x]] at the end and returns 1 if

@@ -56,6 +56,9 @@ def load() -> ctypes.CDLL:
         "orc_gain": (ctypes.c_float, [ctypes.c_float] * 5),
         "orc_modularity": (ctypes.c_double, [P, ctypes.c_double, I32P]),
         "orc_sweep": (ctypes.c_int64, [P, ctypes.c_double, I32P, ctypes.c_int]),
+        "orc_decide_group_obs": (None, [P, ctypes.c_double, I32P, ctypes.c_int, I32P, F32P, F32P, ctypes.c_int]),
+        "orc_sweep_masked": (ctypes.c_int64, [P, ctypes.c_double, I32P, ctypes.POINTER(ctypes.c_uint8),
+                                              ctypes.c_int]),
         "orc_phase": (ctypes.c_int, [P, ctypes.c_double, I32P, ctypes.c_float, I32P, ctypes.c_int32, I32P,
                                      ctypes.c_int]),
         "orc_aggregate": (ctypes.c_int, [P, I32P, P, I32P, ctypes.c_int32, I32P]),
@@ -136,6 +139,29 @@ def decide(g: Graph, comm: np.ndarray, group: int, M: float | None = None, threa
     out = np.zeros(max(g.V, 1), np.int32)
     lib.orc_decide_group(g.ref, M, _p(comm, ctypes.c_int32), group, _p(out, ctypes.c_int32), threads)
     return out[: g.V]
+
+
+def decide_obs(g: Graph, comm: np.ndarray, group: int, M: float | None = None, threads: int = 1):
+    """decide() plus the margin and fp32 K[cur] of each decided vertex (NaN elsewhere)."""
+    lib = load()
+    M = total_weight(g) if M is None else M
+    comm = np.ascontiguousarray(comm, dtype=np.int32)
+    out = np.zeros(max(g.V, 1), np.int32)
+    margin = np.full(max(g.V, 1), np.nan, np.float32)
+    kown = np.full(max(g.V, 1), np.nan, np.float32)
+    lib.orc_decide_group_obs(g.ref, M, _p(comm, ctypes.c_int32), group, _p(out, ctypes.c_int32),
+                             _p(margin, ctypes.c_float), _p(kown, ctypes.c_float), threads)
+    return out[: g.V], margin[: g.V], kown[: g.V]
+
+
+def sweep_masked(g: Graph, comm: np.ndarray, active: np.ndarray, M: float | None = None,
+                 threads: int = 1) -> int:
+    """One sweep in place deciding only vertices with active != 0."""
+    M = total_weight(g) if M is None else M
+    assert comm.dtype == np.int32 and comm.flags.c_contiguous
+    active = np.ascontiguousarray(active, dtype=np.uint8)
+    return int(load().orc_sweep_masked(g.ref, M, _p(comm, ctypes.c_int32), _p(active, ctypes.c_uint8),
+                                       threads))
 
 
 def modularity(g: Graph, comm: np.ndarray, M: float | None = None) -> float:
