@@ -48,6 +48,89 @@ def cpu_baseline(csr, threads):
                       f"from singletons, {moved} moves, {dt:.2f} s"}
 
 
+def calibrate_copy():
+    """A plain device-to-device copy of 2^28 fp32 (1 GiB read, 1 GiB written),
+    three times, for scaling the FETCH_SIZE / WRITE_SIZE counters."""
+    import torch
+
+    n = 1 << 28
+    x = torch.ones(n, dtype=torch.float32, device="cuda")
+    y = torch.empty_like(x)
+    for _ in range(3):
+        y.copy_(x)
+    torch.cuda.synchronize()
+    print(json.dumps({"copy_bytes_read": 4 * n, "copy_bytes_written": 4 * n, "launches": 3}))
+
+
+def pruned_comparison(L, csr, min_gain=1e-6):
+    """Learned-pruned vs plain on the same graph: the model is trained on the
+    device from plain-run traces of 8 seeded RMAT-18 graphs (seeds 1801-1808),
+    its threshold set for 97% recall of movers on a held-out RMAT-18 (seed
+    1809); training is reported apart from the run times."""
+    import numpy as np
+
+    import frontier as F
+
+    t0 = time.perf_counter()
+    train = [L.gen_rmat(18, 16, 0.57, 0.19, 0.19, seed=s) for s in range(1801, 1809)]
+    X, y, tinfo = F.collect_traces(train, min_gain=min_gain)
+    params, losses, info = F.train(X, y)
+    val = L.gen_rmat(18, 16, 0.57, 0.19, 0.19, seed=1809)
+    Xv, yv, _ = F.collect_traces([val], min_gain=min_gain, per_sweep=1 << 30, seed=1)
+    thr, frac = F.recall_threshold(params, Xv, yv, recall=0.97)
+    train_s = time.perf_counter() - t0
+    log(f"model trained in {train_s:.1f}s: {tinfo['samples']} samples, loss {info['loss_first']:.4f} -> "
+        f"{info['loss_last']:.4f}, threshold {thr:.3f} selects {frac:.3f}")
+    out = {"training": {"graphs": "RMAT-18 seeds 1801-1808", "samples": tinfo["samples"],
+                        "positives": tinfo["positives"], "trace_s": tinfo["seconds"], "train_s": info["seconds"],
+                        "epochs": info["epochs"], "loss_first": info["loss_first"], "loss_last": info["loss_last"],
+                        "threshold": thr, "val_selected": frac, "total_s": train_s}}
+    with L.Graph(csr) as G:
+        G.set_model(params)
+        G.run(min_gain)   # warm
+        runs = {"plain": None,
+                "learned": L.frontier(kind=L.FRONTIER_MODEL, threshold=thr, full_every=4),
+                "neighbor": L.frontier(kind=L.FRONTIER_NEIGHBOR, full_every=4),
+                "random": L.frontier(kind=L.FRONTIER_RANDOM, random_p=min(1.0, frac), seed=7, full_every=4)}
+        res = {}
+        for name, fr in runs.items():
+            G.reset()
+            st = G.run(min_gain) if fr is None else G.run_pruned(min_gain, fr)
+            res[name] = {"modularity": st["modularity"], "time_ms": st["algo_ms"], "sweeps": st["sweeps"],
+                         "full_sweeps": st["full_sweeps"], "evaluated": st["evaluated"],
+                         "levels": st["levels"], "sweep_ms": st["sweep_ms"], "aggregate_ms": st["aggregate_ms"]}
+            log(f"{name}: Q={st['modularity']:.6f} {st['algo_ms']:.1f} ms, {st['sweeps']} sweeps "
+                f"({st['full_sweeps']} full), {st['evaluated']} evaluations")
+        qp, tp, ep = res["plain"]["modularity"], res["plain"]["time_ms"], res["plain"]["evaluated"]
+        for name in ("learned", "neighbor", "random"):
+            r = res[name]
+            r["dQ_rel"] = (qp - r["modularity"]) / qp
+            r["time_ratio"] = r["time_ms"] / tp
+            r["evaluated_ratio"] = r["evaluated"] / max(ep, 1)
+        out.update(res)
+        out["target"] = {"dQ_rel_max": 0.005, "time_ratio_max": 0.5,
+                         "met": bool(res["learned"]["dQ_rel"] <= 0.005 and res["learned"]["time_ratio"] <= 0.5)}
+    return out
+
+
+def traffic_for(kernel, scale):
+    """Measured HBM bytes per launch of `kernel` on RMAT-`scale` from the
+    committed rocprofv3 PMC summary (FETCH_SIZE + WRITE_SIZE, copy-calibrated),
+    or None when no summary covers it."""
+    path = os.path.join(ROOT, "profiles", f"pmc_traffic_rmat{scale}.md")
+    if not os.path.exists(path):
+        return None
+    for line in open(path):
+        # table rows: | kernel | fetch bytes | write bytes | traffic bytes | ...
+        cells = [c.strip() for c in line.strip().strip("|").split("|")]
+        if len(cells) >= 4 and cells[0] == f"`{kernel}`":
+            try:
+                return float(cells[3])
+            except ValueError:
+                return None
+    return None
+
+
 def csr_scale(csr):
     return int(round(__import__("math").log2(max(csr.V, 1))))
 
@@ -62,7 +145,13 @@ def main():
     ap.add_argument("--cpu-threads", type=int, default=1)
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--no-converge", action="store_true")
+    ap.add_argument("--no-pruned", action="store_true")
+    ap.add_argument("--calibrate-copy", action="store_true",
+                    help="run only a plain 1 GiB device copy (PMC byte-counter calibration)")
     args = ap.parse_args()
+    if args.calibrate_copy:
+        calibrate_copy()
+        return
     if args.gpus != 1:
         print(f"bench.py: this build runs on one GPU only (asked for {args.gpus}); "
               f"multi-GPU sharding is a design note, not built", file=sys.stderr)
@@ -100,14 +189,29 @@ def main():
         kernels[name] = {"launches": kt["launches"], "avg_ms": avg_ms,
                          "ms_per_sweep": kt["total_ms"] / args.steps,
                          "gbs": kt["bytes"] / (kt["total_ms"] * 1e-3) / 1e9 if kt["total_ms"] > 0 else None}
-    dom = max(kernels, key=lambda k: kernels[k]["ms_per_sweep"]) if kernels else None
+    details = {}
+    for kid, name in enumerate(L.DETAIL_NAMES):
+        kt = G.kernel_timing_detail(kid)
+        if kt["launches"] == 0:
+            continue
+        details[name] = {"kernel": L.DETAIL_KERNELS[kid], "launches": kt["launches"],
+                         "avg_ms": kt["total_ms"] / kt["launches"],
+                         "ms_per_sweep": kt["total_ms"] / args.steps,
+                         "bytes_per_launch": kt["bytes"] / kt["launches"],
+                         "gbs": kt["bytes"] / (kt["total_ms"] * 1e-3) / 1e9 if kt["total_ms"] > 0 else None}
+    # roofline of the single kernel with the most time among those whose
+    # algorithmic bytes are counted
+    counted = {k: v for k, v in details.items() if v["bytes_per_launch"] > 0}
     roofline = None
-    if dom:
-        kt = G.kernel_timing(L.KERNEL_NAMES.index(dom))
-        achieved = kt["bytes"] / (kt["total_ms"] * 1e-3) / 1e9
-        roofline = {"kernel": dom, "bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS,
-                    "unit": "GB/s", "frac": achieved / HBM_PEAK_GBS, "traffic": None,
-                    "bytes_per_launch": kt["bytes"] / kt["launches"], "avg_launch_ms": kt["total_ms"] / kt["launches"]}
+    if counted:
+        dom = max(counted, key=lambda k: counted[k]["ms_per_sweep"])
+        d = counted[dom]
+        achieved = d["bytes_per_launch"] / (d["avg_ms"] * 1e-3) / 1e9
+        roofline = {"kernel": d["kernel"], "bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS,
+                    "unit": "GB/s", "frac": achieved / HBM_PEAK_GBS,
+                    "traffic": traffic_for(d["kernel"], args.scale),
+                    "bytes_per_launch": d["bytes_per_launch"], "avg_launch_ms": d["avg_ms"],
+                    "share_of_sweep": d["ms_per_sweep"] / ms_per_step}
 
     converge = None
     if not args.no_converge:
@@ -122,6 +226,10 @@ def main():
         log(f"converged: Q={st['modularity']:.6f} in {st['algo_ms']:.1f} ms, {st['levels']} levels, "
             f"{st['sweeps']} sweeps")
     G.close()
+
+    pruned = None
+    if not args.no_pruned:
+        pruned = pruned_comparison(L, csr)
 
     cpu = None
     if not args.no_cpu_baseline:
@@ -148,8 +256,9 @@ def main():
         "roofline": roofline,
         "cpu_baseline": cpu,
         "kernels": kernels,
+        "kernel_detail": details,
         "converge": converge,
-        "pruned": None,
+        "pruned": pruned,
     }
     print(json.dumps(out))
 

@@ -271,12 +271,14 @@ struct lv_graph {
     // timing
     struct Rec {
         hipEvent_t a, b;
-        int kid;
+        int kid;       // LV_K_* (or -1)
+        int kd;        // LV_KD_* (or -1)
         double bytes;
     };
     std::vector<hipEvent_t> ev_pool;
     std::vector<Rec> pending;
     lv_kernel_time ktime[LV_K_COUNT]{};
+    lv_kernel_time ktd[LV_KD_COUNT]{};
     double memory_ms = 0.0;
 
     // ---------------------------------------------------------------- util
@@ -310,20 +312,29 @@ struct lv_graph {
         if (e) (void)hipEventRecord(e, stream);
         return e;
     }
-    void tend(hipEvent_t a, int kid, double bytes) {
+    // Closes a timed region opened by tbegin: counted under kid (coarse
+    // family, -1 = none) and kd (single kernel, -1 = none).
+    void tend(hipEvent_t a, int kid, double bytes, int kd = -1) {
         if (!opt.timing || !a) return;
         hipEvent_t b = take_event();
         if (!b) return;
         (void)hipEventRecord(b, stream);
-        pending.push_back({a, b, kid, bytes});
+        pending.push_back({a, b, kid, kd, bytes});
     }
     lv_status flush_timers() {
         for (auto &r : pending) {
             float ms = 0.0f;
             LV_HIP(hipEventElapsedTime(&ms, r.a, r.b));
-            ktime[r.kid].launches++;
-            ktime[r.kid].total_ms += ms;
-            ktime[r.kid].bytes += r.bytes;
+            if (r.kid >= 0) {
+                ktime[r.kid].launches++;
+                ktime[r.kid].total_ms += ms;
+                ktime[r.kid].bytes += r.bytes;
+            }
+            if (r.kd >= 0) {
+                ktd[r.kd].launches++;
+                ktd[r.kd].total_ms += ms;
+                ktd[r.kd].bytes += r.bytes;
+            }
             ev_pool.push_back(r.a);
             ev_pool.push_back(r.b);
         }
@@ -757,43 +768,61 @@ struct lv_graph {
             case 0: case 1: case 2: case 3:
                 t0 = tbegin();
                 lv::launch_move_sub(4 << b, a, list, n, s);
-                tend(t0, LV_K_MOVE_SUB, bytes);
+                tend(t0, LV_K_MOVE_SUB, bytes, LV_KD_MOVE_BUCKET0 + b);
                 break;
             case 4: case 5:
                 t0 = tbegin();
                 lv::launch_move_wave(b == 4 ? 256 : 512, a, list, n, s);
-                tend(t0, LV_K_MOVE_WAVE, bytes);
+                tend(t0, LV_K_MOVE_WAVE, bytes, LV_KD_MOVE_BUCKET0 + b);
                 break;
             case 6:
                 t0 = tbegin();
                 lv::launch_move_wg(256, wg6_lg, a, list, n, s);
-                tend(t0, LV_K_MOVE_WG, bytes);
+                tend(t0, LV_K_MOVE_WG, bytes, LV_KD_MOVE_BUCKET0 + b);
                 break;
-            default:
+            default: {
+                // chunked tier: the family and each of its three kernels;
+                // stage A streams the heavy rows (col, w, comm[col])
+                const lv::ChunkedTier t = chunked_tier();
                 t0 = tbegin();
-                lv::launch_move_chunked(a, chunked_tier(), nins, nitems, order + h_off[b], nh, s);
+                hipEvent_t ta = tbegin();
+                lv::launch_chunk_aggregate(a, t, nins, s);
+                tend(ta, -1, 12.0 * (double)h_bedges[b] * ((double)n / (double)nfull),
+                     LV_KD_CHUNK_AGGREGATE);
+                hipEvent_t tb = tbegin();
+                lv::launch_chunk_merge(a, t, nitems, s);
+                tend(tb, -1, 0.0, LV_KD_CHUNK_MERGE);
+                hipEvent_t tc = tbegin();
+                lv::launch_heavy_final(a, t, nh, s);
+                tend(tc, -1, 0.0, LV_KD_CHUNK_FINAL);
                 tend(t0, LV_K_MOVE_GLOBAL, bytes);
                 break;
+            }
             }
         }
         LV_LAUNCHED("move kernels");
         return LV_OK;
     }
 
-    void group_range(int g, int32_t &lo, int32_t &hi) const {
+    // Range of group g's buckets in a bucket-ordered list with offsets off.
+    void group_range(int g, int32_t &lo, int32_t &hi, const int32_t *off = nullptr) const {
+        if (!off) off = h_off;
         lo = -1;
         hi = -1;
         for (int b = 0; b < lv::kNumBuckets; ++b) {
             if (lv::kBucketGroup[b] != g) continue;
-            if (lo < 0) lo = h_off[b];
-            hi = h_off[b + 1];
+            if (lo < 0) lo = off[b];
+            hi = off[b + 1];
         }
     }
 
     lv_status enqueue_modularity() {
         hipEvent_t t0 = tbegin();
-        lv::launch_modularity(rowptr(), col(), w(), comm, sigma, V, order, st, ins.p, M, qpart.p, d_q,
-                              stream);
+        hipEvent_t tr = tbegin();
+        lv::launch_modularity_rows(rowptr(), col(), w(), comm, sigma, V, order, st, ins.p, qpart.p,
+                                   stream);
+        tend(tr, -1, 12.0 * (double)E + 16.0 * (double)V, LV_KD_MODULARITY_ROWS);
+        lv::launch_modularity_finish(st, M, qpart.p, d_q, stream);
         tend(t0, LV_K_MODULARITY, 12.0 * (double)E + 16.0 * (double)V);
         LV_LAUNCHED("modularity");
         own_valid = true;
@@ -818,7 +847,11 @@ struct lv_graph {
         const lv::FeatureArgs fa = feature_args();
         if (fr.kind == LV_FRONTIER_MODEL) {
             if (!mlp_set) return fail(LV_ERR_STATE, "frontier model not set (lv_set_model)");
+            hipEvent_t t0 = tbegin();
             lv::launch_frontier(fa, mlp, fr.threshold, bb, fkeys, active, nullptr, s);
+            // per vertex: rowptr 8, k 8, comm 4, record 8, margin 4, kown 4,
+            // hist 1, nmv 4 read; keys 1, active 1 written
+            tend(t0, -1, 43.0 * (double)V, LV_KD_FRONTIER);
         } else {
             lv::launch_frontier_rule(fr.kind, fa, fr.random_p, fr.seed + 0x9e37ull * sweep_count,
                                      bb, fkeys, active, s);
@@ -865,9 +898,13 @@ struct lv_graph {
             LV_TRY(run_group(g, pruned));
             hipEvent_t t0 = tbegin();
             const lv::GroupMarks gm = marks();
-            lv::launch_apply(a, order, lo, hi, &gst[g], ins.p, comm, size, sigma, crec,
+            // pruned: only frontier vertices can move
+            int32_t alo = lo, ahi = hi;
+            if (pruned) group_range(g, alo, ahi, h_foff);
+            const int32_t *alist = (pruned ? forder : order) + alo;
+            lv::launch_apply(a, alist, ahi - alo, order, &gst[g], ins.p, comm, size, sigma, crec,
                              incremental ? &gm : nullptr, hist, d_moved, d_q + 1, s);
-            tend(t0, LV_K_APPLY, 16.0 * (double)(hi - lo));
+            tend(t0, LV_K_APPLY, 16.0 * (double)(ahi - alo));
             LV_LAUNCHED("apply");
             if (incremental) step++;
         }
@@ -951,10 +988,18 @@ struct lv_graph {
         int32_t ch = 0, sweeps = 0, fulls = 0;
         int64_t evaluated = 0, edges = 0;
         double ms_total = 0.0;
+        // Every sweep is judged by the reference's rule against the sweep
+        // before it.  A pruned sweep that would end the phase is followed by
+        // a full sweep (the convergence check), judged against the state
+        // before that pruned sweep, so that a full sweep undoing a pruned
+        // one cannot keep the phase alive.  Without a frontier every sweep is
+        // full and this is exactly the reference's loop.
+        double q_prev = q0, q_before_pruned = q0;
         bool confirm = false;
         for (int64_t i = 0;; ++i) {
             const bool full = fr == nullptr || confirm || i % k_full == 0 || i >= kMaxPrunedSweeps;
-            const bool track = trace != nullptr || (fr != nullptr && (i + 1) % k_full != 0);
+            const bool next_may_prune = fr != nullptr && (i + 1) % k_full != 0;
+            const bool track = trace != nullptr || next_may_prune;
             if (trace && trace->n < trace->cap) LV_TRY(trace_features(trace));
             double q1 = 0.0, ms = 0.0;
             int64_t mv = 0, ev = 0;
@@ -968,18 +1013,15 @@ struct lv_graph {
             evaluated += ev;
             edges += full ? E : 0;
             ms_total += ms;
-            const double gain = q1 - q0;
-            if (gain > 0.0) ch = 1;
+            if (q1 - q_prev > 0.0) ch = 1;
             // A sweep without positive gain also ends the phase (identical
             // for min_gain > 0; guarantees termination for min_gain <= 0).
+            const double gain = q1 - (confirm ? q_before_pruned : q_prev);
             const bool stop = mv == 0 || !(gain > 0.0) || !(gain >= (double)min_gain);
-            q0 = q1;
-            if (stop) {
-                if (full) break;
-                confirm = true;
-                continue;
-            }
-            confirm = false;
+            if (stop && full) break;
+            confirm = stop;  // a pruned sweep wants to stop: check with a full sweep
+            if (stop) q_before_pruned = q_prev;
+            q_prev = q1;
         }
         lv::launch_compose(orig2comm, V0, comm, stream);
         LV_LAUNCHED("compose");
@@ -1532,6 +1574,14 @@ lv_status lv_kernel_timing_reset(lv_graph *g) {
     if (!g) return fail(LV_ERR_ARG, "null graph");
     LV_TRY(g->sync());
     for (auto &t : g->ktime) t = lv_kernel_time{};
+    for (auto &t : g->ktd) t = lv_kernel_time{};
+    return LV_OK;
+}
+
+lv_status lv_kernel_timing_detail(lv_graph *g, int32_t kernel, lv_kernel_time *out) {
+    if (!g || !out || kernel < 0 || kernel >= LV_KD_COUNT) return fail(LV_ERR_ARG, "bad argument");
+    LV_TRY(g->sync());
+    *out = g->ktd[kernel];
     return LV_OK;
 }
 

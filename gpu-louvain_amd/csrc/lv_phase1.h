@@ -99,6 +99,11 @@ void launch_move_wg(int threads, int cap_log2, const MoveArgs &a, const int32_t 
 void set_wg_lds_limits(int bytes);
 void launch_move_chunked(const MoveArgs &a, const ChunkedTier &t, int32_t nchunks,
                          int32_t nitems, const int32_t *heavy_list, int32_t nh, hipStream_t s);
+// The three kernels of launch_move_chunked, for callers that time them apart.
+void launch_chunk_aggregate(const MoveArgs &a, const ChunkedTier &t, int32_t nchunks,
+                            hipStream_t s);
+void launch_chunk_merge(const MoveArgs &a, const ChunkedTier &t, int32_t nitems, hipStream_t s);
+void launch_heavy_final(const MoveArgs &a, const ChunkedTier &t, int32_t nh, hipStream_t s);
 void launch_clear_tables(int32_t *keys, double *vals, int64_t n, hipStream_t s);
 
 // Bitmaps of the movers of a group step (two, alternating by step, so that
@@ -110,13 +115,13 @@ struct GroupMarks {
 };
 
 // --- apply ---
-// Applies the decisions of the group's vertices order[lo..hi) (newc[v] then
-// holds the community v left) and refreshes the records of the touched
-// communities.  With gm (incremental modularity) the movers are marked and
-// the group's change of the weight inside communities is added to *own
-// (gst: the group's row segments).
+// Applies the decisions of the vertices list[0..n) (a superset of the
+// group's movers; newc[v] then holds the community v left) and refreshes the
+// records of the touched communities.  With gm (incremental modularity) the
+// movers are marked and the group's change of the weight inside communities
+// is added to *own (gst: the group's row segments over order).
 // hist[v] gets bit 0 set for a mover (the move history of the frontier model).
-void launch_apply(const MoveArgs &a, const int32_t *order, int32_t lo, int32_t hi,
+void launch_apply(const MoveArgs &a, const int32_t *list, int32_t n, const int32_t *order,
                   const SegTable *gst, const Chunk *chunks, int32_t *comm, int32_t *size,
                   double *sigma, uint64_t *crec, const GroupMarks *gm, uint8_t *hist,
                   unsigned long long *moved, double *own, hipStream_t s);
@@ -127,6 +132,15 @@ void launch_modularity(const int32_t *rowptr, const int32_t *col, const float *w
                        const int32_t *comm, const double *sigma, int32_t V, const int32_t *order,
                        const SegTable &st, const Chunk *chunks, double M, double *partials,
                        double *q, hipStream_t s);
+
+// The two halves of launch_modularity: the row/square pass (per-block
+// partials) and the fixed-order reduction into q.
+void launch_modularity_rows(const int32_t *rowptr, const int32_t *col, const float *w,
+                            const int32_t *comm, const double *sigma, int32_t V,
+                            const int32_t *order, const SegTable &st, const Chunk *chunks,
+                            double *partials, hipStream_t s);
+void launch_modularity_finish(const SegTable &st, double M, double *partials, double *q,
+                              hipStream_t s);
 
 // Incremental: one pass for sum of S_c^2; q[0] = Q from q[1] (the weight
 // inside communities, kept current by the applies), q[2] = sq.

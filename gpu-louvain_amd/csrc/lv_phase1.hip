@@ -661,7 +661,8 @@ constexpr int kClearWords = 4096;
 // rows (incremental sweeps only); [fix_start, clear_start): records of the
 // touched communities; [clear_start, ...): clear the other step's bitmap.
 __global__ __launch_bounds__(kRowThreads) void k_apply_finish(
-    MoveArgs a, const int32_t *__restrict__ order, int32_t lo, int32_t n, SegTable st,
+    MoveArgs a, const int32_t *__restrict__ order, const int32_t *__restrict__ list, int32_t n,
+    SegTable st,
     const Chunk *__restrict__ chunks, const int32_t *__restrict__ comm,
     const int32_t *__restrict__ size, const double *__restrict__ sigma, uint64_t *crec,
     const uint32_t *__restrict__ movers, uint32_t *clear_bits, int32_t words, double *own,
@@ -679,7 +680,7 @@ __global__ __launch_bounds__(kRowThreads) void k_apply_finish(
         bool mv = false;
         int32_t oc = 0, nc = 0;
         if (i < n) {
-            const int32_t v = order[lo + i];
+            const int32_t v = list[i];
             oc = a.newc[v];
             nc = comm[v];
             mv = oc != nc;
@@ -1006,12 +1007,29 @@ void launch_move_wg(int threads, int cap_log2, const MoveArgs &a, const int32_t 
     hipLaunchKernelGGL(k_move_wg<256>, dim3(g), dim3(256), lds, s, a, list, n, cap_log2);
 }
 
+void launch_chunk_aggregate(const MoveArgs &a, const ChunkedTier &t, int32_t nchunks,
+                            hipStream_t s) {
+    if (nchunks <= 0) return;
+    hipLaunchKernelGGL(k_chunk_aggregate, dim3(nchunks), dim3(256), 0, s, a, t);
+}
+
+void launch_chunk_merge(const MoveArgs &a, const ChunkedTier &t, int32_t nitems, hipStream_t s) {
+    if (nitems <= 0) return;
+    hipLaunchKernelGGL(k_chunk_merge, dim3(nitems), dim3(kMergeThreads), 0, s, a, t);
+}
+
+void launch_heavy_final(const MoveArgs &a, const ChunkedTier &t, int32_t nh, hipStream_t s) {
+    if (nh <= 0) return;
+    hipLaunchKernelGGL(k_heavy_final, dim3(blocks_for(nh, 256)), dim3(256), 0, s, a, nh, t);
+}
+
 void launch_move_chunked(const MoveArgs &a, const ChunkedTier &t, int32_t nchunks,
                          int32_t nitems, const int32_t *heavy_list, int32_t nh, hipStream_t s) {
+    (void)heavy_list;
     if (nh <= 0) return;
-    hipLaunchKernelGGL(k_chunk_aggregate, dim3(nchunks), dim3(256), 0, s, a, t);
-    hipLaunchKernelGGL(k_chunk_merge, dim3(nitems), dim3(kMergeThreads), 0, s, a, t);
-    hipLaunchKernelGGL(k_heavy_final, dim3(blocks_for(nh, 256)), dim3(256), 0, s, a, nh, t);
+    launch_chunk_aggregate(a, t, nchunks, s);
+    launch_chunk_merge(a, t, nitems, s);
+    launch_heavy_final(a, t, nh, s);
 }
 
 void launch_clear_tables(int32_t *keys, double *vals, int64_t n, hipStream_t s) {
@@ -1021,37 +1039,51 @@ void launch_clear_tables(int32_t *keys, double *vals, int64_t n, hipStream_t s) 
     hipLaunchKernelGGL(k_clear_tables, dim3((unsigned)g), dim3(256), 0, s, keys, vals, n);
 }
 
-void launch_apply(const MoveArgs &a, const int32_t *order, int32_t lo, int32_t hi,
+void launch_apply(const MoveArgs &a, const int32_t *list, int32_t n, const int32_t *order,
                   const SegTable *gst, const Chunk *chunks, int32_t *comm, int32_t *size,
                   double *sigma, uint64_t *crec, const GroupMarks *gm, uint8_t *hist,
                   unsigned long long *moved, double *own, hipStream_t s) {
-    const int32_t n = hi - lo;
-    if (n <= 0) return;
-    const int32_t *list = order + lo;
-    hipLaunchKernelGGL(k_apply_join, dim3(blocks_for(n, 256)), dim3(256), 0, s, list, n, a.newc,
-                       comm, size, sigma, a.k, crec, gm ? gm->movers : nullptr, hist, moved);
-    hipLaunchKernelGGL(k_apply_leave, dim3(blocks_for(n, 256)), dim3(256), 0, s, list, n, a.newc,
-                       comm, size, sigma, a.k, crec);
+    if (n <= 0 && !gm) return;
+    if (n > 0) {
+        hipLaunchKernelGGL(k_apply_join, dim3(blocks_for(n, 256)), dim3(256), 0, s, list, n,
+                           a.newc, comm, size, sigma, a.k, crec, gm ? gm->movers : nullptr, hist,
+                           moved);
+        hipLaunchKernelGGL(k_apply_leave, dim3(blocks_for(n, 256)), dim3(256), 0, s, list, n,
+                           a.newc, comm, size, sigma, a.k, crec);
+    }
     const int32_t corr = gm ? gst->total_blocks : 0;
-    const int32_t fix = (int32_t)blocks_for(n, kRowThreads);
+    const int32_t fix = n > 0 ? (int32_t)blocks_for(n, kRowThreads) : 0;
     const int32_t clear = gm ? (int32_t)blocks_for(gm->words, kClearWords) : 0;
     const SegTable empty{};
+    if (corr + fix + clear <= 0) return;
     hipLaunchKernelGGL(k_apply_finish, dim3(corr + fix + clear), dim3(kRowThreads), 0, s, a, order,
-                       lo, n, gm ? *gst : empty, chunks, comm, size, sigma, crec,
+                       list, n, gm ? *gst : empty, chunks, comm, size, sigma, crec,
                        gm ? gm->movers : nullptr, gm ? gm->clear_movers : nullptr,
                        gm ? gm->words : 0, own, corr, corr + fix);
+}
+
+void launch_modularity_rows(const int32_t *rowptr, const int32_t *col, const float *w,
+                            const int32_t *comm, const double *sigma, int32_t V,
+                            const int32_t *order, const SegTable &st, const Chunk *chunks,
+                            double *partials, hipStream_t s) {
+    hipLaunchKernelGGL(k_modularity, dim3(st.total_blocks), dim3(kRowThreads), 0, s, rowptr, col,
+                       w, comm, sigma, V, order, st, chunks, partials);
+}
+
+void launch_modularity_finish(const SegTable &st, double M, double *partials, double *q,
+                              hipStream_t s) {
+    double *partials2 = partials + 2 * (int64_t)st.total_blocks;
+    hipLaunchKernelGGL(k_modularity_reduce, dim3(kQReduceBlocks), dim3(256), 0, s, partials,
+                       st.total_blocks, partials2);
+    hipLaunchKernelGGL(k_modularity_final, dim3(1), dim3(256), 0, s, partials2, M, q, 0);
 }
 
 void launch_modularity(const int32_t *rowptr, const int32_t *col, const float *w,
                        const int32_t *comm, const double *sigma, int32_t V, const int32_t *order,
                        const SegTable &st, const Chunk *chunks, double M, double *partials,
                        double *q, hipStream_t s) {
-    hipLaunchKernelGGL(k_modularity, dim3(st.total_blocks), dim3(kRowThreads), 0, s, rowptr, col,
-                       w, comm, sigma, V, order, st, chunks, partials);
-    double *partials2 = partials + 2 * (int64_t)st.total_blocks;
-    hipLaunchKernelGGL(k_modularity_reduce, dim3(kQReduceBlocks), dim3(256), 0, s, partials,
-                       st.total_blocks, partials2);
-    hipLaunchKernelGGL(k_modularity_final, dim3(1), dim3(256), 0, s, partials2, M, q, 0);
+    launch_modularity_rows(rowptr, col, w, comm, sigma, V, order, st, chunks, partials, s);
+    launch_modularity_finish(st, M, partials, q, s);
 }
 
 void launch_modularity_from_own(const double *sigma, int32_t V, double M, double *partials,

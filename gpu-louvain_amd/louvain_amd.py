@@ -39,6 +39,12 @@ LV_ERR_NODEVICE = -6
 
 K_MOVE_SUB, K_MOVE_WAVE, K_MOVE_WG, K_MOVE_GLOBAL, K_APPLY, K_MODULARITY, K_AGGREGATE = range(7)
 KERNEL_NAMES = ["move_sub", "move_wave", "move_wg", "move_chunked", "apply", "modularity", "aggregate"]
+# single HIP kernels (lv_kernel_timing_detail) and their device function names
+DETAIL_NAMES = ["move_b0", "move_b1", "move_b2", "move_b3", "move_b4", "move_b5", "move_b6",
+                "chunk_aggregate", "chunk_merge", "chunk_final", "modularity_rows", "frontier"]
+DETAIL_KERNELS = ["k_move_sub<4>", "k_move_sub<8>", "k_move_sub<16>", "k_move_sub<32>",
+                  "k_move_wave<256>", "k_move_wave<512>", "k_move_wg<256>",
+                  "k_chunk_aggregate", "k_chunk_merge", "k_heavy_final", "k_modularity", "k_mlp<false>"]
 
 I32P = ctypes.POINTER(ctypes.c_int32)
 I64P = ctypes.POINTER(ctypes.c_int64)
@@ -161,6 +167,7 @@ def load() -> ctypes.CDLL:
         "lv_sweep": (ctypes.c_int32, [ctypes.c_void_p, F64P, I64P]),
         "lv_kernel_timing": (ctypes.c_int32, [ctypes.c_void_p, ctypes.c_int32, ctypes.POINTER(KernelTime)]),
         "lv_kernel_timing_reset": (ctypes.c_int32, [ctypes.c_void_p]),
+        "lv_kernel_timing_detail": (ctypes.c_int32, [ctypes.c_void_p, ctypes.c_int32, ctypes.POINTER(KernelTime)]),
         "lv_last_error": (ctypes.c_char_p, []),
         "lv_read_mtx": (ctypes.c_int32, [ctypes.c_char_p, ctypes.POINTER(I32P), ctypes.POINTER(I32P),
                                          ctypes.POINTER(F32P), I32P, I64P]),
@@ -392,6 +399,11 @@ class Graph:
     def kernel_timing(self, kernel: int) -> dict:
         kt = KernelTime()
         _check(self._lib.lv_kernel_timing(self._h, kernel, ctypes.byref(kt)))
+        return {"launches": kt.launches, "total_ms": kt.total_ms, "bytes": kt.bytes}
+
+    def kernel_timing_detail(self, kernel: int) -> dict:
+        kt = KernelTime()
+        _check(self._lib.lv_kernel_timing_detail(self._h, kernel, ctypes.byref(kt)))
         return {"launches": kt.launches, "total_ms": kt.total_ms, "bytes": kt.bytes}
 
     def kernel_timing_reset(self) -> None:

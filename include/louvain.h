@@ -86,6 +86,15 @@ typedef struct {
 #define LV_K_AGGREGATE 6     /* whole aggregation (phase 2)                         */
 #define LV_K_COUNT 7
 
+/* Single-kernel timing ids (lv_kernel_timing_detail): one HIP kernel each. */
+#define LV_KD_MOVE_BUCKET0 0     /* +b, b = 0..6: move kernel of execution bucket b     */
+#define LV_KD_CHUNK_AGGREGATE 7  /* chunked tier stage A (bytes: 12 per heavy edge)      */
+#define LV_KD_CHUNK_MERGE 8      /* chunked tier stage B (bytes not counted)             */
+#define LV_KD_CHUNK_FINAL 9      /* chunked tier decisions (bytes not counted)           */
+#define LV_KD_MODULARITY_ROWS 10 /* full modularity row pass (12 E + 16 V bytes)         */
+#define LV_KD_FRONTIER 11        /* frontier model inference (43 bytes per vertex)       */
+#define LV_KD_COUNT 12
+
 /* ---- graph handle (utils.cu:64-106 copyStructures + initM mo.cu:420-422) ---- */
 /* Uploads the CSR (rowptr[V+1], col[E], w[E]; host pointers) and sets up the
  * singleton state comm = iota, size = 1 (utils.cu:84-89).  M = (sum w)/2. */
@@ -132,6 +141,8 @@ lv_status lv_sweep(lv_graph *g, double *q_after, int64_t *moved);
 /* Per-kernel timing, id in [0, LV_K_COUNT). */
 lv_status lv_kernel_timing(lv_graph *g, int32_t kernel, lv_kernel_time *out);
 lv_status lv_kernel_timing_reset(lv_graph *g);
+/* Per-kernel timing of one HIP kernel, id in [0, LV_KD_COUNT). */
+lv_status lv_kernel_timing_detail(lv_graph *g, int32_t kernel, lv_kernel_time *out);
 
 const char *lv_last_error(void);
 
@@ -170,10 +181,13 @@ typedef struct {
     int32_t reserved[9];
 } lv_train_options;
 
-/* Levels like lv_run, with pruned phases: sweep i of a phase is full when
- * i % full_every == 0, pruned (frontier only) otherwise; a pruned sweep that
- * would end the phase is followed by a full sweep, and only a full sweep
- * ends it (termination rule of modularity_optimisation.cu:284-363). */
+/* Levels like lv_run, with pruned phases: sweep i of a phase is full
+ * (every vertex) when i % full_every == 0 and pruned (the frontier only)
+ * otherwise.  Each sweep is judged by the termination rule of
+ * modularity_optimisation.cu:284-363 against the sweep before it; when a
+ * pruned sweep would end the phase a full sweep follows, judged against the
+ * state before that pruned sweep, and only a full sweep ends the phase.
+ * Without pruning this is lv_run. */
 lv_status lv_run_pruned(lv_graph *g, float min_gain, const lv_frontier *fr, lv_stats *stats);
 /* One pruned phase (as lv_optimise). */
 lv_status lv_optimise_pruned(lv_graph *g, float min_gain, const lv_frontier *fr,
