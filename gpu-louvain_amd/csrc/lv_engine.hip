@@ -247,6 +247,12 @@ struct lv_graph {
 
     // row-reduction segments
     lv::SegTable st{};
+    // XCD-local modularity: light rows (buckets 0..3) by st_light, heavy rows
+    // (buckets 4..7) by column block with the split table
+    bool xcd_ok = false;
+    lv::SegTable st_light{}, xst{};
+    int32_t xh_lo = 0, xh_n = 0;
+    DevVec<int32_t> xsplit;
     lv::SegTable gst[lv::kNumGroups]{};
     DevVec<double> qpart;
     double *d_q = nullptr;
@@ -707,7 +713,9 @@ struct lv_graph {
         st.sq_block_start = blk;
         blk += (int32_t)((V + lv::kSqItems - 1) / lv::kSqItems);
         st.total_blocks = std::max(blk, 1);
-        LV_TRY(qpart.reserve(2 * (int64_t)st.total_blocks + 2 * 256));
+        LV_TRY(setup_xcd());
+        LV_TRY(qpart.reserve(2 * ((int64_t)st.total_blocks + st_light.total_blocks +
+                                  8 * (int64_t)xst.total_blocks) + 2 * 256));
         // per-group row segments for the modularity correction of the applies
         for (int g = 0; g < lv::kNumGroups; ++g) {
             lv::SegTable &t = gst[g];
@@ -744,6 +752,65 @@ struct lv_graph {
         state_changed();
         LV_LAUNCHED("level state");
         level_ready = true;
+        return LV_OK;
+    }
+
+    // Split table and segment tables of the XCD-local modularity pass; off
+    // when disabled, when the level has no heavy rows or when one is unsorted.
+    lv_status setup_xcd() {
+        xcd_ok = false;
+        st_light = lv::SegTable{};
+        xst = lv::SegTable{};
+        constexpr int XB = 4;  // first heavy bucket (degree > 32)
+        xh_lo = h_off[XB];
+        xh_n = h_off[lv::kNumBuckets] - h_off[XB];
+        if (opt.xcd_rows < 0 || xh_n <= 0 || V < 8) return LV_OK;
+        hipStream_t s = stream;
+        LV_TRY(xsplit.reserve(9 * (int64_t)xh_n));
+        int32_t *flag = a_scalars + 8;
+        lv::fill_i32(flag, 0, 1, s);
+        lv::launch_xcd_splits(rowptr(), col(), order + xh_lo, xh_n, V, xsplit.p, flag, s);
+        LV_LAUNCHED("xcd splits");
+        LV_TRY(stage(64));
+        LV_HIP(hipMemcpyAsync(h_stage, flag, sizeof(int32_t), hipMemcpyDeviceToHost, s));
+        LV_TRY(sync());
+        int32_t unsorted = 1;
+        std::memcpy(&unsorted, h_stage, sizeof unsorted);
+        if (unsorted) return LV_OK;
+        static const int lanes_light[XB] = {2, 3, 4, 5};
+        int32_t blk = 0;
+        for (int b = 0; b < XB; ++b) {
+            const int32_t cnt = h_off[b + 1] - h_off[b];
+            if (cnt <= 0) continue;
+            const int per = lv::kRowThreads >> lanes_light[b];
+            st_light.s[st_light.nseg++] = lv::Seg{h_off[b], cnt, blk, lanes_light[b]};
+            blk += (cnt + per - 1) / per;
+        }
+        if (st_light.nseg == 0) {
+            st_light.s[0] = lv::Seg{0, 0, 0, 2};
+            st_light.nseg = 1;
+        }
+        st_light.chunk_block_start = blk;
+        st_light.nchunks = 0;
+        st_light.sq_block_start = blk;
+        blk += (int32_t)((V + lv::kSqItems - 1) / lv::kSqItems);
+        st_light.total_blocks = blk;
+        // heavy rows: a column-block segment of a row has about deg/8 entries
+        static const int lanes_heavy[lv::kNumBuckets - XB] = {3, 4, 6, 8};
+        int32_t j = 0;
+        for (int b = XB; b < lv::kNumBuckets; ++b) {
+            const int32_t cnt = h_off[b + 1] - h_off[b];
+            if (cnt <= 0) continue;
+            const int lg = lanes_heavy[b - XB];
+            const int per = lg >= 8 ? 1 : (lv::kRowThreads >> lg);
+            xst.s[xst.nseg++] = lv::Seg{h_off[b] - xh_lo, cnt, j, lg};
+            j += (cnt + per - 1) / per;
+        }
+        xst.chunk_block_start = j;
+        xst.nchunks = 0;
+        xst.sq_block_start = j;
+        xst.total_blocks = j;
+        xcd_ok = true;
         return LV_OK;
     }
 
@@ -817,6 +884,25 @@ struct lv_graph {
     }
 
     lv_status enqueue_modularity() {
+        if (xcd_ok) {
+            double e_light = 0.0, e_heavy = 0.0;
+            for (int b = 0; b < lv::kNumBuckets; ++b) (b < 4 ? e_light : e_heavy) += (double)h_bedges[b];
+            hipEvent_t t0 = tbegin();
+            hipEvent_t tr = tbegin();
+            lv::launch_modularity_rows(rowptr(), col(), w(), comm, sigma, V, order, st_light, ins.p,
+                                       qpart.p, stream);
+            tend(tr, -1, 12.0 * e_light + 16.0 * (double)V, LV_KD_MODULARITY_ROWS);
+            hipEvent_t tx = tbegin();
+            lv::launch_modularity_xcd(col(), w(), comm, order + xh_lo, xst, xsplit.p, qpart.p,
+                                      st_light.total_blocks, stream);
+            tend(tx, -1, 12.0 * e_heavy, LV_KD_MODULARITY_XCD);
+            lv::launch_modularity_reduce(st_light.total_blocks + 8 * xst.total_blocks, M, qpart.p,
+                                         d_q, stream);
+            tend(t0, LV_K_MODULARITY, 12.0 * (double)E + 16.0 * (double)V);
+            LV_LAUNCHED("modularity");
+            own_valid = true;
+            return LV_OK;
+        }
         hipEvent_t t0 = tbegin();
         hipEvent_t tr = tbegin();
         lv::launch_modularity_rows(rowptr(), col(), w(), comm, sigma, V, order, st, ins.p, qpart.p,
@@ -989,11 +1075,10 @@ struct lv_graph {
         int64_t evaluated = 0, edges = 0;
         double ms_total = 0.0;
         // Every sweep is judged by the reference's rule against the sweep
-        // before it.  A pruned sweep that would end the phase is followed by
-        // a full sweep (the convergence check), judged against the state
-        // before that pruned sweep, so that a full sweep undoing a pruned
-        // one cannot keep the phase alive.  Without a frontier every sweep is
-        // full and this is exactly the reference's loop.
+        // before it.  A pruned sweep that converged is followed by a full
+        // sweep (the convergence check), judged against the state before that
+        // pruned sweep.  Without a frontier every sweep is full and this is
+        // exactly the reference's loop.
         double q_prev = q0, q_before_pruned = q0;
         bool confirm = false;
         for (int64_t i = 0;; ++i) {
@@ -1019,7 +1104,12 @@ struct lv_graph {
             const double gain = q1 - (confirm ? q_before_pruned : q_prev);
             const bool stop = mv == 0 || !(gain > 0.0) || !(gain >= (double)min_gain);
             if (stop && full) break;
-            confirm = stop;  // a pruned sweep wants to stop: check with a full sweep
+            // A pruned sweep that lowered Q ends the phase as a full one
+            // would; one that converged (no move, or a gain below min_gain)
+            // is checked by a full sweep, since the frontier may have missed
+            // movers.
+            if (stop && q1 - q_prev < 0.0) break;
+            confirm = stop;
             if (stop) q_before_pruned = q_prev;
             q_prev = q1;
         }
@@ -1184,6 +1274,7 @@ struct lv_graph {
         dev_free(fd_off);
         dev_free(mlp);
         tfeat.release();
+        xsplit.release();
         dev_free(bits);
         dev_free(keys8);
         dev_free(order);

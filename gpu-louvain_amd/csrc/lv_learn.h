@@ -39,10 +39,14 @@ struct FeatureArgs {
 // Frontier selection rules.
 enum FrontierKind { kFrontierModel = 0, kFrontierNeighbor = 1, kFrontierRandom = 2, kFrontierAll = 3 };
 
+// Movers of degree above this do not count in nmv: a hub's move says little
+// about any one neighbour and its row would dominate the pass.
+constexpr int32_t kNmvMaxDegree = 1024;
+
 // hist[v] <<= 1 for every vertex (start of a sweep).
 void launch_hist_shift(uint8_t *hist, int32_t V, hipStream_t s);
 // nmv[u] += 1 for every row entry (u, v) of a vertex v that moved in the last
-// sweep (hist bit 0); nmv must be zero before.  Rows are walked with the
+// sweep (hist bit 0) and has degree <= kNmvMaxDegree; nmv must be zero before.  Rows are walked with the
 // segment table of the level (heavy rows by chunks).
 void launch_neighbor_moves(const int32_t *rowptr, const int32_t *col, const int32_t *order,
                            const SegTable &st, const Chunk *chunks, const uint8_t *hist,

@@ -8,6 +8,7 @@
 //   x4 log2(1+size(cur))/24                   community size
 //   x5 log2(1 + V*S_cur/(2M))/24              community total relative to the mean vertex
 //   x6 min(1, nmv/deg)                        share of row entries whose neighbour moved
+//                                             (neighbours of degree <= kNmvMaxDegree)
 //   x7 min(sweeps since the last move, 8)/8
 // where margin = bestGain - K[cur]/M of the vertex's last evaluation (> 0 iff
 // it moved then) and S_cur, size(cur) are read from the community record.
@@ -89,6 +90,7 @@ __global__ __launch_bounds__(kRowThreads) void k_neighbor_moves(
         }
     }
     if (v < 0 || !(hist[v] & 1u)) return;
+    if (rowptr[v + 1] - rowptr[v] > kNmvMaxDegree) return;
     for (int64_t e = lo; e < hi; e += stride) atomicAdd(&nmv[col[e]], 1);
 }
 

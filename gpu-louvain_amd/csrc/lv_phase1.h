@@ -142,6 +142,20 @@ void launch_modularity_rows(const int32_t *rowptr, const int32_t *col, const flo
 void launch_modularity_finish(const SegTable &st, double M, double *partials, double *q,
                               hipStream_t s);
 
+// XCD-local row passes over the heavy rows (degree > 32, bucket-ordered
+// list): split[i*9 .. i*9+9) = the row of list[i] cut at the column-block
+// boundaries x*ceil(V/8); *unsorted |= 1 when a listed row is not sorted
+// by column (the split is then unusable).
+void launch_xcd_splits(const int32_t *rowptr, const int32_t *col, const int32_t *list, int32_t n,
+                       int32_t V, int32_t *split, int32_t *unsorted, hipStream_t s);
+// Weight inside communities of the heavy rows, one column block per XCD;
+// writes (own, 0) partials at blocks [base, base + 8*xst.total_blocks).
+void launch_modularity_xcd(const int32_t *col, const float *w, const int32_t *comm,
+                           const int32_t *list, const SegTable &xst, const int32_t *split,
+                           double *partials, int32_t base, hipStream_t s);
+// Fixed-order reduction of nb partial pairs into q (as launch_modularity_finish).
+void launch_modularity_reduce(int32_t nb, double M, double *partials, double *q, hipStream_t s);
+
 // Incremental: one pass for sum of S_c^2; q[0] = Q from q[1] (the weight
 // inside communities, kept current by the applies), q[2] = sq.
 void launch_modularity_from_own(const double *sigma, int32_t V, double M, double *partials,

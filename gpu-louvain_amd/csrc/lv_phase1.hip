@@ -882,6 +882,119 @@ __global__ __launch_bounds__(kRowThreads) void k_modularity(
     }
 }
 
+// ---------------------------------------------------- XCD-local row passes
+//
+// Workgroups are dispatched to the 8 XCDs round-robin (workgroup b runs on
+// XCD b % 8), and each XCD has its own L2.  The heavy rows (degree > 32,
+// sorted by column) are cut at the column-block boundaries x*ceil(V/8):
+// workgroup b handles column block b % 8 of its rows only, so every random
+// comm[u] gather of an XCD falls in one eighth of comm (2 MB at V = 4M),
+// which stays in that XCD's L2.  Row data is streamed with non-temporal
+// loads so that it does not evict the slice.
+
+// split[i*9 + x] = first edge of row list[i] with column >= x*blk (x = 0..8;
+// entry 0 = row start, 8 = row end).  Meaningful for sorted rows only
+// (k_rows_sorted checks).
+__global__ void k_xcd_splits(const int32_t *__restrict__ rowptr, const int32_t *__restrict__ col,
+                             const int32_t *__restrict__ list, int32_t n, int32_t blk,
+                             int32_t *split) {
+    const int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= (int64_t)n * 9) return;
+    const int32_t i = (int32_t)(t / 9), x = (int32_t)(t % 9);
+    const int32_t v = list[i];
+    const int32_t lo = rowptr[v], hi = rowptr[v + 1];
+    int32_t r;
+    if (x == 0) {
+        r = lo;
+    } else if (x == 8) {
+        r = hi;
+    } else {
+        const int64_t bound = (int64_t)x * blk;
+        int32_t a = lo, z = hi;  // first e in [lo, hi) with col[e] >= bound
+        while (a < z) {
+            const int32_t m = a + ((z - a) >> 1);
+            if ((int64_t)col[m] < bound) a = m + 1; else z = m;
+        }
+        r = a;
+    }
+    split[t] = r;
+}
+
+// Sortedness of the listed rows: one thread per edge pair.
+__global__ void k_rows_sorted(const int32_t *__restrict__ rowptr, const int32_t *__restrict__ col,
+                              const int32_t *__restrict__ list, int32_t n, int32_t *unsorted) {
+    const int32_t i = blockIdx.x;
+    if (i >= n) return;
+    const int32_t v = list[i];
+    const int32_t lo = rowptr[v], hi = rowptr[v + 1];
+    bool bad = false;
+    for (int32_t e = lo + 1 + threadIdx.x; e < hi; e += blockDim.x) bad |= col[e - 1] > col[e];
+    if (__any(bad) && (threadIdx.x & 63) == 0) atomicOr(unsorted, 1);
+}
+
+__device__ __forceinline__ double own_weight_nt(const int32_t *__restrict__ col,
+                                                const float *__restrict__ w,
+                                                const int32_t *__restrict__ comm, int32_t cv,
+                                                int64_t lo, int64_t hi, int st) {
+    double own = 0.0;
+    int64_t e = lo;
+    for (; e + 3 * st < hi; e += 4 * st) {
+        const int32_t u0 = __builtin_nontemporal_load(&col[e]);
+        const int32_t u1 = __builtin_nontemporal_load(&col[e + st]);
+        const int32_t u2 = __builtin_nontemporal_load(&col[e + 2 * st]);
+        const int32_t u3 = __builtin_nontemporal_load(&col[e + 3 * st]);
+        const float w0 = __builtin_nontemporal_load(&w[e]);
+        const float w1 = __builtin_nontemporal_load(&w[e + st]);
+        const float w2 = __builtin_nontemporal_load(&w[e + 2 * st]);
+        const float w3 = __builtin_nontemporal_load(&w[e + 3 * st]);
+        const int32_t c0 = comm[u0], c1 = comm[u1], c2 = comm[u2], c3 = comm[u3];
+        own += c0 == cv ? (double)w0 : 0.0;
+        own += c1 == cv ? (double)w1 : 0.0;
+        own += c2 == cv ? (double)w2 : 0.0;
+        own += c3 == cv ? (double)w3 : 0.0;
+    }
+    for (; e < hi; e += st) {
+        const int32_t u = __builtin_nontemporal_load(&col[e]);
+        const float wf = __builtin_nontemporal_load(&w[e]);
+        if (comm[u] == cv) own += (double)wf;
+    }
+    return own;
+}
+
+// Weight inside communities over the heavy rows, by column block: grid =
+// 8 * xst.total_blocks; partials of block b at partials[2*(base + b)].
+__global__ __launch_bounds__(kRowThreads) void k_modularity_xcd(
+    const int32_t *__restrict__ col, const float *__restrict__ w,
+    const int32_t *__restrict__ comm, const int32_t *__restrict__ list, SegTable xst,
+    const int32_t *__restrict__ split, double *partials, int32_t base) {
+    __shared__ double lds[kRowThreads / 64];
+    const int b = blockIdx.x;
+    const int x = b & 7, j = b >> 3;
+    const Seg sg = xst.s[find_seg(xst, j)];
+    const int lg = sg.lanes_log2;
+    double own = 0.0;
+    if (lg <= 6) {
+        const int X = 1 << lg;
+        const int32_t idx = (j - sg.block_start) * (kRowThreads >> lg) + (threadIdx.x >> lg);
+        if (idx < sg.count) {
+            const int32_t i = sg.list_off + idx;
+            const int32_t v = list[i];
+            own = own_weight_nt(col, w, comm, comm[v], split[i * 9 + x] + (threadIdx.x & (X - 1)),
+                                split[i * 9 + x + 1], X);
+        }
+    } else {
+        const int32_t i = sg.list_off + (j - sg.block_start);
+        const int32_t v = list[i];
+        own = own_weight_nt(col, w, comm, comm[v], split[i * 9 + x] + threadIdx.x,
+                            split[i * 9 + x + 1], kRowThreads);
+    }
+    own = block_sum_f64(own, lds);
+    if (threadIdx.x == 0) {
+        partials[2 * (base + b)] = own;
+        partials[2 * (base + b) + 1] = 0.0;
+    }
+}
+
 // Sum of S_c^2 only (incremental sweeps keep the weight inside communities
 // current on their own).
 __global__ __launch_bounds__(kRowThreads) void k_sum_squares(const double *__restrict__ sigma,
@@ -1084,6 +1197,30 @@ void launch_modularity(const int32_t *rowptr, const int32_t *col, const float *w
                        double *q, hipStream_t s) {
     launch_modularity_rows(rowptr, col, w, comm, sigma, V, order, st, chunks, partials, s);
     launch_modularity_finish(st, M, partials, q, s);
+}
+
+void launch_xcd_splits(const int32_t *rowptr, const int32_t *col, const int32_t *list, int32_t n,
+                       int32_t V, int32_t *split, int32_t *unsorted, hipStream_t s) {
+    if (n <= 0) return;
+    const int32_t blk = (V + 7) / 8;
+    hipLaunchKernelGGL(k_xcd_splits, dim3(blocks_for((int64_t)n * 9, 256)), dim3(256), 0, s, rowptr,
+                       col, list, n, blk, split);
+    hipLaunchKernelGGL(k_rows_sorted, dim3(n), dim3(256), 0, s, rowptr, col, list, n, unsorted);
+}
+
+void launch_modularity_xcd(const int32_t *col, const float *w, const int32_t *comm,
+                           const int32_t *list, const SegTable &xst, const int32_t *split,
+                           double *partials, int32_t base, hipStream_t s) {
+    if (xst.total_blocks <= 0) return;
+    hipLaunchKernelGGL(k_modularity_xcd, dim3(8 * xst.total_blocks), dim3(kRowThreads), 0, s, col,
+                       w, comm, list, xst, split, partials, base);
+}
+
+void launch_modularity_reduce(int32_t nb, double M, double *partials, double *q, hipStream_t s) {
+    double *partials2 = partials + 2 * (int64_t)nb;
+    hipLaunchKernelGGL(k_modularity_reduce, dim3(kQReduceBlocks), dim3(256), 0, s, partials, nb,
+                       partials2);
+    hipLaunchKernelGGL(k_modularity_final, dim3(1), dim3(256), 0, s, partials2, M, q, 0);
 }
 
 void launch_modularity_from_own(const double *sigma, int32_t V, double M, double *partials,

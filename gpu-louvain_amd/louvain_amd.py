@@ -41,10 +41,12 @@ K_MOVE_SUB, K_MOVE_WAVE, K_MOVE_WG, K_MOVE_GLOBAL, K_APPLY, K_MODULARITY, K_AGGR
 KERNEL_NAMES = ["move_sub", "move_wave", "move_wg", "move_chunked", "apply", "modularity", "aggregate"]
 # single HIP kernels (lv_kernel_timing_detail) and their device function names
 DETAIL_NAMES = ["move_b0", "move_b1", "move_b2", "move_b3", "move_b4", "move_b5", "move_b6",
-                "chunk_aggregate", "chunk_merge", "chunk_final", "modularity_rows", "frontier"]
+                "chunk_aggregate", "chunk_merge", "chunk_final", "modularity_rows", "frontier",
+                "modularity_xcd"]
 DETAIL_KERNELS = ["k_move_sub<4>", "k_move_sub<8>", "k_move_sub<16>", "k_move_sub<32>",
                   "k_move_wave<256>", "k_move_wave<512>", "k_move_wg<256>",
-                  "k_chunk_aggregate", "k_chunk_merge", "k_heavy_final", "k_modularity", "k_mlp<false>"]
+                  "k_chunk_aggregate", "k_chunk_merge", "k_heavy_final", "k_modularity", "k_mlp<false>",
+                  "k_modularity_xcd"]
 
 I32P = ctypes.POINTER(ctypes.c_int32)
 I64P = ctypes.POINTER(ctypes.c_int64)
@@ -69,7 +71,8 @@ class Options(ctypes.Structure):
         ("part_target", ctypes.c_int32),
         ("merge_limit", ctypes.c_int32),
         ("q_incremental", ctypes.c_int32),
-        ("reserved", ctypes.c_int32 * 8),
+        ("xcd_rows", ctypes.c_int32),
+        ("reserved", ctypes.c_int32 * 7),
     ]
 
 
@@ -289,7 +292,7 @@ class Graph:
 
     def __init__(self, g: CSR, device: int = 0, timing: bool = False, heavy_wg_max: int = 0,
                  chunk_edges: int = 0, part_target: int = 0, merge_limit: int = 0, agg_lds_max: int = 0,
-                 q_incremental: int = 0):
+                 q_incremental: int = 0, xcd_rows: int = 0):
         lib = load()
         self._lib = lib
         self._rowptr = np.ascontiguousarray(g.rowptr, dtype=np.int32)
@@ -305,6 +308,7 @@ class Graph:
         opt.merge_limit = merge_limit
         opt.agg_lds_max = agg_lds_max
         opt.q_incremental = q_incremental
+        opt.xcd_rows = xcd_rows
         h = ctypes.c_void_p()
         col = self._col if self._col.size else np.zeros(1, np.int32)
         w = self._w if self._w.size else np.zeros(1, np.float32)

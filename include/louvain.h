@@ -47,7 +47,9 @@ typedef struct {
     int32_t merge_limit;     /* pairs an LDS merge table may take (0 = 1536)          */
     int32_t q_incremental;   /* track modularity incrementally after a sweep that moved
                                 at most V/q_incremental vertices (0 = 8, -1 = never)   */
-    int32_t reserved[8];
+    int32_t xcd_rows;        /* heavy-row modularity pass by column block per XCD
+                                (0 = on when rows are sorted, -1 = off)             */
+    int32_t reserved[7];
 } lv_options;
 
 /* Run statistics (lv_run).  Times are device times from HIP events. */
@@ -91,9 +93,10 @@ typedef struct {
 #define LV_KD_CHUNK_AGGREGATE 7  /* chunked tier stage A (bytes: 12 per heavy edge)      */
 #define LV_KD_CHUNK_MERGE 8      /* chunked tier stage B (bytes not counted)             */
 #define LV_KD_CHUNK_FINAL 9      /* chunked tier decisions (bytes not counted)           */
-#define LV_KD_MODULARITY_ROWS 10 /* full modularity row pass (12 E + 16 V bytes)         */
+#define LV_KD_MODULARITY_ROWS 10 /* modularity row pass (12 B per edge it reads + 16 V)  */
 #define LV_KD_FRONTIER 11        /* frontier model inference (43 bytes per vertex)       */
-#define LV_KD_COUNT 12
+#define LV_KD_MODULARITY_XCD 12  /* heavy-row modularity pass by column block (12 B/edge)  */
+#define LV_KD_COUNT 13
 
 /* ---- graph handle (utils.cu:64-106 copyStructures + initM mo.cu:420-422) ---- */
 /* Uploads the CSR (rowptr[V+1], col[E], w[E]; host pointers) and sets up the
@@ -184,10 +187,11 @@ typedef struct {
 /* Levels like lv_run, with pruned phases: sweep i of a phase is full
  * (every vertex) when i % full_every == 0 and pruned (the frontier only)
  * otherwise.  Each sweep is judged by the termination rule of
- * modularity_optimisation.cu:284-363 against the sweep before it; when a
- * pruned sweep would end the phase a full sweep follows, judged against the
- * state before that pruned sweep, and only a full sweep ends the phase.
- * Without pruning this is lv_run. */
+ * modularity_optimisation.cu:284-363 against the sweep before it.  A pruned
+ * sweep that lowered Q ends the phase (as a full sweep would); one that
+ * converged (no move, or a gain in [0, min_gain)) is followed by a full
+ * sweep judged against the state before that pruned sweep, which ends the
+ * phase or continues it.  Without pruning this is lv_run. */
 lv_status lv_run_pruned(lv_graph *g, float min_gain, const lv_frontier *fr, lv_stats *stats);
 /* One pruned phase (as lv_optimise). */
 lv_status lv_optimise_pruned(lv_graph *g, float min_gain, const lv_frontier *fr,

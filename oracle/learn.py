@@ -109,12 +109,17 @@ def adam(params, g, m, v, t, lr, beta1=0.9, beta2=0.999, eps=1e-8):
     return params - lr * mh / (np.sqrt(vh) + eps), m, v
 
 
-def neighbor_moves(rowptr: np.ndarray, col: np.ndarray, moved: np.ndarray) -> np.ndarray:
-    """nmv[u] = number of row entries (v, u) with v a mover (moved[v] != 0)."""
+NMV_MAX_DEGREE = 1024
+
+
+def neighbor_moves(rowptr: np.ndarray, col: np.ndarray, moved: np.ndarray,
+                   max_degree: int = NMV_MAX_DEGREE) -> np.ndarray:
+    """nmv[u] = number of row entries (v, u) with v a mover (moved[v] != 0)
+    of degree <= max_degree."""
     V = len(rowptr) - 1
     deg = np.diff(rowptr)
     src = np.repeat(np.arange(V), deg)
-    sel = np.asarray(moved, dtype=bool)[src]
+    sel = (np.asarray(moved, dtype=bool) & (deg <= max_degree))[src]
     return np.bincount(col[: len(src)][sel], minlength=V).astype(np.int32)
 
 

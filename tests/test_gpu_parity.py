@@ -213,3 +213,20 @@ def test_edge_cases(lvlib, oracle, kind):
     assert lv == lv_o and np.array_equal(o2c, o2c_o)
     if g.E > 0:
         assert abs(st["modularity"] - st_o["modularity"]) <= Q_TOL
+
+
+@pytest.mark.parametrize("scale,seed", [(12, 61), (14, 62)])
+def test_xcd_modularity_pass_matches_plain_pass(lvlib, oracle, scale, seed):
+    """The column-blocked heavy-row pass gives the same Q as the row pass."""
+    g = lvlib.gen_rmat(scale, 16, seed=seed)
+    og = oracle.Graph.from_csr(g)
+    with lvlib.Graph(g, timing=True) as A, lvlib.Graph(g, xcd_rows=-1) as B:
+        for s in range(3):
+            qa, ma = A.sweep()
+            qb, mb = B.sweep()
+            assert ma == mb and qa == qb
+            assert abs(A.modularity() - oracle.modularity(og, A.communities())) <= 1e-9
+        assert A.kernel_timing_detail(lvlib.DETAIL_NAMES.index("modularity_xcd"))["launches"] > 0
+        A.aggregate()
+        B.aggregate()
+        assert abs(A.modularity() - B.modularity()) == 0.0

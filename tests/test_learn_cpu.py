@@ -85,6 +85,13 @@ def test_neighbor_moves_against_loop():
             for e in range(rowptr[v], rowptr[v + 1]):
                 want[col[e]] += 1
     assert np.array_equal(learn.neighbor_moves(rowptr, col, moved), want)
+    # movers above the degree cap do not count
+    deg = np.diff(rowptr)
+    hub = int(np.argmax(deg))
+    moved2 = np.zeros(60, bool)
+    moved2[hub] = True
+    assert learn.neighbor_moves(rowptr, col, moved2, max_degree=int(deg[hub]) - 1).sum() == 0
+    assert learn.neighbor_moves(rowptr, col, moved2, max_degree=int(deg[hub])).sum() == deg[hub]
 
 
 def test_features_hand_computed_two_triangles():
