@@ -50,7 +50,23 @@ struct MoveArgs {
     const uint8_t *active; // frontier flags (chunked tier; null = every vertex)
     float M32;  // (float)M
     float den;  // (2.0f * M32) * M32, fp32 as the reference evaluates it
+    // Batches of independent graphs: gid[v] = graph of vertex v and the
+    // graph's own M32 / den (null for a single graph).
+    const int32_t *gid;
+    const float *gM32;
+    const float *gden;
 };
+
+__device__ __forceinline__ void vertex_mden(const MoveArgs &a, int32_t v, float &M32, float &den) {
+    if (a.gid) {
+        const int32_t g = a.gid[v];
+        M32 = a.gM32[g];
+        den = a.gden[g];
+    } else {
+        M32 = a.M32;
+        den = a.den;
+    }
+}
 
 // Community record read by the move kernels: the fp32 rounding of the FP64
 // community total (the value the gain uses) and the community size, in one

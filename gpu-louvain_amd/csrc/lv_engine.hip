@@ -23,6 +23,7 @@
 #include "../../include/louvain.h"
 #include "lv_common.h"
 #include "lv_error.h"
+#include "lv_batch.h"
 #include "lv_learn.h"
 #include "lv_phase1.h"
 #include "lv_phase2.h"
@@ -162,6 +163,8 @@ constexpr int kAggChunk = 4096;   // flat member edges per aggregation chunk
 // A phase with a frontier sweeps only full after this many sweeps (bounds a
 // phase whose pruned and full sweeps keep undoing each other).
 constexpr int64_t kMaxPrunedSweeps = 4096;
+// Internal frontier kind: the vertices of the active graphs of a batch.
+constexpr int32_t kFrontierBatchGraphs = 100;
 
 int ceil_log2(int64_t x) {
     int lg = 0;
@@ -210,6 +213,26 @@ struct lv_graph {
     bool mlp_set = false;
     uint64_t sweep_count = 0;
     DevVec<float> tfeat;         // feature staging of traced sweeps
+
+    // batch of independent graphs (block-diagonal union); G = 0: one graph
+    int32_t G = 0;
+    std::vector<int32_t> h_vptr0;
+    std::vector<double> gM;
+    int32_t *vptr = nullptr, *vptr_next = nullptr, *vptr0 = nullptr;
+    int32_t *gid = nullptr, *gid0 = nullptr, *gfinal = nullptr;
+    float *gM32 = nullptr, *gden = nullptr;
+    uint8_t *gact = nullptr, *gdone = nullptr;
+    double *gsums = nullptr, *own_v = nullptr;
+    std::vector<double> h_gsums;     // per graph: own, sq, moved of the last sweep
+    uint8_t *h_up = nullptr;         // pinned staging of per-graph flag uploads
+
+    // Per-graph flags to the device through pinned memory; the caller syncs
+    // the stream before the next upload.
+    lv_status upload_flags(uint8_t *dst, const std::vector<uint8_t> &src) {
+        std::memcpy(h_up, src.data(), src.size());
+        LV_HIP(hipMemcpyAsync(dst, h_up, src.size(), hipMemcpyHostToDevice, stream));
+        return LV_OK;
+    }
     uint32_t *bits = nullptr;    // two mover bitmaps (alternating group steps)
     int32_t bit_words = 0;
     uint64_t step = 0;           // group steps applied with marks
@@ -369,6 +392,9 @@ struct lv_graph {
         a.active = nullptr;
         a.M32 = M32;
         a.den = den;
+        a.gid = G > 0 ? gid : nullptr;
+        a.gM32 = G > 0 ? gM32 : nullptr;
+        a.gden = G > 0 ? gden : nullptr;
         return a;
     }
 
@@ -931,7 +957,9 @@ struct lv_graph {
     lv_status select_frontier(const lv_frontier &fr) {
         hipStream_t s = stream;
         const lv::FeatureArgs fa = feature_args();
-        if (fr.kind == LV_FRONTIER_MODEL) {
+        if (fr.kind == kFrontierBatchGraphs) {
+            lv::launch_graph_frontier(rowptr(), V, gid, gact, bb, fkeys, active, s);
+        } else if (fr.kind == LV_FRONTIER_MODEL) {
             if (!mlp_set) return fail(LV_ERR_STATE, "frontier model not set (lv_set_model)");
             hipEvent_t t0 = tbegin();
             lv::launch_frontier(fa, mlp, fr.threshold, bb, fkeys, active, nullptr, s);
@@ -994,7 +1022,9 @@ struct lv_graph {
             LV_LAUNCHED("apply");
             if (incremental) step++;
         }
-        if (incremental) {
+        if (G > 0) {
+            LV_TRY(enqueue_graph_sums());
+        } else if (incremental) {
             hipEvent_t tq = tbegin();
             lv::launch_modularity_from_own(sigma, V, M, qpart.p, d_q, s);
             tend(tq, LV_K_MODULARITY, 8.0 * (double)V);
@@ -1012,6 +1042,9 @@ struct lv_graph {
         LV_HIP(hipMemcpyAsync(h_stage, d_q, sizeof(double), hipMemcpyDeviceToHost, s));
         LV_HIP(hipMemcpyAsync((char *)h_stage + 8, d_moved, sizeof(unsigned long long),
                               hipMemcpyDeviceToHost, s));
+        if (G > 0)
+            LV_HIP(hipMemcpyAsync(h_gsums.data(), gsums, sizeof(double) * 3 * (size_t)G,
+                                  hipMemcpyDeviceToHost, s));
         LV_TRY(sync());
         std::memcpy(q_after, h_stage, sizeof(double));
         unsigned long long mv = 0;
@@ -1135,6 +1168,7 @@ struct lv_graph {
         // renumbering (community_aggregation.cu:267-275)
         lv::launch_present(size, V, a_newid, s);
         lv::exclusive_scan(a_newid, a_newid, V, newV_d, scratch, s);
+        if (G > 0) lv::launch_graph_offsets(vptr, G, V, a_newid, newV_d, vptr_next, s);
         lv::exclusive_scan(size, a_mstart, V, nullptr, scratch, s);
         lv::fill_i32(a_fill, 0, V, s);
         lv::launch_members(comm, V, a_mstart, a_fill, a_members, s);
@@ -1243,6 +1277,184 @@ struct lv_graph {
         V = newV;
         E = newE;
         level_ready = false;
+        if (G > 0) {
+            std::swap(vptr, vptr_next);
+            lv::launch_graph_ids(vptr, G, gid, s);
+            LV_LAUNCHED("graph ids");
+        }
+        return LV_OK;
+    }
+
+    // ---------------------------------------------------------------- batch
+    // Per-graph (own, sq, moved) of the current state into gsums (device).
+    lv_status enqueue_graph_sums() {
+        hipStream_t s = stream;
+        hipEvent_t t0 = tbegin();
+        lv::fill_f64(own_v, 0.0, V, s);
+        lv::launch_vertex_own(rowptr(), col(), w(), comm, order, st, ins.p, own_v, s);
+        lv::launch_graph_sums(vptr, G, own_v, sigma, hist, gsums, s);
+        tend(t0, LV_K_MODULARITY, 12.0 * (double)E + 16.0 * (double)V);
+        LV_LAUNCHED("graph sums");
+        return LV_OK;
+    }
+
+    double graph_q(int32_t g) const {
+        const double Mg = gM[g];
+        return h_gsums[3 * g] / (2.0 * Mg) - h_gsums[3 * g + 1] / (4.0 * Mg * Mg);
+    }
+
+    lv_status setup_batch(int32_t nG, const int32_t *vp, const int32_t *rowptr_h, const float *w_h) {
+        G = nG;
+        h_vptr0.assign(vp, vp + G + 1);
+        gM.assign(G, 0.0);
+        std::vector<float> m32(G), dn(G);
+        for (int32_t g = 0; g < G; ++g) {
+            double s = 0.0;  // (sum of the graph's CSR weights)/2 in CSR order, as lv_create
+            for (int64_t e = rowptr_h[vp[g]]; e < rowptr_h[vp[g + 1]]; ++e) s += (double)w_h[e];
+            gM[g] = s / 2.0;
+            m32[g] = (float)gM[g];
+            volatile float two_m = 2.0f * m32[g];
+            volatile float d = two_m * m32[g];
+            dn[g] = d;
+        }
+        LV_TRY(dev_alloc(&vptr, G + 1));
+        LV_TRY(dev_alloc(&vptr_next, G + 1));
+        LV_TRY(dev_alloc(&vptr0, G + 1));
+        LV_TRY(dev_alloc(&gid, (int64_t)V0 + 1));
+        LV_TRY(dev_alloc(&gid0, (int64_t)V0 + 1));
+        LV_TRY(dev_alloc(&gfinal, (int64_t)V0 + 1));
+        LV_TRY(dev_alloc(&gM32, G));
+        LV_TRY(dev_alloc(&gden, G));
+        LV_TRY(dev_alloc(&gact, G + 8));
+        LV_TRY(dev_alloc(&gdone, G + 8));
+        LV_TRY(dev_alloc(&gsums, 3 * (int64_t)G));
+        LV_TRY(dev_alloc(&own_v, (int64_t)V0 + 1));
+        h_gsums.assign(3 * (size_t)G, 0.0);
+        LV_HIP(hipHostMalloc(reinterpret_cast<void **>(&h_up), (size_t)G + 64, hipHostMallocDefault));
+        LV_HIP(hipMemcpy(vptr0, vp, sizeof(int32_t) * (G + 1), hipMemcpyHostToDevice));
+        LV_HIP(hipMemcpy(gM32, m32.data(), sizeof(float) * G, hipMemcpyHostToDevice));
+        LV_HIP(hipMemcpy(gden, dn.data(), sizeof(float) * G, hipMemcpyHostToDevice));
+        lv::launch_graph_ids(vptr0, G, gid0, stream);
+        LV_LAUNCHED("graph ids");
+        return sync();
+    }
+
+    // Per graph: plain phases and aggregations exactly as lv_run would run
+    // them on that graph alone; all graphs advance together, a graph whose
+    // phase ended waits (outside the frontier) for the others.
+    lv_status run_batch(float min_gain, lv_batch_result *res, lv_stats *tot) {
+        hipStream_t s = stream;
+        hipEvent_t e0, e1;
+        LV_HIP(hipEventCreate(&e0));
+        LV_HIP(hipEventCreate(&e1));
+        LV_HIP(hipEventRecord(e0, s));
+        LV_TRY(reset());
+        LV_HIP(hipMemcpyAsync(vptr, vptr0, sizeof(int32_t) * (G + 1), hipMemcpyDeviceToDevice, s));
+        LV_HIP(hipMemcpyAsync(gid, gid0, sizeof(int32_t) * V0, hipMemcpyDeviceToDevice, s));
+        std::vector<uint8_t> done(G, 0), act(G, 0), changed(G, 0);
+        std::vector<double> qprev(G, 0.0);
+        std::vector<lv_batch_result> r(G);
+        for (auto &x : r) x = lv_batch_result{};
+        lv_stats st{};
+        double agg_ms = 0.0;
+        int32_t remaining = G;
+        lv_frontier fr{};
+        fr.kind = kFrontierBatchGraphs;
+        while (remaining > 0) {
+            if (!level_ready) LV_TRY(setup_level());
+            // Q at the phase start
+            LV_TRY(enqueue_graph_sums());
+            LV_HIP(hipMemcpyAsync(h_gsums.data(), gsums, sizeof(double) * 3 * (size_t)G,
+                                  hipMemcpyDeviceToHost, s));
+            LV_TRY(sync());
+            int32_t nact = 0;
+            for (int32_t g = 0; g < G; ++g) {
+                act[g] = done[g] ? 0 : 1;
+                changed[g] = 0;
+                nact += act[g];
+                if (act[g]) qprev[g] = graph_q(g);
+            }
+            LV_TRY(upload_flags(gact, act));
+            st.levels++;
+            while (nact > 0) {
+                double q1 = 0.0, ms = 0.0;
+                int64_t mv = 0, ev = 0;
+                const bool all = nact == G;
+                LV_TRY(sweep(&q1, &mv, &ms, all ? nullptr : &fr, false, &ev));
+                st.sweeps++;
+                st.sweep_ms += ms;
+                st.evaluated += ev;
+                bool flip = false;
+                for (int32_t g = 0; g < G; ++g) {
+                    if (!act[g]) continue;
+                    const double q = graph_q(g);
+                    const double gain = q - qprev[g];
+                    const double moved_g = h_gsums[3 * g + 2];
+                    r[g].sweeps++;
+                    if (gain > 0.0) changed[g] = 1;
+                    qprev[g] = q;
+                    // the phase rule of lv_optimise, per graph
+                    if (moved_g == 0.0 || !(gain > 0.0) || !(gain >= (double)min_gain)) {
+                        act[g] = 0;
+                        nact--;
+                        flip = true;
+                    }
+                }
+                if (flip && nact > 0) LV_TRY(upload_flags(gact, act));
+            }
+            lv::launch_compose(orig2comm, V0, comm, s);
+            LV_LAUNCHED("compose");
+            std::vector<uint8_t> now(G, 0);
+            bool any_now = false;
+            for (int32_t g = 0; g < G; ++g) {
+                if (done[g]) continue;
+                r[g].levels++;
+                if (!changed[g]) {
+                    done[g] = 1;
+                    now[g] = 1;
+                    any_now = true;
+                    remaining--;
+                    r[g].modularity = qprev[g];
+                }
+            }
+            if (any_now) {
+                std::vector<int32_t> vp(G + 1);
+                LV_HIP(hipMemcpyAsync(vp.data(), vptr, sizeof(int32_t) * (G + 1), hipMemcpyDeviceToHost, s));
+                LV_TRY(upload_flags(gdone, now));
+                lv::launch_graph_final(orig2comm, V0, gid0, gdone, vptr, gfinal, s);
+                LV_LAUNCHED("final assignment");
+                LV_TRY(sync());
+                for (int32_t g = 0; g < G; ++g)
+                    if (now[g]) r[g].final_V = vp[g + 1] - vp[g];
+            }
+            if (remaining == 0) break;
+            hipEvent_t a0 = take_event(), a1 = take_event();
+            if (a0) (void)hipEventRecord(a0, s);
+            LV_TRY(aggregate());
+            if (a1) (void)hipEventRecord(a1, s);
+            LV_TRY(sync());
+            float ams = 0.0f;
+            if (a0 && a1) (void)hipEventElapsedTime(&ams, a0, a1);
+            agg_ms += ams;
+            if (a0) ev_pool.push_back(a0);
+            if (a1) ev_pool.push_back(a1);
+        }
+        LV_HIP(hipEventRecord(e1, s));
+        LV_HIP(hipEventSynchronize(e1));
+        float ms = 0.0f;
+        LV_HIP(hipEventElapsedTime(&ms, e0, e1));
+        (void)hipEventDestroy(e0);
+        (void)hipEventDestroy(e1);
+        double qsum = 0.0;
+        for (int32_t g = 0; g < G; ++g) qsum += r[g].modularity;
+        st.modularity = G > 0 ? qsum / G : 0.0;
+        st.algo_ms = ms;
+        st.aggregate_ms = agg_ms;
+        st.memory_ms = memory_ms;
+        st.final_V = V;
+        st.final_E = E;
+        if (res) std::memcpy(res, r.data(), sizeof(lv_batch_result) * (size_t)G);
+        if (tot) *tot = st;
         return LV_OK;
     }
 
@@ -1275,6 +1487,20 @@ struct lv_graph {
         dev_free(mlp);
         tfeat.release();
         xsplit.release();
+        dev_free(vptr);
+        dev_free(vptr_next);
+        dev_free(vptr0);
+        dev_free(gid);
+        dev_free(gid0);
+        dev_free(gfinal);
+        dev_free(gM32);
+        dev_free(gden);
+        dev_free(gact);
+        dev_free(gdone);
+        dev_free(gsums);
+        dev_free(own_v);
+        if (h_up) (void)hipHostFree(h_up);
+        h_up = nullptr;
         dev_free(bits);
         dev_free(keys8);
         dev_free(order);
@@ -1801,5 +2027,73 @@ lv_status lv_mlp_train(int32_t device, const float *feat, const float *labels, i
     return LV_OK;
 }
 
+// ------------------------------------------------------------ batches
+
+struct lv_batch {
+    lv_graph *g;
+};
+
+lv_status lv_batch_create(int32_t G, const int32_t *vptr, const int32_t *rowptr, const int32_t *col,
+                          const float *w, int64_t E, const lv_options *opt, lv_batch **out) {
+    if (!out) return fail(LV_ERR_ARG, "out is null");
+    *out = nullptr;
+    if (G <= 0 || !vptr) return fail(LV_ERR_ARG, "no graphs");
+    if (vptr[0] != 0) return fail(LV_ERR_ARG, "vptr[0] must be 0");
+    for (int32_t g = 0; g < G; ++g)
+        if (vptr[g + 1] < vptr[g]) return fail(LV_ERR_ARG, "vptr not monotone");
+    const int32_t V = vptr[G];
+    if (!rowptr || (E > 0 && (!col || !w)) || rowptr[0] != 0 || (int64_t)rowptr[V] != E)
+        return fail(LV_ERR_ARG, "bad CSR");
+    for (int32_t g = 0; g < G; ++g)
+        for (int32_t v = vptr[g]; v < vptr[g + 1]; ++v)
+            for (int32_t e = rowptr[v]; e < rowptr[v + 1]; ++e)
+                if (col[e] < vptr[g] || col[e] >= vptr[g + 1])
+                    return fail(LV_ERR_ARG, "an edge leaves its graph's vertex range");
+    lv_graph *h = nullptr;
+    LV_TRY(lv_create(rowptr, col, w, V, E, opt, &h));
+    lv_status st = h->setup_batch(G, vptr, rowptr, w);
+    if (st != LV_OK) {
+        std::string msg = g_last_error;
+        lv_destroy(h);
+        g_last_error = msg;
+        return st;
+    }
+    lv_batch *b = new (std::nothrow) lv_batch{h};
+    if (!b) {
+        lv_destroy(h);
+        return fail(LV_ERR_OOM, "host allocation failed");
+    }
+    *out = b;
+    return LV_OK;
+}
+
+void lv_batch_destroy(lv_batch *b) {
+    if (!b) return;
+    lv_destroy(b->g);
+    delete b;
+}
+
+lv_status lv_batch_run(lv_batch *b, float min_gain, lv_batch_result *per_graph, lv_stats *total) {
+    if (!b) return fail(LV_ERR_ARG, "null batch");
+    LV_HIP(hipSetDevice(b->g->device));
+    return b->g->run_batch(min_gain, per_graph, total);
+}
+
+lv_status lv_batch_assignment(lv_batch *b, int32_t *comm) {
+    if (!b || !comm) return fail(LV_ERR_ARG, "null argument");
+    lv_graph *g = b->g;
+    LV_HIP(hipSetDevice(g->device));
+    if (g->V0 > 0)
+        LV_HIP(hipMemcpyAsync(comm, g->gfinal, sizeof(int32_t) * g->V0, hipMemcpyDeviceToHost,
+                              g->stream));
+    return g->sync();
+}
+
+lv_status lv_batch_kernel_timing(lv_batch *b, int32_t kernel, lv_kernel_time *out) {
+    if (!b) return fail(LV_ERR_ARG, "null batch");
+    return lv_kernel_timing(b->g, kernel, out);
+}
+
 }  // extern "C"
+
 

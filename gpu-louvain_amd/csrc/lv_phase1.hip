@@ -69,13 +69,15 @@ __device__ __forceinline__ void lanes_max_pair(uint64_t &best, double &bestK) {
 __device__ __forceinline__ void write_decision(const MoveArgs &a, int32_t v, int32_t cur,
                                                uint64_t best, double bestK, double kc) {
 #pragma clang fp contract(off)
-    const int32_t nc = decide_from(best, kc, a.M32, cur);
+    float vM32, vden;
+    vertex_mden(a, v, vM32, vden);
+    const int32_t nc = decide_from(best, kc, vM32, cur);
     a.newc[v] = nc;
     a.dk[v] = nc != cur ? bestK - kc : 0.0;
     // margin as computeMove sees it: bestGain starts at 0 (no candidate or
     // only negative gains leave it there)
     const float bg = best ? fmaxf(best_gain(best), 0.0f) : 0.0f;
-    const float t = (float)kc / a.M32;
+    const float t = (float)kc / vM32;
     a.margin[v] = bg - t;
     a.kown[v] = (float)kc;
 }
@@ -123,8 +125,10 @@ __global__ __launch_bounds__(256) void k_move_sub(MoveArgs a, const int32_t *__r
     if (owner && c != cur) {
         const uint64_t rcur = a.crec[cur];
         const uint64_t rc = a.crec[c];
+        float vM32, vden;
+        vertex_mden(a, v, vM32, vden);
         consider_k(best, bestK, c, K, rc, cur, rec_size(rcur), (float)a.k[v], rec_sigma(rcur),
-                   a.M32, a.den);
+                   vM32, vden);
     }
     lanes_max_pair<X>(best, bestK);
     if (lane == 0) write_decision(a, v, cur, best, bestK, kc);
@@ -172,6 +176,8 @@ __global__ __launch_bounds__(256) void k_move_wave(MoveArgs a, const int32_t *__
     __syncthreads();
     if (!active) return;  // only wave-level communication below
     const float k32 = (float)a.k[v];
+    float vM32, vden;
+    vertex_mden(a, v, vM32, vden);
     const uint64_t rcur = a.crec[cur];
     const float scur = rec_sigma(rcur);
     const int32_t size_cur = rec_size(rcur);
@@ -194,7 +200,7 @@ __global__ __launch_bounds__(256) void k_move_wave(MoveArgs a, const int32_t *__
             kc = vv[j];
             continue;
         }
-        consider_k(best, bestK, kk[j], vv[j], rr[j], cur, size_cur, k32, scur, a.M32, a.den);
+        consider_k(best, bestK, kk[j], vv[j], rr[j], cur, size_cur, k32, scur, vM32, vden);
     }
     lanes_max_pair<64>(best, bestK);
     kc = wave_sum_f64(kc);
@@ -240,6 +246,8 @@ __global__ __launch_bounds__(T) void k_move_wg(MoveArgs a, const int32_t *__rest
         }
         __syncthreads();
         const float k32 = (float)a.k[v];
+        float vM32, vden;
+        vertex_mden(a, v, vM32, vden);
         const uint64_t rcur = a.crec[cur];
         const float scur = rec_sigma(rcur);
         const int32_t size_cur = rec_size(rcur);
@@ -264,8 +272,8 @@ __global__ __launch_bounds__(T) void k_move_wg(MoveArgs a, const int32_t *__rest
                     kc = vv[j];
                     continue;
                 }
-                consider_k(best, bestK, kk[j], vv[j], rr[j], cur, size_cur, k32, scur, a.M32,
-                           a.den);
+                consider_k(best, bestK, kk[j], vv[j], rr[j], cur, size_cur, k32, scur, vM32,
+                           vden);
             }
         }
         lanes_max_pair<64>(best, bestK);
@@ -432,6 +440,8 @@ __global__ __launch_bounds__(kMergeThreads) void k_chunk_merge(MoveArgs a, Chunk
     if (a.active && !a.active[v]) return;  // not in this sweep's frontier
     const int32_t cur = a.comm[v];
     const float k32 = (float)a.k[v];
+    float vM32, vden;
+    vertex_mden(a, v, vM32, vden);
     const uint64_t rcur = a.crec[cur];
     const float scur = rec_sigma(rcur);
     const int32_t size_cur = rec_size(rcur);
@@ -466,8 +476,8 @@ __global__ __launch_bounds__(kMergeThreads) void k_chunk_merge(MoveArgs a, Chunk
                     kc = vv[j];
                     continue;
                 }
-                consider_k(best, bestK, kk[j], vv[j], rr[j], cur, size_cur, k32, scur, a.M32,
-                           a.den);
+                consider_k(best, bestK, kk[j], vv[j], rr[j], cur, size_cur, k32, scur, vM32,
+                           vden);
             }
         }
     } else {
@@ -489,7 +499,7 @@ __global__ __launch_bounds__(kMergeThreads) void k_chunk_merge(MoveArgs a, Chunk
                 kc = Kc;
                 continue;
             }
-            consider_k(best, bestK, c, Kc, a.crec[c], cur, size_cur, k32, scur, a.M32, a.den);
+            consider_k(best, bestK, c, Kc, a.crec[c], cur, size_cur, k32, scur, vM32, vden);
         }
     }
     lanes_max_pair<64>(best, bestK);

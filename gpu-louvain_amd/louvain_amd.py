@@ -522,3 +522,108 @@ def mlp_init(seed: int = 0) -> np.ndarray:
     w2 = rng.normal(0.0, np.sqrt(2.0 / HIDDEN), (HIDDEN, HIDDEN))
     w3 = rng.normal(0.0, np.sqrt(1.0 / HIDDEN), HIDDEN)
     return np.concatenate([w1.ravel(), np.zeros(HIDDEN), w2.ravel(), np.zeros(HIDDEN), w3, [0.0]]).astype(np.float32)
+
+
+class BatchResult(ctypes.Structure):
+    _fields_ = [("modularity", ctypes.c_double), ("levels", ctypes.c_int32), ("sweeps", ctypes.c_int32),
+                ("final_V", ctypes.c_int32), ("reserved", ctypes.c_int32)]
+
+
+def union(graphs) -> tuple[np.ndarray, CSR]:
+    """Block-diagonal union of CSRs: (vptr [G+1], union CSR with global ids)."""
+    vptr = np.zeros(len(graphs) + 1, np.int64)
+    eptr = np.zeros(len(graphs) + 1, np.int64)
+    for i, g in enumerate(graphs):
+        vptr[i + 1] = vptr[i] + g.V
+        eptr[i + 1] = eptr[i] + g.E
+    if vptr[-1] >= 2**31 or eptr[-1] >= 2**31:
+        raise ValueError("union exceeds int32 indices")
+    rowptr = np.zeros(int(vptr[-1]) + 1, np.int32)
+    col = np.zeros(int(eptr[-1]), np.int32)
+    w = np.zeros(int(eptr[-1]), np.float32)
+    for i, g in enumerate(graphs):
+        v0, e0 = int(vptr[i]), int(eptr[i])
+        rowptr[v0 + 1:v0 + g.V + 1] = g.rowptr[1:] + e0
+        col[e0:e0 + g.E] = g.col + v0
+        w[e0:e0 + g.E] = g.w
+    return vptr.astype(np.int32), CSR(rowptr, col, w)
+
+
+class Batch:
+    """Independent graphs solved together (lv_batch_*)."""
+
+    def __init__(self, graphs, device: int = 0, timing: bool = False):
+        lib = load()
+        _batch_sigs(lib)
+        self._lib = lib
+        self.vptr, u = union(graphs)
+        self.G = len(graphs)
+        self._keep = (u.rowptr, u.col if u.col.size else np.zeros(1, np.int32),
+                      u.w if u.w.size else np.zeros(1, np.float32))
+        opt = Options()
+        opt.device = device
+        opt.timing = int(timing)
+        h = ctypes.c_void_p()
+        _check(lib.lv_batch_create(self.G, _p(self.vptr, ctypes.c_int32), _p(self._keep[0], ctypes.c_int32),
+                                   _p(self._keep[1], ctypes.c_int32), _p(self._keep[2], ctypes.c_float),
+                                   int(u.col.size), ctypes.byref(opt), ctypes.byref(h)))
+        self._h = h
+        self.V = int(self.vptr[-1])
+
+    def run(self, min_gain: float) -> tuple[list[dict], dict]:
+        res = (BatchResult * self.G)()
+        st = Stats()
+        _check(self._lib.lv_batch_run(self._h, min_gain, res, ctypes.byref(st)))
+        per = [{"modularity": r.modularity, "levels": r.levels, "sweeps": r.sweeps, "final_V": r.final_V}
+               for r in res]
+        return per, st.as_dict()
+
+    def assignment(self) -> list[np.ndarray]:
+        comm = np.zeros(max(self.V, 1), np.int32)
+        _check(self._lib.lv_batch_assignment(self._h, _p(comm, ctypes.c_int32)))
+        return [comm[self.vptr[i]:self.vptr[i + 1]] for i in range(self.G)]
+
+    def kernel_timing(self, kernel: int) -> dict:
+        kt = KernelTime()
+        _check(self._lib.lv_batch_kernel_timing(self._h, kernel, ctypes.byref(kt)))
+        return {"launches": kt.launches, "total_ms": kt.total_ms, "bytes": kt.bytes}
+
+    def close(self) -> None:
+        if getattr(self, "_h", None):
+            self._lib.lv_batch_destroy(self._h)
+            self._h = None
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *exc):
+        self.close()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
+_batch_done = False
+
+
+def _batch_sigs(lib) -> None:
+    global _batch_done
+    if _batch_done:
+        return
+    sig = {
+        "lv_batch_create": (ctypes.c_int32, [ctypes.c_int32, I32P, I32P, I32P, F32P, ctypes.c_int64,
+                                             ctypes.POINTER(Options), ctypes.POINTER(ctypes.c_void_p)]),
+        "lv_batch_destroy": (None, [ctypes.c_void_p]),
+        "lv_batch_run": (ctypes.c_int32, [ctypes.c_void_p, ctypes.c_float, ctypes.POINTER(BatchResult),
+                                          ctypes.POINTER(Stats)]),
+        "lv_batch_assignment": (ctypes.c_int32, [ctypes.c_void_p, I32P]),
+        "lv_batch_kernel_timing": (ctypes.c_int32, [ctypes.c_void_p, ctypes.c_int32, ctypes.POINTER(KernelTime)]),
+    }
+    for name, (res, args) in sig.items():
+        fn = getattr(lib, name)
+        fn.restype = res
+        fn.argtypes = args
+    _batch_done = True

@@ -149,6 +149,30 @@ lv_status lv_kernel_timing_detail(lv_graph *g, int32_t kernel, lv_kernel_time *o
 
 const char *lv_last_error(void);
 
+/* ---- batches of independent graphs (no counterpart in the reference) ----
+ * G graphs given as one block-diagonal CSR: graph g owns vertices
+ * vptr[g] .. vptr[g+1] and its rows may only reference them.  lv_batch_run
+ * solves each graph exactly as lv_run would alone (its own M, phases,
+ * termination and levels); all graphs share every launch (one kernel per
+ * degree bucket covers the vertices of all graphs in that bucket). */
+typedef struct lv_batch lv_batch;
+typedef struct {
+    double modularity;       /* final Q of the graph (FP64)                          */
+    int32_t levels;          /* phases run on the graph                               */
+    int32_t sweeps;          /* sweeps run on the graph                               */
+    int32_t final_V;         /* vertices of the graph's last level                    */
+    int32_t reserved;
+} lv_batch_result;
+lv_status lv_batch_create(int32_t G, const int32_t *vptr, const int32_t *rowptr, const int32_t *col,
+                          const float *w, int64_t E, const lv_options *opt, lv_batch **out);
+void lv_batch_destroy(lv_batch *b);
+/* per_graph[G] (may be null); total: summed sweeps/levels, device times, mean Q. */
+lv_status lv_batch_run(lv_batch *b, float min_gain, lv_batch_result *per_graph, lv_stats *total);
+/* comm[V]: final community of each vertex, numbered inside its graph as
+ * lv_assignment numbers a single graph's communities. */
+lv_status lv_batch_assignment(lv_batch *b, int32_t *comm);
+lv_status lv_batch_kernel_timing(lv_batch *b, int32_t kernel, lv_kernel_time *out);
+
 /* ---- learned frontier (no counterpart in the reference) ----
  * A move predictor (MLP 8 -> 32 -> 32 -> 1, ReLU, sigmoid; 1377 fp32
  * parameters W1[32][8], b1[32], W2[32][32], b2[32], w3[32], b3) chooses

@@ -153,3 +153,18 @@ def test_cli_usage(oracle):
     r = subprocess.run([exe, "-g", "1"], capture_output=True, text=True)
     assert r.returncode == 1
     assert r.stdout.strip() == "Usage: ./gpulouvain -f mtx-matrix-file -g min-gain [-v]"
+
+
+def test_batch_rejects_edges_across_graphs(lvlib):
+    import graphs
+    a = lvlib.CSR(*graphs.two_triangles())
+    vptr, u = lvlib.union([a, a])
+    u.col[0] = 7   # vertex 0 of graph 0 points into graph 1
+    import ctypes
+    lib = lvlib.load()
+    lvlib._batch_sigs(lib)
+    h = ctypes.c_void_p()
+    st = lib.lv_batch_create(2, lvlib._p(vptr, ctypes.c_int32), lvlib._p(u.rowptr, ctypes.c_int32),
+                             lvlib._p(u.col, ctypes.c_int32), lvlib._p(u.w, ctypes.c_float), u.E, None,
+                             ctypes.byref(h))
+    assert st == lvlib.LV_ERR_ARG
