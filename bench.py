@@ -113,6 +113,57 @@ def pruned_comparison(L, csr, min_gain=1e-6):
     return out
 
 
+def batch_comparison(L, n_graphs=4096, seed=4, sample=64, min_gain=1e-6):
+    """4,096 independent planted-partition graphs solved as one batch: sizes
+    log-uniform in [200, 50000], blocks uniform in 4..64, p_in for mean
+    in-block degree 12, p_out for mean cross-block degree 2, unit weights.
+    Compared with running the first `sample` graphs one by one (lv_run),
+    whose results must equal the batch's."""
+    import numpy as np
+
+    rng = np.random.default_rng(seed)
+    t0 = time.perf_counter()
+    gs = []
+    for _ in range(n_graphs):
+        n = int(np.exp(rng.uniform(np.log(200), np.log(50000))))
+        b = int(rng.integers(4, 65))
+        p_in = min(1.0, 12.0 / max(n / b - 1.0, 1.0))
+        p_out = 2.0 / max(n - n / b, 1.0)
+        gs.append(L.gen_planted(n, b, p_in, p_out, seed=int(rng.integers(1 << 62)), unit_weights=True))
+    gen_s = time.perf_counter() - t0
+    V = sum(g.V for g in gs)
+    E = sum(g.E for g in gs)
+    log(f"batch: {n_graphs} planted graphs, V={V} E={E}, generated in {gen_s:.1f}s")
+    with L.Batch(gs) as B:
+        B.run(min_gain)  # warm
+        per, tot = B.run(min_gain)
+        comms = B.assignment()
+    log(f"batch: solved in {tot['algo_ms']:.1f} ms ({tot['levels']} levels, {tot['sweeps']} union sweeps)")
+    single_ms = 0.0
+    mismatches = 0
+    for i in range(sample):
+        with L.Graph(gs[i]) as G:
+            G.run(min_gain)
+            G.reset()
+            st = G.run(min_gain)
+            o2c, lv = G.assignment()
+        single_ms += st["algo_ms"]
+        if (abs(st["modularity"] - per[i]["modularity"]) > 1e-12 or not np.array_equal(o2c, comms[i])
+                or lv != per[i]["final_V"]):
+            mismatches += 1
+    est = single_ms / sample * n_graphs
+    log(f"batch: one-by-one {single_ms / sample:.2f} ms per graph (first {sample}), "
+        f"{est:.0f} ms estimated for all; {mismatches} mismatches")
+    return {"graphs": n_graphs, "V": V, "E": E, "time_ms": tot["algo_ms"],
+            "graphs_per_s": n_graphs / (tot["algo_ms"] * 1e-3), "edges_per_s": E / (tot["algo_ms"] * 1e-3),
+            "union_levels": tot["levels"], "union_sweeps": tot["sweeps"],
+            "aggregate_ms": tot["aggregate_ms"], "upload_ms": tot["memory_ms"],
+            "mean_modularity": float(np.mean([r["modularity"] for r in per])),
+            "one_by_one_ms_per_graph": single_ms / sample, "one_by_one_estimate_ms": est,
+            "speedup_vs_one_by_one": est / tot["algo_ms"], "sample_checked": sample,
+            "sample_mismatches": mismatches}
+
+
 def traffic_for(kernel, scale):
     """Measured HBM bytes per launch of `kernel` on RMAT-`scale` from the
     committed rocprofv3 PMC summary (FETCH_SIZE + WRITE_SIZE, copy-calibrated),
@@ -146,6 +197,8 @@ def main():
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--no-converge", action="store_true")
     ap.add_argument("--no-pruned", action="store_true")
+    ap.add_argument("--no-batch", action="store_true")
+    ap.add_argument("--no-tune", action="store_true")
     ap.add_argument("--calibrate-copy", action="store_true",
                     help="run only a plain 1 GiB device copy (PMC byte-counter calibration)")
     args = ap.parse_args()
@@ -165,7 +218,14 @@ def main():
     log(f"generated RMAT-{args.scale}: V={csr.V} E={csr.E} maxdeg={int(csr.degrees().max())} "
         f"in {time.perf_counter() - t0:.1f}s")
 
-    G = L.Graph(csr, timing=True)
+    tuned = None
+    if not args.no_tune:
+        import autotune as T
+
+        tuned = T.tune(csr, cache_path=None)
+        log(f"autotune: {tuned['tried']} candidates, default {tuned['default_ms']:.3f} ms/sweep -> "
+            f"{tuned['ms']:.3f} ms with {tuned['options']}")
+    G = L.Graph(csr, timing=True, **(tuned["options"] if tuned else {}))
     for i in range(args.warmup):
         G.reset()
         G.sweep()
@@ -230,6 +290,9 @@ def main():
     pruned = None
     if not args.no_pruned:
         pruned = pruned_comparison(L, csr)
+    batch = None
+    if not args.no_batch:
+        batch = batch_comparison(L)
 
     cpu = None
     if not args.no_cpu_baseline:
@@ -259,6 +322,8 @@ def main():
         "kernel_detail": details,
         "converge": converge,
         "pruned": pruned,
+        "batch": batch,
+        "autotune": tuned,
     }
     print(json.dumps(out))
 

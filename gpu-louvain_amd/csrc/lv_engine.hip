@@ -467,6 +467,31 @@ struct lv_graph {
         return LV_OK;
     }
 
+    // Tier bounds and sizes from opt (zero fields take the defaults).  They
+    // change the kernel mapping only, never a result.
+    void apply_options() {
+        // LDS aggregation tiers the device can hold; the others stay empty.
+        for (int t = 0; t < lv::kAggTiers - 1; ++t) {
+            const bool fits = lv::merge_lds_bytes(t) <= max_lds;
+            agg_bounds.hi[t] = fits ? lv::kAggTierMax[t] : (t > 0 ? agg_bounds.hi[t - 1] : 0);
+        }
+        if (opt.agg_lds_max > 0)
+            for (int t = 0; t < lv::kAggTiers - 1; ++t)
+                agg_bounds.hi[t] = std::min(agg_bounds.hi[t], opt.agg_lds_max);
+        // Bucket bounds: 0..5 are the reference groups; group 6 splits into
+        // a one-workgroup tier (LDS table of 2^wg6_lg slots at load <= 1/2)
+        // and the chunked tier.
+        int wg_max = opt.heavy_wg_max > 0 ? opt.heavy_wg_max : (1 << wg6_lg) / 2;
+        wg_max = std::max(320, std::min(wg_max, (1 << wg6_lg) / 2));
+        const int32_t his[lv::kNumBuckets] = {4, 8, 16, 32, 84, 319, wg_max, 0x7fffffff};
+        for (int b = 0; b < lv::kNumBuckets; ++b) bb.hi[b] = his[b];
+        chunk_edges = opt.chunk_edges > 0 ? std::max(16, std::min(opt.chunk_edges, lv::kChunkEdges))
+                                          : lv::kChunkEdges;
+        part_target = opt.part_target > 0 ? std::max(16, opt.part_target) : lv::kPartTarget;
+        merge_limit = opt.merge_limit > 0 ? std::min(opt.merge_limit, lv::kMergeLimit) : lv::kMergeLimit;
+        incr_den = opt.q_incremental != 0 ? opt.q_incremental : 8;
+    }
+
     // ---------------------------------------------------------------- setup
     lv_status create(const int32_t *rowptr_h, const int32_t *col_h, const float *w_h, int32_t nV,
                      int64_t nE, const lv_options *o) {
@@ -486,28 +511,9 @@ struct lv_graph {
         lv::set_wg_lds_limits(max_lds);
         lv::set_merge_lds_limits(max_lds);
         (void)hipGetLastError();
-        // LDS aggregation tiers the device can hold; the others stay empty.
-        for (int t = 0; t < lv::kAggTiers - 1; ++t) {
-            const bool fits = lv::merge_lds_bytes(t) <= max_lds;
-            agg_bounds.hi[t] = fits ? lv::kAggTierMax[t] : (t > 0 ? agg_bounds.hi[t - 1] : 0);
-        }
-        if (opt.agg_lds_max > 0)
-            for (int t = 0; t < lv::kAggTiers - 1; ++t)
-                agg_bounds.hi[t] = std::min(agg_bounds.hi[t], opt.agg_lds_max);
-
+        apply_options();
         V0 = V = nV;
         E0 = E = nE;
-        // Bucket bounds: 0..5 are the reference groups; group 6 splits into
-        // a one-workgroup tier (LDS table of 2^wg6_lg slots at load <= 1/2)
-        // and the chunked tier.
-        int wg_max = opt.heavy_wg_max > 0 ? opt.heavy_wg_max : (1 << wg6_lg) / 2;
-        wg_max = std::max(320, std::min(wg_max, (1 << wg6_lg) / 2));
-        const int32_t his[lv::kNumBuckets] = {4, 8, 16, 32, 84, 319, wg_max, 0x7fffffff};
-        for (int b = 0; b < lv::kNumBuckets; ++b) bb.hi[b] = his[b];
-        if (opt.chunk_edges > 0) chunk_edges = std::max(16, std::min(opt.chunk_edges, lv::kChunkEdges));
-        if (opt.part_target > 0) part_target = std::max(16, opt.part_target);
-        if (opt.merge_limit > 0) merge_limit = std::min(opt.merge_limit, lv::kMergeLimit);
-        if (opt.q_incremental != 0) incr_den = opt.q_incremental;
 
         // M = (sum of CSR weights)/2 in FP64, CSR order (utils.cu:36-46).
         double s = 0.0;
@@ -2092,6 +2098,28 @@ lv_status lv_batch_assignment(lv_batch *b, int32_t *comm) {
 lv_status lv_batch_kernel_timing(lv_batch *b, int32_t kernel, lv_kernel_time *out) {
     if (!b) return fail(LV_ERR_ARG, "null batch");
     return lv_kernel_timing(b->g, kernel, out);
+}
+
+lv_status lv_set_options(lv_graph *g, const lv_options *opt) {
+    if (!g || !opt) return fail(LV_ERR_ARG, "null argument");
+    if (opt->device != g->device) return fail(LV_ERR_ARG, "the device of a handle cannot change");
+    LV_HIP(hipSetDevice(g->device));
+    LV_TRY(g->sync());
+    const int32_t timing = g->opt.timing;
+    g->opt = *opt;
+    g->opt.timing = timing;
+    g->apply_options();
+    // the current level is bucketed again before its next use; its
+    // community state is kept
+    if (g->level_ready) {
+        std::vector<int32_t> keep((size_t)std::max(g->V, 1));
+        if (g->V > 0)
+            LV_HIP(hipMemcpy(keep.data(), g->comm, sizeof(int32_t) * g->V, hipMemcpyDeviceToHost));
+        g->level_ready = false;
+        LV_TRY(g->setup_level());
+        if (g->V > 0) LV_TRY(lv_set_communities(g, keep.data()));
+    }
+    return LV_OK;
 }
 
 }  // extern "C"

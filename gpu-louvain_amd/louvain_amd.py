@@ -171,6 +171,7 @@ def load() -> ctypes.CDLL:
         "lv_kernel_timing": (ctypes.c_int32, [ctypes.c_void_p, ctypes.c_int32, ctypes.POINTER(KernelTime)]),
         "lv_kernel_timing_reset": (ctypes.c_int32, [ctypes.c_void_p]),
         "lv_kernel_timing_detail": (ctypes.c_int32, [ctypes.c_void_p, ctypes.c_int32, ctypes.POINTER(KernelTime)]),
+        "lv_set_options": (ctypes.c_int32, [ctypes.c_void_p, ctypes.POINTER(Options)]),
         "lv_last_error": (ctypes.c_char_p, []),
         "lv_read_mtx": (ctypes.c_int32, [ctypes.c_char_p, ctypes.POINTER(I32P), ctypes.POINTER(I32P),
                                          ctypes.POINTER(F32P), I32P, I64P]),
@@ -299,6 +300,7 @@ class Graph:
         self._col = np.ascontiguousarray(g.col, dtype=np.int32)
         self._w = np.ascontiguousarray(g.w, dtype=np.float32)
         self.V0 = int(self._rowptr.shape[0] - 1)
+        self.device = device
         opt = Options()
         opt.device = device
         opt.timing = int(timing)
@@ -404,6 +406,21 @@ class Graph:
         kt = KernelTime()
         _check(self._lib.lv_kernel_timing(self._h, kernel, ctypes.byref(kt)))
         return {"launches": kt.launches, "total_ms": kt.total_ms, "bytes": kt.bytes}
+
+    def set_options(self, heavy_wg_max: int = 0, chunk_edges: int = 0, part_target: int = 0,
+                    merge_limit: int = 0, agg_lds_max: int = 0, q_incremental: int = 0,
+                    xcd_rows: int = 0) -> None:
+        """New tier options for this handle (results do not depend on them)."""
+        opt = Options()
+        opt.device = self.device
+        opt.heavy_wg_max = heavy_wg_max
+        opt.chunk_edges = chunk_edges
+        opt.part_target = part_target
+        opt.merge_limit = merge_limit
+        opt.agg_lds_max = agg_lds_max
+        opt.q_incremental = q_incremental
+        opt.xcd_rows = xcd_rows
+        _check(self._lib.lv_set_options(self._h, ctypes.byref(opt)))
 
     def kernel_timing_detail(self, kernel: int) -> dict:
         kt = KernelTime()

@@ -141,6 +141,11 @@ lv_status lv_decide(lv_graph *g, int32_t group, int32_t *new_comm);
 /* One sweep over groups 0..6 with the modularity after it (the body of the
  * loop at modularity_optimisation.cu:308-354); *moved = vertices moved. */
 lv_status lv_sweep(lv_graph *g, double *q_after, int64_t *moved);
+/* New tier options for an existing handle (no counterpart in the reference;
+ * for autotuning).  The device and the timing flag are kept; the current
+ * level is re-bucketed and keeps its community state.  Results do not
+ * depend on these options. */
+lv_status lv_set_options(lv_graph *g, const lv_options *opt);
 /* Per-kernel timing, id in [0, LV_K_COUNT). */
 lv_status lv_kernel_timing(lv_graph *g, int32_t kernel, lv_kernel_time *out);
 lv_status lv_kernel_timing_reset(lv_graph *g);
