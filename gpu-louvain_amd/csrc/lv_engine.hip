@@ -238,7 +238,8 @@ struct lv_graph {
     uint64_t step = 0;           // group steps applied with marks
     bool own_valid = false;      // d_q[1] holds the weight inside communities
     int64_t last_moved = -1;     // movers of the previous sweep (-1: unknown)
-    int32_t incr_den = 8;        // incremental modularity when movers <= V/incr_den
+    int64_t last_moved_edges = -1;  // row entries of those movers
+    int32_t incr_den = 2;        // incremental modularity when their rows hold <= E/incr_den entries
 
     // bucketing
     lv::BucketBounds bb{};
@@ -280,6 +281,7 @@ struct lv_graph {
     DevVec<double> qpart;
     double *d_q = nullptr;
     unsigned long long *d_moved = nullptr;
+    double *d_own_slots = nullptr;   // the applies' modularity corrections (kept zero between sweeps)
 
     // aggregation scratch
     int32_t *a_newid = nullptr, *a_mstart = nullptr, *a_fill = nullptr, *a_members = nullptr,
@@ -404,6 +406,7 @@ struct lv_graph {
         lv::fill_i32(reinterpret_cast<int32_t *>(bits), 0, 2 * (int64_t)bit_words, stream);
         own_valid = false;
         last_moved = -1;
+        last_moved_edges = -1;
         const int64_t Vp = (int64_t)V + 1;
         lv::fill_i32(reinterpret_cast<int32_t *>(margin), 0, Vp, stream);
         lv::fill_i32(reinterpret_cast<int32_t *>(kown), 0, Vp, stream);
@@ -489,7 +492,7 @@ struct lv_graph {
                                           : lv::kChunkEdges;
         part_target = opt.part_target > 0 ? std::max(16, opt.part_target) : lv::kPartTarget;
         merge_limit = opt.merge_limit > 0 ? std::min(opt.merge_limit, lv::kMergeLimit) : lv::kMergeLimit;
-        incr_den = opt.q_incremental != 0 ? opt.q_incremental : 8;
+        incr_den = opt.q_incremental != 0 ? opt.q_incremental : 2;
     }
 
     // ---------------------------------------------------------------- setup
@@ -563,7 +566,9 @@ struct lv_graph {
                         64;
         LV_TRY(dev_alloc(&scratch, scratch_elems));
         LV_TRY(dev_alloc(&d_q, 8));
-        LV_TRY(dev_alloc(&d_moved, 2));
+        LV_TRY(dev_alloc(&d_moved, lv::kMoveSlots * lv::kMoveSlotStride));
+        LV_TRY(dev_alloc(&d_own_slots, lv::kMoveSlots * lv::kMoveSlotStride));
+        lv::fill_f64(d_own_slots, 0.0, lv::kMoveSlots * lv::kMoveSlotStride, stream);
         LV_TRY(dev_alloc(&a_newid, Vp));
         LV_TRY(dev_alloc(&a_mstart, Vp));
         LV_TRY(dev_alloc(&a_fill, Vp));
@@ -999,8 +1004,9 @@ struct lv_graph {
                     int64_t *evaluated = nullptr) {
         if (!level_ready) LV_TRY(setup_level());
         hipStream_t s = stream;
-        const bool incremental = incr_den > 0 && own_valid && last_moved >= 0 &&
-                                 last_moved * (int64_t)incr_den <= (int64_t)V;
+        // the correction walks the movers' rows; a full pass walks all rows
+        const bool incremental = incr_den > 0 && own_valid && last_moved_edges >= 0 &&
+                                 last_moved_edges * (int64_t)incr_den <= E;
         hipEvent_t e0 = take_event(), e1 = take_event();
         if (e0) (void)hipEventRecord(e0, s);
         const bool pruned = fr != nullptr && fr->kind != LV_FRONTIER_ALL;
@@ -1009,7 +1015,8 @@ struct lv_graph {
             *evaluated = pruned ? h_foff[lv::kNumBuckets] : h_off[lv::kNumBuckets];
         sweep_count++;
         lv::launch_hist_shift(hist, V, s);
-        hipLaunchKernelGGL(k_zero_u64, dim3(1), dim3(1), 0, s, d_moved);
+        hipLaunchKernelGGL(k_zero_u64_n, dim3(blocks_for(lv::kMoveSlots * lv::kMoveSlotStride, 256)),
+                           dim3(256), 0, s, d_moved, (int64_t)lv::kMoveSlots * lv::kMoveSlotStride);
         const lv::MoveArgs a = move_args();
         for (int g = 0; g < lv::kNumGroups; ++g) {
             int32_t lo, hi;
@@ -1023,7 +1030,7 @@ struct lv_graph {
             if (pruned) group_range(g, alo, ahi, h_foff);
             const int32_t *alist = (pruned ? forder : order) + alo;
             lv::launch_apply(a, alist, ahi - alo, order, &gst[g], ins.p, comm, size, sigma, crec,
-                             incremental ? &gm : nullptr, hist, d_moved, d_q + 1, s);
+                             incremental ? &gm : nullptr, hist, d_moved, d_own_slots, s);
             tend(t0, LV_K_APPLY, 16.0 * (double)(ahi - alo));
             LV_LAUNCHED("apply");
             if (incremental) step++;
@@ -1032,7 +1039,7 @@ struct lv_graph {
             LV_TRY(enqueue_graph_sums());
         } else if (incremental) {
             hipEvent_t tq = tbegin();
-            lv::launch_modularity_from_own(sigma, V, M, qpart.p, d_q, s);
+            lv::launch_modularity_from_own(sigma, V, M, qpart.p, d_q, d_own_slots, s);
             tend(tq, LV_K_MODULARITY, 8.0 * (double)V);
             LV_LAUNCHED("modularity");
         } else {
@@ -1044,19 +1051,27 @@ struct lv_graph {
             LV_LAUNCHED("neighbour moves");
         }
         if (e1) (void)hipEventRecord(e1, s);
-        LV_TRY(stage(64));
+        const size_t mv_bytes = sizeof(unsigned long long) * lv::kMoveSlots * lv::kMoveSlotStride;
+        LV_TRY(stage(64 + mv_bytes));
         LV_HIP(hipMemcpyAsync(h_stage, d_q, sizeof(double), hipMemcpyDeviceToHost, s));
-        LV_HIP(hipMemcpyAsync((char *)h_stage + 8, d_moved, sizeof(unsigned long long),
-                              hipMemcpyDeviceToHost, s));
+        LV_HIP(hipMemcpyAsync((char *)h_stage + 64, d_moved, mv_bytes, hipMemcpyDeviceToHost, s));
         if (G > 0)
             LV_HIP(hipMemcpyAsync(h_gsums.data(), gsums, sizeof(double) * 3 * (size_t)G,
                                   hipMemcpyDeviceToHost, s));
         LV_TRY(sync());
         std::memcpy(q_after, h_stage, sizeof(double));
-        unsigned long long mv = 0;
-        std::memcpy(&mv, (char *)h_stage + 8, sizeof mv);
+        unsigned long long mv = 0, me = 0;
+        {
+            const unsigned long long *slots =
+                reinterpret_cast<const unsigned long long *>((char *)h_stage + 64);
+            for (int i = 0; i < lv::kMoveSlots; ++i) {
+                mv += slots[i * lv::kMoveSlotStride];
+                me += slots[i * lv::kMoveSlotStride + 1];
+            }
+        }
         *moved = (int64_t)mv;
         last_moved = (int64_t)mv;
+        last_moved_edges = (int64_t)me;
         if (sweep_ms) {
             float ms = 0.0f;
             if (e0 && e1) (void)hipEventElapsedTime(&ms, e0, e1);
@@ -1515,6 +1530,7 @@ struct lv_graph {
         dev_free(scratch);
         dev_free(d_q);
         dev_free(d_moved);
+        dev_free(d_own_slots);
         dev_free(a_newid);
         dev_free(a_mstart);
         dev_free(a_fill);

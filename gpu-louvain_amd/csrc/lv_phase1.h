@@ -108,6 +108,9 @@ void launch_clear_tables(int32_t *keys, double *vals, int64_t n, hipStream_t s);
 
 // Bitmaps of the movers of a group step (two, alternating by step, so that
 // one is cleared while the other is read).
+constexpr int kMoveSlots = 64;
+constexpr int kMoveSlotStride = 16;  // 128 bytes between slots
+
 struct GroupMarks {
     uint32_t *movers;        // this step's bitmap over vertices
     uint32_t *clear_movers;  // the other step's, cleared by this step
@@ -119,8 +122,11 @@ struct GroupMarks {
 // group's movers; newc[v] then holds the community v left) and refreshes the
 // records of the touched communities.  With gm (incremental modularity) the
 // movers are marked and the group's change of the weight inside communities
-// is added to *own (gst: the group's row segments over order).
-// hist[v] gets bit 0 set for a mover (the move history of the frontier model).
+// is added to own (kMoveSlots slots kMoveSlotStride apart; gst: the group's
+// row segments over order).
+// hist[v] gets bit 0 set for a mover (the move history of the frontier model);
+// moved holds kMoveSlots counter pairs kMoveSlotStride apart: summed over
+// the slots, [0] counts the movers and [1] their row entries.
 void launch_apply(const MoveArgs &a, const int32_t *list, int32_t n, const int32_t *order,
                   const SegTable *gst, const Chunk *chunks, int32_t *comm, int32_t *size,
                   double *sigma, uint64_t *crec, const GroupMarks *gm, uint8_t *hist,
@@ -156,10 +162,11 @@ void launch_modularity_xcd(const int32_t *col, const float *w, const int32_t *co
 // Fixed-order reduction of nb partial pairs into q (as launch_modularity_finish).
 void launch_modularity_reduce(int32_t nb, double M, double *partials, double *q, hipStream_t s);
 
-// Incremental: one pass for sum of S_c^2; q[0] = Q from q[1] (the weight
-// inside communities, kept current by the applies), q[2] = sq.
+// Incremental: one pass for sum of S_c^2; q[0] = Q from q[1] plus the
+// applies' corrections in own_slots (kMoveSlots slots kMoveSlotStride apart,
+// folded into q[1] and cleared), q[2] = sq.
 void launch_modularity_from_own(const double *sigma, int32_t V, double M, double *partials,
-                                double *q, hipStream_t s);
+                                double *q, double *own_slots, hipStream_t s);
 
 // --- original-vertex map ---
 void launch_compose(int32_t *orig2comm, int32_t V0, const int32_t *map, hipStream_t s);

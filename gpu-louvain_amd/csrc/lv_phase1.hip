@@ -620,6 +620,7 @@ __device__ __forceinline__ bool test_bit(const uint32_t *bits, int32_t idx) {
 }
 
 __global__ __launch_bounds__(256) void k_apply_join(const int32_t *__restrict__ list, int32_t n,
+                                                    const int32_t *__restrict__ rowptr,
                                                     int32_t *newc, int32_t *comm, int32_t *size,
                                                     double *sigma, const double *__restrict__ k,
                                                     const uint64_t *__restrict__ crec,
@@ -646,7 +647,16 @@ __global__ __launch_bounds__(256) void k_apply_join(const int32_t *__restrict__ 
     wave_peel_update(mv && shared_old, oc, -kv, -1, size, sigma, 1);
     if (movers) wave_set_bit(mv, v, movers);
     const uint64_t m = __ballot(mv);
-    if ((threadIdx.x & 63) == 0 && m) atomicAdd(moved, (unsigned long long)__popcll(m));
+    // moved[0] += movers, moved[1] += their row entries
+    const double deg = mv ? (double)(rowptr[v + 1] - rowptr[v]) : 0.0;
+    const double wdeg = wave_sum_f64(deg);
+    if ((threadIdx.x & 63) == 0 && m) {
+        // spread over kMoveSlots cache lines: one counter would serialise
+        // every wave of the launch on one address
+        unsigned long long *slot = moved + (int64_t)(blockIdx.x % kMoveSlots) * kMoveSlotStride;
+        atomicAdd(slot, (unsigned long long)__popcll(m));
+        atomicAdd(slot + 1, (unsigned long long)wdeg);
+    }
 }
 
 __global__ __launch_bounds__(256) void k_apply_leave(const int32_t *__restrict__ list, int32_t n,
@@ -743,7 +753,8 @@ __global__ __launch_bounds__(kRowThreads) void k_apply_finish(
         }
     }
     acc = block_sum_f64(acc, lds);
-    if (threadIdx.x == 0 && acc != 0.0) atomicAdd(own, acc);
+    if (threadIdx.x == 0 && acc != 0.0)
+        atomicAdd(&own[(int64_t)(blockIdx.x % kMoveSlots) * kMoveSlotStride], acc);
 }
 
 __global__ void k_records_all(int32_t V, const double *__restrict__ sigma,
@@ -1048,16 +1059,25 @@ __global__ __launch_bounds__(256) void k_modularity_reduce(const double *__restr
     }
 }
 
-// use_own: the weight inside communities comes from q[1], not the partials.
+// own_slots (incremental): the weight inside communities is q[1] plus the
+// corrections the applies left in the slots, which are cleared here.
 __global__ __launch_bounds__(256) void k_modularity_final(const double *__restrict__ partials2,
-                                                          double M, double *q, int use_own) {
+                                                          double M, double *q, double *own_slots) {
     __shared__ double lds[4];
     double own = partials2[2 * threadIdx.x];
     double sq = partials2[2 * threadIdx.x + 1];
+    double d = 0.0;
+    if (own_slots && threadIdx.x < kMoveSlots) {
+        d = own_slots[threadIdx.x * kMoveSlotStride];
+        own_slots[threadIdx.x * kMoveSlotStride] = 0.0;
+    }
     own = block_sum_f64(own, lds);
     sq = block_sum_f64(sq, lds);
+    // fixed-order sum of the slots (the additions inside a slot are exact
+    // for weights on a dyadic grid)
+    d = block_sum_f64(d, lds);
     if (threadIdx.x == 0) {
-        if (use_own) own = q[1];
+        if (own_slots) own = q[1] + d;
         q[0] = own / (2.0 * M) - sq / (4.0 * M * M);
         q[1] = own;
         q[2] = sq;
@@ -1169,8 +1189,8 @@ void launch_apply(const MoveArgs &a, const int32_t *list, int32_t n, const int32
     if (n <= 0 && !gm) return;
     if (n > 0) {
         hipLaunchKernelGGL(k_apply_join, dim3(blocks_for(n, 256)), dim3(256), 0, s, list, n,
-                           a.newc, comm, size, sigma, a.k, crec, gm ? gm->movers : nullptr, hist,
-                           moved);
+                           a.rowptr, a.newc, comm, size, sigma, a.k, crec,
+                           gm ? gm->movers : nullptr, hist, moved);
         hipLaunchKernelGGL(k_apply_leave, dim3(blocks_for(n, 256)), dim3(256), 0, s, list, n,
                            a.newc, comm, size, sigma, a.k, crec);
     }
@@ -1198,7 +1218,7 @@ void launch_modularity_finish(const SegTable &st, double M, double *partials, do
     double *partials2 = partials + 2 * (int64_t)st.total_blocks;
     hipLaunchKernelGGL(k_modularity_reduce, dim3(kQReduceBlocks), dim3(256), 0, s, partials,
                        st.total_blocks, partials2);
-    hipLaunchKernelGGL(k_modularity_final, dim3(1), dim3(256), 0, s, partials2, M, q, 0);
+    hipLaunchKernelGGL(k_modularity_final, dim3(1), dim3(256), 0, s, partials2, M, q, nullptr);
 }
 
 void launch_modularity(const int32_t *rowptr, const int32_t *col, const float *w,
@@ -1230,17 +1250,17 @@ void launch_modularity_reduce(int32_t nb, double M, double *partials, double *q,
     double *partials2 = partials + 2 * (int64_t)nb;
     hipLaunchKernelGGL(k_modularity_reduce, dim3(kQReduceBlocks), dim3(256), 0, s, partials, nb,
                        partials2);
-    hipLaunchKernelGGL(k_modularity_final, dim3(1), dim3(256), 0, s, partials2, M, q, 0);
+    hipLaunchKernelGGL(k_modularity_final, dim3(1), dim3(256), 0, s, partials2, M, q, nullptr);
 }
 
 void launch_modularity_from_own(const double *sigma, int32_t V, double M, double *partials,
-                                double *q, hipStream_t s) {
+                                double *q, double *own_slots, hipStream_t s) {
     const int32_t nb = (int32_t)(V > 0 ? blocks_for(V, kSqItems) : 1);
     hipLaunchKernelGGL(k_sum_squares, dim3(nb), dim3(kRowThreads), 0, s, sigma, V, partials);
     double *partials2 = partials + 2 * (int64_t)nb;
     hipLaunchKernelGGL(k_modularity_reduce, dim3(kQReduceBlocks), dim3(256), 0, s, partials, nb,
                        partials2);
-    hipLaunchKernelGGL(k_modularity_final, dim3(1), dim3(256), 0, s, partials2, M, q, 1);
+    hipLaunchKernelGGL(k_modularity_final, dim3(1), dim3(256), 0, s, partials2, M, q, own_slots);
 }
 
 void launch_compose(int32_t *orig2comm, int32_t V0, const int32_t *map, hipStream_t s) {

@@ -45,8 +45,9 @@ typedef struct {
     int32_t agg_lds_max;     /* summed-degree bound of LDS aggregation (0 = auto)     */
     int32_t part_target;     /* pairs per merge partition of the chunked tier (0=512)  */
     int32_t merge_limit;     /* pairs an LDS merge table may take (0 = 1536)          */
-    int32_t q_incremental;   /* track modularity incrementally after a sweep that moved
-                                at most V/q_incremental vertices (0 = 8, -1 = never)   */
+    int32_t q_incremental;   /* track modularity incrementally after a sweep whose
+                                movers' rows hold at most E/q_incremental entries
+                                (0 = 2, -1 = never)                                    */
     int32_t xcd_rows;        /* heavy-row modularity pass by column block per XCD
                                 (0 = on when rows are sorted, -1 = off)             */
     int32_t reserved[7];
