@@ -243,7 +243,8 @@ struct lv_graph {
 
     // bucketing
     lv::BucketBounds bb{};
-    int wg6_lg = 12;             // LDS slots (log2) of the one-workgroup tier
+    int wg6_lg = 12;             // LDS slots (log2) of the one-workgroup tier (max)
+    int wg_lg = 12;              // slots allocated per launch: 2 * the tier's degree bound
     int chunk_edges = lv::kChunkEdges, part_target = lv::kPartTarget, merge_limit = lv::kMergeLimit;
     lv::AggBounds agg_bounds{};
     int8_t *keys8 = nullptr;
@@ -488,6 +489,9 @@ struct lv_graph {
         wg_max = std::max(320, std::min(wg_max, (1 << wg6_lg) / 2));
         const int32_t his[lv::kNumBuckets] = {4, 8, 16, 32, 84, 319, wg_max, 0x7fffffff};
         for (int b = 0; b < lv::kNumBuckets; ++b) bb.hi[b] = his[b];
+        // smaller tables for a lower degree bound: more workgroups per CU
+        wg_lg = 9;
+        while ((1 << wg_lg) < 2 * wg_max) ++wg_lg;
         chunk_edges = opt.chunk_edges > 0 ? std::max(16, std::min(opt.chunk_edges, lv::kChunkEdges))
                                           : lv::kChunkEdges;
         part_target = opt.part_target > 0 ? std::max(16, opt.part_target) : lv::kPartTarget;
@@ -881,7 +885,7 @@ struct lv_graph {
                 break;
             case 6:
                 t0 = tbegin();
-                lv::launch_move_wg(256, wg6_lg, a, list, n, s);
+                lv::launch_move_wg(256, wg_lg, a, list, n, s);
                 tend(t0, LV_K_MOVE_WG, bytes, LV_KD_MOVE_BUCKET0 + b);
                 break;
             default: {

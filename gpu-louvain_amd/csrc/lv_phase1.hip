@@ -211,22 +211,26 @@ template <int T>
 __global__ __launch_bounds__(T) void k_move_wg(MoveArgs a, const int32_t *__restrict__ list,
                                                int32_t n, int cap_log2) {
     extern __shared__ double dyn_lds[];
-    const int C = 1 << cap_log2;
+    const int C = 1 << cap_log2;  // allocation: W[C] then K[C]
     double *W = dyn_lds;
     int32_t *K = reinterpret_cast<int32_t *>(dyn_lds + C);
     __shared__ uint64_t red_best[T / 64];
     __shared__ double red_K[T / 64];
     __shared__ double red_kc[T / 64];
     const int lane = threadIdx.x & 63, wid = threadIdx.x >> 6;
-    const uint32_t shift = 32 - cap_log2, mask = (uint32_t)C - 1;
     for (int32_t vi = blockIdx.x; vi < n; vi += gridDim.x) {
-        for (int i = threadIdx.x; i < C; i += T) {
+        const int32_t v = list[vi];
+        const int32_t rs = a.rowptr[v], re = a.rowptr[v + 1];
+        // table of the vertex: the smallest power of two >= 2*degree (load
+        // <= 1/2), at least 512 slots and at most the allocation
+        const int cl = min(cap_log2, max(9, 32 - __clz(2 * (re - rs) - 1)));
+        const int Cv = 1 << cl;
+        const uint32_t shift = 32 - cl, mask = (uint32_t)Cv - 1;
+        for (int i = threadIdx.x; i < Cv; i += T) {
             K[i] = -1;
             W[i] = 0.0;
         }
         __syncthreads();
-        const int32_t v = list[vi];
-        const int32_t rs = a.rowptr[v], re = a.rowptr[v + 1];
         const int32_t cur = a.comm[v];
         for (int32_t e0 = rs + threadIdx.x; e0 < re; e0 += 4 * T) {
             int32_t u[4];
@@ -253,15 +257,15 @@ __global__ __launch_bounds__(T) void k_move_wg(MoveArgs a, const int32_t *__rest
         const int32_t size_cur = rec_size(rcur);
         uint64_t best = pack_best(0.0f, cur);
         double bestK = 0.0, kc = 0.0;
-        for (int i0 = threadIdx.x; i0 < C; i0 += 4 * T) {
+        for (int i0 = threadIdx.x; i0 < Cv; i0 += 4 * T) {
             int32_t kk[4];
             double vv[4];
             uint64_t rr[4];
 #pragma unroll
             for (int j = 0; j < 4; ++j) {
                 const int i = i0 + j * T;
-                kk[j] = i < C ? K[i] : -1;
-                vv[j] = i < C ? W[i] : 0.0;
+                kk[j] = i < Cv ? K[i] : -1;
+                vv[j] = i < Cv ? W[i] : 0.0;
             }
 #pragma unroll
             for (int j = 0; j < 4; ++j) rr[j] = (kk[j] >= 0 && kk[j] != cur) ? a.crec[kk[j]] : 0ull;

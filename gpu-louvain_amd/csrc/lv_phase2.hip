@@ -131,7 +131,13 @@ __global__ __launch_bounds__(T) void k_merge_lds(AggArgs a, const int32_t *__res
         const int32_t msz = a.size[c];
         const int32_t f0 = a.mscan[m0];
         for (int i = threadIdx.x; i <= msz; i += T) mseg[i] = a.mscan[m0 + i] - f0;
-        for (int i = threadIdx.x; i < C; i += T) {
+        // table of the community: smallest power of two >= 2*D (load <= 1/2),
+        // at least 64 slots, at most the allocation
+        const int32_t Dc = a.mscan[m0 + msz] - f0;
+        const int cl = min(CAP_LOG2, max(6, Dc > 1 ? 32 - __clz(2 * Dc - 1) : 1));
+        const int Cv = 1 << cl;
+        const uint32_t shift = 32 - cl;
+        for (int i = threadIdx.x; i < Cv; i += T) {
             keys[i] = -1;
             vals[i] = 0.0;
         }
@@ -155,11 +161,11 @@ __global__ __launch_bounds__(T) void k_merge_lds(AggArgs a, const int32_t *__res
                 }
                 const int32_t e = base + f;
                 const int32_t key = a.newid[a.comm[a.col[e]]];
-                lds_insert(keys, vals, 32 - CAP_LOG2, C - 1, key, (double)a.w[e]);
+                lds_insert(keys, vals, shift, (uint32_t)Cv - 1, key, (double)a.w[e]);
             }
         }
         __syncthreads();
-        for (int s = threadIdx.x; s < C; s += T) {
+        for (int s = threadIdx.x; s < Cv; s += T) {
             const int32_t k = keys[s];
             if (k >= 0) sbuf[atomicAdd(counter, 1)] = ((uint64_t)(uint32_t)k << 32) | (uint32_t)s;
         }
