@@ -13,6 +13,11 @@ second per sweep.  Also reported: time to converge (full multi-level run),
 per-kernel HIP-event timings with the roofline of the dominant kernel, and
 the CPU oracle timed on a bounded sample (one sweep, same graph).
 
+With --level N the step is one sweep of level N (reached by plain phases
+and aggregations) from that level's singleton state, value = that level's
+CSR entries per second.  Also reported by default: the autotuner's choice,
+the learned-pruned vs plain comparison and the 4,096-graph batch.
+
 Prints ONE JSON line on stdout; progress goes to stderr.
 """
 import argparse
@@ -199,6 +204,8 @@ def main():
     ap.add_argument("--no-pruned", action="store_true")
     ap.add_argument("--no-batch", action="store_true")
     ap.add_argument("--no-tune", action="store_true")
+    ap.add_argument("--level", type=int, default=0,
+                    help="time sweeps of this level (reached by plain phases) from its singleton state")
     ap.add_argument("--calibrate-copy", action="store_true",
                     help="run only a plain 1 GiB device copy (PMC byte-counter calibration)")
     args = ap.parse_args()
@@ -226,18 +233,38 @@ def main():
         log(f"autotune: {tuned['tried']} candidates, default {tuned['default_ms']:.3f} ms/sweep -> "
             f"{tuned['ms']:.3f} ms with {tuned['options']}")
     G = L.Graph(csr, timing=True, **(tuned["options"] if tuned else {}))
+    E_step = csr.E
+    if args.level > 0:
+        import numpy as np
+
+        for _ in range(args.level):
+            G.optimise(1e-6)
+            G.aggregate()
+        V_l, E_step = G.level_size()
+        start = np.arange(V_l, dtype=np.int32)
+        log(f"level {args.level}: V={V_l} E={E_step}")
+
+        def step():
+            G.set_communities(start)
+            t = time.perf_counter()
+            r = G.sweep()
+            return r, time.perf_counter() - t
+    else:
+        def step():
+            t = time.perf_counter()
+            G.reset()
+            r = G.sweep()
+            return r, time.perf_counter() - t
     for i in range(args.warmup):
-        G.reset()
-        G.sweep()
+        step()
     log(f"warmup done ({args.warmup} sweeps)")
     G.kernel_timing_reset()
-    t0 = time.perf_counter()
     moved = 0
+    elapsed = 0.0
     for i in range(args.steps):
-        G.reset()
-        q, moved = G.sweep()
-    t1 = time.perf_counter()
-    ms_per_step = (t1 - t0) * 1e3 / max(args.steps, 1)
+        (q, moved), dt = step()
+        elapsed += dt
+    ms_per_step = elapsed * 1e3 / max(args.steps, 1)
     log(f"timed {args.steps} sweeps: {ms_per_step:.3f} ms/sweep, Q after sweep {q:.6f}, moved {moved}")
 
     kernels = {}
@@ -269,7 +296,7 @@ def main():
         achieved = d["bytes_per_launch"] / (d["avg_ms"] * 1e-3) / 1e9
         roofline = {"kernel": d["kernel"], "bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS,
                     "unit": "GB/s", "frac": achieved / HBM_PEAK_GBS,
-                    "traffic": traffic_for(d["kernel"], args.scale),
+                    "traffic": traffic_for(d["kernel"], args.scale) if args.level == 0 else None,
                     "bytes_per_launch": d["bytes_per_launch"], "avg_launch_ms": d["avg_ms"],
                     "share_of_sweep": d["ms_per_sweep"] / ms_per_step}
 
@@ -300,7 +327,7 @@ def main():
         cpu = cpu_baseline(csr, args.cpu_threads)
         log(f"cpu baseline: {cpu['value']:.3e} edges/s")
 
-    value = csr.E / (ms_per_step * 1e-3)
+    value = E_step / (ms_per_step * 1e-3)
     out = {
         "metric": "Louvain edges/s per sweep (plain, level 0) + time-to-converge",
         "value": value,
@@ -314,7 +341,8 @@ def main():
         "vs_baseline": None,
         "dtype": "fp32 gains / fp64 sums / int32 indices",
         "data": "synthetic (seeded RMAT generator)",
-        "config": {"workload": f"RMAT-{args.scale} ef16 a.57 b.19 c.19 seed {seed}, fp32 weights in (0,1]",
+        "config": {"workload": f"RMAT-{args.scale} ef16 a.57 b.19 c.19 seed {seed}, fp32 weights in (0,1]"
+                               + (f", sweeps of level {args.level}" if args.level else ""),
                    "V": csr.V, "E": csr.E, "min_gain": 1e-6, "parallelism": "single GPU"},
         "roofline": roofline,
         "cpu_baseline": cpu,

@@ -15,7 +15,7 @@ constexpr int kChunkEdges = 1024;    // max edges per chunk (stage A LDS: 2048 s
 constexpr int kChunkCapLog2 = 11;
 constexpr int kMergeCapLog2 = 11;    // stage B LDS: 2048 slots
 constexpr int kMergeLimit = 1536;    // pairs a stage-B table may take (else global)
-constexpr int kPartTarget = 512;     // expected pairs per partition
+constexpr int kPartTarget = 1024;    // expected pairs per partition
 constexpr int kMaxParts = 1024;
 constexpr int kMergeTile = 1024;     // chunks staged per pass in stage B
 

@@ -43,7 +43,7 @@ typedef struct {
     int32_t heavy_wg_max;    /* degree bound of the one-workgroup LDS tier (0 = 2048)  */
     int32_t chunk_edges;     /* edges per chunk of the chunked tier (0 = 1024)        */
     int32_t agg_lds_max;     /* summed-degree bound of LDS aggregation (0 = auto)     */
-    int32_t part_target;     /* pairs per merge partition of the chunked tier (0=512)  */
+    int32_t part_target;     /* pairs per merge partition of the chunked tier (0=1024) */
     int32_t merge_limit;     /* pairs an LDS merge table may take (0 = 1536)          */
     int32_t q_incremental;   /* track modularity incrementally after a sweep whose
                                 movers' rows hold at most E/q_incremental entries
