@@ -267,6 +267,10 @@ struct lv_graph {
     DevVec<double> item_K;
     DevVec<uint64_t> hbest;
     DevVec<double> hkcur;
+    DevVec<int32_t> hovf;
+    DevVec<int32_t> corder, iorder;  // XCD-aligned dispatch orders of the chunked tier
+    int32_t n_corder = 0, n_iorder = 0;
+    bool tier_xcd = false;
     DevVec<int32_t> gkeys;  // global hash arena, kept all-empty between uses
     DevVec<double> gvals;
 
@@ -456,6 +460,12 @@ struct lv_graph {
         t.best = hbest.p;
         t.kcur = hkcur.p;
         t.merge_limit = merge_limit;
+        t.ovf = hovf.p;
+        t.chunk_order = tier_xcd ? corder.p : nullptr;
+        t.item_order = tier_xcd ? iorder.p : nullptr;
+        t.n_chunk_order = n_corder;
+        t.n_item_order = n_iorder;
+        t.range_blk = tier_xcd ? (V + 7) / 8 : 0;
         return t;
     }
 
@@ -651,11 +661,16 @@ struct lv_graph {
             for (int b = 0; b < lv::kNumBuckets; ++b) h_bedges[b] = (int64_t)tmp[b];
         }
 
+        // split table of the heavy rows (XCD-local passes); needs only h_off
+        LV_TRY(setup_xcd());
+
         // chunked tier: row chunks (stage A, strength, modularity), per-vertex
         // partitions (stage B) and overflow tables
         const int HB = lv::kNumBuckets - 1;
         nh = h_off[HB + 1] - h_off[HB];
         nins = nitems = 0;
+        tier_xcd = false;
+        n_corder = n_iorder = 0;
         if (nh > 0) {
             LV_TRY(hrows.reserve(2 * (int64_t)nh));
             hipLaunchKernelGGL(k_heavy_rows, dim3(blocks_for(nh, 256)), dim3(256), 0, s, rp,
@@ -668,6 +683,17 @@ struct lv_graph {
             LV_HIP(hipMemcpyAsync(ids.data(), order + h_off[HB], sizeof(int32_t) * nh,
                                   hipMemcpyDeviceToHost, s));
             LV_TRY(sync());
+            // with sorted rows: chunks cut at the column-block boundaries and
+            // partitions by id range, both in an XCD-aligned dispatch order
+            // (measured slower than the hash partitions on RMAT-22: opt-in)
+            tier_xcd = xcd_ok && opt.xcd_rows == 2;
+            std::vector<int32_t> sp;
+            if (tier_xcd) {
+                sp.resize(9 * (size_t)nh);
+                LV_HIP(hipMemcpy(sp.data(), xsplit.p + 9 * (int64_t)(h_off[HB] - xh_lo),
+                                 sizeof(int32_t) * 9 * nh, hipMemcpyDeviceToHost));
+            }
+            std::vector<std::vector<int32_t>> xl(8), il(8);
             std::vector<lv::Chunk> hin;
             std::vector<lv::HeavyVertex> hh(nh);
             std::vector<lv::MergeItem> hit;
@@ -675,8 +701,15 @@ struct lv_graph {
             int32_t pbase = 0, tbase = 0;
             for (int32_t i = 0; i < nh; ++i) {
                 const int64_t lo = rows[2 * i], hi = rows[2 * i + 1], deg = hi - lo;
-                int32_t P = (int32_t)std::min<int64_t>(lv::kMaxParts, (deg + part_target - 1) / part_target);
-                P = std::max(P, 1);
+                int32_t P;
+                if (tier_xcd) {
+                    const int64_t m = std::max<int64_t>(1, (deg + 8 * (int64_t)part_target - 1) /
+                                                               (8 * (int64_t)part_target));
+                    P = (int32_t)std::min<int64_t>(lv::kMaxParts, 8 * m);
+                } else {
+                    P = (int32_t)std::min<int64_t>(lv::kMaxParts, (deg + part_target - 1) / part_target);
+                    P = std::max(P, 1);
+                }
                 lv::HeavyVertex &h = hh[i];
                 h.v = ids[i];
                 h.P = P;
@@ -684,24 +717,60 @@ struct lv_graph {
                 h.ptot_base = tbase;
                 h.tab_lg = std::max(6, ceil_log2(2 * deg));
                 h.tab_off = slots;
-                for (int64_t e = lo; e < hi; e += chunk_edges) {
+                auto add_chunk = [&](int64_t clo, int64_t chi, int x) {
                     lv::Chunk c{};
                     c.hv = i;
                     c.v = ids[i];
-                    c.lo = e;
-                    c.hi = std::min<int64_t>(hi, e + chunk_edges);
+                    c.lo = clo;
+                    c.hi = chi;
                     c.out = outs;
                     c.pbase = pbase;
                     c.P = P;
                     c.ptot_base = tbase;
-                    outs += c.hi - c.lo;
+                    c.first = hin.size() == (size_t)h.chunk0 ? 1 : 0;
+                    outs += chi - clo;
                     pbase += P + 1;
+                    if (x >= 0) xl[x].push_back((int32_t)hin.size());
                     hin.push_back(c);
+                };
+                if (tier_xcd) {
+                    for (int x = 0; x < 8; ++x) {
+                        const int64_t s0 = sp[9 * (size_t)i + x], s1 = sp[9 * (size_t)i + x + 1];
+                        for (int64_t e = s0; e < s1; e += chunk_edges)
+                            add_chunk(e, std::min<int64_t>(s1, e + chunk_edges), x);
+                    }
+                } else {
+                    for (int64_t e = lo; e < hi; e += chunk_edges)
+                        add_chunk(e, std::min<int64_t>(hi, e + chunk_edges), -1);
                 }
                 h.nchunks = (int32_t)hin.size() - h.chunk0;
-                for (int32_t p = 0; p < P; ++p) hit.push_back({i, p});
+                for (int32_t p = 0; p < P; ++p) {
+                    if (tier_xcd) il[p / (P / 8)].push_back((int32_t)hit.size());
+                    hit.push_back({i, p});
+                }
                 tbase += P;
                 slots += (int64_t)1 << h.tab_lg;
+            }
+            n_corder = n_iorder = 0;
+            if (tier_xcd) {
+                // workgroup 8j + x takes the j-th entry of XCD x's list
+                auto interleave = [](const std::vector<std::vector<int32_t>> &l,
+                                     std::vector<int32_t> &out) {
+                    size_t L = 0;
+                    for (const auto &q : l) L = std::max(L, q.size());
+                    out.assign(8 * L, -1);
+                    for (int x = 0; x < 8; ++x)
+                        for (size_t j = 0; j < l[x].size(); ++j) out[8 * j + x] = l[x][j];
+                };
+                std::vector<int32_t> co, io;
+                interleave(xl, co);
+                interleave(il, io);
+                n_corder = (int32_t)co.size();
+                n_iorder = (int32_t)io.size();
+                LV_TRY(corder.reserve(n_corder));
+                LV_TRY(iorder.reserve(n_iorder));
+                LV_HIP(hipMemcpy(corder.p, co.data(), sizeof(int32_t) * co.size(), hipMemcpyHostToDevice));
+                LV_HIP(hipMemcpy(iorder.p, io.data(), sizeof(int32_t) * io.size(), hipMemcpyHostToDevice));
             }
             nins = (int32_t)hin.size();
             nitems = (int32_t)hit.size();
@@ -717,12 +786,14 @@ struct lv_graph {
             LV_TRY(ptot.reserve(tbase));
             LV_TRY(hbest.reserve(nh));
             LV_TRY(hkcur.reserve(nh));
+            LV_TRY(hovf.reserve(nh));
             LV_HIP(hipMemcpy(ins.p, hin.data(), sizeof(lv::Chunk) * nins, hipMemcpyHostToDevice));
             LV_HIP(hipMemcpy(hvs.p, hh.data(), sizeof(lv::HeavyVertex) * nh, hipMemcpyHostToDevice));
             LV_HIP(hipMemcpy(items.p, hit.data(), sizeof(lv::MergeItem) * nitems,
                              hipMemcpyHostToDevice));
             lv::fill_i32(ptot.p, 0, tbase, s);
             lv::fill_f64(hkcur.p, 0.0, nh, s);
+            lv::fill_i32(hovf.p, 0, nh, s);
             hipLaunchKernelGGL(k_zero_u64_n, dim3(blocks_for(nh, 256)), dim3(256), 0, s,
                                reinterpret_cast<unsigned long long *>(hbest.p), (int64_t)nh);
             LV_LAUNCHED("chunked tier tables");
@@ -754,7 +825,6 @@ struct lv_graph {
         st.sq_block_start = blk;
         blk += (int32_t)((V + lv::kSqItems - 1) / lv::kSqItems);
         st.total_blocks = std::max(blk, 1);
-        LV_TRY(setup_xcd());
         LV_TRY(qpart.reserve(2 * ((int64_t)st.total_blocks + st_light.total_blocks +
                                   8 * (int64_t)xst.total_blocks) + 2 * 256));
         // per-group row segments for the modularity correction of the applies
@@ -1558,6 +1628,9 @@ struct lv_graph {
         hrows.release();
         hbest.release();
         hkcur.release();
+        hovf.release();
+        corder.release();
+        iorder.release();
         gkeys.release();
         gvals.release();
         item_best.release();

@@ -8,9 +8,12 @@ namespace lv {
 // Chunked tier (rows too long for one workgroup).  A heavy vertex's row is
 // cut into chunks of at most kChunkEdges edges; stage A aggregates each
 // chunk in LDS and writes its distinct (community, weight) pairs bucketed
-// by partition part_of(c, P); stage B merges one partition of one vertex in
-// LDS and evaluates its gains.  Chunks also drive the heavy rows of the
-// strength and modularity reductions.
+// by partition; stage B merges one partition of one vertex in LDS and
+// evaluates its gains.  With sorted rows the chunks are cut at the 8
+// column-block boundaries and dispatched to the XCD of their block, and the
+// partitions are community-id ranges dispatched to the XCD of their range,
+// so that both stages gather from one eighth of comm / crec.  Chunks also
+// drive the heavy rows of the strength and modularity reductions.
 constexpr int kChunkEdges = 1024;    // max edges per chunk (stage A LDS: 2048 slots)
 constexpr int kChunkCapLog2 = 11;
 constexpr int kMergeCapLog2 = 11;    // stage B LDS: 2048 slots
@@ -27,7 +30,7 @@ struct Chunk {
     int32_t pbase;      // offsets of its partitions in poff[pbase .. pbase+P]
     int32_t P;          // partitions of the vertex
     int32_t ptot_base;  // per-partition totals of the vertex in ptot[]
-    int32_t pad;
+    int32_t first;      // 1 for the vertex's first chunk
 };
 
 struct HeavyVertex {
@@ -54,11 +57,18 @@ struct ChunkedTier {
     int32_t *ptot;      // per vertex and partition: pairs (zeroed by stage B)
     int32_t *gkeys;     // overflow tables (kept all-empty between uses)
     double *gvals;
+    int32_t *ovf;       // per heavy vertex: its overflow table was used (cleared after the sweep step)
     uint64_t *best;     // per vertex packed argmax, 0 = none
     double *kcur;       // per vertex weight to its own community
     uint64_t *item_best;  // per merge item: its packed argmax
     double *item_K;       // ... and the weight to that candidate
     int32_t merge_limit;
+    // XCD-aligned dispatch (null = identity): workgroup b of stage A takes
+    // chunk chunk_order[b], of stage B item item_order[b]; -1 = padding.
+    const int32_t *chunk_order;
+    const int32_t *item_order;
+    int32_t n_chunk_order, n_item_order;
+    int32_t range_blk;    // > 0: partitions by community-id range (see tier_part)
 };
 
 // Segment of the vertex list processed with a fixed lane count per vertex

@@ -308,15 +308,39 @@ __global__ __launch_bounds__(T) void k_move_wg(MoveArgs a, const int32_t *__rest
 
 // Stage A: aggregate one chunk of a heavy row in LDS; write its distinct
 // (community, weight) pairs grouped by partition.
+// Partition of community c among the P merge items of a heavy vertex: a hash
+// (part_of), or with range_blk > 0 id ranges, P/8 per column block of
+// range_blk ids, so that the merge items of block x can run on XCD x.
+__device__ __forceinline__ uint32_t tier_part(const ChunkedTier &t, int32_t c, uint32_t P) {
+    if (t.range_blk > 0) {
+        const uint32_t blk = (uint32_t)t.range_blk;
+        const uint32_t m = P >> 3;
+        const uint32_t x = (uint32_t)c / blk;
+        const uint32_t r = (uint32_t)c - x * blk;
+        return x * m + (uint32_t)(((uint64_t)r * m) / blk);
+    }
+    return part_of(c, P);
+}
+
+// smallest power of two >= 2n (load <= 1/2), between 2^6 and 2^cap_lg
+__device__ __forceinline__ int table_log2(int64_t n, int cap_lg) {
+    const int lg = n > 1 ? 64 - __clzll((unsigned long long)(2 * n - 1)) : 1;
+    return min(cap_lg, max(6, lg));
+}
+
 __global__ __launch_bounds__(256) void k_chunk_aggregate(MoveArgs a, ChunkedTier t) {
     constexpr int C = 1 << kChunkCapLog2;
     __shared__ int32_t keys[C];
     __shared__ double vals[C];
     __shared__ int32_t pcnt[kMaxParts];
-    const Chunk ch = t.chunks[blockIdx.x];
+    const int32_t ci = t.chunk_order ? t.chunk_order[blockIdx.x] : (int32_t)blockIdx.x;
+    if (ci < 0) return;  // padding of the XCD dispatch order
+    const Chunk ch = t.chunks[ci];
     if (a.active && !a.active[ch.v]) return;  // not in this sweep's frontier
     const int P = ch.P;
-    for (int i = threadIdx.x; i < C; i += 256) {
+    const int cl = table_log2(ch.hi - ch.lo, kChunkCapLog2);
+    const int Cv = 1 << cl;
+    for (int i = threadIdx.x; i < Cv; i += 256) {
         keys[i] = -1;
         vals[i] = 0.0;
     }
@@ -324,15 +348,15 @@ __global__ __launch_bounds__(256) void k_chunk_aggregate(MoveArgs a, ChunkedTier
     __syncthreads();
     const int32_t v = ch.v;
     for (int64_t e = ch.lo + threadIdx.x; e < ch.hi; e += 256) {
-        const int32_t u = a.col[e];
-        const float wf = a.w[e];
+        const int32_t u = __builtin_nontemporal_load(&a.col[e]);
+        const float wf = __builtin_nontemporal_load(&a.w[e]);
         if (u == v) continue;
-        lds_insert(keys, vals, 32 - kChunkCapLog2, C - 1, a.comm[u], (double)wf);
+        lds_insert(keys, vals, 32 - cl, (uint32_t)Cv - 1, a.comm[u], (double)wf);
     }
     __syncthreads();
-    for (int s = threadIdx.x; s < C; s += 256) {
+    for (int s = threadIdx.x; s < Cv; s += 256) {
         const int32_t k = keys[s];
-        if (k >= 0) atomicAdd(&pcnt[part_of(k, (uint32_t)P)], 1);
+        if (k >= 0) atomicAdd(&pcnt[tier_part(t, k, (uint32_t)P)], 1);
     }
     __syncthreads();
     if (threadIdx.x < 64) {  // wave 0: exclusive scan of the partition counts
@@ -357,10 +381,10 @@ __global__ __launch_bounds__(256) void k_chunk_aggregate(MoveArgs a, ChunkedTier
         if (lane == 0) t.poff[ch.pbase + P] = carry;
     }
     __syncthreads();
-    for (int s = threadIdx.x; s < C; s += 256) {
+    for (int s = threadIdx.x; s < Cv; s += 256) {
         const int32_t k = keys[s];
         if (k < 0) continue;
-        const int32_t pos = atomicAdd(&pcnt[part_of(k, (uint32_t)P)], 1);
+        const int32_t pos = atomicAdd(&pcnt[tier_part(t, k, (uint32_t)P)], 1);
         t.out_key[ch.out + pos] = k;
         t.out_val[ch.out + pos] = vals[s];
     }
@@ -437,7 +461,8 @@ __global__ __launch_bounds__(kMergeThreads) void k_chunk_merge(MoveArgs a, Chunk
     __shared__ uint64_t red_best[kMergeThreads / 64];
     __shared__ double red_K[kMergeThreads / 64];
     __shared__ double red_kc[kMergeThreads / 64];
-    const int32_t item = blockIdx.x;
+    const int32_t item = t.item_order ? t.item_order[blockIdx.x] : (int32_t)blockIdx.x;
+    if (item < 0) return;  // padding of the XCD dispatch order
     const MergeItem it = t.items[item];
     const HeavyVertex h = t.hvs[it.hv];
     const int32_t v = h.v;
@@ -453,23 +478,25 @@ __global__ __launch_bounds__(kMergeThreads) void k_chunk_merge(MoveArgs a, Chunk
     uint64_t best = 0;
     double bestK = 0.0, kc = 0.0;
     if (n <= t.merge_limit) {
-        for (int i = threadIdx.x; i < C; i += kMergeThreads) {
+        const int cl = table_log2(n, kMergeCapLog2);
+        const int Cv = 1 << cl;
+        for (int i = threadIdx.x; i < Cv; i += kMergeThreads) {
             keys[i] = -1;
             vals[i] = 0.0;
         }
         __syncthreads();
         for_partition_pairs(t, h, it.p, seg_base, seg_off, [&](int32_t c, double x) {
-            lds_insert(keys, vals, 32 - kMergeCapLog2, C - 1, c, x);
+            lds_insert(keys, vals, 32 - cl, (uint32_t)Cv - 1, c, x);
         });
-        for (int i0 = threadIdx.x; i0 < C; i0 += 4 * kMergeThreads) {
+        for (int i0 = threadIdx.x; i0 < Cv; i0 += 4 * kMergeThreads) {
             int32_t kk[4];
             double vv[4];
             uint64_t rr[4];
 #pragma unroll
             for (int j = 0; j < 4; ++j) {
                 const int i = i0 + j * kMergeThreads;
-                kk[j] = i < C ? keys[i] : -1;
-                vv[j] = i < C ? vals[i] : 0.0;
+                kk[j] = i < Cv ? keys[i] : -1;
+                vv[j] = i < Cv ? vals[i] : 0.0;
             }
 #pragma unroll
             for (int j = 0; j < 4; ++j) rr[j] = (kk[j] >= 0 && kk[j] != cur) ? a.crec[kk[j]] : 0ull;
@@ -487,6 +514,7 @@ __global__ __launch_bounds__(kMergeThreads) void k_chunk_merge(MoveArgs a, Chunk
     } else {
         int32_t *K = t.gkeys + h.tab_off;
         double *W = t.gvals + h.tab_off;
+        if (threadIdx.x == 0) t.ovf[it.hv] = 1;
         const uint32_t cap = 1u << h.tab_lg;
         for_partition_pairs(t, h, it.p, seg_base, seg_off, [&](int32_t c, double x) {
             global_insert(K, W, 32 - h.tab_lg, cap - 1, c, x);
@@ -495,10 +523,10 @@ __global__ __launch_bounds__(kMergeThreads) void k_chunk_merge(MoveArgs a, Chunk
         __syncthreads();
         for (uint32_t i = threadIdx.x; i < cap; i += kMergeThreads) {
             const int32_t c = __hip_atomic_load(&K[i], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-            if (c < 0 || part_of(c, (uint32_t)h.P) != (uint32_t)it.p) continue;
+            if (c < 0 || tier_part(t, c, (uint32_t)h.P) != (uint32_t)it.p) continue;
             const double Kc = __hip_atomic_load(&W[i], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-            K[i] = -1;  // leave the table clean
-            W[i] = 0.0;
+            // the table stays as it is: other partitions of the vertex may
+            // still be inserting; k_clear_overflow empties it afterwards
             if (c == cur) {
                 kc = Kc;
                 continue;
@@ -530,7 +558,7 @@ __global__ __launch_bounds__(kMergeThreads) void k_chunk_merge(MoveArgs a, Chunk
         if (b != 0)
             atomicMax(reinterpret_cast<unsigned long long *>(&t.best[it.hv]),
                       (unsigned long long)b);
-        if (part_of(cur, (uint32_t)h.P) == (uint32_t)it.p) t.kcur[it.hv] = kk;
+        if (tier_part(t, cur, (uint32_t)h.P) == (uint32_t)it.p) t.kcur[it.hv] = kk;
         t.ptot[h.ptot_base + it.p] = 0;  // ready for the next sweep
     }
 }
@@ -555,6 +583,20 @@ __global__ void k_heavy_final(MoveArgs a, int32_t nh, ChunkedTier t) {
     write_decision(a, v, cur, best, bestK, t.kcur[hv]);
     t.best[hv] = 0;
     t.kcur[hv] = 0.0;
+}
+
+// Empties the overflow tables the merge items used (one workgroup per heavy
+// vertex), restoring the all-empty arena for the next sweep.
+__global__ void k_clear_overflow(ChunkedTier t, int32_t nh) {
+    const int32_t hv = blockIdx.x;
+    if (hv >= nh || !t.ovf[hv]) return;
+    const HeavyVertex h = t.hvs[hv];
+    const int64_t cap = (int64_t)1 << h.tab_lg;
+    for (int64_t i = threadIdx.x; i < cap; i += blockDim.x) {
+        t.gkeys[h.tab_off + i] = -1;
+        t.gvals[h.tab_off + i] = 0.0;
+    }
+    if (threadIdx.x == 0) t.ovf[hv] = 0;
 }
 
 __global__ void k_clear_tables(int32_t *keys, double *vals, int64_t n) {
@@ -736,8 +778,7 @@ __global__ __launch_bounds__(kRowThreads) void k_apply_finish(
     };
     if (b >= st.chunk_block_start) {
         const Chunk ch = chunks[b - st.chunk_block_start];
-        row(ch.v, ch.lo + threadIdx.x, ch.hi, kRowThreads,
-            threadIdx.x == 0 && ch.lo == a.rowptr[ch.v]);
+        row(ch.v, ch.lo + threadIdx.x, ch.hi, kRowThreads, threadIdx.x == 0 && ch.first);
     } else {
         int si = 0;
         while (si + 1 < st.nseg && b >= st.s[si + 1].block_start) ++si;
@@ -1156,18 +1197,21 @@ void launch_move_wg(int threads, int cap_log2, const MoveArgs &a, const int32_t 
 
 void launch_chunk_aggregate(const MoveArgs &a, const ChunkedTier &t, int32_t nchunks,
                             hipStream_t s) {
-    if (nchunks <= 0) return;
-    hipLaunchKernelGGL(k_chunk_aggregate, dim3(nchunks), dim3(256), 0, s, a, t);
+    const int32_t g = t.chunk_order ? t.n_chunk_order : nchunks;
+    if (g <= 0) return;
+    hipLaunchKernelGGL(k_chunk_aggregate, dim3(g), dim3(256), 0, s, a, t);
 }
 
 void launch_chunk_merge(const MoveArgs &a, const ChunkedTier &t, int32_t nitems, hipStream_t s) {
-    if (nitems <= 0) return;
-    hipLaunchKernelGGL(k_chunk_merge, dim3(nitems), dim3(kMergeThreads), 0, s, a, t);
+    const int32_t g = t.item_order ? t.n_item_order : nitems;
+    if (g <= 0) return;
+    hipLaunchKernelGGL(k_chunk_merge, dim3(g), dim3(kMergeThreads), 0, s, a, t);
 }
 
 void launch_heavy_final(const MoveArgs &a, const ChunkedTier &t, int32_t nh, hipStream_t s) {
     if (nh <= 0) return;
     hipLaunchKernelGGL(k_heavy_final, dim3(blocks_for(nh, 256)), dim3(256), 0, s, a, nh, t);
+    hipLaunchKernelGGL(k_clear_overflow, dim3(nh), dim3(256), 0, s, t, nh);
 }
 
 void launch_move_chunked(const MoveArgs &a, const ChunkedTier &t, int32_t nchunks,

@@ -48,8 +48,9 @@ typedef struct {
     int32_t q_incremental;   /* track modularity incrementally after a sweep whose
                                 movers' rows hold at most E/q_incremental entries
                                 (0 = 2, -1 = never)                                    */
-    int32_t xcd_rows;        /* heavy-row modularity pass by column block per XCD
-                                (0 = on when rows are sorted, -1 = off)             */
+    int32_t xcd_rows;        /* XCD-local passes over heavy rows when rows are sorted:
+                                0 = modularity pass, 2 = modularity pass and chunked
+                                tier, -1 = off                                      */
     int32_t reserved[7];
 } lv_options;
 

@@ -42,6 +42,13 @@ def _cases(lvlib):
     # force the chunked tier's global-table fallback (LDS merge limit 16 pairs)
     out.append(("rmat13_merge_overflow", lvlib.gen_rmat(13, 16, seed=43),
                 {"heavy_wg_max": 320, "chunk_edges": 128, "part_target": 100000, "merge_limit": 16}))
+    # the same two with the XCD-aligned chunked tier (column-block chunks,
+    # partitions by community-id range)
+    out.append(("rmat14_chunked_tier_xcd", lvlib.gen_rmat(14, 16, seed=41),
+                {"heavy_wg_max": 320, "chunk_edges": 64, "part_target": 64, "xcd_rows": 2}))
+    out.append(("rmat13_merge_overflow_xcd", lvlib.gen_rmat(13, 16, seed=43),
+                {"heavy_wg_max": 320, "chunk_edges": 128, "part_target": 100000, "merge_limit": 16,
+                 "xcd_rows": 2}))
     return out
 
 
@@ -53,7 +60,8 @@ def cases(lvlib):
 def _names():
     return ["two_triangles", "planted_c1_unit", "rmat12_real", "rmat14_real", "rmat13_unit",
             "random_selfloops_real", "star3000", "star20000", "rmat14_chunked_tier",
-            "rmat13_merge_overflow", "rmat14_incremental_q", "planted_incremental_q"]
+            "rmat13_merge_overflow", "rmat14_incremental_q", "planted_incremental_q",
+            "rmat14_chunked_tier_xcd", "rmat13_merge_overflow_xcd"]
 
 
 def _get(cases, name):
