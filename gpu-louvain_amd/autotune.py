@@ -31,7 +31,7 @@ SPACE = {
     "chunk_edges": [256, 512, 1024],
     "part_target": [256, 512, 1024],
     "merge_limit": [1024, 1536],
-    "xcd_rows": [0, 2, -1],
+    "xcd_rows": [0, 1, 2, -1],
 }
 _BOUNDS = [4, 8, 16, 32, 84, 319, 2048]
 

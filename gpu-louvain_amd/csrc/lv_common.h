@@ -11,14 +11,15 @@ namespace lv {
 // ---------------------------------------------------------------------
 // Phase-1 execution buckets.  Buckets 0..5 coincide with the reference's
 // degree groups 0..5 (modularity_optimisation.cuh:7-19); group 6 (deg > 319)
-// is split by degree into a one-workgroup LDS tier and a chunked tier
-// (rows split over many workgroups, merged by hash partition in LDS).
-// Splitting a group changes only the kernel mapping: decisions inside a
-// group are simultaneous.
+// is split by degree into two one-workgroup LDS tiers (tables sized for
+// their degree bound, so the lower one runs more workgroups per CU) and a
+// chunked tier (rows split over many workgroups, merged by partition in
+// LDS).  Splitting a group changes only the kernel mapping: decisions inside
+// a group are simultaneous.
 // ---------------------------------------------------------------------
-constexpr int kNumBuckets = 8;
+constexpr int kNumBuckets = 9;
 constexpr int kNumGroups = 7;
-constexpr int kBucketGroup[kNumBuckets] = {0, 1, 2, 3, 4, 5, 6, 6};
+constexpr int kBucketGroup[kNumBuckets] = {0, 1, 2, 3, 4, 5, 6, 6, 6};
 constexpr int kGroupBound[kNumGroups + 1] = {0, 4, 8, 16, 32, 84, 319, 0x7fffffff};
 
 struct BucketBounds {

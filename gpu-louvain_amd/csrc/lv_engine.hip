@@ -160,6 +160,8 @@ unsigned blocks_for(int64_t n, int per) { return (unsigned)((n + per - 1) / per)
 
 constexpr int kSlotChunk = 4096;  // hash slots per evaluation chunk
 constexpr int kAggChunk = 4096;   // flat member edges per aggregation chunk
+constexpr int32_t kWgSplit = 1024;  // degree bound of the lower one-workgroup bucket
+constexpr int32_t kXcdMinVertices = 1 << 21;  // XCD-local row passes from this level size
 // A phase with a frontier sweeps only full after this many sweeps (bounds a
 // phase whose pruned and full sweeps keep undoing each other).
 constexpr int64_t kMaxPrunedSweeps = 4096;
@@ -245,6 +247,7 @@ struct lv_graph {
     lv::BucketBounds bb{};
     int wg6_lg = 12;             // LDS slots (log2) of the one-workgroup tier (max)
     int wg_lg = 12;              // slots allocated per launch: 2 * the tier's degree bound
+    int wg_lg_lo = 11;           // ... of the lower one-workgroup bucket
     int chunk_edges = lv::kChunkEdges, part_target = lv::kPartTarget, merge_limit = lv::kMergeLimit;
     lv::AggBounds agg_bounds{};
     int8_t *keys8 = nullptr;
@@ -497,11 +500,18 @@ struct lv_graph {
         // and the chunked tier.
         int wg_max = opt.heavy_wg_max > 0 ? opt.heavy_wg_max : (1 << wg6_lg) / 2;
         wg_max = std::max(320, std::min(wg_max, (1 << wg6_lg) / 2));
-        const int32_t his[lv::kNumBuckets] = {4, 8, 16, 32, 84, 319, wg_max, 0x7fffffff};
+        const int32_t wg_mid = std::min(wg_max, kWgSplit);
+        const int32_t his[lv::kNumBuckets] = {4, 8, 16, 32, 84, 319, wg_mid, wg_max, 0x7fffffff};
         for (int b = 0; b < lv::kNumBuckets; ++b) bb.hi[b] = his[b];
-        // smaller tables for a lower degree bound: more workgroups per CU
-        wg_lg = 9;
-        while ((1 << wg_lg) < 2 * wg_max) ++wg_lg;
+        // tables of 2 * the bucket's degree bound: the lower bucket runs more
+        // workgroups per CU
+        auto lg_for = [](int32_t bound) {
+            int lg = 9;
+            while ((1 << lg) < 2 * bound) ++lg;
+            return lg;
+        };
+        wg_lg_lo = lg_for(wg_mid);
+        wg_lg = lg_for(wg_max);
         chunk_edges = opt.chunk_edges > 0 ? std::max(16, std::min(opt.chunk_edges, lv::kChunkEdges))
                                           : lv::kChunkEdges;
         part_target = opt.part_target > 0 ? std::max(16, opt.part_target) : lv::kPartTarget;
@@ -800,7 +810,7 @@ struct lv_graph {
         }
 
         // row-reduction segments: lanes per vertex by bucket
-        static const int lanes_lg[lv::kNumBuckets - 1] = {2, 3, 4, 5, 6, 6, 8};
+        static const int lanes_lg[lv::kNumBuckets - 1] = {2, 3, 4, 5, 6, 6, 8, 8};
         st = lv::SegTable{};
         int32_t blk = 0;
         for (int b = 0; b < HB; ++b) {
@@ -875,7 +885,10 @@ struct lv_graph {
         constexpr int XB = 4;  // first heavy bucket (degree > 32)
         xh_lo = h_off[XB];
         xh_n = h_off[lv::kNumBuckets] - h_off[XB];
+        // worth it only when comm (4V bytes) is well beyond one XCD's 4 MB
+        // L2; below that the plain row pass already hits L2 often
         if (opt.xcd_rows < 0 || xh_n <= 0 || V < 8) return LV_OK;
+        if (opt.xcd_rows == 0 && V < kXcdMinVertices) return LV_OK;
         hipStream_t s = stream;
         LV_TRY(xsplit.reserve(9 * (int64_t)xh_n));
         int32_t *flag = a_scalars + 8;
@@ -907,7 +920,7 @@ struct lv_graph {
         blk += (int32_t)((V + lv::kSqItems - 1) / lv::kSqItems);
         st_light.total_blocks = blk;
         // heavy rows: a column-block segment of a row has about deg/8 entries
-        static const int lanes_heavy[lv::kNumBuckets - XB] = {3, 4, 6, 8};
+        static const int lanes_heavy[lv::kNumBuckets - XB] = {3, 4, 6, 8, 8};
         int32_t j = 0;
         for (int b = XB; b < lv::kNumBuckets; ++b) {
             const int32_t cnt = h_off[b + 1] - h_off[b];
@@ -953,9 +966,9 @@ struct lv_graph {
                 lv::launch_move_wave(b == 4 ? 256 : 512, a, list, n, s);
                 tend(t0, LV_K_MOVE_WAVE, bytes, LV_KD_MOVE_BUCKET0 + b);
                 break;
-            case 6:
+            case 6: case 7:
                 t0 = tbegin();
-                lv::launch_move_wg(256, wg_lg, a, list, n, s);
+                lv::launch_move_wg(b - 6, b == 6 ? wg_lg_lo : wg_lg, a, list, n, s);
                 tend(t0, LV_K_MOVE_WG, bytes, LV_KD_MOVE_BUCKET0 + b);
                 break;
             default: {

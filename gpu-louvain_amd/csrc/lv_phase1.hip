@@ -15,7 +15,8 @@
 //                   the X lanes (row order, so sums are bitwise the
 //                   sequential ones), argmax by xor-shuffle of packed keys.
 //   k_move_wave<C>  one wavefront per vertex, C-slot LDS hash per wave.
-//   k_move_wg<T>    one T-lane workgroup per vertex, LDS hash sized per launch.
+//   k_move_wg<T,i>  one T-lane workgroup per vertex, LDS hash sized per launch
+//                   (i = 0, 1: the two workgroup buckets).
 //   k_chunk_*       rows beyond one workgroup: stage A aggregates 2048-edge
 //                   chunks in LDS and buckets the distinct pairs by hash
 //                   partition; stage B merges each (vertex, partition) in
@@ -207,7 +208,8 @@ __global__ __launch_bounds__(256) void k_move_wave(MoveArgs a, const int32_t *__
     if (lane == 0) write_decision(a, v, cur, best, bestK, kc);
 }
 
-template <int T>
+// TIER only names the instance (one per workgroup bucket) for profiles.
+template <int T, int TIER>
 __global__ __launch_bounds__(T) void k_move_wg(MoveArgs a, const int32_t *__restrict__ list,
                                                int32_t n, int cap_log2) {
     extern __shared__ double dyn_lds[];
@@ -1186,13 +1188,15 @@ void launch_move_wave(int cap, const MoveArgs &a, const int32_t *list, int32_t n
         hipLaunchKernelGGL(k_move_wave<512>, dim3(g), dim3(256), 0, s, a, list, n);
 }
 
-void launch_move_wg(int threads, int cap_log2, const MoveArgs &a, const int32_t *list, int32_t n,
+void launch_move_wg(int tier, int cap_log2, const MoveArgs &a, const int32_t *list, int32_t n,
                     hipStream_t s) {
-    (void)threads;
     if (n <= 0) return;
     const size_t lds = (size_t)12 << cap_log2;
     const unsigned g = (unsigned)(n < 1048576 ? n : 1048576);
-    hipLaunchKernelGGL(k_move_wg<256>, dim3(g), dim3(256), lds, s, a, list, n, cap_log2);
+    if (tier == 0)
+        hipLaunchKernelGGL((k_move_wg<256, 0>), dim3(g), dim3(256), lds, s, a, list, n, cap_log2);
+    else
+        hipLaunchKernelGGL((k_move_wg<256, 1>), dim3(g), dim3(256), lds, s, a, list, n, cap_log2);
 }
 
 void launch_chunk_aggregate(const MoveArgs &a, const ChunkedTier &t, int32_t nchunks,
@@ -1317,7 +1321,9 @@ void launch_compose(int32_t *orig2comm, int32_t V0, const int32_t *map, hipStrea
 }
 
 void set_wg_lds_limits(int bytes) {
-    (void)hipFuncSetAttribute(reinterpret_cast<const void *>(&k_move_wg<256>),
+    (void)hipFuncSetAttribute(reinterpret_cast<const void *>(&k_move_wg<256, 0>),
+                              hipFuncAttributeMaxDynamicSharedMemorySize, bytes);
+    (void)hipFuncSetAttribute(reinterpret_cast<const void *>(&k_move_wg<256, 1>),
                               hipFuncAttributeMaxDynamicSharedMemorySize, bytes);
 }
 

@@ -40,7 +40,7 @@ typedef struct lv_graph lv_graph;
 typedef struct {
     int32_t device;          /* HIP device ordinal (default 0)                       */
     int32_t timing;          /* 1: record per-kernel HIP events (lv_kernel_time)      */
-    int32_t heavy_wg_max;    /* degree bound of the one-workgroup LDS tier (0 = 2048)  */
+    int32_t heavy_wg_max;    /* degree bound of the one-workgroup LDS tiers (0 = 2048) */
     int32_t chunk_edges;     /* edges per chunk of the chunked tier (0 = 1024)        */
     int32_t agg_lds_max;     /* summed-degree bound of LDS aggregation (0 = auto)     */
     int32_t part_target;     /* pairs per merge partition of the chunked tier (0=1024) */
@@ -49,8 +49,8 @@ typedef struct {
                                 movers' rows hold at most E/q_incremental entries
                                 (0 = 2, -1 = never)                                    */
     int32_t xcd_rows;        /* XCD-local passes over heavy rows when rows are sorted:
-                                0 = modularity pass, 2 = modularity pass and chunked
-                                tier, -1 = off                                      */
+                                0 = modularity pass on levels with V >= 2^21, 1 = it
+                                on every level, 2 = it and the chunked tier, -1 = off */
     int32_t reserved[7];
 } lv_options;
 
@@ -91,14 +91,15 @@ typedef struct {
 #define LV_K_COUNT 7
 
 /* Single-kernel timing ids (lv_kernel_timing_detail): one HIP kernel each. */
-#define LV_KD_MOVE_BUCKET0 0     /* +b, b = 0..6: move kernel of execution bucket b     */
-#define LV_KD_CHUNK_AGGREGATE 7  /* chunked tier stage A (bytes: 12 per heavy edge)      */
-#define LV_KD_CHUNK_MERGE 8      /* chunked tier stage B (bytes not counted)             */
-#define LV_KD_CHUNK_FINAL 9      /* chunked tier decisions (bytes not counted)           */
-#define LV_KD_MODULARITY_ROWS 10 /* modularity row pass (12 B per edge it reads + 16 V)  */
-#define LV_KD_FRONTIER 11        /* frontier model inference (43 bytes per vertex)       */
-#define LV_KD_MODULARITY_XCD 12  /* heavy-row modularity pass by column block (12 B/edge)  */
-#define LV_KD_COUNT 13
+#define LV_KD_MOVE_BUCKET0 0     /* +b, b = 0..7: move kernel of execution bucket b
+                                    (0-3 sub-wave, 4-5 wave, 6-7 workgroup)          */
+#define LV_KD_CHUNK_AGGREGATE 8  /* chunked tier stage A (bytes: 12 per heavy edge)      */
+#define LV_KD_CHUNK_MERGE 9      /* chunked tier stage B (bytes not counted)             */
+#define LV_KD_CHUNK_FINAL 10     /* chunked tier decisions (bytes not counted)           */
+#define LV_KD_MODULARITY_ROWS 11 /* modularity row pass (12 B per edge it reads + 16 V)  */
+#define LV_KD_FRONTIER 12        /* frontier model inference (43 bytes per vertex)       */
+#define LV_KD_MODULARITY_XCD 13  /* heavy-row modularity pass by column block (12 B/edge)  */
+#define LV_KD_COUNT 14
 
 /* ---- graph handle (utils.cu:64-106 copyStructures + initM mo.cu:420-422) ---- */
 /* Uploads the CSR (rowptr[V+1], col[E], w[E]; host pointers) and sets up the

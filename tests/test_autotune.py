@@ -43,7 +43,7 @@ def test_every_candidate_gives_the_same_sweep(lvlib):
         ref_q, ref_m = G.sweep()
         ref_c = G.communities().copy()
         space = {"heavy_wg_max": [320, 2048], "chunk_edges": [64, 1024], "part_target": [32, 512],
-                 "merge_limit": [16, 1536], "xcd_rows": [0, 2, -1]}
+                 "merge_limit": [16, 1536], "xcd_rows": [1, 2, -1]}
         for opt in T.candidates(space):
             G.set_options(**opt)
             G.reset()

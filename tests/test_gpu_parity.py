@@ -42,6 +42,8 @@ def _cases(lvlib):
     # force the chunked tier's global-table fallback (LDS merge limit 16 pairs)
     out.append(("rmat13_merge_overflow", lvlib.gen_rmat(13, 16, seed=43),
                 {"heavy_wg_max": 320, "chunk_edges": 128, "part_target": 100000, "merge_limit": 16}))
+    # the column-blocked (XCD-local) modularity pass forced on a small graph
+    out.append(("rmat14_xcd_q", lvlib.gen_rmat(14, 16, seed=46), {"xcd_rows": 1}))
     # the same two with the XCD-aligned chunked tier (column-block chunks,
     # partitions by community-id range)
     out.append(("rmat14_chunked_tier_xcd", lvlib.gen_rmat(14, 16, seed=41),
@@ -61,7 +63,7 @@ def _names():
     return ["two_triangles", "planted_c1_unit", "rmat12_real", "rmat14_real", "rmat13_unit",
             "random_selfloops_real", "star3000", "star20000", "rmat14_chunked_tier",
             "rmat13_merge_overflow", "rmat14_incremental_q", "planted_incremental_q",
-            "rmat14_chunked_tier_xcd", "rmat13_merge_overflow_xcd"]
+            "rmat14_xcd_q", "rmat14_chunked_tier_xcd", "rmat13_merge_overflow_xcd"]
 
 
 def _get(cases, name):
@@ -228,7 +230,7 @@ def test_xcd_modularity_pass_matches_plain_pass(lvlib, oracle, scale, seed):
     """The column-blocked heavy-row pass gives the same Q as the row pass."""
     g = lvlib.gen_rmat(scale, 16, seed=seed)
     og = oracle.Graph.from_csr(g)
-    with lvlib.Graph(g, timing=True) as A, lvlib.Graph(g, xcd_rows=-1) as B:
+    with lvlib.Graph(g, timing=True, xcd_rows=1) as A, lvlib.Graph(g, xcd_rows=-1) as B:
         for s in range(3):
             qa, ma = A.sweep()
             qb, mb = B.sweep()
