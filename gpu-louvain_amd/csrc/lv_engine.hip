@@ -23,6 +23,7 @@
 #include "../../include/louvain.h"
 #include "lv_common.h"
 #include "lv_error.h"
+#include "lv_aggsort.h"
 #include "lv_batch.h"
 #include "lv_learn.h"
 #include "lv_phase1.h"
@@ -302,6 +303,11 @@ struct lv_graph {
     DevVec<int64_t> a_toff, a_soff;
     DevVec<int32_t> a_lg, a_D;
     DevVec<uint64_t> sortA, sortB;
+    // sort-based aggregation: key/value ping-pong, run heads and slots, sums
+    DevVec<uint64_t> skA, skB;
+    DevVec<float> svA, svB;
+    DevVec<int32_t> shead, spos, sscratch;
+    DevVec<double> sacc;
 
     // pinned host staging
     void *h_stage = nullptr;
@@ -1268,6 +1274,7 @@ struct lv_graph {
     // ---------------------------------------------------------------- phase 2
     lv_status aggregate() {
         if (!level_ready) LV_TRY(setup_level());
+        if (opt.agg_mode == 0) return aggregate_sorted();
         hipStream_t s = stream;
         hipEvent_t t0 = tbegin();
         const int32_t *rp = rowptr();
@@ -1566,6 +1573,81 @@ struct lv_graph {
         return LV_OK;
     }
 
+    // Aggregation by radix sort (community_aggregation.cu:248-347 computes
+    // the same CSR by per-community hash merges): keys (new id of the row's
+    // community, new id of the column's community) of every CSR entry are
+    // sorted, equal keys summed in FP64, and the unique keys are the next
+    // level's rows and columns in ascending new id.
+    lv_status aggregate_sorted() {
+        hipStream_t s = stream;
+        hipEvent_t t0 = tbegin();
+        int32_t *newV_d = a_scalars;
+        int32_t *etot_d = a_scalars + 1;
+        int32_t *U_d = a_scalars + 2;
+        // renumbering (community_aggregation.cu:267-275)
+        lv::launch_present(size, V, a_newid, s);
+        lv::exclusive_scan(a_newid, a_newid, V, newV_d, scratch, s);
+        if (G > 0) lv::launch_graph_offsets(vptr, G, V, a_newid, newV_d, vptr_next, s);
+        LV_LAUNCHED("renumbering");
+        LV_TRY(stage(64));
+        LV_HIP(hipMemcpyAsync(h_stage, newV_d, sizeof(int32_t), hipMemcpyDeviceToHost, s));
+        LV_TRY(sync());
+        int32_t newV = 0;
+        std::memcpy(&newV, h_stage, sizeof newV);
+        const int nb = cur_buf == 0 ? 1 : 0;
+        if (E > 0) {
+            LV_TRY(skA.reserve(E));
+            LV_TRY(skB.reserve(E));
+            LV_TRY(svA.reserve(E));
+            LV_TRY(svB.reserve(E));
+            LV_TRY(shead.reserve(E));
+            LV_TRY(spos.reserve(E));
+            LV_TRY(sacc.reserve(E));
+            LV_TRY(sscratch.reserve(std::max(lv::radix_scratch_elems(E), lv::scan_scratch_elems(E) + 8)));
+            lv::launch_agg_pairs(rowptr(), col(), w(), comm, a_newid, order, st, ins.p, skA.p, svA.p,
+                                 s);
+            // digits of the column id, then of the row id (stable LSD)
+            int bits = 1;
+            while ((1ll << bits) < (int64_t)newV) ++bits;
+            int shifts[8], np = 0;
+            for (int b = 0; b < bits; b += 8) shifts[np++] = b;
+            for (int b = 0; b < bits; b += 8) shifts[np++] = 32 + b;
+            const bool in_b = lv::radix_sort_pairs(skA.p, svA.p, skB.p, svB.p, E, shifts, np,
+                                                   sscratch.p, s);
+            uint64_t *keys = in_b ? skB.p : skA.p;
+            float *vals = in_b ? svB.p : svA.p;
+            uint64_t *ukey = in_b ? skA.p : skB.p;  // the free key buffer
+            lv::launch_run_heads(keys, E, shead.p, s);
+            lv::exclusive_scan(shead.p, spos.p, E, U_d, sscratch.p, s);
+            lv::fill_f64(sacc.p, 0.0, E, s);
+            lv::launch_run_reduce(keys, vals, shead.p, spos.p, E, ukey, sacc.p, s);
+            lv::fill_i32(a_cnt, 0, (int64_t)newV + 1, s);
+            lv::launch_unique_rows(ukey, sacc.p, U_d, a_cnt, colb[nb], wb[nb], E, s);
+        } else {
+            lv::fill_i32(a_cnt, 0, (int64_t)newV + 1, s);
+            lv::fill_i32(U_d, 0, 1, s);
+        }
+        lv::exclusive_scan(a_cnt, rpb[nb], (int64_t)newV + 1, etot_d, scratch, s);
+        // updateOriginalToCommunity (community_aggregation.cu:222-228)
+        lv::launch_compose(orig2comm, V0, a_newid, s);
+        LV_LAUNCHED("sort aggregation");
+        tend(t0, LV_K_AGGREGATE, 24.0 * (double)E + 40.0 * (double)V);
+        LV_HIP(hipMemcpyAsync(h_stage, U_d, sizeof(int32_t), hipMemcpyDeviceToHost, s));
+        LV_TRY(sync());
+        int32_t newE = 0;
+        std::memcpy(&newE, h_stage, sizeof newE);
+        cur_buf = nb;
+        V = newV;
+        E = newE;
+        level_ready = false;
+        if (G > 0) {
+            std::swap(vptr, vptr_next);
+            lv::launch_graph_ids(vptr, G, gid, s);
+            LV_LAUNCHED("graph ids");
+        }
+        return LV_OK;
+    }
+
     lv_status destroy() {
         if (stream) (void)hipStreamSynchronize(stream);
         dev_free(rp0);
@@ -1657,6 +1739,14 @@ struct lv_graph {
         a_D.release();
         sortA.release();
         sortB.release();
+        skA.release();
+        skB.release();
+        svA.release();
+        svB.release();
+        shead.release();
+        spos.release();
+        sscratch.release();
+        sacc.release();
         if (h_stage) (void)hipHostFree(h_stage);
         h_stage = nullptr;
         for (auto &r : pending) {

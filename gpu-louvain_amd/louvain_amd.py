@@ -72,7 +72,8 @@ class Options(ctypes.Structure):
         ("merge_limit", ctypes.c_int32),
         ("q_incremental", ctypes.c_int32),
         ("xcd_rows", ctypes.c_int32),
-        ("reserved", ctypes.c_int32 * 7),
+        ("agg_mode", ctypes.c_int32),
+        ("reserved", ctypes.c_int32 * 6),
     ]
 
 
@@ -293,7 +294,7 @@ class Graph:
 
     def __init__(self, g: CSR, device: int = 0, timing: bool = False, heavy_wg_max: int = 0,
                  chunk_edges: int = 0, part_target: int = 0, merge_limit: int = 0, agg_lds_max: int = 0,
-                 q_incremental: int = 0, xcd_rows: int = 0):
+                 q_incremental: int = 0, xcd_rows: int = 0, agg_mode: int = 0):
         lib = load()
         self._lib = lib
         self._rowptr = np.ascontiguousarray(g.rowptr, dtype=np.int32)
@@ -311,6 +312,7 @@ class Graph:
         opt.agg_lds_max = agg_lds_max
         opt.q_incremental = q_incremental
         opt.xcd_rows = xcd_rows
+        opt.agg_mode = agg_mode
         h = ctypes.c_void_p()
         col = self._col if self._col.size else np.zeros(1, np.int32)
         w = self._w if self._w.size else np.zeros(1, np.float32)
@@ -409,7 +411,7 @@ class Graph:
 
     def set_options(self, heavy_wg_max: int = 0, chunk_edges: int = 0, part_target: int = 0,
                     merge_limit: int = 0, agg_lds_max: int = 0, q_incremental: int = 0,
-                    xcd_rows: int = 0) -> None:
+                    xcd_rows: int = 0, agg_mode: int = 0) -> None:
         """New tier options for this handle (results do not depend on them)."""
         opt = Options()
         opt.device = self.device
@@ -420,6 +422,7 @@ class Graph:
         opt.agg_lds_max = agg_lds_max
         opt.q_incremental = q_incremental
         opt.xcd_rows = xcd_rows
+        opt.agg_mode = agg_mode
         _check(self._lib.lv_set_options(self._h, ctypes.byref(opt)))
 
     def kernel_timing_detail(self, kernel: int) -> dict:

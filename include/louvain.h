@@ -51,7 +51,9 @@ typedef struct {
     int32_t xcd_rows;        /* XCD-local passes over heavy rows when rows are sorted:
                                 0 = modularity pass on levels with V >= 2^21, 1 = it
                                 on every level, 2 = it and the chunked tier, -1 = off */
-    int32_t reserved[7];
+    int32_t agg_mode;        /* aggregation: 0 = radix sort of (row, column) community
+                                keys, 1 = per-community hash merges                 */
+    int32_t reserved[6];
 } lv_options;
 
 /* Run statistics (lv_run).  Times are device times from HIP events. */

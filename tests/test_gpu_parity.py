@@ -42,6 +42,9 @@ def _cases(lvlib):
     # force the chunked tier's global-table fallback (LDS merge limit 16 pairs)
     out.append(("rmat13_merge_overflow", lvlib.gen_rmat(13, 16, seed=43),
                 {"heavy_wg_max": 320, "chunk_edges": 128, "part_target": 100000, "merge_limit": 16}))
+    # aggregation by per-community hash merges instead of the radix sort
+    out.append(("rmat14_hash_agg", lvlib.gen_rmat(14, 16, seed=47), {"agg_mode": 1}))
+    out.append(("planted_hash_agg", lvlib.gen_planted(8000, 40, 0.02, 0.0008, seed=48), {"agg_mode": 1}))
     # the column-blocked (XCD-local) modularity pass forced on a small graph
     out.append(("rmat14_xcd_q", lvlib.gen_rmat(14, 16, seed=46), {"xcd_rows": 1}))
     # the same two with the XCD-aligned chunked tier (column-block chunks,
@@ -63,7 +66,8 @@ def _names():
     return ["two_triangles", "planted_c1_unit", "rmat12_real", "rmat14_real", "rmat13_unit",
             "random_selfloops_real", "star3000", "star20000", "rmat14_chunked_tier",
             "rmat13_merge_overflow", "rmat14_incremental_q", "planted_incremental_q",
-            "rmat14_xcd_q", "rmat14_chunked_tier_xcd", "rmat13_merge_overflow_xcd"]
+            "rmat14_xcd_q", "rmat14_chunked_tier_xcd", "rmat13_merge_overflow_xcd",
+            "rmat14_hash_agg", "planted_hash_agg"]
 
 
 def _get(cases, name):
@@ -184,16 +188,18 @@ def test_rerun_after_reset_is_identical(lvlib):
     assert np.array_equal(o1, o2)
 
 
-def test_global_tier_aggregation_with_large_sort(lvlib, oracle):
-    """A community with > 8192 distinct neighbour communities goes through
-    the global hash and the merge-path sort."""
+@pytest.mark.parametrize("agg_mode", [0, 1])
+def test_global_tier_aggregation_with_large_sort(lvlib, oracle, agg_mode):
+    """A community with > 8192 distinct neighbour communities: the hash path
+    (agg_mode 1) takes the global hash and the merge-path sort; the sort
+    path a long run of one row."""
     rowptr, col, w = graphs.star(20000, extra_ring=False)
     g = _csr(lvlib, rowptr, col, w)
     og = oracle.Graph.from_csr(g)
     comm = np.arange(g.V, dtype=np.int32)
     comm[1:50] = 0  # hub community holds 49 leaves; 19951 leaf singletons remain
     o2c = np.arange(g.V, dtype=np.int32)
-    with lvlib.Graph(g) as G:
+    with lvlib.Graph(g, agg_mode=agg_mode) as G:
         G.set_communities(comm)
         G.aggregate()
         h = G.level_graph()
