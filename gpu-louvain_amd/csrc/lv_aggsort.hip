@@ -32,21 +32,27 @@ __global__ __launch_bounds__(kSortThreads) void k_radix_hist(const uint64_t *__r
 // Stable scatter of one tile.  Wave w owns items [w*512, w*512+512) of the
 // tile, taken as 8 rounds of 64 consecutive items; a lane's rank among the
 // equal digits of its round comes from 8 ballots (one per digit bit), and
-// per-wave digit counters carry the ranks across rounds.  Waves are then
-// offset by the counts of the waves before them, tiles by the scanned
-// histogram.
+// per-wave digit counters carry the ranks across rounds.  The tile is then
+// reordered by digit in LDS, so that the stores of consecutive lanes go to
+// consecutive addresses of each digit's output range (offset by the scanned
+// histogram).
 __global__ __launch_bounds__(kSortThreads) void k_radix_scatter(
     const uint64_t *__restrict__ keys, const float *__restrict__ vals, int64_t n, int shift,
     int64_t ntiles, const int32_t *__restrict__ offs, uint64_t *okeys, float *ovals) {
     constexpr int NW = kSortThreads / 64;
     __shared__ int32_t wcnt[NW][256];
+    __shared__ int32_t tstart[256];     // first tile-local slot of each digit
+    __shared__ int32_t gbase[256];      // output offset of each digit minus tstart
+    __shared__ uint64_t lk[kSortTile];
+    __shared__ float lv[kSortTile];
     const int lane = threadIdx.x & 63, wid = threadIdx.x >> 6;
     for (int d = lane; d < 256; d += 64) wcnt[wid][d] = 0;
     __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
     __builtin_amdgcn_wave_barrier();
     __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
     const uint64_t lt = lane == 0 ? 0ull : (~0ull >> (64 - lane));
-    const int64_t base = (int64_t)blockIdx.x * kSortTile + (int64_t)wid * (kSortItems * 64);
+    const int64_t tile0 = (int64_t)blockIdx.x * kSortTile;
+    const int64_t base = tile0 + (int64_t)wid * (kSortItems * 64);
     uint64_t k[kSortItems];
     float v[kSortItems];
     int dg[kSortItems], rk[kSortItems];
@@ -79,15 +85,49 @@ __global__ __launch_bounds__(kSortThreads) void k_radix_scatter(
         __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
     }
     __syncthreads();
+    // per digit: tile count, tile-local start (block exclusive scan), and
+    // the wave offsets within the digit (counts of the earlier waves)
+    {
+        const int d = threadIdx.x;  // 256 threads = 256 digits
+        int32_t c = 0;
+        for (int w2 = 0; w2 < NW; ++w2) {
+            const int32_t x = wcnt[w2][d];
+            wcnt[w2][d] = c;  // becomes the wave offset
+            c += x;
+        }
+        // exclusive scan of c over the 256 digits
+        int32_t inc = c;
+#pragma unroll
+        for (int off = 1; off < 64; off <<= 1) {
+            const int32_t y = __shfl_up(inc, off, 64);
+            if (lane >= off) inc += y;
+        }
+        __shared__ int32_t wsum[NW];
+        if (lane == 63) wsum[wid] = inc;
+        __syncthreads();
+        int32_t pre = 0;
+        for (int w2 = 0; w2 < wid; ++w2) pre += wsum[w2];
+        const int32_t start = pre + inc - c;
+        tstart[d] = start;
+        gbase[d] = offs[(int64_t)d * ntiles + blockIdx.x] - start;
+    }
+    __syncthreads();
 #pragma unroll
     for (int j = 0; j < kSortItems; ++j) {
         const int d = dg[j];
         if (d < 0) continue;
-        int32_t off = offs[(int64_t)d * ntiles + blockIdx.x];
-        for (int w2 = 0; w2 < wid; ++w2) off += wcnt[w2][d];
-        const int64_t o = (int64_t)off + rk[j];
-        okeys[o] = k[j];
-        ovals[o] = v[j];
+        const int slot = tstart[d] + wcnt[wid][d] + rk[j];
+        lk[slot] = k[j];
+        lv[slot] = v[j];
+    }
+    __syncthreads();
+    const int64_t cnt = n - tile0 < kSortTile ? n - tile0 : kSortTile;
+    for (int slot = threadIdx.x; slot < cnt; slot += kSortThreads) {
+        const uint64_t key = lk[slot];
+        const int d = (int)((key >> shift) & 255u);
+        const int64_t o = (int64_t)gbase[d] + slot;
+        okeys[o] = key;
+        ovals[o] = lv[slot];
     }
 }
 
