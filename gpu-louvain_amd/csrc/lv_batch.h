@@ -13,15 +13,18 @@
 namespace lv {
 
 // own[v] = weight of v's row entries inside comm[v] (FP64; heavy rows by chunk
-// with atomics, exact for grid weights).  own must be zero before.
+// with atomics, exact for grid weights) for the rows st lists over order
+// (chunks of vertices with active[v] == 0 are skipped; active may be null).
+// own must be zero before.
 void launch_vertex_own(const int32_t *rowptr, const int32_t *col, const float *w,
                        const int32_t *comm, const int32_t *order, const SegTable &st,
-                       const Chunk *chunks, double *own, hipStream_t s);
+                       const Chunk *chunks, const uint8_t *active, double *own, hipStream_t s);
 // Per graph g (one workgroup each): out[3g] = sum of own over the graph's
 // vertices, out[3g+1] = sum of sigma^2 over its communities, out[3g+2] =
 // vertices that moved in the last sweep (hist bit 0).  Fixed order.
+// Graphs with gact[g] == 0 are skipped (gact may be null).
 void launch_graph_sums(const int32_t *vptr, int32_t G, const double *own, const double *sigma,
-                       const uint8_t *hist, double *out, hipStream_t s);
+                       const uint8_t *hist, const uint8_t *gact, double *out, hipStream_t s);
 // Frontier of the active graphs: active[v] = gact[gid[v]] and deg > 0;
 // keys[v] = bucket or -1.
 void launch_graph_frontier(const int32_t *rowptr, int32_t V, const int32_t *gid,

@@ -33,11 +33,12 @@ __global__ __launch_bounds__(kRowThreads) void k_vertex_own(
     const int32_t *__restrict__ rowptr, const int32_t *__restrict__ col,
     const float *__restrict__ w, const int32_t *__restrict__ comm,
     const int32_t *__restrict__ order, SegTable st, const Chunk *__restrict__ chunks,
-    double *own) {
+    const uint8_t *__restrict__ active, double *own) {
     __shared__ double lds[kRowThreads / 64];
     const int b = blockIdx.x;
     if (b >= st.chunk_block_start) {
         const Chunk ch = chunks[b - st.chunk_block_start];
+        if (active && !active[ch.v]) return;
         double x = row_own(col, w, comm, comm[ch.v], ch.lo + threadIdx.x, ch.hi, kRowThreads);
         x = block_sum(x, lds);
         if (threadIdx.x == 0 && x != 0.0) atomicAdd(&own[ch.v], x);
@@ -72,9 +73,11 @@ __global__ __launch_bounds__(256) void k_graph_sums(const int32_t *__restrict__ 
                                                     const double *__restrict__ own,
                                                     const double *__restrict__ sigma,
                                                     const uint8_t *__restrict__ hist,
+                                                    const uint8_t *__restrict__ gact,
                                                     double *out) {
     __shared__ double lds[4];
     const int32_t g = blockIdx.x;
+    if (gact && !gact[g]) return;
     const int32_t lo = vptr[g], hi = vptr[g + 1];
     double o = 0.0, q = 0.0, m = 0.0;
     for (int32_t v = lo + threadIdx.x; v < hi; v += 256) {
@@ -131,17 +134,17 @@ __global__ void k_graph_final(const int32_t *__restrict__ orig2comm, int32_t V0,
 
 void launch_vertex_own(const int32_t *rowptr, const int32_t *col, const float *w,
                        const int32_t *comm, const int32_t *order, const SegTable &st,
-                       const Chunk *chunks, double *own, hipStream_t s) {
+                       const Chunk *chunks, const uint8_t *active, double *own, hipStream_t s) {
     const int32_t nb = st.chunk_block_start + st.nchunks;
     if (nb <= 0) return;
     hipLaunchKernelGGL(k_vertex_own, dim3(nb), dim3(kRowThreads), 0, s, rowptr, col, w, comm, order,
-                       st, chunks, own);
+                       st, chunks, active, own);
 }
 
 void launch_graph_sums(const int32_t *vptr, int32_t G, const double *own, const double *sigma,
-                       const uint8_t *hist, double *out, hipStream_t s) {
+                       const uint8_t *hist, const uint8_t *gact, double *out, hipStream_t s) {
     if (G <= 0) return;
-    hipLaunchKernelGGL(k_graph_sums, dim3(G), dim3(256), 0, s, vptr, own, sigma, hist, out);
+    hipLaunchKernelGGL(k_graph_sums, dim3(G), dim3(256), 0, s, vptr, own, sigma, hist, gact, out);
 }
 
 void launch_graph_frontier(const int32_t *rowptr, int32_t V, const int32_t *gid,

@@ -817,30 +817,7 @@ struct lv_graph {
 
         // row-reduction segments: lanes per vertex by bucket
         static const int lanes_lg[lv::kNumBuckets - 1] = {2, 3, 4, 5, 6, 6, 8, 8};
-        st = lv::SegTable{};
-        int32_t blk = 0;
-        for (int b = 0; b < HB; ++b) {
-            const int32_t cnt = h_off[b + 1] - h_off[b];
-            if (cnt <= 0) continue;
-            lv::Seg sg;
-            sg.list_off = h_off[b];
-            sg.count = cnt;
-            sg.block_start = blk;
-            sg.lanes_log2 = lanes_lg[b];
-            const int per = lanes_lg[b] >= 8 ? 1 : (lv::kRowThreads >> lanes_lg[b]);
-            blk += (cnt + per - 1) / per;
-            st.s[st.nseg++] = sg;
-        }
-        if (st.nseg == 0) {  // keep find_seg well-defined
-            st.s[0] = lv::Seg{0, 0, 0, 2};
-            st.nseg = 1;
-        }
-        st.chunk_block_start = blk;
-        st.nchunks = nins;
-        blk += nins;
-        st.sq_block_start = blk;
-        blk += (int32_t)((V + lv::kSqItems - 1) / lv::kSqItems);
-        st.total_blocks = std::max(blk, 1);
+        st = make_segs(h_off, nins);
         LV_TRY(qpart.reserve(2 * ((int64_t)st.total_blocks + st_light.total_blocks +
                                   8 * (int64_t)xst.total_blocks) + 2 * 256));
         // per-group row segments for the modularity correction of the applies
@@ -942,6 +919,33 @@ struct lv_graph {
         xst.total_blocks = j;
         xcd_ok = true;
         return LV_OK;
+    }
+
+    // Row-reduction segments over a bucket-ordered list with offsets off
+    // (buckets below the chunked tier), then the chunk blocks and the
+    // sum-of-squares blocks.
+    lv::SegTable make_segs(const int32_t *off, int32_t nchunks) const {
+        static const int lanes[lv::kNumBuckets - 1] = {2, 3, 4, 5, 6, 6, 8, 8};
+        lv::SegTable t{};
+        int32_t blk = 0;
+        for (int b = 0; b < lv::kNumBuckets - 1; ++b) {
+            const int32_t cnt = off[b + 1] - off[b];
+            if (cnt <= 0) continue;
+            const int per = lanes[b] >= 8 ? 1 : (lv::kRowThreads >> lanes[b]);
+            t.s[t.nseg++] = lv::Seg{off[b], cnt, blk, lanes[b]};
+            blk += (cnt + per - 1) / per;
+        }
+        if (t.nseg == 0) {  // keep the segment search well-defined
+            t.s[0] = lv::Seg{0, 0, 0, 2};
+            t.nseg = 1;
+        }
+        t.chunk_block_start = blk;
+        t.nchunks = nchunks;
+        blk += nchunks;
+        t.sq_block_start = blk;
+        blk += (int32_t)((V + lv::kSqItems - 1) / lv::kSqItems);
+        t.total_blocks = std::max(blk, 1);
+        return t;
     }
 
     // ---------------------------------------------------------------- phase 1
@@ -1129,7 +1133,7 @@ struct lv_graph {
             if (incremental) step++;
         }
         if (G > 0) {
-            LV_TRY(enqueue_graph_sums());
+            LV_TRY(enqueue_graph_sums(pruned));
         } else if (incremental) {
             hipEvent_t tq = tbegin();
             lv::launch_modularity_from_own(sigma, V, M, qpart.p, d_q, d_own_slots, s);
@@ -1402,12 +1406,20 @@ struct lv_graph {
 
     // ---------------------------------------------------------------- batch
     // Per-graph (own, sq, moved) of the current state into gsums (device).
-    lv_status enqueue_graph_sums() {
+    // pruned: only the graphs of the current frontier (their rows are the
+    // frontier lists; the others' sums are not read).
+    lv_status enqueue_graph_sums(bool pruned = false) {
         hipStream_t s = stream;
         hipEvent_t t0 = tbegin();
         lv::fill_f64(own_v, 0.0, V, s);
-        lv::launch_vertex_own(rowptr(), col(), w(), comm, order, st, ins.p, own_v, s);
-        lv::launch_graph_sums(vptr, G, own_v, sigma, hist, gsums, s);
+        if (pruned) {
+            const lv::SegTable fst = make_segs(h_foff, nins);
+            lv::launch_vertex_own(rowptr(), col(), w(), comm, forder, fst, ins.p, active, own_v, s);
+            lv::launch_graph_sums(vptr, G, own_v, sigma, hist, gact, gsums, s);
+        } else {
+            lv::launch_vertex_own(rowptr(), col(), w(), comm, order, st, ins.p, nullptr, own_v, s);
+            lv::launch_graph_sums(vptr, G, own_v, sigma, hist, nullptr, gsums, s);
+        }
         tend(t0, LV_K_MODULARITY, 12.0 * (double)E + 16.0 * (double)V);
         LV_LAUNCHED("graph sums");
         return LV_OK;
