@@ -450,7 +450,27 @@ struct lv_graph {
         gm.movers = bits + (int64_t)cur * bit_words;
         gm.clear_movers = bits + (int64_t)(cur ^ 1) * bit_words;
         gm.words = bit_words;
+        gm.rounds = 1;
         return gm;
+    }
+
+    // The group's correction table for `rounds` (block starts by
+    // lv::corr_items).
+    lv::SegTable corr_table(int g, int32_t rounds) const {
+        lv::SegTable t = gst[g];
+        int32_t gb = 0;
+        for (int i = 0; i < t.nseg; ++i) {
+            const int lg = t.s[i].lanes_log2;
+            const int32_t per = lv::corr_items(1 << (lg < 8 ? lg : 8), rounds);
+            t.s[i].block_start = gb;
+            gb += (t.s[i].count + per - 1) / per;
+        }
+        t.chunk_block_start = gb;
+        const int32_t per = lv::corr_items(lv::kRowThreads, rounds);
+        gb += (t.nchunks + per - 1) / per;
+        t.sq_block_start = gb;
+        t.total_blocks = gb;
+        return t;
     }
 
     lv::ChunkedTier chunked_tier() {
@@ -656,6 +676,10 @@ struct lv_graph {
         const int32_t *rp = rowptr();
         lv::launch_degree_keys(rp, V, bb, keys8, s);
         lv::stable_partition(keys8, V, lv::kNumBuckets, order, d_off, scratch, s);
+        // frontier kernels write the entries of listed (positive-degree)
+        // vertices only; the others stay out of every frontier
+        lv::fill_u8(reinterpret_cast<uint8_t *>(fkeys), 0xFF, V, s);
+        lv::fill_u8(active, 0, V, s);
         hipLaunchKernelGGL(k_zero_u64, dim3(1), dim3(1), 0, s, d_bedges);
         for (int b = 1; b < lv::kNumBuckets; ++b)
             hipLaunchKernelGGL(k_zero_u64, dim3(1), dim3(1), 0, s, d_bedges + b);
@@ -820,7 +844,8 @@ struct lv_graph {
         st = make_segs(h_off, nins);
         LV_TRY(qpart.reserve(2 * ((int64_t)st.total_blocks + st_light.total_blocks +
                                   8 * (int64_t)xst.total_blocks) + 2 * 256));
-        // per-group row segments for the modularity correction of the applies
+        // per-group segments for the modularity correction of the applies
+        // (block starts are set per sweep by corr_table)
         for (int g = 0; g < lv::kNumGroups; ++g) {
             lv::SegTable &t = gst[g];
             t = lv::SegTable{};
@@ -834,8 +859,7 @@ struct lv_graph {
                 sg.count = cnt;
                 sg.block_start = gb;
                 sg.lanes_log2 = lanes_lg[b];
-                const int per = lanes_lg[b] >= 8 ? 1 : (lv::kRowThreads >> lanes_lg[b]);
-                gb += (cnt + per - 1) / per;
+                gb += cnt;
                 t.s[t.nseg++] = sg;
             }
             if (t.nseg == 0) {
@@ -1070,13 +1094,15 @@ struct lv_graph {
         } else if (fr.kind == LV_FRONTIER_MODEL) {
             if (!mlp_set) return fail(LV_ERR_STATE, "frontier model not set (lv_set_model)");
             hipEvent_t t0 = tbegin();
-            lv::launch_frontier(fa, mlp, fr.threshold, bb, fkeys, active, nullptr, s);
+            lv::launch_frontier(fa, order, h_off[lv::kNumBuckets], mlp, fr.threshold,
+                                fr.heavy_threshold, fr.heavy_degree, bb, fkeys, active, nullptr,
+                                s);
             // per vertex: rowptr 8, k 8, comm 4, record 8, margin 4, kown 4,
             // hist 1, nmv 4 read; keys 1, active 1 written
             tend(t0, -1, 43.0 * (double)V, LV_KD_FRONTIER);
         } else {
-            lv::launch_frontier_rule(fr.kind, fa, fr.random_p, fr.seed + 0x9e37ull * sweep_count,
-                                     bb, fkeys, active, s);
+            lv::launch_frontier_rule(fr.kind, fa, order, h_off[lv::kNumBuckets], fr.random_p,
+                                     fr.seed + 0x9e37ull * sweep_count, bb, fkeys, active, s);
         }
         lv::stable_partition(fkeys, V, lv::kNumBuckets, forder, fd_off, scratch, s);
         if (V > 0)
@@ -1115,18 +1141,25 @@ struct lv_graph {
         hipLaunchKernelGGL(k_zero_u64_n, dim3(blocks_for(lv::kMoveSlots * lv::kMoveSlotStride, 256)),
                            dim3(256), 0, s, d_moved, (int64_t)lv::kMoveSlots * lv::kMoveSlotStride);
         const lv::MoveArgs a = move_args();
+        // correction rows per lane group: about one mover per block when the
+        // last sweep's share of movers holds
+        const int64_t nact = std::max<int64_t>(h_off[lv::kNumBuckets], 1);
+        const int32_t rounds = (int32_t)std::min<int64_t>(
+            1024, std::max<int64_t>(1, nact / std::max<int64_t>(last_moved, 1)));
         for (int g = 0; g < lv::kNumGroups; ++g) {
             int32_t lo, hi;
             group_range(g, lo, hi);
             if (lo < 0 || hi <= lo) continue;
             LV_TRY(run_group(g, pruned));
             hipEvent_t t0 = tbegin();
-            const lv::GroupMarks gm = marks();
+            lv::GroupMarks gm = marks();
+            gm.rounds = rounds;
+            const lv::SegTable ct = corr_table(g, rounds);
             // pruned: only frontier vertices can move
             int32_t alo = lo, ahi = hi;
             if (pruned) group_range(g, alo, ahi, h_foff);
             const int32_t *alist = (pruned ? forder : order) + alo;
-            lv::launch_apply(a, alist, ahi - alo, order, &gst[g], ins.p, comm, size, sigma, crec,
+            lv::launch_apply(a, alist, ahi - alo, order, &ct, ins.p, comm, size, sigma, crec,
                              incremental ? &gm : nullptr, hist, d_moved, d_own_slots, s);
             tend(t0, LV_K_APPLY, 16.0 * (double)(ahi - alo));
             LV_LAUNCHED("apply");
@@ -1144,7 +1177,8 @@ struct lv_graph {
         }
         if (track) {
             lv::fill_i32(nmv, 0, V, s);
-            lv::launch_neighbor_moves(rowptr(), col(), order, st, ins.p, hist, nmv, s);
+            lv::launch_neighbor_moves(rowptr(), col(), order, h_off[lv::kNumBuckets], hist, nmv,
+                                      s);
             LV_LAUNCHED("neighbour moves");
         }
         if (e1) (void)hipEventRecord(e1, s);
@@ -1913,6 +1947,7 @@ lv_status check_frontier(const lv_frontier *fr) {
     if (fr->kind < LV_FRONTIER_MODEL || fr->kind > LV_FRONTIER_ALL)
         return fail(LV_ERR_ARG, "unknown frontier kind");
     if (fr->full_every < 0) return fail(LV_ERR_ARG, "full_every must be >= 0");
+    if (fr->heavy_degree < 0) return fail(LV_ERR_ARG, "heavy_degree must be >= 0");
     return LV_OK;
 }
 
@@ -2006,8 +2041,8 @@ lv_status lv_predict(lv_graph *g, float threshold, float *prob, uint8_t *active)
     if (!g->level_ready) LV_TRY(g->setup_level());
     if (g->V <= 0) return LV_OK;
     LV_TRY(g->tfeat.reserve(g->V));
-    lv::launch_frontier(g->feature_args(), g->mlp, threshold, g->bb, g->fkeys, g->active,
-                        g->tfeat.p, g->stream);
+    lv::launch_frontier(g->feature_args(), nullptr, 0, g->mlp, threshold, threshold, 0, g->bb,
+                        g->fkeys, g->active, g->tfeat.p, g->stream);
     LV_LAUNCHED("frontier model");
     if (prob)
         LV_HIP(hipMemcpyAsync(prob, g->tfeat.p, sizeof(float) * g->V, hipMemcpyDeviceToHost,

@@ -46,23 +46,29 @@ constexpr int32_t kNmvMaxDegree = 1024;
 // hist[v] <<= 1 for every vertex (start of a sweep).
 void launch_hist_shift(uint8_t *hist, int32_t V, hipStream_t s);
 // nmv[u] += 1 for every row entry (u, v) of a vertex v that moved in the last
-// sweep (hist bit 0) and has degree <= kNmvMaxDegree; nmv must be zero before.  Rows are walked with the
-// segment table of the level (heavy rows by chunks).
-void launch_neighbor_moves(const int32_t *rowptr, const int32_t *col, const int32_t *order,
-                           const SegTable &st, const Chunk *chunks, const uint8_t *hist,
-                           int32_t *nmv, hipStream_t s);
+// sweep (hist bit 0) and has degree <= kNmvMaxDegree; nmv must be zero
+// before.  The n vertices of list are scanned 64 per wave; the wave then
+// walks each mover's row in turn, one entry per lane.
+void launch_neighbor_moves(const int32_t *rowptr, const int32_t *col, const int32_t *list,
+                           int32_t n, const uint8_t *hist, int32_t *nmv, hipStream_t s);
 // feat[v*8 .. v*8+8) for every vertex.
 void launch_features(const FeatureArgs &f, float *feat, hipStream_t s);
 // Frontier from the model: active[v] = deg > 0 and sigmoid(MLP(features)) >
 // threshold; keys[v] = execution bucket of an active vertex, -1 otherwise;
-// prob (optional) = the predicted probability.
-void launch_frontier(const FeatureArgs &f, const float *params, float threshold, BucketBounds bb,
-                     int8_t *keys, uint8_t *active, float *prob, hipStream_t s);
+// prob (optional) = the predicted probability.  Vertices of degree above
+// heavy_degree (when positive) use heavy_threshold instead.  Over the n
+// vertices of list, or over all f.V vertices when list is null (entries of
+// vertices outside the list are left as they are).
+void launch_frontier(const FeatureArgs &f, const int32_t *list, int32_t n, const float *params,
+                     float threshold, float heavy_threshold, int32_t heavy_degree,
+                     BucketBounds bb, int8_t *keys, uint8_t *active, float *prob, hipStream_t s);
 // Frontier from a rule: kFrontierNeighbor = moved in the last sweep or a
 // neighbour did; kFrontierRandom = seeded Bernoulli(p) per vertex and sweep;
-// kFrontierAll = every vertex of positive degree.
-void launch_frontier_rule(int kind, const FeatureArgs &f, float p, uint64_t seed, BucketBounds bb,
-                          int8_t *keys, uint8_t *active, hipStream_t s);
+// kFrontierAll = every vertex of positive degree.  Over the n vertices of
+// list (null: all f.V vertices).
+void launch_frontier_rule(int kind, const FeatureArgs &f, const int32_t *list, int32_t n, float p,
+                          uint64_t seed, BucketBounds bb, int8_t *keys, uint8_t *active,
+                          hipStream_t s);
 // prob[i] = sigmoid(MLP(feat[i*8 .. i*8+8))) on MFMA (bf16 operands, fp32 accumulation).
 void launch_mlp_forward(const float *feat, int64_t n, const float *params, float *prob,
                         hipStream_t s);

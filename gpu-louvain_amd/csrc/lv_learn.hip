@@ -52,46 +52,32 @@ __global__ void k_hist_shift(uint8_t *hist, int32_t V) {
     if (v < V) hist[v] = (uint8_t)(hist[v] << 1);
 }
 
-__device__ __forceinline__ int seg_of(const SegTable &st, int b) {
-    int si = 0;
-    while (si + 1 < st.nseg && b >= st.s[si + 1].block_start) ++si;
-    return si;
-}
-
 // Rows of last-sweep movers: one count per row entry at the neighbour.
-__global__ __launch_bounds__(kRowThreads) void k_neighbor_moves(
-    const int32_t *__restrict__ rowptr, const int32_t *__restrict__ col,
-    const int32_t *__restrict__ order, SegTable st, const Chunk *__restrict__ chunks,
-    const uint8_t *__restrict__ hist, int32_t *nmv) {
-    const int b = blockIdx.x;
-    int32_t v = -1;
-    int64_t lo = 0, hi = 0;
-    int stride = kRowThreads;
-    if (b >= st.chunk_block_start) {
-        const Chunk ch = chunks[b - st.chunk_block_start];
-        v = ch.v;
-        lo = ch.lo + threadIdx.x;
-        hi = ch.hi;
-    } else {
-        const Seg sg = st.s[seg_of(st, b)];
-        const int lg = sg.lanes_log2;
-        if (lg <= 6) {
-            const int32_t idx = (b - sg.block_start) * (kRowThreads >> lg) + (threadIdx.x >> lg);
-            if (idx < sg.count) {
-                v = order[sg.list_off + idx];
-                lo = rowptr[v] + (threadIdx.x & ((1 << lg) - 1));
-                hi = rowptr[v + 1];
-                stride = 1 << lg;
-            }
-        } else {
-            v = order[sg.list_off + (b - sg.block_start)];
-            lo = rowptr[v] + threadIdx.x;
+__global__ __launch_bounds__(256) void k_neighbor_moves(const int32_t *__restrict__ rowptr,
+                                                        const int32_t *__restrict__ col,
+                                                        const int32_t *__restrict__ list,
+                                                        int32_t n,
+                                                        const uint8_t *__restrict__ hist,
+                                                        int32_t *nmv) {
+    const int lane = threadIdx.x & 63;
+    const int64_t i = (int64_t)blockIdx.x * 256 + threadIdx.x;
+    int32_t lo = 0, hi = 0;
+    bool mv = false;
+    if (i < n) {
+        const int32_t v = list[i];
+        if (hist[v] & 1u) {
+            lo = rowptr[v];
             hi = rowptr[v + 1];
+            mv = hi - lo <= kNmvMaxDegree;
         }
     }
-    if (v < 0 || !(hist[v] & 1u)) return;
-    if (rowptr[v + 1] - rowptr[v] > kNmvMaxDegree) return;
-    for (int64_t e = lo; e < hi; e += stride) atomicAdd(&nmv[col[e]], 1);
+    uint64_t m = __ballot(mv);
+    while (m) {
+        const int src = __ffsll((long long)m) - 1;
+        m &= m - 1;
+        const int32_t rlo = __shfl(lo, src, 64), rhi = __shfl(hi, src, 64);
+        for (int32_t e = rlo + lane; e < rhi; e += 64) atomicAdd(&nmv[col[e]], 1);
+    }
 }
 
 __global__ void k_features(FeatureArgs f, float *feat) {
@@ -174,9 +160,11 @@ __device__ __forceinline__ float sigmoid(float z) { return 1.0f / (1.0f + expf(-
 // Each wave handles 64 vertices per iteration: features to LDS, then four
 // 16-vertex MFMA tiles.
 template <bool FROM_FEAT>
-__global__ __launch_bounds__(256) void k_mlp(FeatureArgs f, const float *__restrict__ feat,
+__global__ __launch_bounds__(256) void k_mlp(FeatureArgs f, const int32_t *__restrict__ list,
+                                             const float *__restrict__ feat,
                                              int64_t n, const float *__restrict__ P,
-                                             float threshold, BucketBounds bb, int8_t *keys,
+                                             float threshold, float heavy_threshold,
+                                             int32_t heavy_degree, BucketBounds bb, int8_t *keys,
                                              uint8_t *active, float *prob) {
     __shared__ float xs[4][64][kFeat + 1];
     __shared__ float h1s[4][16][kHid + 1];
@@ -193,7 +181,7 @@ __global__ __launch_bounds__(256) void k_mlp(FeatureArgs f, const float *__restr
 #pragma unroll
                 for (int j = 0; j < kFeat; ++j) x[j] = feat[vl * kFeat + j];
             } else {
-                features_of(f, (int32_t)vl, x);
+                features_of(f, list ? list[vl] : (int32_t)vl, x);
             }
         } else {
 #pragma unroll
@@ -210,13 +198,17 @@ __global__ __launch_bounds__(256) void k_mlp(FeatureArgs f, const float *__restr
             if (col == 0) {
 #pragma unroll
                 for (int r = 0; r < 4; ++r) {
-                    const int64_t v = base + 16 * q + 4 * grp + r;
-                    if (v >= n) continue;
+                    const int64_t i = base + 16 * q + 4 * grp + r;
+                    if (i >= n) continue;
                     const float p = sigmoid(z[r]);
+                    const int64_t v = (!FROM_FEAT && list) ? (int64_t)list[i] : i;
                     if (prob) prob[v] = p;
                     if (!FROM_FEAT) {
                         const int32_t deg = f.rowptr[v + 1] - f.rowptr[v];
-                        const bool act = deg > 0 && p > threshold;
+                        const float thr = heavy_degree > 0 && deg > heavy_degree
+                                              ? heavy_threshold
+                                              : threshold;
+                        const bool act = deg > 0 && p > thr;
                         active[v] = act ? 1 : 0;
                         keys[v] = act ? (int8_t)bucket_of(deg, bb) : (int8_t)-1;
                     }
@@ -233,10 +225,12 @@ __device__ __forceinline__ uint64_t mix64(uint64_t x) {
     return x ^ (x >> 31);
 }
 
-__global__ void k_frontier_rule(int kind, FeatureArgs f, float p, uint64_t seed, BucketBounds bb,
-                                int8_t *keys, uint8_t *active) {
-    const int32_t v = blockIdx.x * blockDim.x + threadIdx.x;
-    if (v >= f.V) return;
+__global__ void k_frontier_rule(int kind, FeatureArgs f, const int32_t *__restrict__ list,
+                                int32_t n, float p, uint64_t seed, BucketBounds bb, int8_t *keys,
+                                uint8_t *active) {
+    const int32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    const int32_t v = list ? list[i] : i;
     const int32_t deg = f.rowptr[v + 1] - f.rowptr[v];
     bool act = deg > 0;
     if (kind == kFrontierNeighbor) {
@@ -378,13 +372,11 @@ void launch_hist_shift(uint8_t *hist, int32_t V, hipStream_t s) {
     hipLaunchKernelGGL(k_hist_shift, dim3(blocks_for(V, 256)), dim3(256), 0, s, hist, V);
 }
 
-void launch_neighbor_moves(const int32_t *rowptr, const int32_t *col, const int32_t *order,
-                           const SegTable &st, const Chunk *chunks, const uint8_t *hist,
-                           int32_t *nmv, hipStream_t s) {
-    const int32_t nb = st.chunk_block_start + st.nchunks;
-    if (nb <= 0) return;
-    hipLaunchKernelGGL(k_neighbor_moves, dim3(nb), dim3(kRowThreads), 0, s, rowptr, col, order, st,
-                       chunks, hist, nmv);
+void launch_neighbor_moves(const int32_t *rowptr, const int32_t *col, const int32_t *list,
+                           int32_t n, const uint8_t *hist, int32_t *nmv, hipStream_t s) {
+    if (n <= 0) return;
+    hipLaunchKernelGGL(k_neighbor_moves, dim3(blocks_for(n, 256)), dim3(256), 0, s, rowptr, col,
+                       list, n, hist, nmv);
 }
 
 void launch_features(const FeatureArgs &f, float *feat, hipStream_t s) {
@@ -392,20 +384,25 @@ void launch_features(const FeatureArgs &f, float *feat, hipStream_t s) {
     hipLaunchKernelGGL(k_features, dim3(blocks_for(f.V, 256)), dim3(256), 0, s, f, feat);
 }
 
-void launch_frontier(const FeatureArgs &f, const float *params, float threshold, BucketBounds bb,
-                     int8_t *keys, uint8_t *active, float *prob, hipStream_t s) {
-    if (f.V <= 0) return;
-    int64_t g = blocks_for(f.V, 256);
+void launch_frontier(const FeatureArgs &f, const int32_t *list, int32_t n, const float *params,
+                     float threshold, float heavy_threshold, int32_t heavy_degree,
+                     BucketBounds bb, int8_t *keys, uint8_t *active, float *prob, hipStream_t s) {
+    if (!list) n = f.V;
+    if (n <= 0) return;
+    int64_t g = blocks_for(n, 256);
     if (g > 8192) g = 8192;
-    hipLaunchKernelGGL(k_mlp<false>, dim3((unsigned)g), dim3(256), 0, s, f, nullptr,
-                       (int64_t)f.V, params, threshold, bb, keys, active, prob);
+    hipLaunchKernelGGL(k_mlp<false>, dim3((unsigned)g), dim3(256), 0, s, f, list, nullptr,
+                       (int64_t)n, params, threshold, heavy_threshold, heavy_degree, bb, keys,
+                       active, prob);
 }
 
-void launch_frontier_rule(int kind, const FeatureArgs &f, float p, uint64_t seed, BucketBounds bb,
-                          int8_t *keys, uint8_t *active, hipStream_t s) {
-    if (f.V <= 0) return;
-    hipLaunchKernelGGL(k_frontier_rule, dim3(blocks_for(f.V, 256)), dim3(256), 0, s, kind, f, p,
-                       seed, bb, keys, active);
+void launch_frontier_rule(int kind, const FeatureArgs &f, const int32_t *list, int32_t n, float p,
+                          uint64_t seed, BucketBounds bb, int8_t *keys, uint8_t *active,
+                          hipStream_t s) {
+    if (!list) n = f.V;
+    if (n <= 0) return;
+    hipLaunchKernelGGL(k_frontier_rule, dim3(blocks_for(n, 256)), dim3(256), 0, s, kind, f, list, n,
+                       p, seed, bb, keys, active);
 }
 
 void launch_mlp_forward(const float *feat, int64_t n, const float *params, float *prob,
@@ -414,8 +411,8 @@ void launch_mlp_forward(const float *feat, int64_t n, const float *params, float
     int64_t g = blocks_for(n, 256);
     if (g > 8192) g = 8192;
     FeatureArgs f{};
-    hipLaunchKernelGGL(k_mlp<true>, dim3((unsigned)g), dim3(256), 0, s, f, feat, n, params, 0.0f,
-                       BucketBounds{}, nullptr, nullptr, prob);
+    hipLaunchKernelGGL(k_mlp<true>, dim3((unsigned)g), dim3(256), 0, s, f, nullptr, feat, n, params,
+                       0.0f, 0.0f, 0, BucketBounds{}, nullptr, nullptr, prob);
 }
 
 int64_t grad_blocks(int64_t n) { return (n + kTrainBlock - 1) / kTrainBlock; }

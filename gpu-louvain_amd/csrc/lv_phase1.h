@@ -126,7 +126,17 @@ struct GroupMarks {
     uint32_t *movers;        // this step's bitmap over vertices
     uint32_t *clear_movers;  // the other step's, cleared by this step
     int32_t words;
+    int32_t rounds;  // correction: rows walked per lane group of a block (see corr_items)
 };
+
+// Items one correction block checks: rounds rows per X-lane group, at most
+// kCorrItems.  rounds = 1 gives one row per lane group (most vertices
+// moved); late sweeps, where few move, use more so that blocks stay few.
+constexpr int32_t kCorrItems = 1024;
+__host__ __device__ inline int32_t corr_items(int X, int32_t rounds) {
+    const int64_t c = (int64_t)(kRowThreads / X) * rounds;
+    return (int32_t)(c < kCorrItems ? c : kCorrItems);
+}
 
 // --- apply ---
 // Applies the decisions of the vertices list[0..n) (a superset of the
@@ -134,7 +144,8 @@ struct GroupMarks {
 // records of the touched communities.  With gm (incremental modularity) the
 // movers are marked and the group's change of the weight inside communities
 // is added to own (kMoveSlots slots kMoveSlotStride apart; gst: the group's
-// row segments over order).
+// segments over order, then its heavy-row chunks, corr_items(X, gm->rounds)
+// items per block).
 // hist[v] gets bit 0 set for a mover (the move history of the frontier model);
 // moved holds kMoveSlots counter pairs kMoveSlotStride apart: summed over
 // the slots, [0] counts the movers and [1] their row entries.

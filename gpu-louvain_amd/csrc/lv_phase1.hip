@@ -734,7 +734,7 @@ __global__ __launch_bounds__(kRowThreads) void k_apply_finish(
     const Chunk *__restrict__ chunks, const int32_t *__restrict__ comm,
     const int32_t *__restrict__ size, const double *__restrict__ sigma, uint64_t *crec,
     const uint32_t *__restrict__ movers, uint32_t *clear_bits, int32_t words, double *own,
-    int32_t fix_start, int32_t clear_start) {
+    int32_t fix_start, int32_t clear_start, int32_t rounds) {
     __shared__ double lds[kRowThreads / 64];
     const int b = blockIdx.x;
     if (b >= clear_start) {
@@ -767,7 +767,6 @@ __global__ __launch_bounds__(kRowThreads) void k_apply_finish(
     double acc = 0.0;
     auto row = [&](int32_t v, int64_t elo, int64_t ehi, int stride, bool first) {
         const int32_t cv_old = a.newc[v], cv_new = comm[v];
-        if (cv_old == cv_new) return;
         if (first) acc += 2.0 * a.dk[v];
         for (int64_t e = elo; e < ehi; e += stride) {
             const int32_t u = a.col[e];
@@ -778,25 +777,52 @@ __global__ __launch_bounds__(kRowThreads) void k_apply_finish(
             if (d != 0) acc += (double)d * (double)a.w[e];
         }
     };
-    if (b >= st.chunk_block_start) {
-        const Chunk ch = chunks[b - st.chunk_block_start];
-        row(ch.v, ch.lo + threadIdx.x, ch.hi, kRowThreads, threadIdx.x == 0 && ch.first);
+    // The block checks its items (rounds x 256/X vertices of one segment, or
+    // rounds heavy-row chunks; at most kCorrItems) at once, keeps the
+    // movers' items in LDS and walks only their rows, X lanes per row.
+    __shared__ int32_t items[kCorrItems];
+    __shared__ int32_t nitems;
+    const bool chunked = b >= st.chunk_block_start;
+    int X = kRowThreads;
+    int32_t first = 0, cnt = 0, list_off = 0;
+    if (chunked) {
+        const int32_t per = corr_items(kRowThreads, rounds);
+        first = (b - st.chunk_block_start) * per;
+        cnt = min(per, st.nchunks - first);
     } else {
         int si = 0;
         while (si + 1 < st.nseg && b >= st.s[si + 1].block_start) ++si;
         const Seg sg = st.s[si];
-        const int lg = sg.lanes_log2;
-        if (lg <= 6) {
-            const int X = 1 << lg;
-            const int32_t idx = (b - sg.block_start) * (kRowThreads >> lg) + (threadIdx.x >> lg);
-            const int lane = threadIdx.x & (X - 1);
-            if (idx < sg.count) {
-                const int32_t v = order[sg.list_off + idx];
-                row(v, a.rowptr[v] + lane, a.rowptr[v + 1], X, lane == 0);
-            }
+        X = 1 << (sg.lanes_log2 < 8 ? sg.lanes_log2 : 8);
+        const int32_t per = corr_items(X, rounds);
+        first = (b - sg.block_start) * per;
+        cnt = min(per, sg.count - first);
+        list_off = sg.list_off;
+    }
+    if (threadIdx.x == 0) nitems = 0;
+    __syncthreads();
+    for (int32_t t = (int32_t)threadIdx.x; t < cnt; t += kRowThreads) {
+        int32_t item = -1;
+        if (chunked) {
+            const int32_t v = chunks[first + t].v;
+            if (a.newc[v] != comm[v]) item = first + t;
         } else {
-            const int32_t v = order[sg.list_off + (b - sg.block_start)];
-            row(v, a.rowptr[v] + threadIdx.x, a.rowptr[v + 1], kRowThreads, threadIdx.x == 0);
+            const int32_t v = order[list_off + first + t];
+            if (a.newc[v] != comm[v]) item = v;
+        }
+        if (item >= 0) items[atomicAdd(&nitems, 1)] = item;
+    }
+    __syncthreads();
+    const int32_t nm = nitems;
+    if (nm == 0) return;  // block-uniform
+    const int lane = (int)threadIdx.x & (X - 1);
+    for (int32_t m = (int32_t)threadIdx.x / X; m < nm; m += kRowThreads / X) {
+        if (chunked) {
+            const Chunk ch = chunks[items[m]];
+            row(ch.v, ch.lo + lane, ch.hi, X, lane == 0 && ch.first);
+        } else {
+            const int32_t v = items[m];
+            row(v, a.rowptr[v] + lane, a.rowptr[v + 1], X, lane == 0);
         }
     }
     acc = block_sum_f64(acc, lds);
@@ -1254,7 +1280,7 @@ void launch_apply(const MoveArgs &a, const int32_t *list, int32_t n, const int32
     hipLaunchKernelGGL(k_apply_finish, dim3(corr + fix + clear), dim3(kRowThreads), 0, s, a, order,
                        list, n, gm ? *gst : empty, chunks, comm, size, sigma, crec,
                        gm ? gm->movers : nullptr, gm ? gm->clear_movers : nullptr,
-                       gm ? gm->words : 0, own, corr, corr + fix);
+                       gm ? gm->words : 0, own, corr, corr + fix, gm ? gm->rounds : 1);
 }
 
 void launch_modularity_rows(const int32_t *rowptr, const int32_t *col, const float *w,

@@ -30,6 +30,7 @@ void stable_partition(const int8_t *keys, int32_t n, int nk, int32_t *items, int
 
 // Fill helpers.
 void fill_i32(int32_t *p, int32_t v, int64_t n, hipStream_t s);
+void fill_u8(uint8_t *p, uint8_t v, int64_t n, hipStream_t s);
 void fill_f64(double *p, double v, int64_t n, hipStream_t s);
 void iota_i32(int32_t *p, int64_t n, hipStream_t s);
 

@@ -136,6 +136,10 @@ __global__ void k_fill_i32(int32_t *p, int32_t v, int64_t n) {
     int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
     for (; i < n; i += (int64_t)gridDim.x * blockDim.x) p[i] = v;
 }
+__global__ void k_fill_u8(uint8_t *p, uint8_t v, int64_t n) {
+    int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    for (; i < n; i += (int64_t)gridDim.x * blockDim.x) p[i] = v;
+}
 __global__ void k_fill_f64(double *p, double v, int64_t n) {
     int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
     for (; i < n; i += (int64_t)gridDim.x * blockDim.x) p[i] = v;
@@ -196,6 +200,10 @@ void stable_partition(const int8_t *keys, int32_t n, int nk, int32_t *items, int
 void fill_i32(int32_t *p, int32_t v, int64_t n, hipStream_t s) {
     if (n <= 0) return;
     hipLaunchKernelGGL(k_fill_i32, dim3(grid_for(n, 256)), dim3(256), 0, s, p, v, n);
+}
+void fill_u8(uint8_t *p, uint8_t v, int64_t n, hipStream_t s) {
+    if (n <= 0) return;
+    hipLaunchKernelGGL(k_fill_u8, dim3(grid_for(n, 256)), dim3(256), 0, s, p, v, n);
 }
 void fill_f64(double *p, double v, int64_t n, hipStream_t s) {
     if (n <= 0) return;

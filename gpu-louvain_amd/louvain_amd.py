@@ -115,7 +115,9 @@ class Frontier(ctypes.Structure):
         ("threshold", ctypes.c_float),
         ("random_p", ctypes.c_float),
         ("seed", ctypes.c_uint64),
-        ("reserved", ctypes.c_int32 * 8),
+        ("heavy_threshold", ctypes.c_float),
+        ("heavy_degree", ctypes.c_int32),
+        ("reserved", ctypes.c_int32 * 6),
     ]
 
 
@@ -133,9 +135,10 @@ class TrainOptions(ctypes.Structure):
 
 
 def frontier(kind: int = FRONTIER_MODEL, full_every: int = 4, threshold: float = 0.5,
-             random_p: float = 0.25, seed: int = 1) -> Frontier:
+             random_p: float = 0.25, seed: int = 1, heavy_threshold: float = 0.0,
+             heavy_degree: int = 0) -> Frontier:
     return Frontier(kind=kind, full_every=full_every, threshold=threshold, random_p=random_p,
-                    seed=seed)
+                    seed=seed, heavy_threshold=heavy_threshold, heavy_degree=heavy_degree)
 
 
 class KernelTime(ctypes.Structure):

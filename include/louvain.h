@@ -206,7 +206,9 @@ typedef struct {
     float threshold;         /* model: evaluate when p(move) > threshold             */
     float random_p;          /* random: evaluation probability                       */
     uint64_t seed;           /* random: seed                                         */
-    int32_t reserved[8];
+    float heavy_threshold;   /* model: threshold for vertices of degree above        */
+    int32_t heavy_degree;    /*   heavy_degree (0 = one threshold for every degree)  */
+    int32_t reserved[6];
 } lv_frontier;
 
 typedef struct {
