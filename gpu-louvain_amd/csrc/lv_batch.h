@@ -19,12 +19,16 @@ namespace lv {
 void launch_vertex_own(const int32_t *rowptr, const int32_t *col, const float *w,
                        const int32_t *comm, const int32_t *order, const SegTable &st,
                        const Chunk *chunks, const uint8_t *active, double *own, hipStream_t s);
-// Per graph g (one workgroup each): out[3g] = sum of own over the graph's
-// vertices, out[3g+1] = sum of sigma^2 over its communities, out[3g+2] =
-// vertices that moved in the last sweep (hist bit 0).  Fixed order.
-// Graphs with gact[g] == 0 are skipped (gact may be null).
+// Per graph g (one workgroup each): out[3g] = the weight inside communities,
+// out[3g+1] = sum of sigma^2 over its communities, out[3g+2] = vertices that
+// moved in the last sweep (hist bit 0).  Fixed order.  With own, the weight
+// inside is the sum of own over the graph's vertices and is also stored in
+// gin[g]; with own null it is read from gin[g] (kept up to date by the
+// applies' corrections).  Graphs with gact[g] == 0 are skipped (gact may be
+// null).
 void launch_graph_sums(const int32_t *vptr, int32_t G, const double *own, const double *sigma,
-                       const uint8_t *hist, const uint8_t *gact, double *out, hipStream_t s);
+                       const uint8_t *hist, const uint8_t *gact, double *gin, double *out,
+                       hipStream_t s);
 // Frontier of the active graphs: active[v] = gact[gid[v]] and deg > 0;
 // keys[v] = bucket or -1.
 void launch_graph_frontier(const int32_t *rowptr, int32_t V, const int32_t *gid,

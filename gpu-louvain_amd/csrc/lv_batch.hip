@@ -74,22 +74,26 @@ __global__ __launch_bounds__(256) void k_graph_sums(const int32_t *__restrict__ 
                                                     const double *__restrict__ sigma,
                                                     const uint8_t *__restrict__ hist,
                                                     const uint8_t *__restrict__ gact,
-                                                    double *out) {
+                                                    double *gin, double *out) {
     __shared__ double lds[4];
     const int32_t g = blockIdx.x;
     if (gact && !gact[g]) return;
     const int32_t lo = vptr[g], hi = vptr[g + 1];
     double o = 0.0, q = 0.0, m = 0.0;
     for (int32_t v = lo + threadIdx.x; v < hi; v += 256) {
-        o += own[v];
+        if (own) o += own[v];
         const double x = sigma[v];
         q += x * x;
         m += (double)(hist[v] & 1u);
     }
-    o = block_sum(o, lds);
+    if (own) o = block_sum(o, lds);
     q = block_sum(q, lds);
     m = block_sum(m, lds);
     if (threadIdx.x == 0) {
+        if (own)
+            gin[g] = o;
+        else
+            o = gin[g];
         out[3 * g] = o;
         out[3 * g + 1] = q;
         out[3 * g + 2] = m;
@@ -142,9 +146,11 @@ void launch_vertex_own(const int32_t *rowptr, const int32_t *col, const float *w
 }
 
 void launch_graph_sums(const int32_t *vptr, int32_t G, const double *own, const double *sigma,
-                       const uint8_t *hist, const uint8_t *gact, double *out, hipStream_t s) {
+                       const uint8_t *hist, const uint8_t *gact, double *gin, double *out,
+                       hipStream_t s) {
     if (G <= 0) return;
-    hipLaunchKernelGGL(k_graph_sums, dim3(G), dim3(256), 0, s, vptr, own, sigma, hist, gact, out);
+    hipLaunchKernelGGL(k_graph_sums, dim3(G), dim3(256), 0, s, vptr, own, sigma, hist, gact, gin,
+                       out);
 }
 
 void launch_graph_frontier(const int32_t *rowptr, int32_t V, const int32_t *gid,

@@ -226,6 +226,7 @@ struct lv_graph {
     float *gM32 = nullptr, *gden = nullptr;
     uint8_t *gact = nullptr, *gdone = nullptr;
     double *gsums = nullptr, *own_v = nullptr;
+    double *gin = nullptr;  // per graph: weight inside communities (incremental sweeps)
     std::vector<double> h_gsums;     // per graph: own, sq, moved of the last sweep
     uint8_t *h_up = nullptr;         // pinned staging of per-graph flag uploads
 
@@ -451,6 +452,8 @@ struct lv_graph {
         gm.clear_movers = bits + (int64_t)(cur ^ 1) * bit_words;
         gm.words = bit_words;
         gm.rounds = 1;
+        gm.gid = nullptr;
+        gm.gin = nullptr;
         return gm;
     }
 
@@ -1154,6 +1157,10 @@ struct lv_graph {
             hipEvent_t t0 = tbegin();
             lv::GroupMarks gm = marks();
             gm.rounds = rounds;
+            if (G > 0) {
+                gm.gid = gid;
+                gm.gin = gin;
+            }
             const lv::SegTable ct = corr_table(g, rounds);
             // pruned: only frontier vertices can move
             int32_t alo = lo, ahi = hi;
@@ -1166,7 +1173,7 @@ struct lv_graph {
             if (incremental) step++;
         }
         if (G > 0) {
-            LV_TRY(enqueue_graph_sums(pruned));
+            LV_TRY(enqueue_graph_sums(pruned, incremental));
         } else if (incremental) {
             hipEvent_t tq = tbegin();
             lv::launch_modularity_from_own(sigma, V, M, qpart.p, d_q, d_own_slots, s);
@@ -1442,19 +1449,31 @@ struct lv_graph {
     // Per-graph (own, sq, moved) of the current state into gsums (device).
     // pruned: only the graphs of the current frontier (their rows are the
     // frontier lists; the others' sums are not read).
-    lv_status enqueue_graph_sums(bool pruned = false) {
+    // Per-graph sums after a union sweep.  incremental: the weight inside
+    // communities was carried forward by the applies' corrections (gin);
+    // otherwise it is recomputed from the rows (of the active graphs when
+    // pruned) and stored in gin.
+    lv_status enqueue_graph_sums(bool pruned = false, bool incremental = false) {
         hipStream_t s = stream;
         hipEvent_t t0 = tbegin();
-        lv::fill_f64(own_v, 0.0, V, s);
-        if (pruned) {
-            const lv::SegTable fst = make_segs(h_foff, nins);
-            lv::launch_vertex_own(rowptr(), col(), w(), comm, forder, fst, ins.p, active, own_v, s);
-            lv::launch_graph_sums(vptr, G, own_v, sigma, hist, gact, gsums, s);
+        const uint8_t *ga = pruned ? gact : nullptr;
+        if (incremental) {
+            lv::launch_graph_sums(vptr, G, nullptr, sigma, hist, ga, gin, gsums, s);
+            tend(t0, LV_K_MODULARITY, 17.0 * (double)V);
         } else {
-            lv::launch_vertex_own(rowptr(), col(), w(), comm, order, st, ins.p, nullptr, own_v, s);
-            lv::launch_graph_sums(vptr, G, own_v, sigma, hist, nullptr, gsums, s);
+            lv::fill_f64(own_v, 0.0, V, s);
+            if (pruned) {
+                const lv::SegTable fst = make_segs(h_foff, nins);
+                lv::launch_vertex_own(rowptr(), col(), w(), comm, forder, fst, ins.p, active, own_v,
+                                      s);
+            } else {
+                lv::launch_vertex_own(rowptr(), col(), w(), comm, order, st, ins.p, nullptr, own_v,
+                                      s);
+            }
+            lv::launch_graph_sums(vptr, G, own_v, sigma, hist, ga, gin, gsums, s);
+            tend(t0, LV_K_MODULARITY, 12.0 * (double)E + 16.0 * (double)V);
+            own_valid = true;
         }
-        tend(t0, LV_K_MODULARITY, 12.0 * (double)E + 16.0 * (double)V);
         LV_LAUNCHED("graph sums");
         return LV_OK;
     }
@@ -1490,6 +1509,7 @@ struct lv_graph {
         LV_TRY(dev_alloc(&gdone, G + 8));
         LV_TRY(dev_alloc(&gsums, 3 * (int64_t)G));
         LV_TRY(dev_alloc(&own_v, (int64_t)V0 + 1));
+        LV_TRY(dev_alloc(&gin, (int64_t)G + 1));
         h_gsums.assign(3 * (size_t)G, 0.0);
         LV_HIP(hipHostMalloc(reinterpret_cast<void **>(&h_up), (size_t)G + 64, hipHostMallocDefault));
         LV_HIP(hipMemcpy(vptr0, vp, sizeof(int32_t) * (G + 1), hipMemcpyHostToDevice));
@@ -1735,6 +1755,7 @@ struct lv_graph {
         dev_free(gdone);
         dev_free(gsums);
         dev_free(own_v);
+        dev_free(gin);
         if (h_up) (void)hipHostFree(h_up);
         h_up = nullptr;
         dev_free(bits);

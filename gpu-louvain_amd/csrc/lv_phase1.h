@@ -127,6 +127,10 @@ struct GroupMarks {
     uint32_t *clear_movers;  // the other step's, cleared by this step
     int32_t words;
     int32_t rounds;  // correction: rows walked per lane group of a block (see corr_items)
+    // batches: the correction of each mover's row goes to gin[gid[v]] (the
+    // weight inside communities of v's graph) instead of own
+    const int32_t *gid;
+    double *gin;
 };
 
 // Items one correction block checks: rounds rows per X-lane group, at most

@@ -734,7 +734,8 @@ __global__ __launch_bounds__(kRowThreads) void k_apply_finish(
     const Chunk *__restrict__ chunks, const int32_t *__restrict__ comm,
     const int32_t *__restrict__ size, const double *__restrict__ sigma, uint64_t *crec,
     const uint32_t *__restrict__ movers, uint32_t *clear_bits, int32_t words, double *own,
-    int32_t fix_start, int32_t clear_start, int32_t rounds) {
+    int32_t fix_start, int32_t clear_start, int32_t rounds, const int32_t *__restrict__ gid,
+    double *gin) {
     __shared__ double lds[kRowThreads / 64];
     const int b = blockIdx.x;
     if (b >= clear_start) {
@@ -812,20 +813,45 @@ __global__ __launch_bounds__(kRowThreads) void k_apply_finish(
         }
         if (item >= 0) items[atomicAdd(&nitems, 1)] = item;
     }
+    // batches: the block's items lie in ascending vertex order, so one graph
+    // holds them all unless the first and the last differ
+    __shared__ int32_t graph_of_block;
+    if (gin && threadIdx.x == 0 && cnt > 0) {
+        const int32_t v0 = chunked ? chunks[first].v : order[list_off + first];
+        const int32_t v1 = chunked ? chunks[first + cnt - 1].v : order[list_off + first + cnt - 1];
+        graph_of_block = gid[v0] == gid[v1] ? gid[v0] : -1;
+    }
     __syncthreads();
     const int32_t nm = nitems;
     if (nm == 0) return;  // block-uniform
+    const int32_t gblock = gin ? graph_of_block : -1;
     const int lane = (int)threadIdx.x & (X - 1);
     for (int32_t m = (int32_t)threadIdx.x / X; m < nm; m += kRowThreads / X) {
+        int32_t v;
         if (chunked) {
             const Chunk ch = chunks[items[m]];
-            row(ch.v, ch.lo + lane, ch.hi, X, lane == 0 && ch.first);
+            v = ch.v;
+            row(v, ch.lo + lane, ch.hi, X, lane == 0 && ch.first);
         } else {
-            const int32_t v = items[m];
+            v = items[m];
             row(v, a.rowptr[v] + lane, a.rowptr[v + 1], X, lane == 0);
         }
+        if (gin && gblock < 0) {
+            // the row's sum over its lane group (over each wave when X = 256;
+            // the X lanes of a group are active together)
+            const int W = X < 64 ? X : 64;
+            double r = acc;
+            for (int o = W >> 1; o > 0; o >>= 1) r += __shfl_xor(r, o, 64);
+            if ((threadIdx.x & (W - 1)) == 0 && r != 0.0) atomicAdd(&gin[gid[v]], r);
+            acc = 0.0;
+        }
     }
+    if (gin && gblock < 0) return;  // block-uniform
     acc = block_sum_f64(acc, lds);
+    if (gin) {
+        if (threadIdx.x == 0 && acc != 0.0) atomicAdd(&gin[gblock], acc);
+        return;
+    }
     if (threadIdx.x == 0 && acc != 0.0)
         atomicAdd(&own[(int64_t)(blockIdx.x % kMoveSlots) * kMoveSlotStride], acc);
 }
@@ -1280,7 +1306,8 @@ void launch_apply(const MoveArgs &a, const int32_t *list, int32_t n, const int32
     hipLaunchKernelGGL(k_apply_finish, dim3(corr + fix + clear), dim3(kRowThreads), 0, s, a, order,
                        list, n, gm ? *gst : empty, chunks, comm, size, sigma, crec,
                        gm ? gm->movers : nullptr, gm ? gm->clear_movers : nullptr,
-                       gm ? gm->words : 0, own, corr, corr + fix, gm ? gm->rounds : 1);
+                       gm ? gm->words : 0, own, corr, corr + fix, gm ? gm->rounds : 1,
+                       gm ? gm->gid : nullptr, gm ? gm->gin : nullptr);
 }
 
 void launch_modularity_rows(const int32_t *rowptr, const int32_t *col, const float *w,

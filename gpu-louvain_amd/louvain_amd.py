@@ -575,7 +575,8 @@ def union(graphs) -> tuple[np.ndarray, CSR]:
 class Batch:
     """Independent graphs solved together (lv_batch_*)."""
 
-    def __init__(self, graphs, device: int = 0, timing: bool = False):
+    def __init__(self, graphs, device: int = 0, timing: bool = False, **options):
+        """options: lv_options fields (heavy_wg_max, q_incremental, ...)."""
         lib = load()
         _batch_sigs(lib)
         self._lib = lib
@@ -586,6 +587,10 @@ class Batch:
         opt = Options()
         opt.device = device
         opt.timing = int(timing)
+        for k, v in options.items():
+            if k not in dict(Options._fields_) or k in ("device", "timing", "reserved"):
+                raise TypeError(f"unknown option {k}")
+            setattr(opt, k, int(v))
         h = ctypes.c_void_p()
         _check(lib.lv_batch_create(self.G, _p(self.vptr, ctypes.c_int32), _p(self._keep[0], ctypes.c_int32),
                                    _p(self._keep[1], ctypes.c_int32), _p(self._keep[2], ctypes.c_float),
