@@ -187,6 +187,55 @@ def traffic_for(kernel, scale):
     return None
 
 
+def io_comparison(L, csr, tmpdir=None):
+    """Reading the graph from disk: the library's MatrixMarket reader on all
+    host threads and on one thread (a child process with OMP_NUM_THREADS=1),
+    and its binary CSR file, on the bench graph written as MatrixMarket."""
+    import subprocess
+    import tempfile
+
+    import numpy as np
+
+    d = tempfile.mkdtemp(dir=tmpdir)
+    mtx, bin_ = os.path.join(d, "g.mtx"), os.path.join(d, "g.lvcsr")
+    try:
+        t = time.perf_counter()
+        L.write_mtx(mtx, csr)
+        write_s = time.perf_counter() - t
+        L.write_csr(bin_, csr)
+        t = time.perf_counter()
+        a = L.read_mtx(mtx)
+        par_s = time.perf_counter() - t
+        t = time.perf_counter()
+        b = L.read_csr(bin_)
+        bin_s = time.perf_counter() - t
+        code = ("import sys, time, hashlib; sys.path.insert(0, sys.argv[1]); import louvain_amd as L; "
+                "t = time.perf_counter(); g = L.read_mtx(sys.argv[2]); dt = time.perf_counter() - t; "
+                "h = hashlib.sha1(g.rowptr.tobytes() + g.col.tobytes() + g.w.tobytes()).hexdigest(); "
+                "print(dt, h)")
+        env = dict(os.environ, OMP_NUM_THREADS="1")
+        r = subprocess.run([sys.executable, "-c", code, os.path.join(ROOT, "gpu-louvain_amd"), mtx],
+                           env=env, capture_output=True, text=True, check=True)
+        one_s, h1 = r.stdout.split()
+        import hashlib
+
+        hN = hashlib.sha1(a.rowptr.tobytes() + a.col.tobytes() + a.w.tobytes()).hexdigest()
+        same = bool(h1 == hN and np.array_equal(b.col, csr.col) and np.array_equal(b.rowptr, csr.rowptr))
+        out = {"mtx_bytes": os.path.getsize(mtx), "write_mtx_s": write_s, "read_mtx_s": par_s,
+               "read_mtx_one_thread_s": float(one_s), "read_binary_csr_s": bin_s,
+               "threads": int(os.environ.get("OMP_NUM_THREADS", os.cpu_count() or 1)),
+               "identical_csr": same}
+        log(f"io: {out['mtx_bytes'] / 1e9:.2f} GB MatrixMarket, read {par_s:.2f}s "
+            f"({out['threads']} threads) vs {float(one_s):.2f}s (1 thread), binary CSR {bin_s:.2f}s, "
+            f"identical {same}")
+        return out
+    finally:
+        for f in (mtx, bin_):
+            if os.path.exists(f):
+                os.remove(f)
+        os.rmdir(d)
+
+
 def csr_scale(csr):
     return int(round(__import__("math").log2(max(csr.V, 1))))
 
@@ -206,6 +255,8 @@ def main():
     ap.add_argument("--no-tune", action="store_true")
     ap.add_argument("--level", type=int, default=0,
                     help="time sweeps of this level (reached by plain phases) from its singleton state")
+    ap.add_argument("--io", action="store_true",
+                    help="also time reading the graph from MatrixMarket and binary CSR files")
     ap.add_argument("--calibrate-copy", action="store_true",
                     help="run only a plain 1 GiB device copy (PMC byte-counter calibration)")
     args = ap.parse_args()
@@ -321,6 +372,8 @@ def main():
     if not args.no_batch:
         batch = batch_comparison(L)
 
+    io = io_comparison(L, csr) if args.io else None
+
     cpu = None
     if not args.no_cpu_baseline:
         log("cpu baseline: one oracle sweep ...")
@@ -352,6 +405,7 @@ def main():
         "pruned": pruned,
         "batch": batch,
         "autotune": tuned,
+        "io": io,
     }
     print(json.dumps(out))
 

@@ -187,6 +187,10 @@ struct lv_graph {
 
     // level 0 (kept) and two buffers for aggregated levels
     int32_t V0 = 0;
+    // original vertices (length of orig2comm): V0, or those of the run a
+    // level dump came from, whose map into level 0 of this handle is o2c0
+    int32_t Vorig = 0;
+    int32_t *o2c0 = nullptr;
     int64_t E0 = 0;
     int32_t *rp0 = nullptr, *col0 = nullptr;
     float *w0 = nullptr;
@@ -569,6 +573,7 @@ struct lv_graph {
         (void)hipGetLastError();
         apply_options();
         V0 = V = nV;
+        Vorig = nV;
         E0 = E = nE;
 
         // M = (sum of CSR weights)/2 in FP64, CSR order (utils.cu:36-46).
@@ -659,7 +664,11 @@ struct lv_graph {
     // Back to the singleton state of level 0.  When level 0 is current and
     // set up, only the community state is rewritten (no re-bucketing).
     lv_status reset() {
-        lv::iota_i32(orig2comm, V0, stream);
+        if (o2c0)
+            LV_HIP(hipMemcpyAsync(orig2comm, o2c0, sizeof(int32_t) * (size_t)Vorig,
+                                  hipMemcpyDeviceToDevice, stream));
+        else
+            lv::iota_i32(orig2comm, V0, stream);
         if (cur_buf < 0 && level_ready) {
             lv::launch_level_state(V, k, comm, size, sigma, crec, stream);
             state_changed();
@@ -1305,7 +1314,7 @@ struct lv_graph {
             if (stop) q_before_pruned = q_prev;
             q_prev = q1;
         }
-        lv::launch_compose(orig2comm, V0, comm, stream);
+        lv::launch_compose(orig2comm, Vorig, comm, stream);
         LV_LAUNCHED("compose");
         if (changed) *changed = ch;
         if (sweeps_out) *sweeps_out = sweeps;
@@ -1426,7 +1435,7 @@ struct lv_graph {
             lv::launch_copy_rows(a_order + toff[t], toff[t + 1] - toff[t], t >= 2, a_mstart,
                                  a_mscan, a_cnt, a_cstart, tmp_col, tmp_w, colb[nb], wb[nb], s);
         // updateOriginalToCommunity (community_aggregation.cu:222-228)
-        lv::launch_compose(orig2comm, V0, a_newid, s);
+        lv::launch_compose(orig2comm, Vorig, a_newid, s);
         LV_LAUNCHED("aggregation output");
         tend(t0, LV_K_AGGREGATE, 24.0 * (double)E + 40.0 * (double)V);
         LV_HIP(hipMemcpyAsync(h_stage, etot_d, sizeof(int32_t), hipMemcpyDeviceToHost, s));
@@ -1583,7 +1592,7 @@ struct lv_graph {
                 }
                 if (flip && nact > 0) LV_TRY(upload_flags(gact, act));
             }
-            lv::launch_compose(orig2comm, V0, comm, s);
+            lv::launch_compose(orig2comm, Vorig, comm, s);
             LV_LAUNCHED("compose");
             std::vector<uint8_t> now(G, 0);
             bool any_now = false;
@@ -1695,7 +1704,7 @@ struct lv_graph {
         }
         lv::exclusive_scan(a_cnt, rpb[nb], (int64_t)newV + 1, etot_d, scratch, s);
         // updateOriginalToCommunity (community_aggregation.cu:222-228)
-        lv::launch_compose(orig2comm, V0, a_newid, s);
+        lv::launch_compose(orig2comm, Vorig, a_newid, s);
         LV_LAUNCHED("sort aggregation");
         tend(t0, LV_K_AGGREGATE, 24.0 * (double)E + 40.0 * (double)V);
         LV_HIP(hipMemcpyAsync(h_stage, U_d, sizeof(int32_t), hipMemcpyDeviceToHost, s));
@@ -1714,6 +1723,34 @@ struct lv_graph {
         return LV_OK;
     }
 
+    // The original vertices of an earlier run map into this handle's level 0
+    // by o2c (n entries in [0, V0)); reset() and lv_assignment use it.  The
+    // run's total weight M_run replaces the one summed from this level.
+    lv_status set_initial_map(const int32_t *o2c, int32_t n, double M_run) {
+        for (int32_t x = 0; x < n; ++x)
+            if (o2c[x] < 0 || o2c[x] >= V0) return fail(LV_ERR_ARG, "map entry out of range");
+        if (!(M_run >= 0.0)) return fail(LV_ERR_ARG, "bad total weight");
+        // the run's M (aggregated fp32 weights need not sum to it exactly)
+        M = M_run;
+        M32 = (float)M;
+        {
+            volatile float two_m = 2.0f * M32;
+            volatile float d = two_m * M32;
+            den = d;
+        }
+        if ((int64_t)n + 1 > (int64_t)V0 + 1) {
+            dev_free(orig2comm);
+            orig2comm = nullptr;
+            LV_TRY(dev_alloc(&orig2comm, (int64_t)n + 1));
+        }
+        dev_free(o2c0);
+        o2c0 = nullptr;
+        LV_TRY(dev_alloc(&o2c0, (int64_t)n + 1));
+        LV_HIP(hipMemcpy(o2c0, o2c, sizeof(int32_t) * (size_t)n, hipMemcpyHostToDevice));
+        Vorig = n;
+        return reset();
+    }
+
     lv_status destroy() {
         if (stream) (void)hipStreamSynchronize(stream);
         dev_free(rp0);
@@ -1730,6 +1767,7 @@ struct lv_graph {
         dev_free(size);
         dev_free(newc);
         dev_free(orig2comm);
+        dev_free(o2c0);
         dev_free(crec);
         dev_free(dk);
         dev_free(margin);
@@ -2076,8 +2114,8 @@ lv_status lv_predict(lv_graph *g, float threshold, float *prob, uint8_t *active)
 lv_status lv_assignment(lv_graph *g, int32_t *orig2comm, int32_t *level_V) {
     if (!g) return fail(LV_ERR_ARG, "null graph");
     LV_HIP(hipSetDevice(g->device));
-    if (orig2comm && g->V0 > 0)
-        LV_HIP(hipMemcpyAsync(orig2comm, g->orig2comm, sizeof(int32_t) * g->V0,
+    if (orig2comm && g->Vorig > 0)
+        LV_HIP(hipMemcpyAsync(orig2comm, g->orig2comm, sizeof(int32_t) * g->Vorig,
                               hipMemcpyDeviceToHost, g->stream));
     LV_TRY(g->sync());
     if (level_V) *level_V = g->V;
@@ -2103,6 +2141,81 @@ lv_status lv_level_graph(lv_graph *g, int32_t *rowptr, int32_t *col, float *w) {
     if (w && g->E > 0)
         LV_HIP(hipMemcpyAsync(w, g->w(), sizeof(float) * g->E, hipMemcpyDeviceToHost, g->stream));
     return g->sync();
+}
+
+namespace {
+const char kMapMagic[8] = {'L', 'V', 'O', '2', 'C', '0', '1', '\0'};
+}
+
+lv_status lv_dump_level(lv_graph *g, const char *prefix, int32_t with_mtx) {
+    if (!g || !prefix) return fail(LV_ERR_ARG, "null argument");
+    LV_HIP(hipSetDevice(g->device));
+    const int32_t V = g->V;
+    const int64_t E = g->E;
+    std::vector<int32_t> rp((size_t)V + 1), cl((size_t)std::max<int64_t>(E, 1));
+    std::vector<float> w((size_t)std::max<int64_t>(E, 1));
+    LV_TRY(lv_level_graph(g, rp.data(), cl.data(), w.data()));
+    std::vector<int32_t> o2c((size_t)std::max(g->Vorig, 1));
+    int32_t lv_V = 0;
+    LV_TRY(lv_assignment(g, o2c.data(), &lv_V));
+    const std::string base(prefix);
+    LV_TRY(lv_write_csr((base + ".lvcsr").c_str(), rp.data(), cl.data(), w.data(), V, E));
+    if (with_mtx) LV_TRY(lv_write_mtx((base + ".mtx").c_str(), rp.data(), cl.data(), w.data(), V));
+    const std::string mp = base + ".o2c";
+    FILE *f = std::fopen(mp.c_str(), "wb");
+    if (!f) return fail(LV_ERR_IO, "cannot create '" + mp + "'");
+    const int64_t hdr[2] = {(int64_t)g->Vorig, (int64_t)V};
+    const double Mrun = g->M;
+    bool ok = std::fwrite(kMapMagic, 1, 8, f) == 8 && std::fwrite(hdr, sizeof hdr, 1, f) == 1 &&
+              std::fwrite(&Mrun, sizeof Mrun, 1, f) == 1 &&
+              std::fwrite(o2c.data(), sizeof(int32_t), (size_t)g->Vorig, f) == (size_t)g->Vorig;
+    ok = (std::fclose(f) == 0) && ok;
+    if (!ok) return fail(LV_ERR_IO, "cannot write '" + mp + "'");
+    return LV_OK;
+}
+
+lv_status lv_create_from_dump(const char *prefix, const lv_options *opt, lv_graph **out) {
+    if (!prefix || !out) return fail(LV_ERR_ARG, "null argument");
+    *out = nullptr;
+    const std::string base(prefix);
+    const std::string mp = base + ".o2c";
+    std::vector<int32_t> o2c;
+    int64_t hdr[2] = {0, 0};
+    double Mrun = 0.0;
+    {
+        FILE *f = std::fopen(mp.c_str(), "rb");
+        if (!f) return fail(LV_ERR_IO, "cannot open '" + mp + "'");
+        char magic[8] = {0};
+        bool ok = std::fread(magic, 1, 8, f) == 8 && std::memcmp(magic, kMapMagic, 8) == 0 &&
+                  std::fread(hdr, sizeof hdr, 1, f) == 1 && hdr[0] >= 0 && hdr[0] <= 0x7fffffffLL &&
+                  std::fread(&Mrun, sizeof Mrun, 1, f) == 1;
+        if (ok) {
+            o2c.resize((size_t)std::max<int64_t>(hdr[0], 1));
+            ok = std::fread(o2c.data(), sizeof(int32_t), (size_t)hdr[0], f) == (size_t)hdr[0];
+        }
+        std::fclose(f);
+        if (!ok) return fail(LV_ERR_ARG, "'" + mp + "' is not a vertex map of a level dump");
+    }
+    int32_t *rp = nullptr, *cl = nullptr;
+    float *w = nullptr;
+    int32_t V = 0;
+    int64_t E = 0;
+    LV_TRY(lv_read_csr((base + ".lvcsr").c_str(), &rp, &cl, &w, &V, &E));
+    lv_status st = (int64_t)V == hdr[1] ? LV_OK : fail(LV_ERR_ARG, "dump files disagree on V");
+    lv_graph *g = nullptr;
+    if (st == LV_OK) st = lv_create(rp, cl, w, V, E, opt, &g);
+    lv_free_host(rp);
+    lv_free_host(cl);
+    lv_free_host(w);
+    if (st == LV_OK) {
+        st = g->set_initial_map(o2c.data(), (int32_t)hdr[0], Mrun);
+        if (st != LV_OK) {
+            lv_destroy(g);
+            return st;
+        }
+        *out = g;
+    }
+    return st;
 }
 
 lv_status lv_get_communities(lv_graph *g, int32_t *comm) {

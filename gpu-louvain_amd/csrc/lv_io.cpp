@@ -20,11 +20,16 @@
 #include <cstdint>
 #include <cstdio>
 #include <cstdlib>
+#include <climits>
 #include <cstring>
 #include <string>
 #include <vector>
 
+#include <fcntl.h>
 #include <omp.h>
+#include <sys/mman.h>
+#include <sys/stat.h>
+#include <unistd.h>
 
 #include "../../include/louvain.h"
 #include "lv_error.h"
@@ -130,6 +135,209 @@ lv_status csr_from_pairs(const std::vector<uint64_t> &pairs, int32_t n, uint64_t
     return LV_OK;
 }
 
+
+// Token parsers of the reader.  parse_int: an optional sign and decimal
+// digits only.  parse_float: the correctly rounded float of a decimal token;
+// tokens of at most 19 significant digits and a decimal exponent within
+// +-22 take one exact double operation (Clinger's fast path), then the
+// double's rounding to float, which is exact unless the double lies on a
+// midpoint between two floats (a midpoint between the token and the double
+// would have been the nearer double); every other token goes to strtof.
+inline bool parse_int(const char *t, size_t len, long long &out) {
+    size_t i = 0;
+    bool neg = false;
+    if (i < len && (t[i] == '+' || t[i] == '-')) neg = t[i++] == '-';
+    if (i >= len || len - i > 18) return false;
+    long long v = 0;
+    for (; i < len; ++i) {
+        const unsigned d = (unsigned)(t[i] - '0');
+        if (d > 9) return false;
+        v = v * 10 + d;
+    }
+    out = neg ? -v : v;
+    return true;
+}
+
+inline bool parse_float(char *t, size_t len, float &out) {
+    static const double p10[23] = {1e0,  1e1,  1e2,  1e3,  1e4,  1e5,  1e6,  1e7,
+                                   1e8,  1e9,  1e10, 1e11, 1e12, 1e13, 1e14, 1e15,
+                                   1e16, 1e17, 1e18, 1e19, 1e20, 1e21, 1e22};
+    size_t i = 0;
+    bool neg = false;
+    if (i < len && (t[i] == '+' || t[i] == '-')) neg = t[i++] == '-';
+    uint64_t m = 0;
+    int digits = 0, e10 = 0;
+    bool any = false, fast = true;
+    for (; i < len && (unsigned)(t[i] - '0') <= 9; ++i) {
+        any = true;
+        if (m == 0 && t[i] == '0') continue;
+        if (digits < 19) {
+            m = m * 10 + (unsigned)(t[i] - '0');
+            ++digits;
+        } else {
+            fast = false;
+        }
+    }
+    if (i < len && t[i] == '.') {
+        for (++i; i < len && (unsigned)(t[i] - '0') <= 9; ++i) {
+            any = true;
+            if (m == 0 && t[i] == '0') {
+                --e10;
+                continue;
+            }
+            if (digits < 19) {
+                m = m * 10 + (unsigned)(t[i] - '0');
+                ++digits;
+                --e10;
+            } else {
+                fast = false;
+            }
+        }
+    }
+    if (any && i < len && (t[i] == 'e' || t[i] == 'E')) {
+        size_t j = i + 1;
+        bool eneg = false;
+        if (j < len && (t[j] == '+' || t[j] == '-')) eneg = t[j++] == '-';
+        int ev = 0;
+        bool ed = false;
+        for (; j < len && (unsigned)(t[j] - '0') <= 9; ++j) {
+            ed = true;
+            if (ev < 10000) ev = ev * 10 + (t[j] - '0');
+        }
+        if (ed) {
+            e10 += eneg ? -ev : ev;
+            i = j;
+        }
+    }
+    if (fast && any && i == len && m < (1ull << 53) && e10 >= -22 && e10 <= 22) {
+        double d = (double)m;
+        d = e10 < 0 ? d / p10[-e10] : d * p10[e10];
+        // a float midpoint has zeros in the 28 low mantissa bits below the
+        // half-ulp bit of float precision
+        uint64_t bits;
+        std::memcpy(&bits, &d, sizeof bits);
+        const uint64_t low = bits & ((1ull << 29) - 1);
+        if (d == 0.0 || low != (1ull << 28)) {
+            out = (float)(neg ? -d : d);
+            return true;
+        }
+    }
+    char *e1 = nullptr;
+    const char save = t[len];
+    t[len] = '\0';
+    out = std::strtof(t, &e1);
+    t[len] = save;
+    return e1 == t + len;
+}
+
+inline bool is_space(char c) { return c == ' ' || c == '\n' || c == '\t' || c == '\r' || c == '\v' || c == '\f'; }
+
+// Read-only mapping of a whole file (empty files map to an empty range).
+struct MappedFile {
+    const char *data = nullptr;
+    size_t size = 0;
+    void *base = nullptr;
+    bool open(const char *path) {
+        const int fd = ::open(path, O_RDONLY);
+        if (fd < 0) return false;
+        struct stat st;
+        if (::fstat(fd, &st) != 0) {
+            ::close(fd);
+            return false;
+        }
+        size = (size_t)st.st_size;
+        if (size > 0) {
+            base = ::mmap(nullptr, size, PROT_READ, MAP_PRIVATE, fd, 0);
+            if (base == MAP_FAILED) {
+                base = nullptr;
+                ::close(fd);
+                return false;
+            }
+            ::madvise(base, size, MADV_SEQUENTIAL);
+            data = (const char *)base;
+        } else {
+            static const char empty = 0;
+            data = &empty;
+        }
+        ::close(fd);
+        return true;
+    }
+    ~MappedFile() {
+        if (base) ::munmap(base, size);
+    }
+};
+
+// CSR of entries (i, j, w) in the reference's order: row i receives (j, w)
+// and, when i != j, row j receives (i, w), each row in entry order
+// (readInputData, utils.cu:29-52).  Threads fill contiguous entry ranges;
+// per-range row counts give each range its place inside every row.
+lv_status csr_in_entry_order(const std::vector<int32_t> &ti, const std::vector<int32_t> &tj,
+                             const std::vector<float> &tw, int32_t n, int32_t **rowptr,
+                             int32_t **col, float **w, int32_t *V, int64_t *E) {
+    const int64_t m = (int64_t)ti.size();
+    // up to 8 ranges of >= 64Ki entries, the counts within 1 GiB
+    int T = (int)std::min<int64_t>(std::min(omp_get_max_threads(), 8), m / 65536);
+    T = std::max(T, 1);
+    while (T > 1 && (int64_t)T * ((int64_t)n + 1) * 4 > (int64_t)1 << 30) --T;
+    std::vector<int32_t> cnt((size_t)T * ((size_t)n + 1), 0);
+    std::vector<int64_t> lo(T + 1);
+    for (int t = 0; t <= T; ++t) lo[t] = m * t / T;
+#pragma omp parallel for num_threads(T) schedule(static, 1)
+    for (int t = 0; t < T; ++t) {
+        int32_t *c = cnt.data() + (size_t)t * ((size_t)n + 1);
+        for (int64_t e = lo[t]; e < lo[t + 1]; ++e) {
+            c[ti[e]]++;
+            if (ti[e] != tj[e]) c[tj[e]]++;
+        }
+    }
+    // rowptr, then each range's start inside every row (cnt turns into offsets)
+    int32_t *rp = (int32_t *)std::malloc(sizeof(int32_t) * ((size_t)n + 1));
+    if (!rp) return io_fail(LV_ERR_OOM, "host allocation failed");
+    int64_t total = 0;
+    for (int32_t v = 0; v < n; ++v) {
+        if (total > 0x7fffffffLL) break;
+        rp[v] = (int32_t)total;
+        for (int t = 0; t < T; ++t) {
+            int32_t &c = cnt[(size_t)t * ((size_t)n + 1) + v];
+            const int32_t k = c;
+            c = (int32_t)total;
+            total += k;
+        }
+    }
+    if (total > 0x7fffffffLL) {
+        std::free(rp);
+        return io_fail(LV_ERR_ARG, "graph exceeds int32 CSR entries");
+    }
+    rp[n] = (int32_t)total;
+    int32_t *cl = (int32_t *)std::malloc(sizeof(int32_t) * (size_t)std::max<int64_t>(total, 1));
+    float *wt = (float *)std::malloc(sizeof(float) * (size_t)std::max<int64_t>(total, 1));
+    if (!cl || !wt) {
+        std::free(rp);
+        std::free(cl);
+        std::free(wt);
+        return io_fail(LV_ERR_OOM, "host allocation failed");
+    }
+#pragma omp parallel for num_threads(T) schedule(static, 1)
+    for (int t = 0; t < T; ++t) {
+        int32_t *pos = cnt.data() + (size_t)t * ((size_t)n + 1);
+        for (int64_t e = lo[t]; e < lo[t + 1]; ++e) {
+            const int32_t a = ti[e], b = tj[e];
+            cl[pos[a]] = b;
+            wt[pos[a]++] = tw[e];
+            if (a != b) {
+                cl[pos[b]] = a;
+                wt[pos[b]++] = tw[e];
+            }
+        }
+    }
+    *rowptr = rp;
+    *col = cl;
+    *w = wt;
+    *V = n;
+    *E = total;
+    return LV_OK;
+}
+
 }  // namespace
 
 extern "C" {
@@ -139,20 +347,12 @@ void lv_free_host(void *p) { std::free(p); }
 lv_status lv_read_mtx(const char *path, int32_t **rowptr, int32_t **col, float **w, int32_t *V,
                       int64_t *E) {
     if (!path || !rowptr || !col || !w || !V || !E) return io_fail(LV_ERR_ARG, "null argument");
-    FILE *f = std::fopen(path, "rb");
-    if (!f) return io_fail(LV_ERR_IO, std::string("cannot open '") + path + "'");
-    std::fseek(f, 0, SEEK_END);
-    long sz = std::ftell(f);
-    std::fseek(f, 0, SEEK_SET);
-    std::string buf;
-    buf.resize(sz > 0 ? (size_t)sz : 0);
-    if (sz > 0 && std::fread(&buf[0], 1, (size_t)sz, f) != (size_t)sz) {
-        std::fclose(f);
-        return io_fail(LV_ERR_IO, std::string("cannot read '") + path + "'");
-    }
-    std::fclose(f);
-    const char *p = buf.c_str();
-    const char *end = p + buf.size();
+    const std::string where = std::string("'") + path + "'";
+    MappedFile mf;
+    if (!mf.open(path)) return io_fail(LV_ERR_IO, std::string("cannot open ") + where);
+    const char *p = mf.data;
+    const char *end = mf.data + mf.size;
+    // header: '%' lines (a "pattern" banner means w = 1), then "rows cols nnz"
     bool pattern = false, have = false;
     long long rows = 0, cols = 0, nnz = 0;
     while (p < end) {
@@ -165,7 +365,7 @@ lv_status lv_read_mtx(const char *path, int32_t **rowptr, int32_t **col, float *
         } else if (p < le && *p != '\r') {
             std::string line(p, le);
             if (std::sscanf(line.c_str(), "%lld %lld %lld", &rows, &cols, &nnz) != 3)
-                return io_fail(LV_ERR_ARG, std::string("bad MatrixMarket size line in '") + path + "'");
+                return io_fail(LV_ERR_ARG, "bad MatrixMarket size line in " + where);
             have = true;
             p = nl ? nl + 1 : end;
             break;
@@ -174,68 +374,193 @@ lv_status lv_read_mtx(const char *path, int32_t **rowptr, int32_t **col, float *
     }
     (void)rows;
     if (!have || cols < 0 || nnz < 0 || cols > 0x7fffffffLL)
-        return io_fail(LV_ERR_ARG, std::string("missing or invalid size line in '") + path + "'");
+        return io_fail(LV_ERR_ARG, "missing or invalid size line in " + where);
     const int32_t n = (int32_t)cols;
-    std::vector<int32_t> ti((size_t)nnz), tj((size_t)nnz);
-    std::vector<float> tw((size_t)nnz);
-    std::vector<int64_t> deg((size_t)n + 1, 0);
-    char *q = const_cast<char *>(p);
-    int64_t Ecount = 0;
-    for (long long t = 0; t < nnz; ++t) {
-        char *e1 = nullptr;
-        errno = 0;
-        long long a = std::strtoll(q, &e1, 10);
-        if (e1 == q) return io_fail(LV_ERR_ARG, "'" + std::string(path) + "': expected " + std::to_string(nnz) + " entries, got " + std::to_string(t));
-        q = e1;
-        long long b = std::strtoll(q, &e1, 10);
-        if (e1 == q) return io_fail(LV_ERR_ARG, "'" + std::string(path) + "': truncated entry " + std::to_string(t + 1));
-        q = e1;
-        float x = 1.0f;
-        if (!pattern) {
-            x = std::strtof(q, &e1);
-            if (e1 == q) return io_fail(LV_ERR_ARG, "'" + std::string(path) + "': missing weight in entry " + std::to_string(t + 1));
-            q = e1;
+    const int K = pattern ? 2 : 3;  // tokens per entry
+    const int64_t need = (int64_t)nnz * K;
+
+    // Entries are whitespace-separated tokens, as the reference's stream
+    // extraction reads them.  Pass 1 counts the tokens of byte chunks cut at
+    // whitespace; their prefix gives every token its entry and field, so
+    // pass 2 parses the chunks independently.
+    const int64_t region = (int64_t)(end - p);
+    const int nt = std::max(1, omp_get_max_threads());
+    int64_t chunk = std::max<int64_t>(region / (8 * (int64_t)nt) + 1, 1 << 16);
+    if (const char *env = std::getenv("LV_MTX_CHUNK_BYTES")) chunk = std::max(1LL, std::atoll(env));
+    std::vector<const char *> cut;
+    cut.push_back(p);
+    for (const char *c = p + chunk; c < end; c += chunk) {
+        const char *x = std::max(c, cut.back());
+        while (x < end && !is_space(*x)) ++x;
+        if (x >= end) break;
+        cut.push_back(x);
+    }
+    cut.push_back(end);
+    const int64_t nch = (int64_t)cut.size() - 1;
+    std::vector<int64_t> ntok(nch + 1, 0);
+#pragma omp parallel for schedule(dynamic, 1)
+    for (int64_t c = 0; c < nch; ++c) {
+        int64_t k = 0;
+        bool in = false;
+        for (const char *x = cut[c]; x < cut[c + 1]; ++x) {
+            const bool sp = is_space(*x);
+            if (!sp && !in) ++k;
+            in = !sp;
         }
-        if (a < 1 || a > n || b < 1 || b > n)
-            return io_fail(LV_ERR_ARG, "'" + std::string(path) + "': entry " + std::to_string(t + 1) + " index out of range");
-        ti[t] = (int32_t)(a - 1);
-        tj[t] = (int32_t)(b - 1);
-        tw[t] = x;
-        deg[ti[t]]++;
-        Ecount++;
-        if (ti[t] != tj[t]) {
-            deg[tj[t]]++;
-            Ecount++;
+        ntok[c + 1] = k;
+    }
+    for (int64_t c = 0; c < nch; ++c) ntok[c + 1] += ntok[c];
+    const int64_t have_tok = ntok[nch];
+    if (have_tok < need) {
+        const int64_t t = have_tok / K, r = have_tok % K;
+        if (r == 0)
+            return io_fail(LV_ERR_ARG, where + ": expected " + std::to_string(nnz) + " entries, got " +
+                                           std::to_string(t));
+        if (r == 1) return io_fail(LV_ERR_ARG, where + ": truncated entry " + std::to_string(t + 1));
+        return io_fail(LV_ERR_ARG, where + ": missing weight in entry " + std::to_string(t + 1));
+    }
+    std::vector<int32_t> ti((size_t)nnz), tj((size_t)nnz);
+    std::vector<float> tw((size_t)nnz, 1.0f);
+    // first bad entry (1-based) and its kind: 1 = not a number, 2 = index out of range
+    int64_t bad_entry = INT64_MAX;
+    int bad_kind = 0;
+#pragma omp parallel for schedule(dynamic, 1)
+    for (int64_t c = 0; c < nch; ++c) {
+        int64_t g = ntok[c];
+        if (g >= need) continue;
+        int64_t my_bad = INT64_MAX;
+        int my_kind = 0;
+        const char *x = cut[c];
+        const char *ce = cut[c + 1];
+        char tok[64];
+        while (x < ce && g < need) {
+            while (x < ce && is_space(*x)) ++x;
+            if (x >= ce) break;
+            const char *t0 = x;
+            while (x < ce && !is_space(*x)) ++x;
+            const size_t len = (size_t)(x - t0);
+            const int64_t entry = g / K;
+            const int field = (int)(g % K);
+            ++g;
+            if (entry + 1 >= my_bad) continue;
+            bool ok = len < sizeof tok;
+            if (ok) {
+                std::memcpy(tok, t0, len);
+                tok[len] = '\0';
+                if (field < 2) {
+                    long long v = 0;
+                    ok = parse_int(tok, len, v);
+                    if (ok && (v < 1 || v > n)) {
+                        my_bad = entry + 1;
+                        my_kind = 2;
+                        continue;
+                    }
+                    if (ok) (field == 0 ? ti : tj)[(size_t)entry] = (int32_t)(v - 1);
+                } else {
+                    float v = 0.0f;
+                    ok = parse_float(tok, len, v);
+                    if (ok) tw[(size_t)entry] = v;
+                }
+            }
+            if (!ok) {
+                my_bad = entry + 1;
+                my_kind = 1;
+            }
+        }
+        if (my_bad != INT64_MAX) {
+#pragma omp critical(lv_mtx_bad)
+            if (my_bad < bad_entry) {
+                bad_entry = my_bad;
+                bad_kind = my_kind;
+            }
         }
     }
-    if (Ecount > 0x7fffffffLL) return io_fail(LV_ERR_ARG, "graph exceeds int32 CSR entries");
+    if (bad_entry != INT64_MAX)
+        return io_fail(LV_ERR_ARG, where + ": entry " + std::to_string(bad_entry) +
+                                       (bad_kind == 2 ? " index out of range" : " is not a number"));
+    return csr_in_entry_order(ti, tj, tw, n, rowptr, col, w, V, E);
+}
+
+static const char kCsrMagic[8] = {'L', 'V', 'C', 'S', 'R', '0', '1', '\0'};
+
+lv_status lv_write_csr(const char *path, const int32_t *rowptr, const int32_t *col,
+                       const float *w, int32_t V, int64_t E) {
+    if (!path || !rowptr || V < 0 || E < 0 || (E > 0 && (!col || !w)))
+        return io_fail(LV_ERR_ARG, "bad argument");
+    if (rowptr[0] != 0 || rowptr[V] != E) return io_fail(LV_ERR_ARG, "rowptr does not span E entries");
+    FILE *f = std::fopen(path, "wb");
+    if (!f) return io_fail(LV_ERR_IO, std::string("cannot create '") + path + "'");
+    const int64_t hdr[2] = {(int64_t)V, E};
+    bool ok = std::fwrite(kCsrMagic, 1, 8, f) == 8 && std::fwrite(hdr, sizeof hdr, 1, f) == 1 &&
+              std::fwrite(rowptr, sizeof(int32_t), (size_t)V + 1, f) == (size_t)V + 1 &&
+              (E == 0 || (std::fwrite(col, sizeof(int32_t), (size_t)E, f) == (size_t)E &&
+                          std::fwrite(w, sizeof(float), (size_t)E, f) == (size_t)E));
+    ok = (std::fclose(f) == 0) && ok;
+    if (!ok) return io_fail(LV_ERR_IO, std::string("cannot write '") + path + "'");
+    return LV_OK;
+}
+
+lv_status lv_read_csr(const char *path, int32_t **rowptr, int32_t **col, float **w, int32_t *V,
+                      int64_t *E) {
+    if (!path || !rowptr || !col || !w || !V || !E) return io_fail(LV_ERR_ARG, "null argument");
+    const std::string where = std::string("'") + path + "'";
+    MappedFile mf;
+    if (!mf.open(path)) return io_fail(LV_ERR_IO, std::string("cannot open ") + where);
+    if (mf.size < 24 || std::memcmp(mf.data, kCsrMagic, 8) != 0)
+        return io_fail(LV_ERR_ARG, where + " is not a binary CSR file");
+    int64_t hdr[2];
+    std::memcpy(hdr, mf.data + 8, sizeof hdr);
+    const int64_t n = hdr[0], m = hdr[1];
+    if (n < 0 || n > 0x7fffffffLL || m < 0 || m > 0x7fffffffLL ||
+        (uint64_t)mf.size != 24 + 4 * ((uint64_t)n + 1) + 8 * (uint64_t)m)
+        return io_fail(LV_ERR_ARG, where + ": sizes do not match the file length");
     int32_t *rp = (int32_t *)std::malloc(sizeof(int32_t) * ((size_t)n + 1));
-    int32_t *cl = (int32_t *)std::malloc(sizeof(int32_t) * (size_t)std::max<int64_t>(Ecount, 1));
-    float *wt = (float *)std::malloc(sizeof(float) * (size_t)std::max<int64_t>(Ecount, 1));
+    int32_t *cl = (int32_t *)std::malloc(sizeof(int32_t) * (size_t)std::max<int64_t>(m, 1));
+    float *wt = (float *)std::malloc(sizeof(float) * (size_t)std::max<int64_t>(m, 1));
     if (!rp || !cl || !wt) {
         std::free(rp);
         std::free(cl);
         std::free(wt);
         return io_fail(LV_ERR_OOM, "host allocation failed");
     }
-    rp[0] = 0;
-    for (int32_t v = 0; v < n; ++v) rp[v + 1] = rp[v] + (int32_t)deg[v];
-    std::vector<int64_t> fill(rp, rp + n);
-    for (long long t = 0; t < nnz; ++t) {
-        const int32_t a = ti[t], b = tj[t];
-        cl[fill[a]] = b;
-        wt[fill[a]++] = tw[t];
-        if (a != b) {
-            cl[fill[b]] = a;
-            wt[fill[b]++] = tw[t];
-        }
+    const char *q = mf.data + 24;
+    std::memcpy(rp, q, sizeof(int32_t) * ((size_t)n + 1));
+    q += sizeof(int32_t) * ((size_t)n + 1);
+    // large arrays copied by all threads
+    const int64_t kPiece = 1 << 22;
+#pragma omp parallel for schedule(static)
+    for (int64_t s0 = 0; s0 < m; s0 += kPiece) {
+        const int64_t k = std::min(kPiece, m - s0);
+        std::memcpy(cl + s0, q + 4 * s0, 4 * (size_t)k);
+        std::memcpy(wt + s0, q + 4 * m + 4 * s0, 4 * (size_t)k);
+    }
+    bool ok = rp[0] == 0 && rp[n] == m;
+    for (int64_t v = 0; ok && v < n; ++v) ok = rp[v] <= rp[v + 1];
+    if (!ok) {
+        std::free(rp);
+        std::free(cl);
+        std::free(wt);
+        return io_fail(LV_ERR_ARG, where + ": rowptr is not monotone from 0 to E");
     }
     *rowptr = rp;
     *col = cl;
     *w = wt;
-    *V = n;
-    *E = Ecount;
+    *V = (int32_t)n;
+    *E = m;
     return LV_OK;
+}
+
+lv_status lv_read_graph(const char *path, int32_t **rowptr, int32_t **col, float **w, int32_t *V,
+                        int64_t *E) {
+    if (!path) return io_fail(LV_ERR_ARG, "null argument");
+    char head[8] = {0};
+    FILE *f = std::fopen(path, "rb");
+    if (!f) return io_fail(LV_ERR_IO, std::string("cannot open '") + path + "'");
+    const size_t got = std::fread(head, 1, 8, f);
+    std::fclose(f);
+    if (got == 8 && std::memcmp(head, kCsrMagic, 8) == 0)
+        return lv_read_csr(path, rowptr, col, w, V, E);
+    return lv_read_mtx(path, rowptr, col, w, V, E);
 }
 
 lv_status lv_write_mtx(const char *path, const int32_t *rowptr, const int32_t *col, const float *w,
@@ -243,23 +568,41 @@ lv_status lv_write_mtx(const char *path, const int32_t *rowptr, const int32_t *c
     if (!path || !rowptr || V < 0) return io_fail(LV_ERR_ARG, "bad argument");
     FILE *f = std::fopen(path, "wb");
     if (!f) return io_fail(LV_ERR_IO, std::string("cannot create '") + path + "'");
+    // rows are formatted by all threads in blocks, written in row order
+    const int32_t kRows = 1 << 14;
+    const int32_t nblk = (V + kRows - 1) / kRows;
+    std::vector<std::string> text((size_t)nblk);
+    std::vector<int64_t> cnt((size_t)nblk, 0);
+#pragma omp parallel for schedule(dynamic, 1)
+    for (int32_t b = 0; b < nblk; ++b) {
+        std::string &out = text[(size_t)b];
+        char line[80];
+        int64_t k = 0;
+        for (int32_t v = b * kRows; v < std::min(V, (b + 1) * kRows); ++v)
+            for (int32_t e = rowptr[v]; e < rowptr[v + 1]; ++e) {
+                if (col[e] > v) continue;
+                const float x = w[e];
+                int len;
+                if (x == std::floor(x) && std::fabs(x) < 1e9f)
+                    len = std::snprintf(line, sizeof line, "%d %d %lld\n", v + 1, col[e] + 1,
+                                        (long long)x);
+                else
+                    len = std::snprintf(line, sizeof line, "%d %d %.9g\n", v + 1, col[e] + 1,
+                                        (double)x);
+                out.append(line, (size_t)len);
+                ++k;
+            }
+        cnt[(size_t)b] = k;
+    }
     int64_t nnz = 0;
-    for (int32_t v = 0; v < V; ++v)
-        for (int32_t e = rowptr[v]; e < rowptr[v + 1]; ++e)
-            if (col[e] <= v) nnz++;
-    std::fprintf(f, "%%%%MatrixMarket matrix coordinate real symmetric\n");
-    std::fprintf(f, "%d %d %lld\n", V, V, (long long)nnz);
-    std::vector<char> line(64);
-    for (int32_t v = 0; v < V; ++v)
-        for (int32_t e = rowptr[v]; e < rowptr[v + 1]; ++e) {
-            if (col[e] > v) continue;
-            const float x = w[e];
-            if (x == std::floor(x) && std::fabs(x) < 1e9f)
-                std::fprintf(f, "%d %d %lld\n", v + 1, col[e] + 1, (long long)x);
-            else
-                std::fprintf(f, "%d %d %.9g\n", v + 1, col[e] + 1, (double)x);
-        }
-    if (std::fclose(f) != 0) return io_fail(LV_ERR_IO, std::string("cannot write '") + path + "'");
+    for (int64_t k : cnt) nnz += k;
+    bool ok = std::fprintf(f, "%%%%MatrixMarket matrix coordinate real symmetric\n") > 0 &&
+              std::fprintf(f, "%d %d %lld\n", V, V, (long long)nnz) > 0;
+    for (int32_t b = 0; ok && b < nblk; ++b)
+        ok = std::fwrite(text[(size_t)b].data(), 1, text[(size_t)b].size(), f) ==
+             text[(size_t)b].size();
+    ok = (std::fclose(f) == 0) && ok;
+    if (!ok) return io_fail(LV_ERR_IO, std::string("cannot write '") + path + "'");
     return LV_OK;
 }
 

@@ -45,7 +45,8 @@ int main(int argc, char **argv) {
     float *w = nullptr;
     int32_t V = 0;
     int64_t E = 0;
-    if (lv_read_mtx(path, &rowptr, &col, &w, &V, &E) != LV_OK) {
+    // a MatrixMarket file, or a binary CSR file written by lv_write_csr
+    if (lv_read_graph(path, &rowptr, &col, &w, &V, &E) != LV_OK) {
         std::fprintf(stderr, "gpulouvain: %s\n", lv_last_error());
         return 2;
     }

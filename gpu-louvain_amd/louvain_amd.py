@@ -168,6 +168,9 @@ def load() -> ctypes.CDLL:
         "lv_assignment": (ctypes.c_int32, [ctypes.c_void_p, I32P, I32P]),
         "lv_level_size": (ctypes.c_int32, [ctypes.c_void_p, I32P, I64P]),
         "lv_level_graph": (ctypes.c_int32, [ctypes.c_void_p, I32P, I32P, F32P]),
+        "lv_dump_level": (ctypes.c_int32, [ctypes.c_void_p, ctypes.c_char_p, ctypes.c_int32]),
+        "lv_create_from_dump": (ctypes.c_int32, [ctypes.c_char_p, ctypes.POINTER(Options),
+                                                 ctypes.POINTER(ctypes.c_void_p)]),
         "lv_get_communities": (ctypes.c_int32, [ctypes.c_void_p, I32P]),
         "lv_set_communities": (ctypes.c_int32, [ctypes.c_void_p, I32P]),
         "lv_decide": (ctypes.c_int32, [ctypes.c_void_p, ctypes.c_int32, I32P]),
@@ -180,6 +183,11 @@ def load() -> ctypes.CDLL:
         "lv_read_mtx": (ctypes.c_int32, [ctypes.c_char_p, ctypes.POINTER(I32P), ctypes.POINTER(I32P),
                                          ctypes.POINTER(F32P), I32P, I64P]),
         "lv_write_mtx": (ctypes.c_int32, [ctypes.c_char_p, I32P, I32P, F32P, ctypes.c_int32]),
+        "lv_write_csr": (ctypes.c_int32, [ctypes.c_char_p, I32P, I32P, F32P, ctypes.c_int32, ctypes.c_int64]),
+        "lv_read_csr": (ctypes.c_int32, [ctypes.c_char_p, ctypes.POINTER(I32P), ctypes.POINTER(I32P),
+                                         ctypes.POINTER(F32P), I32P, I64P]),
+        "lv_read_graph": (ctypes.c_int32, [ctypes.c_char_p, ctypes.POINTER(I32P), ctypes.POINTER(I32P),
+                                           ctypes.POINTER(F32P), I32P, I64P]),
         "lv_gen_rmat": (ctypes.c_int32, [ctypes.c_int32, ctypes.c_int32, ctypes.c_double, ctypes.c_double,
                                          ctypes.c_double, ctypes.c_uint64, ctypes.c_int32,
                                          ctypes.POINTER(I32P), ctypes.POINTER(I32P), ctypes.POINTER(F32P),
@@ -254,13 +262,36 @@ def _take_host_csr(rp, cl, wt, V: ctypes.c_int32, E: ctypes.c_int64) -> CSR:
     return CSR(rowptr, col, w)
 
 
-def read_mtx(path: str) -> CSR:
+def _read(fn: str, path: str) -> CSR:
     lib = load()
     rp, cl, wt = I32P(), I32P(), F32P()
     V, E = ctypes.c_int32(), ctypes.c_int64()
-    _check(lib.lv_read_mtx(path.encode(), ctypes.byref(rp), ctypes.byref(cl), ctypes.byref(wt),
-                           ctypes.byref(V), ctypes.byref(E)))
+    _check(getattr(lib, fn)(path.encode(), ctypes.byref(rp), ctypes.byref(cl), ctypes.byref(wt),
+                            ctypes.byref(V), ctypes.byref(E)))
     return _take_host_csr(rp, cl, wt, V, E)
+
+
+def read_mtx(path: str) -> CSR:
+    return _read("lv_read_mtx", path)
+
+
+def read_csr(path: str) -> CSR:
+    """Binary CSR file (lv_write_csr)."""
+    return _read("lv_read_csr", path)
+
+
+def read_graph(path: str) -> CSR:
+    """Binary CSR or MatrixMarket file, by the file's magic."""
+    return _read("lv_read_graph", path)
+
+
+def write_csr(path: str, g: CSR) -> None:
+    lib = load()
+    rowptr = np.ascontiguousarray(g.rowptr, dtype=np.int32)
+    col = np.ascontiguousarray(g.col, dtype=np.int32)
+    w = np.ascontiguousarray(g.w, dtype=np.float32)
+    _check(lib.lv_write_csr(path.encode(), _p(rowptr, ctypes.c_int32), _p(col, ctypes.c_int32),
+                            _p(w, ctypes.c_float), g.V, g.E))
 
 
 def write_mtx(path: str, g: CSR) -> None:
@@ -323,6 +354,30 @@ class Graph:
                              _p(w, ctypes.c_float), self.V0, int(self._col.size), ctypes.byref(opt),
                              ctypes.byref(h)))
         self._h = h
+        self.Vorig = self.V0
+
+    @classmethod
+    def from_dump(cls, prefix: str, device: int = 0, timing: bool = False) -> "Graph":
+        """A handle on a level written by dump_level (lv_create_from_dump)."""
+        lib = load()
+        self = cls.__new__(cls)
+        self._lib = lib
+        self.device = device
+        opt = Options()
+        opt.device = device
+        opt.timing = int(timing)
+        h = ctypes.c_void_p()
+        _check(lib.lv_create_from_dump(prefix.encode(), ctypes.byref(opt), ctypes.byref(h)))
+        self._h = h
+        V, _ = self.level_size()
+        self.V0 = V
+        with open(prefix + ".o2c", "rb") as f:
+            self.Vorig = int(np.frombuffer(f.read(24)[8:16], np.int64)[0])
+        return self
+
+    def dump_level(self, prefix: str, mtx: bool = False) -> None:
+        """Writes prefix.lvcsr, prefix.o2c (and prefix.mtx) for the current level."""
+        _check(self._lib.lv_dump_level(self._h, prefix.encode(), int(mtx)))
 
     # --- lifecycle
     def close(self) -> None:
@@ -386,10 +441,10 @@ class Graph:
         return CSR(rowptr, col[:E], w[:E])
 
     def assignment(self) -> tuple[np.ndarray, int]:
-        o2c = np.zeros(max(self.V0, 1), np.int32)
+        o2c = np.zeros(max(self.Vorig, 1), np.int32)
         lv = ctypes.c_int32()
         _check(self._lib.lv_assignment(self._h, _p(o2c, ctypes.c_int32), ctypes.byref(lv)))
-        return o2c[: self.V0], lv.value
+        return o2c[: self.Vorig], lv.value
 
     def communities(self) -> np.ndarray:
         V, _ = self.level_size()

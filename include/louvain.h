@@ -135,6 +135,16 @@ lv_status lv_run(lv_graph *g, float min_gain, lv_stats *stats);
 lv_status lv_assignment(lv_graph *g, int32_t *orig2comm, int32_t *level_V);
 lv_status lv_level_size(lv_graph *g, int32_t *V, int64_t *E);
 lv_status lv_level_graph(lv_graph *g, int32_t *rowptr, int32_t *col, float *w);
+/* Level dump (no reference counterpart; for resuming a run and for feeding a
+ * level to the CPU oracle): <prefix>.lvcsr holds the current level's CSR
+ * (lv_write_csr), <prefix>.o2c the map of the original vertices into it
+ * (8-byte magic "LVO2C01\0", int64 original V, int64 level V, fp64 total
+ * weight M of the run, int32 map[original V]), and with with_mtx
+ * <prefix>.mtx the CSR as MatrixMarket.  lv_create_from_dump builds a handle
+ * whose level 0 is the dumped level, with the run's M, and whose
+ * lv_assignment maps the original vertices (levels count from the dump). */
+lv_status lv_dump_level(lv_graph *g, const char *prefix, int32_t with_mtx);
+lv_status lv_create_from_dump(const char *prefix, const lv_options *opt, lv_graph **out);
 lv_status lv_get_communities(lv_graph *g, int32_t *comm);
 /* Replace the current level's community state (sizes and totals are rebuilt). */
 lv_status lv_set_communities(lv_graph *g, const int32_t *comm);
@@ -264,10 +274,22 @@ lv_status lv_mlp_train(int32_t device, const float *feat, const float *labels, i
 
 /* ---- host-side input (utils.cu:9-62 readInputData) and synthetic graphs ---- */
 /* MatrixMarket coordinate reader with the reference's conventions (skip '%'
- * lines, V := cols, each non-loop triple stored in both rows, insertion
- * order kept).  Arrays are malloc'd; free with lv_free_host. */
+ * lines, V := cols, entries are whitespace-separated "i j w" tokens, each
+ * non-loop triple stored in both rows, insertion order kept).  The file is
+ * memory-mapped and parsed by all OpenMP threads; the CSR is the same for
+ * any thread count.  Arrays are malloc'd; free with lv_free_host. */
 lv_status lv_read_mtx(const char *path, int32_t **rowptr, int32_t **col, float **w,
                       int32_t *V, int64_t *E);
+/* Binary CSR file (no reference counterpart; a cache for large inputs):
+ * 8-byte magic "LVCSR01\0", int64 V, int64 E, then rowptr[V+1] (int32),
+ * col[E] (int32), w[E] (fp32), little-endian. */
+lv_status lv_write_csr(const char *path, const int32_t *rowptr, const int32_t *col,
+                       const float *w, int32_t V, int64_t E);
+lv_status lv_read_csr(const char *path, int32_t **rowptr, int32_t **col, float **w,
+                      int32_t *V, int64_t *E);
+/* Reads a binary CSR file (recognised by its magic) or a MatrixMarket file. */
+lv_status lv_read_graph(const char *path, int32_t **rowptr, int32_t **col, float **w,
+                        int32_t *V, int64_t *E);
 /* Writes the lower triangle (i >= j) of a symmetric CSR as a symmetric real
  * (or integer-valued real) MatrixMarket file. */
 lv_status lv_write_mtx(const char *path, const int32_t *rowptr, const int32_t *col,

@@ -74,6 +74,120 @@ def test_reader_errors(lvlib, tmp_path):
         lvlib.read_mtx(str(short))
 
 
+TRICKY = (
+    "%%MatrixMarket matrix coordinate real symmetric\n"
+    "% a comment\n"
+    "\n"
+    "6 6 9\n"
+    "1 2 0.5\n"
+    "2\t3  1.25\r\n"
+    "3 3\n2.0\n"           # an entry across three lines
+    "  4 1 1e-3\n"
+    "5 6 7\t6 6 0.25\n"     # two entries on one line
+    "6\n1\n3.5 2 4 0.125\n"
+    "1 1 9\n"
+    "4 5 1 99 99 99\n"      # tokens after the last entry are not read
+)
+
+
+@pytest.mark.parametrize("chunk", [None, 1, 2, 5, 13, 64])
+def test_reader_chunk_boundaries_match_oracle(lvlib, oracle, tmp_path, monkeypatch, chunk):
+    """The parallel reader cuts the entry region into chunks at whitespace
+    and gives every token its entry by a prefix over the chunks' token counts;
+    tiny chunks put the cuts inside entries and lines."""
+    if chunk is not None:
+        monkeypatch.setenv("LV_MTX_CHUNK_BYTES", str(chunk))
+    tricky = tmp_path / "tricky.mtx"
+    tricky.write_bytes(TRICKY.encode())
+    pattern = tmp_path / "pattern.mtx"
+    pattern.write_text("%%MatrixMarket matrix coordinate pattern symmetric\n4 4 4\n1 2\n2 3 3\n4\n4 1\n")
+    g = lvlib.gen_rmat(9, 8, seed=3)
+    big = str(tmp_path / "g.mtx")
+    lvlib.write_mtx(big, g)
+    paths = [str(tricky), str(pattern), big] + [os.path.join(GOLDEN, n) for n in
+                                                ("karate.mtx", "reader_conventions.mtx")]
+    for path in paths:
+        a = lvlib.read_mtx(path)
+        b = oracle.read_mtx(path)
+        rp, cl, w = b.csr_arrays()
+        assert np.array_equal(a.rowptr, rp), path
+        assert np.array_equal(a.col, cl), path
+        assert np.array_equal(a.w.view(np.uint32), w.view(np.uint32)), path
+
+
+def test_reader_weights_round_like_strtof(lvlib, oracle, tmp_path):
+    """Weights parsed by the reader's fast path equal the C library's
+    correctly rounded strtof (through the oracle's fscanf) on decimals of
+    every length, exponents, and exact midpoints between floats."""
+    rng = np.random.default_rng(11)
+    toks = ["1", "0.5", "1e-3", "2.5E+2", "-0.0", "+3.25", "0000.125", "1.000000059604644775390625",
+            "1.0000000596046448", "16777217", "16777219", "3.4028235e38", "1e-30", "7.006e-40",
+            "0.1", "0.2", "0.3", "123456789012345678901234567890", ".5", "5.", "1e22", "1e23",
+            "9007199254740993", "0.000000000000000000001"]
+    for _ in range(3000):
+        nd = int(rng.integers(1, 22))
+        digits = "".join(str(d) for d in rng.integers(0, 10, nd))
+        dot = int(rng.integers(0, nd + 1))
+        t = digits[:dot] + "." + digits[dot:] if rng.random() < 0.8 else digits
+        if rng.random() < 0.3:
+            t += "e" + str(int(rng.integers(-25, 25)))
+        toks.append(t)
+    for _ in range(2000):  # exact midpoints between adjacent floats
+        f = np.float32(rng.uniform(1e-3, 1e6))
+        lo = float(f)
+        hi = float(np.nextafter(f, np.float32(np.inf)))
+        toks.append(repr((lo + hi) / 2))
+    n = len(toks)
+    body = "".join(f"{i % 50 + 1} {(i * 7) % 50 + 1} {t}\n" for i, t in enumerate(toks))
+    f = tmp_path / "w.mtx"
+    f.write_text(f"%%MatrixMarket matrix coordinate real general\n50 50 {n}\n" + body)
+    a = lvlib.read_mtx(str(f))
+    b = oracle.read_mtx(str(f))
+    rp, cl, w = b.csr_arrays()
+    assert np.array_equal(a.col, cl)
+    assert np.array_equal(a.w.view(np.uint32), w.view(np.uint32))
+
+
+@pytest.mark.parametrize("chunk", [None, 3])
+def test_reader_error_entries(lvlib, tmp_path, monkeypatch, chunk):
+    if chunk is not None:
+        monkeypatch.setenv("LV_MTX_CHUNK_BYTES", str(chunk))
+    head = "%%MatrixMarket matrix coordinate real symmetric\n3 3 4\n"
+    cases = {"1 2 1\n2 3 1\n3 1 1\n": "expected 4 entries, got 3",
+             "1 2 1\n2 3 1\n3 1 1\n1\n": "truncated entry 4",
+             "1 2 1\n2 3 1\n3 1 1\n1 2\n": "missing weight in entry 4",
+             "1 2 1\n2 x 1\n3 1 1\n1 2 1\n": "entry 2 is not a number",
+             "1 2 1\n2 3 1\n0 1 1\n1 4 1\n": "entry 3 index out of range"}
+    for i, (body, msg) in enumerate(cases.items()):
+        f = tmp_path / f"e{i}.mtx"
+        f.write_text(head + body)
+        with pytest.raises(lvlib.LouvainError) as ei:
+            lvlib.read_mtx(str(f))
+        assert ei.value.status == lvlib.LV_ERR_ARG
+        assert msg in str(ei.value), (msg, str(ei.value))
+
+
+def test_binary_csr_round_trip_and_errors(lvlib, tmp_path):
+    g = lvlib.gen_rmat(11, 8, seed=4)
+    p = str(tmp_path / "g.lvcsr")
+    lvlib.write_csr(p, g)
+    for h in (lvlib.read_csr(p), lvlib.read_graph(p)):
+        assert np.array_equal(h.rowptr, g.rowptr) and np.array_equal(h.col, g.col)
+        assert np.array_equal(h.w.view(np.uint32), g.w.view(np.uint32))
+    m = str(tmp_path / "g.mtx")
+    lvlib.write_mtx(m, g)
+    a, b = lvlib.read_graph(m), lvlib.read_mtx(m)
+    assert np.array_equal(a.rowptr, b.rowptr) and np.array_equal(a.col, b.col)
+    raw = open(p, "rb").read()
+    (tmp_path / "cut.lvcsr").write_bytes(raw[:-4])
+    with pytest.raises(lvlib.LouvainError) as ei:
+        lvlib.read_csr(str(tmp_path / "cut.lvcsr"))
+    assert ei.value.status == lvlib.LV_ERR_ARG
+    with pytest.raises(lvlib.LouvainError) as ei:
+        lvlib.read_csr(m)
+    assert ei.value.status == lvlib.LV_ERR_ARG
+
+
 def _edge_multiset(g):
     src = np.repeat(np.arange(g.V), np.diff(g.rowptr))
     key = np.stack([src, g.col, g.w.view(np.int32)], 1)
