@@ -297,6 +297,18 @@ struct lv_graph {
     unsigned long long *d_moved = nullptr;
     double *d_own_slots = nullptr;   // the applies' modularity corrections (kept zero between sweeps)
 
+    // Levels after the first may keep only their vertices of positive degree
+    // (compact ids, ascending in the reference's ids).  ref_of/cmp_of map
+    // compact <-> reference ids of the current level (map_cur); Vref counts
+    // the reference's vertices.  With maps_identity both numberings agree.
+    bool compact_on = true;
+    bool maps_identity = true;
+    int32_t Vref = 0;
+    int32_t *ref_of[2] = {nullptr, nullptr}, *cmp_of[2] = {nullptr, nullptr};
+    int map_cur = 0;
+    int32_t *newid_ref = nullptr;  // reference renumbering at an aggregation
+    int32_t *a_keep = nullptr;     // communities that become vertices of the next level
+
     // aggregation scratch
     int32_t *a_newid = nullptr, *a_mstart = nullptr, *a_fill = nullptr, *a_members = nullptr,
             *a_mscan = nullptr, *a_cnt = nullptr, *a_cstart = nullptr, *a_order = nullptr,
@@ -445,7 +457,8 @@ struct lv_graph {
         f.nmv = nmv;
         f.V = V;
         f.M = M32;
-        f.vscale = M > 0.0 ? (float)((double)V / (2.0 * M)) : 0.0f;
+        // V of the reference's numbering (vertices without edges included)
+        f.vscale = M > 0.0 ? (float)((double)Vref / (2.0 * M)) : 0.0f;
         return f;
     }
 
@@ -550,6 +563,7 @@ struct lv_graph {
         part_target = opt.part_target > 0 ? std::max(16, opt.part_target) : lv::kPartTarget;
         merge_limit = opt.merge_limit > 0 ? std::min(opt.merge_limit, lv::kMergeLimit) : lv::kMergeLimit;
         incr_den = opt.q_incremental != 0 ? opt.q_incremental : 2;
+        compact_on = opt.compact >= 0;
     }
 
     // ---------------------------------------------------------------- setup
@@ -628,6 +642,12 @@ struct lv_graph {
         LV_TRY(dev_alloc(&d_own_slots, lv::kMoveSlots * lv::kMoveSlotStride));
         lv::fill_f64(d_own_slots, 0.0, lv::kMoveSlots * lv::kMoveSlotStride, stream);
         LV_TRY(dev_alloc(&a_newid, Vp));
+        LV_TRY(dev_alloc(&a_keep, Vp));
+        LV_TRY(dev_alloc(&newid_ref, Vp + 1));
+        for (int i = 0; i < 2; ++i) {
+            LV_TRY(dev_alloc(&ref_of[i], Vp));
+            LV_TRY(dev_alloc(&cmp_of[i], Vp));
+        }
         LV_TRY(dev_alloc(&a_mstart, Vp));
         LV_TRY(dev_alloc(&a_fill, Vp));
         LV_TRY(dev_alloc(&a_members, Vp));
@@ -678,6 +698,8 @@ struct lv_graph {
         cur_buf = -1;
         V = V0;
         E = E0;
+        maps_identity = true;
+        Vref = V0;
         LV_LAUNCHED("reset");
         level_ready = false;
         return LV_OK;
@@ -1237,22 +1259,133 @@ struct lv_graph {
         int32_t n;
     };
 
-    lv_status trace_features(TraceSink *t) {
-        if (V <= 0) return LV_OK;
-        LV_TRY(tfeat.reserve((int64_t)V * lv::kFeat));
-        lv::launch_features(feature_args(), tfeat.p, stream);
-        LV_LAUNCHED("features");
-        LV_HIP(hipMemcpyAsync(t->feat + (int64_t)t->n * V * lv::kFeat, tfeat.p,
-                              sizeof(float) * (size_t)V * lv::kFeat, hipMemcpyDeviceToHost, stream));
+    // Community array of the level (comm or decisions) in the reference's ids.
+    lv_status communities_out(const int32_t *dev, int32_t *host) {
+        if (maps_identity) {
+            if (V > 0)
+                LV_HIP(hipMemcpyAsync(host, dev, sizeof(int32_t) * (size_t)V, hipMemcpyDeviceToHost,
+                                      stream));
+            return sync();
+        }
+        std::vector<int32_t> rof, c((size_t)std::max(V, 1));
+        LV_TRY(host_maps(&rof, nullptr));
+        if (V > 0)
+            LV_HIP(hipMemcpyAsync(c.data(), dev, sizeof(int32_t) * (size_t)V, hipMemcpyDeviceToHost,
+                                  stream));
+        LV_TRY(sync());
+        for (int32_t r = 0; r < Vref; ++r) host[r] = r;  // vertices without edges: own community
+        for (int32_t i = 0; i < V; ++i) host[rof[(size_t)i]] = rof[(size_t)c[(size_t)i]];
+        return LV_OK;
+    }
+
+    // Community state from comm (V entries, stored ids): sizes, totals and
+    // records rebuilt.
+    lv_status set_state(const int32_t *c) {
+        if (V > 0) {
+            LV_HIP(hipMemcpyAsync(comm, c, sizeof(int32_t) * V, hipMemcpyHostToDevice, stream));
+            lv::fill_i32(size, 0, V, stream);
+            lv::fill_f64(sigma, 0.0, V, stream);
+            hipLaunchKernelGGL(k_state_from_comm, dim3(blocks_for(V, 256)), dim3(256), 0, stream, V,
+                               comm, k, size, sigma);
+            lv::launch_records_all(V, sigma, size, crec, stream);
+            state_changed();
+            LV_LAUNCHED("state from communities");
+        }
         return sync();
     }
 
-    lv_status trace_labels(TraceSink *t) {
-        if (V <= 0) return LV_OK;
-        uint8_t *dst = t->moved + (int64_t)t->n * V;
-        LV_HIP(hipMemcpyAsync(dst, hist, (size_t)V, hipMemcpyDeviceToHost, stream));
+    // ---- arrays in the reference's numbering at a compact level
+    // Host copies of the level's maps (empty when the numberings agree).
+    lv_status host_maps(std::vector<int32_t> *rof, std::vector<int32_t> *cof) {
+        if (maps_identity) {
+            if (rof) rof->clear();
+            if (cof) cof->clear();
+            return LV_OK;
+        }
+        if (rof) {
+            rof->resize((size_t)std::max(V, 1));
+            if (V > 0)
+                LV_HIP(hipMemcpyAsync(rof->data(), ref_of[map_cur], sizeof(int32_t) * (size_t)V,
+                                      hipMemcpyDeviceToHost, stream));
+        }
+        if (cof) {
+            cof->resize((size_t)std::max(Vref, 1));
+            if (Vref > 0)
+                LV_HIP(hipMemcpyAsync(cof->data(), cmp_of[map_cur], sizeof(int32_t) * (size_t)Vref,
+                                      hipMemcpyDeviceToHost, stream));
+        }
+        return sync();
+    }
+
+    // dst[Vref rows of w] from src[V rows] through rof; rows of vertices
+    // without edges get fill[0..w).
+    template <typename T>
+    static void expand_rows(const T *src, T *dst, const std::vector<int32_t> &rof, int32_t Vr,
+                            int w, const T *fill) {
+        for (int64_t r = 0; r < Vr; ++r)
+            for (int j = 0; j < w; ++j) dst[r * w + j] = fill[j];
+        for (size_t i = 0; i < rof.size() && i < (size_t)INT32_MAX; ++i)
+            for (int j = 0; j < w; ++j) dst[(int64_t)rof[i] * w + j] = src[i * w + j];
+    }
+
+    // Per-vertex device array (V rows of w) to the caller in the reference's
+    // numbering (Vref rows), rows of vertices without edges set to fill.
+    template <typename T>
+    lv_status copy_out_rows(const T *dev, T *host, int w, const T *fill) {
+        if (maps_identity) {
+            if (V > 0)
+                LV_HIP(hipMemcpyAsync(host, dev, sizeof(T) * (size_t)V * w, hipMemcpyDeviceToHost,
+                                      stream));
+            return sync();
+        }
+        std::vector<int32_t> rof;
+        LV_TRY(host_maps(&rof, nullptr));
+        rof.resize((size_t)V);
+        std::vector<T> tmp((size_t)std::max<int64_t>((int64_t)V * w, 1));
+        if (V > 0)
+            LV_HIP(hipMemcpyAsync(tmp.data(), dev, sizeof(T) * (size_t)V * w, hipMemcpyDeviceToHost,
+                                  stream));
         LV_TRY(sync());
-        for (int32_t v = 0; v < V; ++v) dst[v] &= 1u;
+        expand_rows(tmp.data(), host, rof, Vref, w, fill);
+        return LV_OK;
+    }
+
+    // Features of the level's vertices without edges and the model's
+    // probability for them.
+    lv_status isolated_defaults(float *feat8, float *prob) {
+        LV_TRY(tfeat.reserve(16));
+        lv::launch_isolated_features(feature_args(), tfeat.p, stream);
+        if (prob && mlp_set) lv::launch_mlp_forward(tfeat.p, 1, mlp, tfeat.p + 8, stream);
+        LV_LAUNCHED("isolated features");
+        float h[16] = {0};
+        LV_HIP(hipMemcpyAsync(h, tfeat.p, sizeof(float) * 16, hipMemcpyDeviceToHost, stream));
+        LV_TRY(sync());
+        std::memcpy(feat8, h, sizeof(float) * lv::kFeat);
+        if (prob) *prob = mlp_set ? h[8] : 0.0f;
+        return LV_OK;
+    }
+
+    lv_status trace_features(TraceSink *t) {
+        if (V <= 0 && Vref <= 0) return LV_OK;
+        float *dst = t->feat + (int64_t)t->n * Vref * lv::kFeat;
+        float iso[lv::kFeat] = {0};
+        if (!maps_identity) LV_TRY(isolated_defaults(iso, nullptr));
+        if (V <= 0) {
+            expand_rows<float>(nullptr, dst, {}, Vref, lv::kFeat, iso);
+            return LV_OK;
+        }
+        LV_TRY(tfeat.reserve((int64_t)V * lv::kFeat));
+        lv::launch_features(feature_args(), tfeat.p, stream);
+        LV_LAUNCHED("features");
+        return copy_out_rows<float>(tfeat.p, dst, lv::kFeat, iso);
+    }
+
+    lv_status trace_labels(TraceSink *t) {
+        if (Vref <= 0) return LV_OK;
+        uint8_t *dst = t->moved + (int64_t)t->n * Vref;
+        const uint8_t zero = 0;
+        LV_TRY(copy_out_rows<uint8_t>(hist, dst, 1, &zero));
+        for (int32_t v = 0; v < Vref; ++v) dst[v] &= 1u;
         return LV_OK;
     }
 
@@ -1314,7 +1447,7 @@ struct lv_graph {
             if (stop) q_before_pruned = q_prev;
             q_prev = q1;
         }
-        lv::launch_compose(orig2comm, Vorig, comm, stream);
+        compose_phase();
         LV_LAUNCHED("compose");
         if (changed) *changed = ch;
         if (sweeps_out) *sweeps_out = sweeps;
@@ -1326,6 +1459,68 @@ struct lv_graph {
     }
 
     // ---------------------------------------------------------------- phase 2
+    // End of a phase: the original vertices follow their level vertex into its
+    // community (updateOriginalToCommunity, modularity_optimisation.cu:251-257).
+    void compose_phase() {
+        if (maps_identity)
+            lv::launch_compose(orig2comm, Vorig, comm, stream);
+        else
+            lv::launch_compose_ref(orig2comm, Vorig, cmp_of[map_cur], ref_of[map_cur], comm,
+                                   stream);
+    }
+
+    // Renumbering flags of an aggregation (community_aggregation.cu:267-275):
+    // a_newid[c] = 1 for the communities that become vertices of the next
+    // level, the non-empty ones; when dropping, only those holding a vertex
+    // of positive degree (the others are singletons without edges, kept in
+    // the reference's numbering only).  Returns whether it drops.
+    bool renumber_flags(bool allow_drop) {
+        const bool drop = allow_drop && compact_on && G == 0;
+        if (drop) {
+            lv::fill_i32(a_newid, 0, V, stream);
+            lv::launch_has_degree(rowptr(), comm, V, a_newid, stream);
+            lv::launch_present_compact(size, V, a_newid, stream);
+        } else {
+            lv::launch_present(size, V, a_newid, stream);
+        }
+        if (drop || !maps_identity)
+            (void)hipMemcpyAsync(a_keep, a_newid, sizeof(int32_t) * (size_t)V,
+                                 hipMemcpyDeviceToDevice, stream);
+        return drop;
+    }
+
+    // After the scan of the flags (a_newid, newV vertices next): the
+    // reference's renumbering of the same communities, which also counts the
+    // singletons without edges, moves the original vertices' map
+    // (community_aggregation.cu:222-228) and gives the maps of the next
+    // level.  *nVref = the next level's vertex count in the reference's ids.
+    lv_status finish_renumbering(bool drop, int32_t newV, int32_t *nVref) {
+        hipStream_t s = stream;
+        if (!drop && maps_identity) {
+            lv::launch_compose(orig2comm, Vorig, a_newid, s);
+            *nVref = newV;
+            return LV_OK;
+        }
+        if (maps_identity) lv::iota_i32(ref_of[map_cur], V, s);
+        int32_t *nref_d = a_scalars + 3;
+        lv::fill_i32(newid_ref, 1, Vref, s);
+        lv::launch_ref_present(size, ref_of[map_cur], V, newid_ref, s);
+        lv::exclusive_scan(newid_ref, newid_ref, Vref, nref_d, scratch, s);
+        lv::launch_compose(orig2comm, Vorig, newid_ref, s);
+        const int nm = map_cur ^ 1;
+        lv::launch_next_ref_of(a_keep, a_newid, ref_of[map_cur], newid_ref, V, ref_of[nm], s);
+        lv::fill_i32(cmp_of[nm], -1, Vref, s);
+        lv::launch_cmp_of(ref_of[nm], newV, cmp_of[nm], s);
+        LV_LAUNCHED("reference renumbering");
+        LV_TRY(stage(64));
+        LV_HIP(hipMemcpyAsync(h_stage, nref_d, sizeof(int32_t), hipMemcpyDeviceToHost, s));
+        LV_TRY(sync());
+        std::memcpy(nVref, h_stage, sizeof(int32_t));
+        map_cur = nm;
+        maps_identity = *nVref == newV;  // nothing dropped: the maps are the identity
+        return LV_OK;
+    }
+
     lv_status aggregate() {
         if (!level_ready) LV_TRY(setup_level());
         if (opt.agg_mode == 0) return aggregate_sorted();
@@ -1335,7 +1530,7 @@ struct lv_graph {
         int32_t *newV_d = a_scalars;
         int32_t *etot_d = a_scalars + 1;
         // renumbering (community_aggregation.cu:267-275)
-        lv::launch_present(size, V, a_newid, s);
+        const bool drop = renumber_flags(false);
         lv::exclusive_scan(a_newid, a_newid, V, newV_d, scratch, s);
         if (G > 0) lv::launch_graph_offsets(vptr, G, V, a_newid, newV_d, vptr_next, s);
         lv::exclusive_scan(size, a_mstart, V, nullptr, scratch, s);
@@ -1434,9 +1629,10 @@ struct lv_graph {
         for (int t = 0; t < lv::kAggTiers; ++t)
             lv::launch_copy_rows(a_order + toff[t], toff[t + 1] - toff[t], t >= 2, a_mstart,
                                  a_mscan, a_cnt, a_cstart, tmp_col, tmp_w, colb[nb], wb[nb], s);
-        // updateOriginalToCommunity (community_aggregation.cu:222-228)
-        lv::launch_compose(orig2comm, Vorig, a_newid, s);
         LV_LAUNCHED("aggregation output");
+        // updateOriginalToCommunity (community_aggregation.cu:222-228)
+        int32_t nVref = 0;
+        LV_TRY(finish_renumbering(drop, newV, &nVref));
         tend(t0, LV_K_AGGREGATE, 24.0 * (double)E + 40.0 * (double)V);
         LV_HIP(hipMemcpyAsync(h_stage, etot_d, sizeof(int32_t), hipMemcpyDeviceToHost, s));
         LV_TRY(sync());
@@ -1444,6 +1640,7 @@ struct lv_graph {
         std::memcpy(&newE, h_stage, sizeof newE);
         cur_buf = nb;
         V = newV;
+        Vref = nVref;
         E = newE;
         level_ready = false;
         if (G > 0) {
@@ -1660,7 +1857,7 @@ struct lv_graph {
         int32_t *etot_d = a_scalars + 1;
         int32_t *U_d = a_scalars + 2;
         // renumbering (community_aggregation.cu:267-275)
-        lv::launch_present(size, V, a_newid, s);
+        const bool drop = renumber_flags(true);
         lv::exclusive_scan(a_newid, a_newid, V, newV_d, scratch, s);
         if (G > 0) lv::launch_graph_offsets(vptr, G, V, a_newid, newV_d, vptr_next, s);
         LV_LAUNCHED("renumbering");
@@ -1703,9 +1900,10 @@ struct lv_graph {
             lv::fill_i32(U_d, 0, 1, s);
         }
         lv::exclusive_scan(a_cnt, rpb[nb], (int64_t)newV + 1, etot_d, scratch, s);
-        // updateOriginalToCommunity (community_aggregation.cu:222-228)
-        lv::launch_compose(orig2comm, Vorig, a_newid, s);
         LV_LAUNCHED("sort aggregation");
+        // updateOriginalToCommunity (community_aggregation.cu:222-228)
+        int32_t nVref = 0;
+        LV_TRY(finish_renumbering(drop, newV, &nVref));
         tend(t0, LV_K_AGGREGATE, 24.0 * (double)E + 40.0 * (double)V);
         LV_HIP(hipMemcpyAsync(h_stage, U_d, sizeof(int32_t), hipMemcpyDeviceToHost, s));
         LV_TRY(sync());
@@ -1713,6 +1911,7 @@ struct lv_graph {
         std::memcpy(&newE, h_stage, sizeof newE);
         cur_buf = nb;
         V = newV;
+        Vref = nVref;
         E = newE;
         level_ready = false;
         if (G > 0) {
@@ -1806,6 +2005,12 @@ struct lv_graph {
         dev_free(d_moved);
         dev_free(d_own_slots);
         dev_free(a_newid);
+        dev_free(a_keep);
+        dev_free(newid_ref);
+        for (int i = 0; i < 2; ++i) {
+            dev_free(ref_of[i]);
+            dev_free(cmp_of[i]);
+        }
         dev_free(a_mstart);
         dev_free(a_fill);
         dev_free(a_members);
@@ -1995,7 +2200,7 @@ lv_status run_levels(lv_graph *g, float min_gain, const lv_frontier *fr, lv_stat
     st.algo_ms = ms;
     st.aggregate_ms = agg_ms;
     st.memory_ms = g->memory_ms;
-    st.final_V = g->V;
+    st.final_V = g->Vref;
     st.final_E = g->E;
     if (stats) *stats = st;
     return LV_OK;
@@ -2072,14 +2277,14 @@ lv_status lv_move_state(lv_graph *g, float *margin, float *kown, uint8_t *hist, 
     if (!g) return fail(LV_ERR_ARG, "null graph");
     LV_HIP(hipSetDevice(g->device));
     if (!g->level_ready) LV_TRY(g->setup_level());
-    const size_t V = (size_t)g->V;
-    if (V > 0) {
-        if (margin)
-            LV_HIP(hipMemcpyAsync(margin, g->margin, 4 * V, hipMemcpyDeviceToHost, g->stream));
-        if (kown) LV_HIP(hipMemcpyAsync(kown, g->kown, 4 * V, hipMemcpyDeviceToHost, g->stream));
-        if (hist) LV_HIP(hipMemcpyAsync(hist, g->hist, V, hipMemcpyDeviceToHost, g->stream));
-        if (nmv) LV_HIP(hipMemcpyAsync(nmv, g->nmv, 4 * V, hipMemcpyDeviceToHost, g->stream));
-    }
+    // vertices without edges: never evaluated, never moved
+    const float zf = 0.0f;
+    const uint8_t zb = 0;
+    const int32_t zi = 0;
+    if (margin) LV_TRY(g->copy_out_rows<float>(g->margin, margin, 1, &zf));
+    if (kown) LV_TRY(g->copy_out_rows<float>(g->kown, kown, 1, &zf));
+    if (hist) LV_TRY(g->copy_out_rows<uint8_t>(g->hist, hist, 1, &zb));
+    if (nmv) LV_TRY(g->copy_out_rows<int32_t>(g->nmv, nmv, 1, &zi));
     return g->sync();
 }
 
@@ -2098,16 +2303,22 @@ lv_status lv_predict(lv_graph *g, float threshold, float *prob, uint8_t *active)
     if (!g->mlp_set) return fail(LV_ERR_STATE, "frontier model not set (lv_set_model)");
     LV_HIP(hipSetDevice(g->device));
     if (!g->level_ready) LV_TRY(g->setup_level());
-    if (g->V <= 0) return LV_OK;
+    // vertices without edges: the model's output for their features, never active
+    float iso[lv::kFeat], piso = 0.0f;
+    LV_TRY(g->isolated_defaults(iso, &piso));
+    const uint8_t zb = 0;
+    if (g->V <= 0) {
+        std::vector<int32_t> none;
+        if (prob) lv_graph::expand_rows<float>(nullptr, prob, none, g->Vref, 1, &piso);
+        if (active) lv_graph::expand_rows<uint8_t>(nullptr, active, none, g->Vref, 1, &zb);
+        return LV_OK;
+    }
     LV_TRY(g->tfeat.reserve(g->V));
     lv::launch_frontier(g->feature_args(), nullptr, 0, g->mlp, threshold, threshold, 0, g->bb,
                         g->fkeys, g->active, g->tfeat.p, g->stream);
     LV_LAUNCHED("frontier model");
-    if (prob)
-        LV_HIP(hipMemcpyAsync(prob, g->tfeat.p, sizeof(float) * g->V, hipMemcpyDeviceToHost,
-                              g->stream));
-    if (active)
-        LV_HIP(hipMemcpyAsync(active, g->active, (size_t)g->V, hipMemcpyDeviceToHost, g->stream));
+    if (prob) LV_TRY(g->copy_out_rows<float>(g->tfeat.p, prob, 1, &piso));
+    if (active) LV_TRY(g->copy_out_rows<uint8_t>(g->active, active, 1, &zb));
     return g->sync();
 }
 
@@ -2118,13 +2329,13 @@ lv_status lv_assignment(lv_graph *g, int32_t *orig2comm, int32_t *level_V) {
         LV_HIP(hipMemcpyAsync(orig2comm, g->orig2comm, sizeof(int32_t) * g->Vorig,
                               hipMemcpyDeviceToHost, g->stream));
     LV_TRY(g->sync());
-    if (level_V) *level_V = g->V;
+    if (level_V) *level_V = g->Vref;
     return LV_OK;
 }
 
 lv_status lv_level_size(lv_graph *g, int32_t *V, int64_t *E) {
     if (!g) return fail(LV_ERR_ARG, "null graph");
-    if (V) *V = g->V;
+    if (V) *V = g->Vref;
     if (E) *E = g->E;
     return LV_OK;
 }
@@ -2132,15 +2343,35 @@ lv_status lv_level_size(lv_graph *g, int32_t *V, int64_t *E) {
 lv_status lv_level_graph(lv_graph *g, int32_t *rowptr, int32_t *col, float *w) {
     if (!g) return fail(LV_ERR_ARG, "null graph");
     LV_HIP(hipSetDevice(g->device));
-    if (rowptr)
-        LV_HIP(hipMemcpyAsync(rowptr, g->rowptr(), sizeof(int32_t) * ((size_t)g->V + 1),
+    std::vector<int32_t> rof, rp_c;
+    LV_TRY(g->host_maps(&rof, nullptr));
+    const bool expand = !g->maps_identity;
+    int32_t *rp_dst = rowptr;
+    if (expand && rowptr) {
+        rp_c.resize((size_t)g->V + 1);
+        rp_dst = rp_c.data();
+    }
+    if (rp_dst)
+        LV_HIP(hipMemcpyAsync(rp_dst, g->rowptr(), sizeof(int32_t) * ((size_t)g->V + 1),
                               hipMemcpyDeviceToHost, g->stream));
     if (col && g->E > 0)
         LV_HIP(hipMemcpyAsync(col, g->col(), sizeof(int32_t) * g->E, hipMemcpyDeviceToHost,
                               g->stream));
     if (w && g->E > 0)
         LV_HIP(hipMemcpyAsync(w, g->w(), sizeof(float) * g->E, hipMemcpyDeviceToHost, g->stream));
-    return g->sync();
+    LV_TRY(g->sync());
+    if (!expand) return LV_OK;
+    // rows of vertices without edges are empty; columns in reference ids
+    if (col)
+        for (int64_t e = 0; e < g->E; ++e) col[e] = rof[(size_t)col[e]];
+    if (rowptr) {
+        int32_t i = 0;
+        for (int32_t r = 0; r <= g->Vref; ++r) {
+            while (i < g->V && rof[(size_t)i] < r) ++i;
+            rowptr[r] = rp_c[(size_t)i];
+        }
+    }
+    return LV_OK;
 }
 
 namespace {
@@ -2150,8 +2381,9 @@ const char kMapMagic[8] = {'L', 'V', 'O', '2', 'C', '0', '1', '\0'};
 lv_status lv_dump_level(lv_graph *g, const char *prefix, int32_t with_mtx) {
     if (!g || !prefix) return fail(LV_ERR_ARG, "null argument");
     LV_HIP(hipSetDevice(g->device));
-    const int32_t V = g->V;
-    const int64_t E = g->E;
+    int32_t V = 0;
+    int64_t E = 0;
+    LV_TRY(lv_level_size(g, &V, &E));
     std::vector<int32_t> rp((size_t)V + 1), cl((size_t)std::max<int64_t>(E, 1));
     std::vector<float> w((size_t)std::max<int64_t>(E, 1));
     LV_TRY(lv_level_graph(g, rp.data(), cl.data(), w.data()));
@@ -2222,30 +2454,31 @@ lv_status lv_get_communities(lv_graph *g, int32_t *comm) {
     if (!g || !comm) return fail(LV_ERR_ARG, "null argument");
     LV_HIP(hipSetDevice(g->device));
     if (!g->level_ready) LV_TRY(g->setup_level());
-    if (g->V > 0)
-        LV_HIP(hipMemcpyAsync(comm, g->comm, sizeof(int32_t) * g->V, hipMemcpyDeviceToHost,
-                              g->stream));
-    return g->sync();
+    return g->communities_out(g->comm, comm);
 }
 
 lv_status lv_set_communities(lv_graph *g, const int32_t *comm) {
     if (!g || !comm) return fail(LV_ERR_ARG, "null argument");
-    for (int32_t v = 0; v < g->V; ++v)
-        if (comm[v] < 0 || comm[v] >= g->V) return fail(LV_ERR_ARG, "community id out of range");
+    for (int32_t v = 0; v < g->Vref; ++v)
+        if (comm[v] < 0 || comm[v] >= g->Vref) return fail(LV_ERR_ARG, "community id out of range");
     LV_HIP(hipSetDevice(g->device));
     if (!g->level_ready) LV_TRY(g->setup_level());
-    if (g->V > 0) {
-        LV_HIP(hipMemcpyAsync(g->comm, comm, sizeof(int32_t) * g->V, hipMemcpyHostToDevice,
-                              g->stream));
-        lv::fill_i32(g->size, 0, g->V, g->stream);
-        lv::fill_f64(g->sigma, 0.0, g->V, g->stream);
-        hipLaunchKernelGGL(k_state_from_comm, dim3(blocks_for(g->V, 256)), dim3(256), 0, g->stream,
-                           g->V, g->comm, g->k, g->size, g->sigma);
-        lv::launch_records_all(g->V, g->sigma, g->size, g->crec, g->stream);
-        g->state_changed();
-        LV_LAUNCHED("state from communities");
+    if (g->maps_identity) return g->set_state(comm);
+    // a compact level stores only vertices with edges: those without keep
+    // their own singleton community and nobody joins it
+    std::vector<int32_t> rof, cof;
+    LV_TRY(g->host_maps(&rof, &cof));
+    for (int32_t r = 0; r < g->Vref; ++r)
+        if (cof[(size_t)r] < 0 && comm[r] != r)
+            return fail(LV_ERR_ARG, "a vertex without edges must stay in its own community");
+    std::vector<int32_t> c((size_t)std::max(g->V, 1));
+    for (int32_t i = 0; i < g->V; ++i) {
+        const int32_t x = comm[rof[(size_t)i]];
+        if (cof[(size_t)x] < 0)
+            return fail(LV_ERR_ARG, "no vertex can join the community of a vertex without edges");
+        c[(size_t)i] = cof[(size_t)x];
     }
-    return g->sync();
+    return g->set_state(c.data());
 }
 
 lv_status lv_decide(lv_graph *g, int32_t group, int32_t *new_comm) {
@@ -2257,10 +2490,8 @@ lv_status lv_decide(lv_graph *g, int32_t group, int32_t *new_comm) {
         LV_HIP(hipMemcpyAsync(g->newc, g->comm, sizeof(int32_t) * g->V, hipMemcpyDeviceToDevice,
                               g->stream));
         LV_TRY(g->run_group(group));
-        LV_HIP(hipMemcpyAsync(new_comm, g->newc, sizeof(int32_t) * g->V, hipMemcpyDeviceToHost,
-                              g->stream));
     }
-    return g->sync();
+    return g->communities_out(g->newc, new_comm);
 }
 
 lv_status lv_kernel_timing(lv_graph *g, int32_t kernel, lv_kernel_time *out) {
@@ -2494,7 +2725,7 @@ lv_status lv_set_options(lv_graph *g, const lv_options *opt) {
             LV_HIP(hipMemcpy(keep.data(), g->comm, sizeof(int32_t) * g->V, hipMemcpyDeviceToHost));
         g->level_ready = false;
         LV_TRY(g->setup_level());
-        if (g->V > 0) LV_TRY(lv_set_communities(g, keep.data()));
+        if (g->V > 0) LV_TRY(g->set_state(keep.data()));
     }
     return LV_OK;
 }

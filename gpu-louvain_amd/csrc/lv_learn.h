@@ -51,6 +51,9 @@ void launch_hist_shift(uint8_t *hist, int32_t V, hipStream_t s);
 // walks each mover's row in turn, one entry per lane.
 void launch_neighbor_moves(const int32_t *rowptr, const int32_t *col, const int32_t *list,
                            int32_t n, const uint8_t *hist, int32_t *nmv, hipStream_t s);
+// out[0..8) = the features of a vertex without edges (the same for all such
+// vertices of a level).
+void launch_isolated_features(const FeatureArgs &f, float *out, hipStream_t s);
 // feat[v*8 .. v*8+8) for every vertex.
 void launch_features(const FeatureArgs &f, float *feat, hipStream_t s);
 // Frontier from the model: active[v] = deg > 0 and sigmoid(MLP(features)) >

@@ -30,21 +30,34 @@ typedef float f32x4 __attribute__((ext_vector_type(4)));
 
 inline unsigned blocks_for(int64_t n, int per) { return (unsigned)((n + per - 1) / per); }
 
-__device__ __forceinline__ void features_of(const FeatureArgs &f, int32_t v, float *x) {
-    const int32_t deg = f.rowptr[v + 1] - f.rowptr[v];
-    const float kv = (float)f.k[v];
-    const uint64_t r = f.crec[f.comm[v]];
-    const float scur = rec_sigma(r);
-    const int32_t sz = rec_size(r);
-    const uint32_t h = f.hist[v];
+__device__ __forceinline__ void features_from(int32_t deg, float kv, float scur, int32_t sz,
+                                              uint32_t h, float kown, float margin, int32_t nmv,
+                                              float M, float vscale, float *x) {
     x[0] = log2f(1.0f + (float)deg) * (1.0f / 16.0f);
-    x[1] = kv > 0.0f ? f.kown[v] / kv : 0.0f;
-    x[2] = kv > 0.0f ? fminf(fmaxf(f.margin[v] * f.M / kv, -4.0f), 4.0f) * 0.25f : 0.0f;
+    x[1] = kv > 0.0f ? kown / kv : 0.0f;
+    x[2] = kv > 0.0f ? fminf(fmaxf(margin * M / kv, -4.0f), 4.0f) * 0.25f : 0.0f;
     x[3] = (float)__popc(h & 0xfu) * 0.25f;
     x[4] = log2f(1.0f + (float)sz) * (1.0f / 24.0f);
-    x[5] = log2f(1.0f + scur * f.vscale) * (1.0f / 24.0f);
-    x[6] = deg > 0 ? fminf(1.0f, (float)f.nmv[v] / (float)deg) : 0.0f;
+    x[5] = log2f(1.0f + scur * vscale) * (1.0f / 24.0f);
+    x[6] = deg > 0 ? fminf(1.0f, (float)nmv / (float)deg) : 0.0f;
     x[7] = (float)min(__ffs(h | 0x100u) - 1, 8) * 0.125f;
+}
+
+__device__ __forceinline__ void features_of(const FeatureArgs &f, int32_t v, float *x) {
+    const uint64_t r = f.crec[f.comm[v]];
+    features_from(f.rowptr[v + 1] - f.rowptr[v], (float)f.k[v], rec_sigma(r), rec_size(r),
+                  f.hist[v], f.kown[v], f.margin[v], f.nmv[v], f.M, f.vscale, x);
+}
+
+// Features of a vertex without edges (a singleton of total 0 that never
+// moved and was never evaluated), in out[0..8).
+__global__ void k_isolated_features(float M, float vscale, float *out) {
+    float x[kFeat];
+    features_from(0, 0.0f, 0.0f, 1, 0u, 0.0f, 0.0f, 0, M, vscale, x);
+    if (threadIdx.x == 0) {
+#pragma unroll
+        for (int j = 0; j < kFeat; ++j) out[j] = x[j];
+    }
 }
 
 __global__ void k_hist_shift(uint8_t *hist, int32_t V) {
@@ -377,6 +390,10 @@ void launch_neighbor_moves(const int32_t *rowptr, const int32_t *col, const int3
     if (n <= 0) return;
     hipLaunchKernelGGL(k_neighbor_moves, dim3(blocks_for(n, 256)), dim3(256), 0, s, rowptr, col,
                        list, n, hist, nmv);
+}
+
+void launch_isolated_features(const FeatureArgs &f, float *out, hipStream_t s) {
+    hipLaunchKernelGGL(k_isolated_features, dim3(1), dim3(64), 0, s, f.M, f.vscale, out);
 }
 
 void launch_features(const FeatureArgs &f, float *feat, hipStream_t s) {

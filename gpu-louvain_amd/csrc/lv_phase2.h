@@ -38,6 +38,35 @@ struct AggChunk {
 };
 
 void launch_present(const int32_t *size, int32_t V, int32_t *flag, hipStream_t s);
+
+// --- levels without their isolated vertices ---
+// A level may store only its vertices of positive degree ("compact" ids,
+// ascending in the reference's ids); ref_of maps compact -> reference id and
+// cmp_of reference -> compact id (-1 for a vertex without edges, which is
+// always a singleton community that no vertex can join).
+//
+// flag[c] = 1 for every community c holding a vertex of positive degree
+// (flag zero before).
+void launch_has_degree(const int32_t *rowptr, const int32_t *comm, int32_t V, int32_t *flag,
+                       hipStream_t s);
+// flag[c] = size[c] > 0 && flag[c]: the communities that become vertices of
+// the next compact level.
+void launch_present_compact(const int32_t *size, int32_t V, int32_t *flag, hipStream_t s);
+// present_ref (all ones over the reference ids of the level before) gets
+// present_ref[ref_of[c]] = size[c] > 0 for every compact community c.
+void launch_ref_present(const int32_t *size, const int32_t *ref_of, int32_t V,
+                        int32_t *present_ref, hipStream_t s);
+// ref_of_next[newid[c]] = newid_ref[ref_of[c]] for the communities c with
+// keep[c] != 0.
+void launch_next_ref_of(const int32_t *keep, const int32_t *newid, const int32_t *ref_of,
+                        const int32_t *newid_ref, int32_t V, int32_t *ref_of_next, hipStream_t s);
+// cmp_of[ref_of[i]] = i for i < V (cmp_of filled with -1 before).
+void launch_cmp_of(const int32_t *ref_of, int32_t V, int32_t *cmp_of, hipStream_t s);
+// End of a phase at a compact level: orig2comm[x] (a reference id r) becomes
+// the reference id of r's community, ref_of[comm[cmp_of[r]]] (r itself when
+// it has no edges).
+void launch_compose_ref(int32_t *orig2comm, int32_t Vorig, const int32_t *cmp_of,
+                        const int32_t *ref_of, const int32_t *comm, hipStream_t s);
 void launch_members(const int32_t *comm, int32_t V, const int32_t *mstart, int32_t *fillc,
                     int32_t *members, hipStream_t s);
 void launch_member_degrees(const int32_t *rowptr, const int32_t *members, int32_t V,

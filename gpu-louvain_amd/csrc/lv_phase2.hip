@@ -60,6 +60,45 @@ __global__ void k_present(const int32_t *__restrict__ size, int32_t V, int32_t *
     if (c < V) flag[c] = size[c] > 0 ? 1 : 0;
 }
 
+__global__ void k_has_degree(const int32_t *__restrict__ rowptr, const int32_t *__restrict__ comm,
+                             int32_t V, int32_t *flag) {
+    const int32_t v = blockIdx.x * blockDim.x + threadIdx.x;
+    if (v < V && rowptr[v + 1] > rowptr[v]) flag[comm[v]] = 1;
+}
+
+__global__ void k_present_compact(const int32_t *__restrict__ size, int32_t V, int32_t *flag) {
+    const int32_t c = blockIdx.x * blockDim.x + threadIdx.x;
+    if (c < V) flag[c] = (size[c] > 0 && flag[c]) ? 1 : 0;
+}
+
+__global__ void k_ref_present(const int32_t *__restrict__ size, const int32_t *__restrict__ ref_of,
+                              int32_t V, int32_t *present_ref) {
+    const int32_t c = blockIdx.x * blockDim.x + threadIdx.x;
+    if (c < V) present_ref[ref_of[c]] = size[c] > 0 ? 1 : 0;
+}
+
+__global__ void k_next_ref_of(const int32_t *__restrict__ keep, const int32_t *__restrict__ newid,
+                              const int32_t *__restrict__ ref_of,
+                              const int32_t *__restrict__ newid_ref, int32_t V,
+                              int32_t *ref_of_next) {
+    const int32_t c = blockIdx.x * blockDim.x + threadIdx.x;
+    if (c < V && keep[c]) ref_of_next[newid[c]] = newid_ref[ref_of[c]];
+}
+
+__global__ void k_cmp_of(const int32_t *__restrict__ ref_of, int32_t V, int32_t *cmp_of) {
+    const int32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < V) cmp_of[ref_of[i]] = i;
+}
+
+__global__ void k_compose_ref(int32_t *orig2comm, int32_t Vorig, const int32_t *__restrict__ cmp_of,
+                              const int32_t *__restrict__ ref_of,
+                              const int32_t *__restrict__ comm) {
+    const int32_t x = blockIdx.x * blockDim.x + threadIdx.x;
+    if (x >= Vorig) return;
+    const int32_t i = cmp_of[orig2comm[x]];
+    if (i >= 0) orig2comm[x] = ref_of[comm[i]];
+}
+
 __global__ void k_members(const int32_t *__restrict__ comm, int32_t V,
                           const int32_t *__restrict__ mstart, int32_t *fillc, int32_t *members) {
     const int32_t v = blockIdx.x * blockDim.x + threadIdx.x;
@@ -361,6 +400,44 @@ constexpr int merge_bytes() {
 void launch_present(const int32_t *size, int32_t V, int32_t *flag, hipStream_t s) {
     if (V <= 0) return;
     hipLaunchKernelGGL(k_present, dim3(blocks_for(V, 256)), dim3(256), 0, s, size, V, flag);
+}
+
+void launch_has_degree(const int32_t *rowptr, const int32_t *comm, int32_t V, int32_t *flag,
+                       hipStream_t s) {
+    if (V <= 0) return;
+    hipLaunchKernelGGL(k_has_degree, dim3(blocks_for(V, 256)), dim3(256), 0, s, rowptr, comm, V,
+                       flag);
+}
+
+void launch_present_compact(const int32_t *size, int32_t V, int32_t *flag, hipStream_t s) {
+    if (V <= 0) return;
+    hipLaunchKernelGGL(k_present_compact, dim3(blocks_for(V, 256)), dim3(256), 0, s, size, V, flag);
+}
+
+void launch_ref_present(const int32_t *size, const int32_t *ref_of, int32_t V,
+                        int32_t *present_ref, hipStream_t s) {
+    if (V <= 0) return;
+    hipLaunchKernelGGL(k_ref_present, dim3(blocks_for(V, 256)), dim3(256), 0, s, size, ref_of, V,
+                       present_ref);
+}
+
+void launch_next_ref_of(const int32_t *keep, const int32_t *newid, const int32_t *ref_of,
+                        const int32_t *newid_ref, int32_t V, int32_t *ref_of_next, hipStream_t s) {
+    if (V <= 0) return;
+    hipLaunchKernelGGL(k_next_ref_of, dim3(blocks_for(V, 256)), dim3(256), 0, s, keep, newid, ref_of,
+                       newid_ref, V, ref_of_next);
+}
+
+void launch_cmp_of(const int32_t *ref_of, int32_t V, int32_t *cmp_of, hipStream_t s) {
+    if (V <= 0) return;
+    hipLaunchKernelGGL(k_cmp_of, dim3(blocks_for(V, 256)), dim3(256), 0, s, ref_of, V, cmp_of);
+}
+
+void launch_compose_ref(int32_t *orig2comm, int32_t Vorig, const int32_t *cmp_of,
+                        const int32_t *ref_of, const int32_t *comm, hipStream_t s) {
+    if (Vorig <= 0) return;
+    hipLaunchKernelGGL(k_compose_ref, dim3(blocks_for(Vorig, 256)), dim3(256), 0, s, orig2comm,
+                       Vorig, cmp_of, ref_of, comm);
 }
 
 void launch_members(const int32_t *comm, int32_t V, const int32_t *mstart, int32_t *fillc,
