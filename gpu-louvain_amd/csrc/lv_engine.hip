@@ -78,6 +78,39 @@ __global__ void k_heavy_rows(const int32_t *__restrict__ rowptr, const int32_t *
     }
 }
 
+// Level 0 without its vertices of degree 0.  flag[v] = deg(v) > 0.
+__global__ void k_degree_flags(const int32_t *__restrict__ rowptr, int32_t V, int32_t *flag) {
+    const int32_t v = blockIdx.x * blockDim.x + threadIdx.x;
+    if (v < V) flag[v] = rowptr[v + 1] > rowptr[v] ? 1 : 0;
+}
+
+// From the exclusive scan of those flags: ref_of[compact] = vertex,
+// cmp_of[vertex] = compact id or -1.
+__global__ void k_level0_maps(const int32_t *__restrict__ rowptr, int32_t V,
+                              const int32_t *__restrict__ scan, int32_t *ref_of, int32_t *cmp_of) {
+    const int32_t v = blockIdx.x * blockDim.x + threadIdx.x;
+    if (v >= V) return;
+    const bool keep = rowptr[v + 1] > rowptr[v];
+    if (keep) ref_of[scan[v]] = v;
+    cmp_of[v] = keep ? scan[v] : -1;
+}
+
+// rowptr of the kept vertices (their rows are contiguous already: the
+// dropped rows are empty); rp_c[Vc] = E.
+__global__ void k_compact_rowptr(const int32_t *__restrict__ rowptr,
+                                 const int32_t *__restrict__ ref_of, int32_t Vc, int32_t E,
+                                 int32_t *rp_c) {
+    const int32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < Vc) rp_c[i] = rowptr[ref_of[i]];
+    if (i == Vc) rp_c[i] = E;
+}
+
+__global__ void k_relabel(int32_t *col, int64_t E, const int32_t *__restrict__ cmp_of) {
+    for (int64_t e = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; e < E;
+         e += (int64_t)gridDim.x * blockDim.x)
+        col[e] = cmp_of[col[e]];
+}
+
 // Edge counts per bucket (algorithmic-byte accounting): per-block LDS
 // counters, one global atomic per block and bucket.
 __global__ __launch_bounds__(256) void k_bucket_edges(const int32_t *__restrict__ rowptr,
@@ -307,6 +340,11 @@ struct lv_graph {
     int32_t *ref_of[2] = {nullptr, nullptr}, *cmp_of[2] = {nullptr, nullptr};
     int map_cur = 0;
     int32_t *newid_ref = nullptr;  // reference renumbering at an aggregation
+    // level 0 stored without its vertices of degree 0 (when it has some):
+    // its maps, kept for reset()
+    bool compact0 = false;
+    int32_t Vref0 = 0;
+    int32_t *ref_of0 = nullptr, *cmp_of0 = nullptr;
     int32_t *a_keep = nullptr;     // communities that become vertices of the next level
 
     // aggregation scratch
@@ -459,6 +497,7 @@ struct lv_graph {
         f.M = M32;
         // V of the reference's numbering (vertices without edges included)
         f.vscale = M > 0.0 ? (float)((double)Vref / (2.0 * M)) : 0.0f;
+        f.ref_of = maps_identity ? nullptr : ref_of[map_cur];
         return f;
     }
 
@@ -671,14 +710,46 @@ struct lv_graph {
             LV_HIP(hipMemcpyAsync(col0, col_h, sizeof(int32_t) * nE, hipMemcpyHostToDevice, stream));
             LV_HIP(hipMemcpyAsync(w0, w_h, sizeof(float) * nE, hipMemcpyHostToDevice, stream));
         }
+        Vref0 = nV;
+        LV_TRY(compact_level0(rowptr_h));
         LV_HIP(hipEventRecord(e1, stream));
         LV_HIP(hipStreamSynchronize(stream));
         float ms = 0.0f;
         LV_HIP(hipEventElapsedTime(&ms, e0, e1));
-        memory_ms = ms;
+        memory_ms = ms;  // upload, and the compaction of level 0
         (void)hipEventDestroy(e0);
         (void)hipEventDestroy(e1);
         return reset();
+    }
+
+    // Level 0 keeps only its vertices of positive degree when some, but not
+    // all, have none: rowptr compacted, columns renumbered (in place, on the
+    // device), maps to the caller's ids kept for every returned array.
+    lv_status compact_level0(const int32_t *rowptr_h) {
+        if (!compact_on) return LV_OK;
+        int32_t vc = 0;
+        for (int32_t v = 0; v < V0; ++v) vc += rowptr_h[v + 1] > rowptr_h[v] ? 1 : 0;
+        if (vc == 0 || vc == V0) return LV_OK;
+        const int64_t Vp = (int64_t)V0 + 1;
+        LV_TRY(dev_alloc(&ref_of0, Vp));
+        LV_TRY(dev_alloc(&cmp_of0, Vp));
+        hipStream_t s = stream;
+        hipLaunchKernelGGL(k_degree_flags, dim3(blocks_for(V0, 256)), dim3(256), 0, s, rp0, V0,
+                           a_newid);
+        lv::exclusive_scan(a_newid, a_newid, V0, nullptr, scratch, s);
+        hipLaunchKernelGGL(k_level0_maps, dim3(blocks_for(V0, 256)), dim3(256), 0, s, rp0, V0,
+                           a_newid, ref_of0, cmp_of0);
+        hipLaunchKernelGGL(k_compact_rowptr, dim3(blocks_for((int64_t)vc + 1, 256)), dim3(256), 0, s,
+                           rp0, ref_of0, vc, (int32_t)E0, rpb[0]);
+        LV_HIP(hipMemcpyAsync(rp0, rpb[0], sizeof(int32_t) * ((size_t)vc + 1),
+                              hipMemcpyDeviceToDevice, s));
+        if (E0 > 0)
+            hipLaunchKernelGGL(k_relabel, dim3((unsigned)std::min<int64_t>(blocks_for(E0, 256), 65536)),
+                               dim3(256), 0, s, col0, E0, cmp_of0);
+        LV_LAUNCHED("level 0 compaction");
+        compact0 = true;
+        V0 = vc;
+        return LV_OK;
     }
 
     // Back to the singleton state of level 0.  When level 0 is current and
@@ -688,7 +759,7 @@ struct lv_graph {
             LV_HIP(hipMemcpyAsync(orig2comm, o2c0, sizeof(int32_t) * (size_t)Vorig,
                                   hipMemcpyDeviceToDevice, stream));
         else
-            lv::iota_i32(orig2comm, V0, stream);
+            lv::iota_i32(orig2comm, Vorig, stream);
         if (cur_buf < 0 && level_ready) {
             lv::launch_level_state(V, k, comm, size, sigma, crec, stream);
             state_changed();
@@ -698,8 +769,15 @@ struct lv_graph {
         cur_buf = -1;
         V = V0;
         E = E0;
-        maps_identity = true;
-        Vref = V0;
+        map_cur = 0;
+        maps_identity = !compact0;
+        Vref = Vref0;
+        if (compact0) {
+            LV_HIP(hipMemcpyAsync(ref_of[0], ref_of0, sizeof(int32_t) * (size_t)V0,
+                                  hipMemcpyDeviceToDevice, stream));
+            LV_HIP(hipMemcpyAsync(cmp_of[0], cmp_of0, sizeof(int32_t) * (size_t)Vref0,
+                                  hipMemcpyDeviceToDevice, stream));
+        }
         LV_LAUNCHED("reset");
         level_ready = false;
         return LV_OK;
@@ -1475,7 +1553,7 @@ struct lv_graph {
     // of positive degree (the others are singletons without edges, kept in
     // the reference's numbering only).  Returns whether it drops.
     bool renumber_flags(bool allow_drop) {
-        const bool drop = allow_drop && compact_on && G == 0;
+        const bool drop = allow_drop && compact_on && G == 0 && E > 0;
         if (drop) {
             lv::fill_i32(a_newid, 0, V, stream);
             lv::launch_has_degree(rowptr(), comm, V, a_newid, stream);
@@ -1923,11 +2001,12 @@ struct lv_graph {
     }
 
     // The original vertices of an earlier run map into this handle's level 0
-    // by o2c (n entries in [0, V0)); reset() and lv_assignment use it.  The
+    // by o2c (n entries below its vertex count); reset() and lv_assignment
+    // use it.  The
     // run's total weight M_run replaces the one summed from this level.
     lv_status set_initial_map(const int32_t *o2c, int32_t n, double M_run) {
         for (int32_t x = 0; x < n; ++x)
-            if (o2c[x] < 0 || o2c[x] >= V0) return fail(LV_ERR_ARG, "map entry out of range");
+            if (o2c[x] < 0 || o2c[x] >= Vref0) return fail(LV_ERR_ARG, "map entry out of range");
         if (!(M_run >= 0.0)) return fail(LV_ERR_ARG, "bad total weight");
         // the run's M (aggregated fp32 weights need not sum to it exactly)
         M = M_run;
@@ -1937,7 +2016,7 @@ struct lv_graph {
             volatile float d = two_m * M32;
             den = d;
         }
-        if ((int64_t)n + 1 > (int64_t)V0 + 1) {
+        if ((int64_t)n + 1 > (int64_t)Vref0 + 1) {
             dev_free(orig2comm);
             orig2comm = nullptr;
             LV_TRY(dev_alloc(&orig2comm, (int64_t)n + 1));
@@ -1967,6 +2046,8 @@ struct lv_graph {
         dev_free(newc);
         dev_free(orig2comm);
         dev_free(o2c0);
+        dev_free(ref_of0);
+        dev_free(cmp_of0);
         dev_free(crec);
         dev_free(dk);
         dev_free(margin);
@@ -2664,7 +2745,10 @@ lv_status lv_batch_create(int32_t G, const int32_t *vptr, const int32_t *rowptr,
                 if (col[e] < vptr[g] || col[e] >= vptr[g + 1])
                     return fail(LV_ERR_ARG, "an edge leaves its graph's vertex range");
     lv_graph *h = nullptr;
-    LV_TRY(lv_create(rowptr, col, w, V, E, opt, &h));
+    // graph ranges stay in one numbering: batches store every vertex
+    lv_options o = opt ? *opt : lv_options{};
+    o.compact = -1;
+    LV_TRY(lv_create(rowptr, col, w, V, E, &o, &h));
     lv_status st = h->setup_batch(G, vptr, rowptr, w);
     if (st != LV_OK) {
         std::string msg = g_last_error;

@@ -33,7 +33,8 @@ struct FeatureArgs {
     const int32_t *nmv;   // row entries whose neighbour moved in the last sweep
     int32_t V;
     float M;
-    float vscale;         // V / (2M)
+    float vscale;         // V / (2M), V counting the vertices without edges
+    const int32_t *ref_of;  // stored -> caller's vertex id (null: the same)
 };
 
 // Frontier selection rules.

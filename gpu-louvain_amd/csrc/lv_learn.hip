@@ -249,7 +249,9 @@ __global__ void k_frontier_rule(int kind, FeatureArgs f, const int32_t *__restri
     if (kind == kFrontierNeighbor) {
         act = act && ((f.hist[v] & 1u) || f.nmv[v] > 0);
     } else if (kind == kFrontierRandom) {
-        const uint64_t r = mix64(seed ^ mix64((uint64_t)(uint32_t)v));
+        // drawn per vertex id of the caller's numbering
+        const int32_t id = f.ref_of ? f.ref_of[v] : v;
+        const uint64_t r = mix64(seed ^ mix64((uint64_t)(uint32_t)id));
         act = act && (float)(r >> 40) * (1.0f / 16777216.0f) < p;
     }
     active[v] = act ? 1 : 0;
