@@ -1402,7 +1402,7 @@ struct lv_graph {
                             int w, const T *fill) {
         for (int64_t r = 0; r < Vr; ++r)
             for (int j = 0; j < w; ++j) dst[r * w + j] = fill[j];
-        for (size_t i = 0; i < rof.size() && i < (size_t)INT32_MAX; ++i)
+        for (size_t i = 0; i < rof.size(); ++i)
             for (int j = 0; j < w; ++j) dst[(int64_t)rof[i] * w + j] = src[i * w + j];
     }
 
@@ -1551,20 +1551,20 @@ struct lv_graph {
     // a_newid[c] = 1 for the communities that become vertices of the next
     // level, the non-empty ones; when dropping, only those holding a vertex
     // of positive degree (the others are singletons without edges, kept in
-    // the reference's numbering only).  Returns whether it drops.
-    bool renumber_flags(bool allow_drop) {
-        const bool drop = allow_drop && compact_on && G == 0 && E > 0;
-        if (drop) {
+    // the reference's numbering only).  *drop tells whether it drops.
+    lv_status renumber_flags(bool allow_drop, bool *drop) {
+        *drop = allow_drop && compact_on && G == 0 && E > 0;
+        if (*drop) {
             lv::fill_i32(a_newid, 0, V, stream);
             lv::launch_has_degree(rowptr(), comm, V, a_newid, stream);
             lv::launch_present_compact(size, V, a_newid, stream);
         } else {
             lv::launch_present(size, V, a_newid, stream);
         }
-        if (drop || !maps_identity)
-            (void)hipMemcpyAsync(a_keep, a_newid, sizeof(int32_t) * (size_t)V,
-                                 hipMemcpyDeviceToDevice, stream);
-        return drop;
+        if ((*drop || !maps_identity) && V > 0)
+            LV_HIP(hipMemcpyAsync(a_keep, a_newid, sizeof(int32_t) * (size_t)V,
+                                  hipMemcpyDeviceToDevice, stream));
+        return LV_OK;
     }
 
     // After the scan of the flags (a_newid, newV vertices next): the
@@ -1608,7 +1608,8 @@ struct lv_graph {
         int32_t *newV_d = a_scalars;
         int32_t *etot_d = a_scalars + 1;
         // renumbering (community_aggregation.cu:267-275)
-        const bool drop = renumber_flags(false);
+        bool drop = false;
+        LV_TRY(renumber_flags(false, &drop));
         lv::exclusive_scan(a_newid, a_newid, V, newV_d, scratch, s);
         if (G > 0) lv::launch_graph_offsets(vptr, G, V, a_newid, newV_d, vptr_next, s);
         lv::exclusive_scan(size, a_mstart, V, nullptr, scratch, s);
@@ -1935,7 +1936,8 @@ struct lv_graph {
         int32_t *etot_d = a_scalars + 1;
         int32_t *U_d = a_scalars + 2;
         // renumbering (community_aggregation.cu:267-275)
-        const bool drop = renumber_flags(true);
+        bool drop = false;
+        LV_TRY(renumber_flags(true, &drop));
         lv::exclusive_scan(a_newid, a_newid, V, newV_d, scratch, s);
         if (G > 0) lv::launch_graph_offsets(vptr, G, V, a_newid, newV_d, vptr_next, s);
         LV_LAUNCHED("renumbering");

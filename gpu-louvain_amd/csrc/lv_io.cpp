@@ -443,13 +443,21 @@ lv_status lv_read_mtx(const char *path, int32_t **rowptr, int32_t **col, float *
             const int field = (int)(g % K);
             ++g;
             if (entry + 1 >= my_bad) continue;
-            bool ok = len < sizeof tok;
-            if (ok) {
+            // a token is copied and terminated (long ones into a string)
+            std::string big;
+            char *tk = tok;
+            if (len >= sizeof tok) {
+                big.assign(t0, len);
+                tk = &big[0];
+            } else {
                 std::memcpy(tok, t0, len);
                 tok[len] = '\0';
+            }
+            bool ok = true;
+            {
                 if (field < 2) {
                     long long v = 0;
-                    ok = parse_int(tok, len, v);
+                    ok = parse_int(tk, len, v);
                     if (ok && (v < 1 || v > n)) {
                         my_bad = entry + 1;
                         my_kind = 2;
@@ -458,7 +466,7 @@ lv_status lv_read_mtx(const char *path, int32_t **rowptr, int32_t **col, float *
                     if (ok) (field == 0 ? ti : tj)[(size_t)entry] = (int32_t)(v - 1);
                 } else {
                     float v = 0.0f;
-                    ok = parse_float(tok, len, v);
+                    ok = parse_float(tk, len, v);
                     if (ok) tw[(size_t)entry] = v;
                 }
             }

@@ -123,7 +123,7 @@ def test_reader_weights_round_like_strtof(lvlib, oracle, tmp_path):
     toks = ["1", "0.5", "1e-3", "2.5E+2", "-0.0", "+3.25", "0000.125", "1.000000059604644775390625",
             "1.0000000596046448", "16777217", "16777219", "3.4028235e38", "1e-30", "7.006e-40",
             "0.1", "0.2", "0.3", "123456789012345678901234567890", ".5", "5.", "1e22", "1e23",
-            "9007199254740993", "0.000000000000000000001"]
+            "9007199254740993", "0.000000000000000000001", "0." + "3" * 80, "1" * 70 + "e-69"]
     for _ in range(3000):
         nd = int(rng.integers(1, 22))
         digits = "".join(str(d) for d in rng.integers(0, 10, nd))
