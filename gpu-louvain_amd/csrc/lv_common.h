@@ -58,15 +58,13 @@ struct MoveArgs {
     const float *gden;
 };
 
-__device__ __forceinline__ void vertex_mden(const MoveArgs &a, int32_t v, float &M32, float &den) {
+// M32 and den of vertex v's graph (.x, .y).
+__device__ __forceinline__ float2 vertex_mden(const MoveArgs &a, int32_t v) {
     if (a.gid) {
         const int32_t g = a.gid[v];
-        M32 = a.gM32[g];
-        den = a.gden[g];
-    } else {
-        M32 = a.M32;
-        den = a.den;
+        return make_float2(a.gM32[g], a.gden[g]);
     }
+    return make_float2(a.M32, a.den);
 }
 
 // Community record read by the move kernels: the fp32 rounding of the FP64

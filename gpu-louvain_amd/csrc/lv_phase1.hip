@@ -70,8 +70,8 @@ __device__ __forceinline__ void lanes_max_pair(uint64_t &best, double &bestK) {
 __device__ __forceinline__ void write_decision(const MoveArgs &a, int32_t v, int32_t cur,
                                                uint64_t best, double bestK, double kc) {
 #pragma clang fp contract(off)
-    float vM32, vden;
-    vertex_mden(a, v, vM32, vden);
+    const float2 md = vertex_mden(a, v);
+    const float vM32 = md.x, vden = md.y;
     const int32_t nc = decide_from(best, kc, vM32, cur);
     a.newc[v] = nc;
     a.dk[v] = nc != cur ? bestK - kc : 0.0;
@@ -126,8 +126,8 @@ __global__ __launch_bounds__(256) void k_move_sub(MoveArgs a, const int32_t *__r
     if (owner && c != cur) {
         const uint64_t rcur = a.crec[cur];
         const uint64_t rc = a.crec[c];
-        float vM32, vden;
-        vertex_mden(a, v, vM32, vden);
+        const float2 md = vertex_mden(a, v);
+        const float vM32 = md.x, vden = md.y;
         consider_k(best, bestK, c, K, rc, cur, rec_size(rcur), (float)a.k[v], rec_sigma(rcur),
                    vM32, vden);
     }
@@ -177,8 +177,8 @@ __global__ __launch_bounds__(256) void k_move_wave(MoveArgs a, const int32_t *__
     __syncthreads();
     if (!active) return;  // only wave-level communication below
     const float k32 = (float)a.k[v];
-    float vM32, vden;
-    vertex_mden(a, v, vM32, vden);
+    const float2 md = vertex_mden(a, v);
+    const float vM32 = md.x, vden = md.y;
     const uint64_t rcur = a.crec[cur];
     const float scur = rec_sigma(rcur);
     const int32_t size_cur = rec_size(rcur);
@@ -252,8 +252,8 @@ __global__ __launch_bounds__(T) void k_move_wg(MoveArgs a, const int32_t *__rest
         }
         __syncthreads();
         const float k32 = (float)a.k[v];
-        float vM32, vden;
-        vertex_mden(a, v, vM32, vden);
+        const float2 md = vertex_mden(a, v);
+        const float vM32 = md.x, vden = md.y;
         const uint64_t rcur = a.crec[cur];
         const float scur = rec_sigma(rcur);
         const int32_t size_cur = rec_size(rcur);
@@ -471,8 +471,8 @@ __global__ __launch_bounds__(kMergeThreads) void k_chunk_merge(MoveArgs a, Chunk
     if (a.active && !a.active[v]) return;  // not in this sweep's frontier
     const int32_t cur = a.comm[v];
     const float k32 = (float)a.k[v];
-    float vM32, vden;
-    vertex_mden(a, v, vM32, vden);
+    const float2 md = vertex_mden(a, v);
+    const float vM32 = md.x, vden = md.y;
     const uint64_t rcur = a.crec[cur];
     const float scur = rec_sigma(rcur);
     const int32_t size_cur = rec_size(rcur);
