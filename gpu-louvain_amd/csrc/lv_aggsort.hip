@@ -133,14 +133,21 @@ __global__ __launch_bounds__(kSortThreads) void k_radix_scatter(
 
 // ------------------------------------------------------------ aggregation
 
+// cn[v] = newid[comm[v]]: the new id of each vertex's community, so that a
+// row entry costs one gather.
+__global__ void k_comm_newid(const int32_t *__restrict__ comm, const int32_t *__restrict__ newid,
+                             int32_t V, int32_t *cn) {
+    const int32_t v = blockIdx.x * blockDim.x + threadIdx.x;
+    if (v < V) cn[v] = newid[comm[v]];
+}
+
 __device__ __forceinline__ void emit_row(const int32_t *__restrict__ col,
                                          const float *__restrict__ w,
-                                         const int32_t *__restrict__ comm,
-                                         const int32_t *__restrict__ newid, uint64_t src,
+                                         const int32_t *__restrict__ cn, uint64_t src,
                                          int64_t lo, int64_t hi, int st, uint64_t *keys,
                                          float *vals) {
     for (int64_t e = lo; e < hi; e += st) {
-        const uint32_t dst = (uint32_t)newid[comm[col[e]]];
+        const uint32_t dst = (uint32_t)cn[col[e]];
         keys[e] = (src << 32) | dst;
         vals[e] = w[e];
     }
@@ -148,14 +155,14 @@ __device__ __forceinline__ void emit_row(const int32_t *__restrict__ col,
 
 __global__ __launch_bounds__(kRowThreads) void k_agg_pairs(
     const int32_t *__restrict__ rowptr, const int32_t *__restrict__ col,
-    const float *__restrict__ w, const int32_t *__restrict__ comm,
-    const int32_t *__restrict__ newid, const int32_t *__restrict__ order, SegTable st,
-    const Chunk *__restrict__ chunks, uint64_t *keys, float *vals) {
+    const float *__restrict__ w, const int32_t *__restrict__ cn,
+    const int32_t *__restrict__ order, SegTable st, const Chunk *__restrict__ chunks,
+    uint64_t *keys, float *vals) {
     const int b = blockIdx.x;
     if (b >= st.chunk_block_start) {
         const Chunk ch = chunks[b - st.chunk_block_start];
-        const uint64_t src = (uint32_t)newid[comm[ch.v]];
-        emit_row(col, w, comm, newid, src, ch.lo + threadIdx.x, ch.hi, kRowThreads, keys, vals);
+        const uint64_t src = (uint32_t)cn[ch.v];
+        emit_row(col, w, cn, src, ch.lo + threadIdx.x, ch.hi, kRowThreads, keys, vals);
         return;
     }
     int si = 0;
@@ -166,15 +173,15 @@ __global__ __launch_bounds__(kRowThreads) void k_agg_pairs(
         const int32_t idx = (b - sg.block_start) * (kRowThreads >> lg) + (threadIdx.x >> lg);
         if (idx < sg.count) {
             const int32_t v = order[sg.list_off + idx];
-            const uint64_t src = (uint32_t)newid[comm[v]];
-            emit_row(col, w, comm, newid, src, rowptr[v] + (threadIdx.x & ((1 << lg) - 1)),
+            const uint64_t src = (uint32_t)cn[v];
+            emit_row(col, w, cn, src, rowptr[v] + (threadIdx.x & ((1 << lg) - 1)),
                      rowptr[v + 1], 1 << lg, keys, vals);
         }
     } else {
         const int32_t v = order[sg.list_off + (b - sg.block_start)];
-        const uint64_t src = (uint32_t)newid[comm[v]];
-        emit_row(col, w, comm, newid, src, rowptr[v] + threadIdx.x, rowptr[v + 1], kRowThreads,
-                 keys, vals);
+        const uint64_t src = (uint32_t)cn[v];
+        emit_row(col, w, cn, src, rowptr[v] + threadIdx.x, rowptr[v + 1], kRowThreads, keys,
+                 vals);
     }
 }
 
@@ -259,13 +266,14 @@ bool radix_sort_pairs(uint64_t *keys, float *vals, uint64_t *tk, float *tv, int6
 }
 
 void launch_agg_pairs(const int32_t *rowptr, const int32_t *col, const float *w,
-                      const int32_t *comm, const int32_t *newid, const int32_t *order,
-                      const SegTable &st, const Chunk *chunks, uint64_t *keys, float *vals,
-                      hipStream_t s) {
+                      const int32_t *comm, const int32_t *newid, int32_t V, const int32_t *order,
+                      const SegTable &st, const Chunk *chunks, int32_t *cn, uint64_t *keys,
+                      float *vals, hipStream_t s) {
     const int32_t nb = st.chunk_block_start + st.nchunks;
-    if (nb <= 0) return;
-    hipLaunchKernelGGL(k_agg_pairs, dim3(nb), dim3(kRowThreads), 0, s, rowptr, col, w, comm, newid,
-                       order, st, chunks, keys, vals);
+    if (nb <= 0 || V <= 0) return;
+    hipLaunchKernelGGL(k_comm_newid, dim3(blocks_for(V, 256)), dim3(256), 0, s, comm, newid, V, cn);
+    hipLaunchKernelGGL(k_agg_pairs, dim3(nb), dim3(kRowThreads), 0, s, rowptr, col, w, cn, order,
+                       st, chunks, keys, vals);
 }
 
 void launch_run_heads(const uint64_t *keys, int64_t n, int32_t *head, hipStream_t s) {

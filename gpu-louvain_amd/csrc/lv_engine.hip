@@ -1956,8 +1956,8 @@ struct lv_graph {
             LV_TRY(spos.reserve(E));
             LV_TRY(sacc.reserve(E));
             LV_TRY(sscratch.reserve(std::max(lv::radix_scratch_elems(E), lv::scan_scratch_elems(E) + 8)));
-            lv::launch_agg_pairs(rowptr(), col(), w(), comm, a_newid, order, st, ins.p, skA.p, svA.p,
-                                 s);
+            lv::launch_agg_pairs(rowptr(), col(), w(), comm, a_newid, V, order, st, ins.p, a_mstart,
+                                 skA.p, svA.p, s);
             // digits of the column id, then of the row id (stable LSD)
             int bits = 1;
             while ((1ll << bits) < (int64_t)newV) ++bits;
