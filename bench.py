@@ -5,13 +5,16 @@ Workload: RMAT scale 22 (4,194,304 vertices, edge factor 16, a/b/c/d =
 .57/.19/.19/.05, seed 22), self-loops and duplicates removed, symmetrised,
 fp32 weights on the 2^-24 grid in (0,1], seeded vertex permutation.
 
-A step is one plain sweep over the level-0 graph from the singleton state:
-reset of the community state, moves of the 7 degree groups with their
-applies, and the fused modularity reduction (with its host readback, the
-termination test of the reference's sweep loop).  value = CSR entries per
-second per sweep.  Also reported: time to converge (full multi-level run),
-per-kernel HIP-event timings with the roofline of the dominant kernel, and
-the CPU oracle timed on a bounded sample (one sweep, same graph).
+A step is one plain sweep of the level-0 phase: the moves of the 7 degree
+groups with their applies, and the modularity (with its host readback, the
+termination test of the reference's sweep loop).  Steps run through the
+level-0 phase from singletons (4 sweeps on this graph) and start it again
+when the phase ends; a restart (community state reset and the modularity
+before the first sweep) is inside the timed step.  value = CSR entries per
+second over the timed level-0 sweeps.  Also reported: time to converge (full
+multi-level run) with per-level sweep times, per-kernel HIP-event timings
+with the roofline of the dominant kernel, and the CPU oracle timed on a
+bounded sample (one sweep, same graph).
 
 With --level N the step is one sweep of level N (reached by plain phases
 and aggregations) from that level's singleton state, value = that level's
@@ -236,6 +239,41 @@ def io_comparison(L, csr, tmpdir=None):
         os.rmdir(d)
 
 
+def per_level(G, min_gain=1e-6):
+    """The multi-level run sweep by sweep (the library's phase rule: a sweep
+    that moves nothing, or gains less than min_gain or nothing, ends the
+    phase), timed per sweep on the host clock around each synchronous sweep:
+    per level its size, sweeps, time and edges/s, and the median level-0
+    sweep."""
+    import numpy as np
+
+    G.reset()
+    levels, lvl0 = [], []
+    for lvl in range(64):
+        V, E = G.level_size()
+        q_prev = G.modularity()
+        changed, ts = False, []
+        while True:
+            t = time.perf_counter()
+            q, moved = G.sweep()
+            ts.append((time.perf_counter() - t) * 1e3)
+            gain = q - q_prev
+            if gain > 0.0:
+                changed = True
+            q_prev = q
+            if moved == 0 or not (gain > 0.0) or gain < min_gain:
+                break
+        if lvl == 0:
+            lvl0 = ts
+        ms = float(np.sum(ts))
+        levels.append({"level": lvl, "V": V, "E": E, "sweeps": len(ts), "sweep_ms": ms,
+                       "edges_per_s": E * len(ts) / (ms * 1e-3) if ms > 0 else None})
+        if not changed:
+            break
+        G.aggregate()
+    return levels, (float(np.median(lvl0)) if lvl0 else None)
+
+
 def csr_scale(csr):
     return int(round(__import__("math").log2(max(csr.V, 1))))
 
@@ -301,13 +339,22 @@ def main():
             r = G.sweep()
             return r, time.perf_counter() - t
     else:
+        phase = {"ended": True, "q": 0.0}
+
         def step():
             t = time.perf_counter()
-            G.reset()
-            r = G.sweep()
-            return r, time.perf_counter() - t
+            if phase["ended"]:
+                G.reset()
+                phase["q"] = G.modularity()
+            q, moved = G.sweep()
+            gain = q - phase["q"]
+            phase["q"] = q
+            phase["ended"] = moved == 0 or not (gain > 0.0) or gain < 1e-6
+            return (q, moved), time.perf_counter() - t
     for i in range(args.warmup):
         step()
+    if args.level == 0:
+        phase["ended"] = True  # the timed steps start a fresh phase
     log(f"warmup done ({args.warmup} sweeps)")
     G.kernel_timing_reset()
     moved = 0
@@ -363,6 +410,13 @@ def main():
                     "edges_per_s_in_sweeps": st["edges_swept"] / (st["sweep_ms"] * 1e-3) if st["sweep_ms"] else None}
         log(f"converged: Q={st['modularity']:.6f} in {st['algo_ms']:.1f} ms, {st['levels']} levels, "
             f"{st['sweeps']} sweeps")
+        lv, med0 = per_level(G)
+        converge["per_level"] = lv
+        converge["level0_median_sweep_ms"] = med0
+        converge["level0_median_edges_per_s"] = csr.E / (med0 * 1e-3) if med0 else None
+        log("per level: " + ", ".join(f"L{x['level']} {x['sweeps']} sweeps {x['sweep_ms']:.1f} ms "
+                                      f"({x['edges_per_s'] / 1e9:.1f} G edges/s)" for x in lv)
+            + f"; level-0 median sweep {med0:.3f} ms")
     G.close()
 
     pruned = None
