@@ -53,9 +53,10 @@ typedef struct {
                                 on every level, 2 = it and the chunked tier, -1 = off */
     int32_t agg_mode;        /* aggregation: 0 = radix sort of (row, column) community
                                 keys, 1 = per-community hash merges                 */
-    int32_t compact;         /* 0: aggregated levels store only their vertices with
-                                edges (results and every returned array stay in the
-                                reference's numbering); -1: store every vertex     */
+    int32_t compact;         /* 0: every level is stored without its vertices of
+                                degree 0 (results and every returned array stay in
+                                the reference's numbering; batches always store
+                                every vertex); -1: store every vertex               */
     int32_t reserved[5];
 } lv_options;
 
