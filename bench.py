@@ -118,15 +118,17 @@ def pruned_comparison(L, csr, min_gain=1e-6):
         out.update(res)
         out["target"] = {"dQ_rel_max": 0.005, "time_ratio_max": 0.5,
                          "met": bool(res["learned"]["dQ_rel"] <= 0.005 and res["learned"]["time_ratio"] <= 0.5)}
-    return out
+    return out, (params, thr)
 
 
-def batch_comparison(L, n_graphs=4096, seed=4, sample=64, min_gain=1e-6):
+def batch_comparison(L, n_graphs=4096, seed=4, sample=64, min_gain=1e-6, model=None):
     """4,096 independent planted-partition graphs solved as one batch: sizes
     log-uniform in [200, 50000], blocks uniform in 4..64, p_in for mean
     in-block degree 12, p_out for mean cross-block degree 2, unit weights.
     Compared with running the first `sample` graphs one by one (lv_run),
-    whose results must equal the batch's."""
+    whose results must equal the batch's.  With model = (params, threshold)
+    the batch also runs learned-pruned (lv_batch_run_pruned, full sweep every
+    4), checked the same way against lv_run_pruned on the sample."""
     import numpy as np
 
     rng = np.random.default_rng(seed)
@@ -162,7 +164,39 @@ def batch_comparison(L, n_graphs=4096, seed=4, sample=64, min_gain=1e-6):
     est = single_ms / sample * n_graphs
     log(f"batch: one-by-one {single_ms / sample:.2f} ms per graph (first {sample}), "
         f"{est:.0f} ms estimated for all; {mismatches} mismatches")
-    return {"graphs": n_graphs, "V": V, "E": E, "time_ms": tot["algo_ms"],
+    learned = None
+    if model is not None:
+        params, thr = model
+        fr = L.frontier(kind=L.FRONTIER_MODEL, threshold=thr, full_every=4)
+        with L.Batch(gs) as B:
+            B.set_model(params)
+            B.run_pruned(min_gain, fr)  # warm
+            lper, ltot = B.run_pruned(min_gain, fr)
+            lcomms = B.assignment()
+        lsingle_ms, lmis = 0.0, 0
+        for i in range(sample):
+            with L.Graph(gs[i]) as G:
+                G.set_model(params)
+                G.run_pruned(min_gain, fr)
+                G.reset()
+                st = G.run_pruned(min_gain, fr)
+                o2c, lv = G.assignment()
+            lsingle_ms += st["algo_ms"]
+            if (abs(st["modularity"] - lper[i]["modularity"]) > 1e-12 or not np.array_equal(o2c, lcomms[i])
+                    or lv != lper[i]["final_V"]):
+                lmis += 1
+        qp = np.array([r["modularity"] for r in per])
+        ql = np.array([r["modularity"] for r in lper])
+        learned = {"time_ms": ltot["algo_ms"], "time_ratio_vs_plain_batch": ltot["algo_ms"] / tot["algo_ms"],
+                   "union_sweeps": ltot["sweeps"], "graph_sweeps": int(sum(r["sweeps"] for r in lper)),
+                   "graph_full_sweeps": ltot["full_sweeps"], "evaluated": ltot["evaluated"],
+                   "evaluated_plain": tot["evaluated"], "mean_modularity": float(ql.mean()),
+                   "mean_dQ_rel": float(np.mean((qp - ql) / qp)), "max_dQ_rel": float(np.max((qp - ql) / qp)),
+                   "one_by_one_ms_per_graph": lsingle_ms / sample, "sample_mismatches": lmis}
+        log(f"batch learned-pruned: {ltot['algo_ms']:.1f} ms ({learned['time_ratio_vs_plain_batch']:.3f}x plain "
+            f"batch), mean dQ_rel {learned['mean_dQ_rel']:.4f}, {ltot['evaluated']} evaluations vs "
+            f"{tot['evaluated']}; {lmis} mismatches vs one-by-one")
+    return {"learned_pruned": learned, "graphs": n_graphs, "V": V, "E": E, "time_ms": tot["algo_ms"],
             "graphs_per_s": n_graphs / (tot["algo_ms"] * 1e-3), "edges_per_s": E / (tot["algo_ms"] * 1e-3),
             "union_levels": tot["levels"], "union_sweeps": tot["sweeps"],
             "aggregate_ms": tot["aggregate_ms"], "upload_ms": tot["memory_ms"],
@@ -419,12 +453,12 @@ def main():
             + f"; level-0 median sweep {med0:.3f} ms")
     G.close()
 
-    pruned = None
+    pruned = model = None
     if not args.no_pruned:
-        pruned = pruned_comparison(L, csr)
+        pruned, model = pruned_comparison(L, csr)
     batch = None
     if not args.no_batch:
-        batch = batch_comparison(L)
+        batch = batch_comparison(L, model=model)
 
     io = io_comparison(L, csr) if args.io else None
 

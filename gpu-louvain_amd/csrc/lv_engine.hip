@@ -201,6 +201,9 @@ constexpr int32_t kXcdMinVertices = 1 << 21;  // XCD-local row passes from this 
 constexpr int64_t kMaxPrunedSweeps = 4096;
 // Internal frontier kind: the vertices of the active graphs of a batch.
 constexpr int32_t kFrontierBatchGraphs = 100;
+// Internal frontier kind: batches with the model; a graph's vertices follow
+// the model when its sweep is pruned, all of them when it is full.
+constexpr int32_t kFrontierBatchModel = 101;
 
 int ceil_log2(int64_t x) {
     int lg = 0;
@@ -264,14 +267,22 @@ struct lv_graph {
     uint8_t *gact = nullptr, *gdone = nullptr;
     double *gsums = nullptr, *own_v = nullptr;
     double *gin = nullptr;  // per graph: weight inside communities (incremental sweeps)
+    uint8_t *gfull = nullptr;  // per graph: this sweep is full (batched pruned runs)
+    float *gvscale = nullptr;  // per graph: V_g / (2 M_g) at the current level (features)
     std::vector<double> h_gsums;     // per graph: own, sq, moved of the last sweep
-    uint8_t *h_up = nullptr;         // pinned staging of per-graph flag uploads
+    // pinned staging of per-graph flag uploads, one slot per destination: an
+    // upload may still be pending when the next one (to another array) is
+    // staged, since none waits for its copy
+    uint8_t *h_up = nullptr;
+    enum UpSlot { kUpAct = 0, kUpFull = 1, kUpDone = 2, kUpSlots = 3 };
+    size_t up_stride() const { return ((size_t)G + 64 + 63) & ~(size_t)63; }
 
     // Per-graph flags to the device through pinned memory; the caller syncs
     // the stream before the next upload.
-    lv_status upload_flags(uint8_t *dst, const std::vector<uint8_t> &src) {
-        std::memcpy(h_up, src.data(), src.size());
-        LV_HIP(hipMemcpyAsync(dst, h_up, src.size(), hipMemcpyHostToDevice, stream));
+    lv_status upload_flags(uint8_t *dst, const std::vector<uint8_t> &src, UpSlot slot) {
+        uint8_t *h = h_up + (size_t)slot * up_stride();
+        std::memcpy(h, src.data(), src.size());
+        LV_HIP(hipMemcpyAsync(dst, h, src.size(), hipMemcpyHostToDevice, stream));
         return LV_OK;
     }
     uint32_t *bits = nullptr;    // two mover bitmaps (alternating group steps)
@@ -498,6 +509,9 @@ struct lv_graph {
         // V of the reference's numbering (vertices without edges included)
         f.vscale = M > 0.0 ? (float)((double)Vref / (2.0 * M)) : 0.0f;
         f.ref_of = maps_identity ? nullptr : ref_of[map_cur];
+        f.gid = G > 0 ? gid : nullptr;
+        f.gM = G > 0 ? gM32 : nullptr;
+        f.gvscale = G > 0 ? gvscale : nullptr;
         return f;
     }
 
@@ -991,8 +1005,26 @@ struct lv_graph {
         lv::launch_level_state(V, k, comm, size, sigma, crec, s);
         state_changed();
         LV_LAUNCHED("level state");
+        if (G > 0) LV_TRY(update_graph_scales());
         level_ready = true;
         return LV_OK;
+    }
+
+    // Batches: each graph's V_g / (2 M_g) at the current level (the model's
+    // community-total feature uses its own graph's numbers).
+    lv_status update_graph_scales() {
+        std::vector<int32_t> vp((size_t)G + 1);
+        LV_HIP(hipMemcpyAsync(vp.data(), vptr, sizeof(int32_t) * ((size_t)G + 1),
+                              hipMemcpyDeviceToHost, stream));
+        LV_TRY(sync());
+        std::vector<float> sc((size_t)G);
+        for (int32_t g = 0; g < G; ++g)
+            sc[(size_t)g] = gM[(size_t)g] > 0.0
+                                ? (float)((double)(vp[(size_t)g + 1] - vp[(size_t)g]) / (2.0 * gM[(size_t)g]))
+                                : 0.0f;
+        LV_HIP(hipMemcpyAsync(gvscale, sc.data(), sizeof(float) * (size_t)G, hipMemcpyHostToDevice,
+                              stream));
+        return sync();
     }
 
     // Split table and segment tables of the XCD-local modularity pass; off
@@ -1203,6 +1235,12 @@ struct lv_graph {
         const lv::FeatureArgs fa = feature_args();
         if (fr.kind == kFrontierBatchGraphs) {
             lv::launch_graph_frontier(rowptr(), V, gid, gact, bb, fkeys, active, s);
+        } else if (fr.kind == kFrontierBatchModel) {
+            if (!mlp_set) return fail(LV_ERR_STATE, "frontier model not set (lv_batch_set_model)");
+            lv::launch_frontier(fa, order, h_off[lv::kNumBuckets], mlp, fr.threshold,
+                                fr.heavy_threshold, fr.heavy_degree, bb, fkeys, active, nullptr, s);
+            lv::launch_graph_model_frontier(rowptr(), order, h_off[lv::kNumBuckets], gid, gact,
+                                            gfull, bb, fkeys, active, s);
         } else if (fr.kind == LV_FRONTIER_MODEL) {
             if (!mlp_set) return fail(LV_ERR_STATE, "frontier model not set (lv_set_model)");
             hipEvent_t t0 = tbegin();
@@ -1282,7 +1320,9 @@ struct lv_graph {
             if (incremental) step++;
         }
         if (G > 0) {
-            LV_TRY(enqueue_graph_sums(pruned, incremental));
+            // a recount walks whole graphs: the graph frontier lists them, the
+            // model's does not (then every row is walked)
+            LV_TRY(enqueue_graph_sums(pruned && fr->kind == kFrontierBatchGraphs, incremental));
         } else if (incremental) {
             hipEvent_t tq = tbegin();
             lv::launch_modularity_from_own(sigma, V, M, qpart.p, d_q, d_own_slots, s);
@@ -1795,8 +1835,11 @@ struct lv_graph {
         LV_TRY(dev_alloc(&gsums, 3 * (int64_t)G));
         LV_TRY(dev_alloc(&own_v, (int64_t)V0 + 1));
         LV_TRY(dev_alloc(&gin, (int64_t)G + 1));
+        LV_TRY(dev_alloc(&gfull, (int64_t)G + 8));
+        LV_TRY(dev_alloc(&gvscale, (int64_t)G + 1));
         h_gsums.assign(3 * (size_t)G, 0.0);
-        LV_HIP(hipHostMalloc(reinterpret_cast<void **>(&h_up), (size_t)G + 64, hipHostMallocDefault));
+        LV_HIP(hipHostMalloc(reinterpret_cast<void **>(&h_up), kUpSlots * up_stride(),
+                            hipHostMallocDefault));
         LV_HIP(hipMemcpy(vptr0, vp, sizeof(int32_t) * (G + 1), hipMemcpyHostToDevice));
         LV_HIP(hipMemcpy(gM32, m32.data(), sizeof(float) * G, hipMemcpyHostToDevice));
         LV_HIP(hipMemcpy(gden, dn.data(), sizeof(float) * G, hipMemcpyHostToDevice));
@@ -1808,7 +1851,12 @@ struct lv_graph {
     // Per graph: plain phases and aggregations exactly as lv_run would run
     // them on that graph alone; all graphs advance together, a graph whose
     // phase ended waits (outside the frontier) for the others.
-    lv_status run_batch(float min_gain, lv_batch_result *res, lv_stats *tot) {
+    // Batched runs: every graph follows lv_run's phase rule (or, with a model
+    // frontier mfr, lv_run_pruned's: its sweep i of a phase is full when
+    // i % full_every == 0 or to confirm a converged pruned sweep) on its own
+    // sums, and leaves the union's frontier when its phase ends.
+    lv_status run_batch(float min_gain, lv_batch_result *res, lv_stats *tot,
+                        const lv_frontier *mfr = nullptr) {
         hipStream_t s = stream;
         hipEvent_t e0, e1;
         LV_HIP(hipEventCreate(&e0));
@@ -1819,6 +1867,17 @@ struct lv_graph {
         LV_HIP(hipMemcpyAsync(gid, gid0, sizeof(int32_t) * V0, hipMemcpyDeviceToDevice, s));
         std::vector<uint8_t> done(G, 0), act(G, 0), changed(G, 0);
         std::vector<double> qprev(G, 0.0);
+        // pruned runs: per graph the sweep index in its phase, the pending
+        // confirmation and Q before the pruned sweep it confirms
+        std::vector<int64_t> gi(G, 0);
+        std::vector<uint8_t> gconf(G, 0), full(G, 1), full_up(G, 2);
+        std::vector<double> qbp(G, 0.0);
+        const int k_full = mfr ? (mfr->full_every > 0 ? mfr->full_every : 4) : 1;
+        lv_frontier mf{};
+        if (mfr) {
+            mf = *mfr;
+            mf.kind = kFrontierBatchModel;
+        }
         std::vector<lv_batch_result> r(G);
         for (auto &x : r) x = lv_batch_result{};
         lv_stats st{};
@@ -1839,14 +1898,33 @@ struct lv_graph {
                 changed[g] = 0;
                 nact += act[g];
                 if (act[g]) qprev[g] = graph_q(g);
+                gi[g] = 0;
+                gconf[g] = 0;
             }
-            LV_TRY(upload_flags(gact, act));
+            LV_TRY(upload_flags(gact, act, kUpAct));
             st.levels++;
             while (nact > 0) {
                 double q1 = 0.0, ms = 0.0;
                 int64_t mv = 0, ev = 0;
                 const bool all = nact == G;
-                LV_TRY(sweep(&q1, &mv, &ms, all ? nullptr : &fr, false, &ev));
+                const lv_frontier *sfr = all ? nullptr : &fr;
+                if (mfr) {
+                    bool any_pruned = false;
+                    for (int32_t g = 0; g < G; ++g) {
+                        full[g] = (gconf[g] || gi[g] % k_full == 0 || gi[g] >= kMaxPrunedSweeps)
+                                      ? 1 : 0;
+                        if (act[g] && !full[g]) any_pruned = true;
+                    }
+                    if (any_pruned) {
+                        if (full != full_up) {
+                            LV_TRY(upload_flags(gfull, full, kUpFull));
+                            full_up = full;
+                        }
+                        sfr = &mf;
+                    }
+                }
+                // the frontier features need the neighbour moves of each sweep
+                LV_TRY(sweep(&q1, &mv, &ms, sfr, mfr != nullptr, &ev));
                 st.sweeps++;
                 st.sweep_ms += ms;
                 st.evaluated += ev;
@@ -1854,19 +1932,36 @@ struct lv_graph {
                 for (int32_t g = 0; g < G; ++g) {
                     if (!act[g]) continue;
                     const double q = graph_q(g);
-                    const double gain = q - qprev[g];
                     const double moved_g = h_gsums[3 * g + 2];
                     r[g].sweeps++;
-                    if (gain > 0.0) changed[g] = 1;
-                    qprev[g] = q;
-                    // the phase rule of lv_optimise, per graph
-                    if (moved_g == 0.0 || !(gain > 0.0) || !(gain >= (double)min_gain)) {
+                    if (q - qprev[g] > 0.0) changed[g] = 1;
+                    bool end = false;
+                    if (!mfr) {
+                        // the phase rule of lv_optimise, per graph
+                        const double gain = q - qprev[g];
+                        end = moved_g == 0.0 || !(gain > 0.0) || !(gain >= (double)min_gain);
+                        qprev[g] = q;
+                    } else {
+                        // lv_optimise with a frontier, per graph
+                        const double gain = q - (gconf[g] ? qbp[g] : qprev[g]);
+                        const bool stop =
+                            moved_g == 0.0 || !(gain > 0.0) || !(gain >= (double)min_gain);
+                        st.full_sweeps += full[g] ? 1 : 0;
+                        end = (stop && full[g]) || (stop && q - qprev[g] < 0.0);
+                        if (!end) {
+                            gconf[g] = stop ? 1 : 0;
+                            if (stop) qbp[g] = qprev[g];
+                            gi[g]++;
+                        }
+                        qprev[g] = q;
+                    }
+                    if (end) {
                         act[g] = 0;
                         nact--;
                         flip = true;
                     }
                 }
-                if (flip && nact > 0) LV_TRY(upload_flags(gact, act));
+                if (flip && nact > 0) LV_TRY(upload_flags(gact, act, kUpAct));
             }
             lv::launch_compose(orig2comm, Vorig, comm, s);
             LV_LAUNCHED("compose");
@@ -1886,7 +1981,7 @@ struct lv_graph {
             if (any_now) {
                 std::vector<int32_t> vp(G + 1);
                 LV_HIP(hipMemcpyAsync(vp.data(), vptr, sizeof(int32_t) * (G + 1), hipMemcpyDeviceToHost, s));
-                LV_TRY(upload_flags(gdone, now));
+                LV_TRY(upload_flags(gdone, now, kUpDone));
                 lv::launch_graph_final(orig2comm, V0, gid0, gdone, vptr, gfinal, s);
                 LV_LAUNCHED("final assignment");
                 LV_TRY(sync());
@@ -2076,6 +2171,8 @@ struct lv_graph {
         dev_free(gsums);
         dev_free(own_v);
         dev_free(gin);
+        dev_free(gfull);
+        dev_free(gvscale);
         if (h_up) (void)hipHostFree(h_up);
         h_up = nullptr;
         dev_free(bits);
@@ -2777,6 +2874,22 @@ lv_status lv_batch_run(lv_batch *b, float min_gain, lv_batch_result *per_graph, 
     if (!b) return fail(LV_ERR_ARG, "null batch");
     LV_HIP(hipSetDevice(b->g->device));
     return b->g->run_batch(min_gain, per_graph, total);
+}
+
+lv_status lv_batch_set_model(lv_batch *b, const float *params) {
+    if (!b) return fail(LV_ERR_ARG, "null batch");
+    return lv_set_model(b->g, params);
+}
+
+lv_status lv_batch_run_pruned(lv_batch *b, float min_gain, const lv_frontier *fr,
+                              lv_batch_result *per_graph, lv_stats *total) {
+    if (!b) return fail(LV_ERR_ARG, "null batch");
+    LV_TRY(check_frontier(fr));
+    if (fr->kind != LV_FRONTIER_MODEL)
+        return fail(LV_ERR_ARG, "batched pruned runs use the model frontier");
+    if (!b->g->mlp_set) return fail(LV_ERR_STATE, "frontier model not set (lv_batch_set_model)");
+    LV_HIP(hipSetDevice(b->g->device));
+    return b->g->run_batch(min_gain, per_graph, total, fr);
 }
 
 lv_status lv_batch_assignment(lv_batch *b, int32_t *comm) {

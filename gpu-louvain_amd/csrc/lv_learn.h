@@ -35,6 +35,11 @@ struct FeatureArgs {
     float M;
     float vscale;         // V / (2M), V counting the vertices without edges
     const int32_t *ref_of;  // stored -> caller's vertex id (null: the same)
+    // batches: gid[v] = graph of v, whose M (fp32) and V/(2M) replace M and
+    // vscale (null for a single graph)
+    const int32_t *gid;
+    const float *gM;
+    const float *gvscale;
 };
 
 // Frontier selection rules.
@@ -66,6 +71,12 @@ void launch_features(const FeatureArgs &f, float *feat, hipStream_t s);
 void launch_frontier(const FeatureArgs &f, const int32_t *list, int32_t n, const float *params,
                      float threshold, float heavy_threshold, int32_t heavy_degree,
                      BucketBounds bb, int8_t *keys, uint8_t *active, float *prob, hipStream_t s);
+// Batches: the vertices of list[0..n) of the active graphs (gact) keep the
+// frontier flags the model set when their graph's sweep is pruned, and are
+// all in it when their graph's sweep is full (gfull); the others leave it.
+void launch_graph_model_frontier(const int32_t *rowptr, const int32_t *list, int32_t n,
+                                 const int32_t *gid, const uint8_t *gact, const uint8_t *gfull,
+                                 BucketBounds bb, int8_t *keys, uint8_t *active, hipStream_t s);
 // Frontier from a rule: kFrontierNeighbor = moved in the last sweep or a
 // neighbour did; kFrontierRandom = seeded Bernoulli(p) per vertex and sweep;
 // kFrontierAll = every vertex of positive degree.  Over the n vertices of

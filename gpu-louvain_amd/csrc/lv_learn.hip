@@ -45,8 +45,29 @@ __device__ __forceinline__ void features_from(int32_t deg, float kv, float scur,
 
 __device__ __forceinline__ void features_of(const FeatureArgs &f, int32_t v, float *x) {
     const uint64_t r = f.crec[f.comm[v]];
+    float M = f.M, vscale = f.vscale;
+    if (f.gid) {
+        const int32_t g = f.gid[v];
+        M = f.gM[g];
+        vscale = f.gvscale[g];
+    }
     features_from(f.rowptr[v + 1] - f.rowptr[v], (float)f.k[v], rec_sigma(r), rec_size(r),
-                  f.hist[v], f.kown[v], f.margin[v], f.nmv[v], f.M, f.vscale, x);
+                  f.hist[v], f.kown[v], f.margin[v], f.nmv[v], M, vscale, x);
+}
+
+__global__ void k_graph_model_frontier(const int32_t *__restrict__ rowptr,
+                                       const int32_t *__restrict__ list, int32_t n,
+                                       const int32_t *__restrict__ gid,
+                                       const uint8_t *__restrict__ gact,
+                                       const uint8_t *__restrict__ gfull, BucketBounds bb,
+                                       int8_t *keys, uint8_t *active) {
+    const int32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    const int32_t v = list[i];
+    const int32_t g = gid[v];
+    const bool act = gact[g] && (gfull[g] || active[v]);
+    active[v] = act ? 1 : 0;
+    keys[v] = act ? (int8_t)bucket_of(rowptr[v + 1] - rowptr[v], bb) : (int8_t)-1;
 }
 
 // Features of a vertex without edges (a singleton of total 0 that never
@@ -413,6 +434,14 @@ void launch_frontier(const FeatureArgs &f, const int32_t *list, int32_t n, const
     hipLaunchKernelGGL(k_mlp<false>, dim3((unsigned)g), dim3(256), 0, s, f, list, nullptr,
                        (int64_t)n, params, threshold, heavy_threshold, heavy_degree, bb, keys,
                        active, prob);
+}
+
+void launch_graph_model_frontier(const int32_t *rowptr, const int32_t *list, int32_t n,
+                                 const int32_t *gid, const uint8_t *gact, const uint8_t *gfull,
+                                 BucketBounds bb, int8_t *keys, uint8_t *active, hipStream_t s) {
+    if (n <= 0) return;
+    hipLaunchKernelGGL(k_graph_model_frontier, dim3(blocks_for(n, 256)), dim3(256), 0, s, rowptr,
+                       list, n, gid, gact, gfull, bb, keys, active);
 }
 
 void launch_frontier_rule(int kind, const FeatureArgs &f, const int32_t *list, int32_t n, float p,

@@ -663,6 +663,21 @@ class Batch:
                for r in res]
         return per, st.as_dict()
 
+    def set_model(self, params: np.ndarray) -> None:
+        params = np.ascontiguousarray(params, dtype=np.float32)
+        if params.size != MLP_PARAMS:
+            raise ValueError(f"model has {MLP_PARAMS} parameters")
+        _check(self._lib.lv_batch_set_model(self._h, _p(params, ctypes.c_float)))
+
+    def run_pruned(self, min_gain: float, fr: Frontier) -> tuple[list[dict], dict]:
+        """Learned-pruned batched run (lv_batch_run_pruned)."""
+        res = (BatchResult * self.G)()
+        st = Stats()
+        _check(self._lib.lv_batch_run_pruned(self._h, min_gain, ctypes.byref(fr), res, ctypes.byref(st)))
+        per = [{"modularity": r.modularity, "levels": r.levels, "sweeps": r.sweeps, "final_V": r.final_V}
+               for r in res]
+        return per, st.as_dict()
+
     def assignment(self) -> list[np.ndarray]:
         comm = np.zeros(max(self.V, 1), np.int32)
         _check(self._lib.lv_batch_assignment(self._h, _p(comm, ctypes.c_int32)))
@@ -705,6 +720,9 @@ def _batch_sigs(lib) -> None:
         "lv_batch_run": (ctypes.c_int32, [ctypes.c_void_p, ctypes.c_float, ctypes.POINTER(BatchResult),
                                           ctypes.POINTER(Stats)]),
         "lv_batch_assignment": (ctypes.c_int32, [ctypes.c_void_p, I32P]),
+        "lv_batch_set_model": (ctypes.c_int32, [ctypes.c_void_p, F32P]),
+        "lv_batch_run_pruned": (ctypes.c_int32, [ctypes.c_void_p, ctypes.c_float, ctypes.POINTER(Frontier),
+                                                 ctypes.POINTER(BatchResult), ctypes.POINTER(Stats)]),
         "lv_batch_kernel_timing": (ctypes.c_int32, [ctypes.c_void_p, ctypes.c_int32, ctypes.POINTER(KernelTime)]),
     }
     for name, (res, args) in sig.items():

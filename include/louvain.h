@@ -250,6 +250,14 @@ lv_status lv_optimise_pruned(lv_graph *g, float min_gain, const lv_frontier *fr,
  * track = 1 also counts neighbour moves for the next sweep's features. */
 lv_status lv_sweep_pruned(lv_graph *g, const lv_frontier *fr, int32_t track, double *q_after,
                           int64_t *moved, int64_t *evaluated);
+/* Learned-pruned batched run: each graph follows lv_run_pruned's rule with
+ * the model frontier fr (kind LV_FRONTIER_MODEL) on its own sweeps; the
+ * model's features use each graph's own M and vertex count, so every graph
+ * ends as its own lv_run_pruned would.  total->full_sweeps sums the graphs'
+ * full sweeps. */
+lv_status lv_batch_set_model(lv_batch *b, const float *params);
+lv_status lv_batch_run_pruned(lv_batch *b, float min_gain, const lv_frontier *fr,
+                              lv_batch_result *per_graph, lv_stats *total);
 /* One plain phase (as lv_optimise) that records, for up to max_sweeps
  * sweeps, the features before the sweep (feat[s][V][8]) and whether each
  * vertex moved in it (moved[s][V], 0/1). */

@@ -62,3 +62,43 @@ def test_batch_matches_oracle(lvlib, oracle):
         assert per[i]["final_V"] == lv
         assert np.array_equal(comms[i], o2c)
         assert abs(per[i]["modularity"] - ost["modularity"]) <= 1e-9
+
+
+@pytest.mark.parametrize("n,seed,thr", [(10, 4, 0.5), (30, 5, 0.35)])
+def test_batch_pruned_matches_single_pruned_runs(lvlib, n, seed, thr):
+    """Batched learned-pruned runs: every graph follows lv_run_pruned's rule
+    on its own sweeps, with features from its own M and vertex count."""
+    gs = _mixed(lvlib, n, seed)
+    rng = np.random.default_rng(seed)
+    params = rng.normal(0.0, 0.3, lvlib.MLP_PARAMS).astype(np.float32)
+    fr = lvlib.frontier(kind=lvlib.FRONTIER_MODEL, threshold=thr, full_every=4)
+    print(f"pruned batch of {len(gs)} graphs, V={sum(g.V for g in gs)}", flush=True)
+    with lvlib.Batch(gs) as B:
+        B.set_model(params)
+        per, tot = B.run_pruned(1e-6, fr)
+        comms = B.assignment()
+        per2, _ = B.run_pruned(1e-6, fr)
+        assert per2 == per
+    assert tot["full_sweeps"] < sum(p["sweeps"] for p in per)  # some sweeps were pruned
+    for i, g in enumerate(gs):
+        with lvlib.Graph(g) as G:
+            G.set_model(params)
+            st = G.run_pruned(1e-6, fr)
+            o2c, lv = G.assignment()
+        assert per[i]["levels"] == st["levels"], i
+        assert per[i]["sweeps"] == st["sweeps"], i
+        assert per[i]["final_V"] == lv, i
+        assert abs(per[i]["modularity"] - st["modularity"]) <= 1e-12, (i, per[i], st["modularity"])
+        assert np.array_equal(comms[i], o2c), i
+
+
+def test_batch_pruned_needs_the_model(lvlib):
+    gs = _mixed(lvlib, 4, 6)
+    with lvlib.Batch(gs) as B:
+        with pytest.raises(lvlib.LouvainError) as ei:
+            B.run_pruned(1e-6, lvlib.frontier(kind=lvlib.FRONTIER_MODEL))
+        assert ei.value.status == lvlib.LV_ERR_STATE
+        B.set_model(np.zeros(lvlib.MLP_PARAMS, np.float32))
+        with pytest.raises(lvlib.LouvainError) as ei:
+            B.run_pruned(1e-6, lvlib.frontier(kind=lvlib.FRONTIER_NEIGHBOR))
+        assert ei.value.status == lvlib.LV_ERR_ARG
