@@ -268,6 +268,12 @@ struct lv_graph {
     double *gsums = nullptr, *own_v = nullptr;
     double *gin = nullptr;  // per graph: weight inside communities (incremental sweeps)
     uint8_t *gfull = nullptr;  // per graph: this sweep is full (batched pruned runs)
+    // batched pruned runs: the vertices of positive degree of the active
+    // graphs by bucket (rebuilt when the active set changes; the model and
+    // the neighbour-move count run over it instead of the whole union)
+    int32_t *glist = nullptr, *gl_off = nullptr;
+    int32_t glist_n = 0;
+    bool glist_ok = false;
     float *gvscale = nullptr;  // per graph: V_g / (2 M_g) at the current level (features)
     std::vector<double> h_gsums;     // per graph: own, sq, moved of the last sweep
     // pinned staging of per-graph flag uploads, one slot per destination: an
@@ -798,6 +804,7 @@ struct lv_graph {
     }
 
     lv_status setup_level() {
+        glist_ok = false;
         hipStream_t s = stream;
         const int32_t *rp = rowptr();
         lv::launch_degree_keys(rp, V, bb, keys8, s);
@@ -1237,10 +1244,13 @@ struct lv_graph {
             lv::launch_graph_frontier(rowptr(), V, gid, gact, bb, fkeys, active, s);
         } else if (fr.kind == kFrontierBatchModel) {
             if (!mlp_set) return fail(LV_ERR_STATE, "frontier model not set (lv_batch_set_model)");
-            lv::launch_frontier(fa, order, h_off[lv::kNumBuckets], mlp, fr.threshold,
-                                fr.heavy_threshold, fr.heavy_degree, bb, fkeys, active, nullptr, s);
-            lv::launch_graph_model_frontier(rowptr(), order, h_off[lv::kNumBuckets], gid, gact,
-                                            gfull, bb, fkeys, active, s);
+            // the vertices outside glist (inactive graphs) keep key -1 from
+            // the list's build
+            if (!glist_ok) return fail(LV_ERR_STATE, "graph list not built");
+            lv::launch_frontier(fa, glist, glist_n, mlp, fr.threshold, fr.heavy_threshold,
+                                fr.heavy_degree, bb, fkeys, active, nullptr, s);
+            lv::launch_graph_model_frontier(rowptr(), glist, glist_n, gid, gact, gfull, bb, fkeys,
+                                            active, s);
         } else if (fr.kind == LV_FRONTIER_MODEL) {
             if (!mlp_set) return fail(LV_ERR_STATE, "frontier model not set (lv_set_model)");
             hipEvent_t t0 = tbegin();
@@ -1333,8 +1343,11 @@ struct lv_graph {
         }
         if (track) {
             lv::fill_i32(nmv, 0, V, s);
-            lv::launch_neighbor_moves(rowptr(), col(), order, h_off[lv::kNumBuckets], hist, nmv,
-                                      s);
+            // movers' neighbours lie in their own graph: the active graphs'
+            // list covers every count that changes
+            const bool gl = G > 0 && glist_ok;
+            lv::launch_neighbor_moves(rowptr(), col(), gl ? glist : order,
+                                      gl ? glist_n : h_off[lv::kNumBuckets], hist, nmv, s);
             LV_LAUNCHED("neighbour moves");
         }
         if (e1) (void)hipEventRecord(e1, s);
@@ -1803,6 +1816,23 @@ struct lv_graph {
         return LV_OK;
     }
 
+    // glist / gl_off from the uploaded gact: frontier flags and keys of the
+    // active graphs (launch_graph_frontier), partitioned by bucket.
+    lv_status build_graph_list() {
+        hipStream_t s = stream;
+        lv::launch_graph_frontier(rowptr(), V, gid, gact, bb, fkeys, active, s);
+        lv::stable_partition(fkeys, V, lv::kNumBuckets, glist, gl_off, scratch, s);
+        LV_LAUNCHED("graph list");
+        int32_t off[lv::kNumBuckets + 1];
+        LV_TRY(stage(4096));
+        LV_HIP(hipMemcpyAsync(h_stage, gl_off, sizeof(off), hipMemcpyDeviceToHost, s));
+        LV_TRY(sync());
+        std::memcpy(off, h_stage, sizeof(off));
+        glist_n = off[lv::kNumBuckets];
+        glist_ok = true;
+        return LV_OK;
+    }
+
     double graph_q(int32_t g) const {
         const double Mg = gM[g];
         return h_gsums[3 * g] / (2.0 * Mg) - h_gsums[3 * g + 1] / (4.0 * Mg * Mg);
@@ -1836,6 +1866,8 @@ struct lv_graph {
         LV_TRY(dev_alloc(&own_v, (int64_t)V0 + 1));
         LV_TRY(dev_alloc(&gin, (int64_t)G + 1));
         LV_TRY(dev_alloc(&gfull, (int64_t)G + 8));
+        LV_TRY(dev_alloc(&glist, (int64_t)V0 + 1));
+        LV_TRY(dev_alloc(&gl_off, 64));
         LV_TRY(dev_alloc(&gvscale, (int64_t)G + 1));
         h_gsums.assign(3 * (size_t)G, 0.0);
         LV_HIP(hipHostMalloc(reinterpret_cast<void **>(&h_up), kUpSlots * up_stride(),
@@ -1902,6 +1934,7 @@ struct lv_graph {
                 gconf[g] = 0;
             }
             LV_TRY(upload_flags(gact, act, kUpAct));
+            if (mfr) LV_TRY(build_graph_list());
             st.levels++;
             while (nact > 0) {
                 double q1 = 0.0, ms = 0.0;
@@ -1923,8 +1956,14 @@ struct lv_graph {
                         sfr = &mf;
                     }
                 }
-                // the frontier features need the neighbour moves of each sweep
-                LV_TRY(sweep(&q1, &mv, &ms, sfr, mfr != nullptr, &ev));
+                // the frontier features need the neighbour moves of a sweep
+                // when some graph's next sweep may be pruned (lv_optimise's
+                // rule per graph)
+                bool track = false;
+                if (mfr)
+                    for (int32_t g = 0; g < G && !track; ++g)
+                        track = act[g] && (gi[g] + 1) % k_full != 0;
+                LV_TRY(sweep(&q1, &mv, &ms, sfr, track, &ev));
                 st.sweeps++;
                 st.sweep_ms += ms;
                 st.evaluated += ev;
@@ -1961,7 +2000,10 @@ struct lv_graph {
                         flip = true;
                     }
                 }
-                if (flip && nact > 0) LV_TRY(upload_flags(gact, act, kUpAct));
+                if (flip && nact > 0) {
+                    LV_TRY(upload_flags(gact, act, kUpAct));
+                    if (mfr) LV_TRY(build_graph_list());
+                }
             }
             lv::launch_compose(orig2comm, Vorig, comm, s);
             LV_LAUNCHED("compose");
@@ -2172,6 +2214,8 @@ struct lv_graph {
         dev_free(own_v);
         dev_free(gin);
         dev_free(gfull);
+        dev_free(glist);
+        dev_free(gl_off);
         dev_free(gvscale);
         if (h_up) (void)hipHostFree(h_up);
         h_up = nullptr;

@@ -86,7 +86,11 @@ __global__ void k_hist_shift(uint8_t *hist, int32_t V) {
     if (v < V) hist[v] = (uint8_t)(hist[v] << 1);
 }
 
-// Rows of last-sweep movers: one count per row entry at the neighbour.
+// Rows of last-sweep movers: one count per row entry at the neighbour.  The
+// wave's mover rows are laid end to end (inclusive scan of their degrees) and
+// its 64 lanes take 64 consecutive entries at a time; a lane finds the row of
+// its entry by a binary search over the scan (shuffles), so a wave of short
+// rows costs about (sum of degrees) / 64 steps instead of one per mover.
 __global__ __launch_bounds__(256) void k_neighbor_moves(const int32_t *__restrict__ rowptr,
                                                         const int32_t *__restrict__ col,
                                                         const int32_t *__restrict__ list,
@@ -95,22 +99,34 @@ __global__ __launch_bounds__(256) void k_neighbor_moves(const int32_t *__restric
                                                         int32_t *nmv) {
     const int lane = threadIdx.x & 63;
     const int64_t i = (int64_t)blockIdx.x * 256 + threadIdx.x;
-    int32_t lo = 0, hi = 0;
-    bool mv = false;
+    int32_t lo = 0, deg = 0;
     if (i < n) {
         const int32_t v = list[i];
         if (hist[v] & 1u) {
             lo = rowptr[v];
-            hi = rowptr[v + 1];
-            mv = hi - lo <= kNmvMaxDegree;
+            const int32_t d = rowptr[v + 1] - lo;
+            deg = d <= kNmvMaxDegree ? d : 0;
         }
     }
-    uint64_t m = __ballot(mv);
-    while (m) {
-        const int src = __ffsll((long long)m) - 1;
-        m &= m - 1;
-        const int32_t rlo = __shfl(lo, src, 64), rhi = __shfl(hi, src, 64);
-        for (int32_t e = rlo + lane; e < rhi; e += 64) atomicAdd(&nmv[col[e]], 1);
+    int32_t inc = deg;  // at most 64 * kNmvMaxDegree
+#pragma unroll
+    for (int off = 1; off < 64; off <<= 1) {
+        const int32_t t = __shfl_up(inc, off, 64);
+        if (lane >= off) inc += t;
+    }
+    const int32_t total = __shfl(inc, 63, 64);
+    const int32_t pre = inc - deg;  // nondecreasing over the lanes
+    for (int32_t base = 0; base < total; base += 64) {
+        const int32_t t = base + lane;
+        // the last lane whose rows start at or before entry t owns it
+        int src = 0;
+#pragma unroll
+        for (int step = 32; step > 0; step >>= 1) {
+            const int c = src + step;
+            if (__shfl(pre, c, 64) <= t) src = c;
+        }
+        const int32_t slo = __shfl(lo, src, 64), spre = __shfl(pre, src, 64);
+        if (t < total) atomicAdd(&nmv[col[slo + (t - spre)]], 1);
     }
 }
 
