@@ -27,7 +27,7 @@ DEFAULT_CACHE = os.path.join(tempfile.gettempdir(), "gpulouvain_tune.json")
 
 # candidate values per option (0 = the library default)
 SPACE = {
-    "heavy_wg_max": [512, 1024, 2048],
+    "heavy_wg_max": [512, 1024, 2048, 4096],
     "chunk_edges": [256, 512, 1024],
     "part_target": [256, 512, 1024],
     "merge_limit": [1024, 1536],

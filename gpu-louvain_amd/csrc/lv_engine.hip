@@ -195,6 +195,8 @@ unsigned blocks_for(int64_t n, int per) { return (unsigned)((n + per - 1) / per)
 constexpr int kSlotChunk = 4096;  // hash slots per evaluation chunk
 constexpr int kAggChunk = 4096;   // flat member edges per aggregation chunk
 constexpr int32_t kWgSplit = 1024;  // degree bound of the lower one-workgroup bucket
+constexpr int32_t kWgMid = 2048;    // ... of the middle one (256 lanes, 4096 slots)
+constexpr int32_t kWgMax = 4096;    // ... of the upper one (1024 lanes, 8192 slots)
 constexpr int32_t kXcdMinVertices = 1 << 21;  // XCD-local row passes from this level size
 // A phase with a frontier sweeps only full after this many sweeps (bounds a
 // phase whose pruned and full sweeps keep undoing each other).
@@ -301,9 +303,9 @@ struct lv_graph {
 
     // bucketing
     lv::BucketBounds bb{};
-    int wg6_lg = 12;             // LDS slots (log2) of the one-workgroup tier (max)
     int wg_lg = 12;              // slots allocated per launch: 2 * the tier's degree bound
     int wg_lg_lo = 11;           // ... of the lower one-workgroup bucket
+    int wg_lg_hi = 13;           // ... of the upper one
     int chunk_edges = lv::kChunkEdges, part_target = lv::kPartTarget, merge_limit = lv::kMergeLimit;
     lv::AggBounds agg_bounds{};
     int8_t *keys8 = nullptr;
@@ -601,12 +603,16 @@ struct lv_graph {
             for (int t = 0; t < lv::kAggTiers - 1; ++t)
                 agg_bounds.hi[t] = std::min(agg_bounds.hi[t], opt.agg_lds_max);
         // Bucket bounds: 0..5 are the reference groups; group 6 splits into
-        // a one-workgroup tier (LDS table of 2^wg6_lg slots at load <= 1/2)
-        // and the chunked tier.
-        int wg_max = opt.heavy_wg_max > 0 ? opt.heavy_wg_max : (1 << wg6_lg) / 2;
-        wg_max = std::max(320, std::min(wg_max, (1 << wg6_lg) / 2));
-        const int32_t wg_mid = std::min(wg_max, kWgSplit);
-        const int32_t his[lv::kNumBuckets] = {4, 8, 16, 32, 84, 319, wg_mid, wg_max, 0x7fffffff};
+        // three one-workgroup tiers (LDS tables of twice the bound, load
+        // <= 1/2) and the chunked tier.  The upper tier's 96 KB table needs
+        // that much LDS per workgroup.
+        const int32_t cap = (12 << 13) <= max_lds ? kWgMax : kWgMid;
+        int wg_max = opt.heavy_wg_max > 0 ? opt.heavy_wg_max : cap;
+        wg_max = std::max(320, std::min(wg_max, cap));
+        const int32_t wg_lo = std::min(wg_max, kWgSplit);
+        const int32_t wg_mid = std::min(wg_max, kWgMid);
+        const int32_t his[] = {4, 8, 16, 32, 84, 319, wg_lo, wg_mid, wg_max, 0x7fffffff};
+        static_assert(sizeof(his) / sizeof(his[0]) == lv::kNumBuckets, "one bound per bucket");
         for (int b = 0; b < lv::kNumBuckets; ++b) bb.hi[b] = his[b];
         // tables of 2 * the bucket's degree bound: the lower bucket runs more
         // workgroups per CU
@@ -615,8 +621,9 @@ struct lv_graph {
             while ((1 << lg) < 2 * bound) ++lg;
             return lg;
         };
-        wg_lg_lo = lg_for(wg_mid);
-        wg_lg = lg_for(wg_max);
+        wg_lg_lo = lg_for(wg_lo);
+        wg_lg = lg_for(wg_mid);
+        wg_lg_hi = lg_for(wg_max);
         chunk_edges = opt.chunk_edges > 0 ? std::max(16, std::min(opt.chunk_edges, lv::kChunkEdges))
                                           : lv::kChunkEdges;
         part_target = opt.part_target > 0 ? std::max(16, opt.part_target) : lv::kPartTarget;
@@ -973,7 +980,9 @@ struct lv_graph {
         }
 
         // row-reduction segments: lanes per vertex by bucket
-        static const int lanes_lg[lv::kNumBuckets - 1] = {2, 3, 4, 5, 6, 6, 8, 8};
+        static const int lanes_lg[] = {2, 3, 4, 5, 6, 6, 8, 8, 8};
+        static_assert(sizeof(lanes_lg) / sizeof(lanes_lg[0]) == lv::kNumBuckets - 1,
+                      "one entry per bucket below the chunked tier");
         st = make_segs(h_off, nins);
         LV_TRY(qpart.reserve(2 * ((int64_t)st.total_blocks + st_light.total_blocks +
                                   8 * (int64_t)xst.total_blocks) + 2 * 256));
@@ -1078,7 +1087,9 @@ struct lv_graph {
         blk += (int32_t)((V + lv::kSqItems - 1) / lv::kSqItems);
         st_light.total_blocks = blk;
         // heavy rows: a column-block segment of a row has about deg/8 entries
-        static const int lanes_heavy[lv::kNumBuckets - XB] = {3, 4, 6, 8, 8};
+        static const int lanes_heavy[] = {3, 4, 6, 8, 8, 8};
+        static_assert(sizeof(lanes_heavy) / sizeof(lanes_heavy[0]) == lv::kNumBuckets - XB,
+                      "one entry per heavy bucket");
         int32_t j = 0;
         for (int b = XB; b < lv::kNumBuckets; ++b) {
             const int32_t cnt = h_off[b + 1] - h_off[b];
@@ -1100,7 +1111,9 @@ struct lv_graph {
     // (buckets below the chunked tier), then the chunk blocks and the
     // sum-of-squares blocks.
     lv::SegTable make_segs(const int32_t *off, int32_t nchunks) const {
-        static const int lanes[lv::kNumBuckets - 1] = {2, 3, 4, 5, 6, 6, 8, 8};
+        static const int lanes[] = {2, 3, 4, 5, 6, 6, 8, 8, 8};
+        static_assert(sizeof(lanes) / sizeof(lanes[0]) == lv::kNumBuckets - 1,
+                      "one entry per bucket below the chunked tier");
         lv::SegTable t{};
         int32_t blk = 0;
         for (int b = 0; b < lv::kNumBuckets - 1; ++b) {
@@ -1151,10 +1164,11 @@ struct lv_graph {
                 lv::launch_move_wave(b == 4 ? 256 : 512, a, list, n, s);
                 tend(t0, LV_K_MOVE_WAVE, bytes, LV_KD_MOVE_BUCKET0 + b);
                 break;
-            case 6: case 7:
+            case 6: case 7: case 8:
                 t0 = tbegin();
-                lv::launch_move_wg(b - 6, b == 6 ? wg_lg_lo : wg_lg, a, list, n, s);
-                tend(t0, LV_K_MOVE_WG, bytes, LV_KD_MOVE_BUCKET0 + b);
+                lv::launch_move_wg(b - 6, b == 6 ? wg_lg_lo : (b == 7 ? wg_lg : wg_lg_hi), a, list,
+                                   n, s);
+                tend(t0, LV_K_MOVE_WG, bytes, b == 8 ? LV_KD_MOVE_WG2 : LV_KD_MOVE_BUCKET0 + b);
                 break;
             default: {
                 // chunked tier: the family and each of its three kernels;

@@ -104,7 +104,8 @@ void launch_records_all(int32_t V, const double *sigma, const int32_t *size, uin
 // --- moves ---
 void launch_move_sub(int X, const MoveArgs &a, const int32_t *list, int32_t n, hipStream_t s);
 void launch_move_wave(int cap, const MoveArgs &a, const int32_t *list, int32_t n, hipStream_t s);
-// tier 0 / 1: the lower / upper workgroup bucket (separate kernel instances).
+// tier 0 / 1 / 2: the lower / middle / upper workgroup bucket (separate kernel
+// instances; tier 2 runs 1024 lanes per vertex).
 void launch_move_wg(int tier, int cap_log2, const MoveArgs &a, const int32_t *list, int32_t n,
                     hipStream_t s);
 void set_wg_lds_limits(int bytes);

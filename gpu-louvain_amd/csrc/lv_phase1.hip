@@ -1247,8 +1247,10 @@ void launch_move_wg(int tier, int cap_log2, const MoveArgs &a, const int32_t *li
     const unsigned g = (unsigned)(n < 1048576 ? n : 1048576);
     if (tier == 0)
         hipLaunchKernelGGL((k_move_wg<256, 0>), dim3(g), dim3(256), lds, s, a, list, n, cap_log2);
-    else
+    else if (tier == 1)
         hipLaunchKernelGGL((k_move_wg<256, 1>), dim3(g), dim3(256), lds, s, a, list, n, cap_log2);
+    else
+        hipLaunchKernelGGL((k_move_wg<1024, 2>), dim3(g), dim3(1024), lds, s, a, list, n, cap_log2);
 }
 
 void launch_chunk_aggregate(const MoveArgs &a, const ChunkedTier &t, int32_t nchunks,
@@ -1377,6 +1379,8 @@ void set_wg_lds_limits(int bytes) {
     (void)hipFuncSetAttribute(reinterpret_cast<const void *>(&k_move_wg<256, 0>),
                               hipFuncAttributeMaxDynamicSharedMemorySize, bytes);
     (void)hipFuncSetAttribute(reinterpret_cast<const void *>(&k_move_wg<256, 1>),
+                              hipFuncAttributeMaxDynamicSharedMemorySize, bytes);
+    (void)hipFuncSetAttribute(reinterpret_cast<const void *>(&k_move_wg<1024, 2>),
                               hipFuncAttributeMaxDynamicSharedMemorySize, bytes);
 }
 

@@ -40,7 +40,7 @@ typedef struct lv_graph lv_graph;
 typedef struct {
     int32_t device;          /* HIP device ordinal (default 0)                       */
     int32_t timing;          /* 1: record per-kernel HIP events (lv_kernel_time)      */
-    int32_t heavy_wg_max;    /* degree bound of the one-workgroup LDS tiers (0 = 2048) */
+    int32_t heavy_wg_max;    /* degree bound of the one-workgroup LDS tiers (0 = 4096) */
     int32_t chunk_edges;     /* edges per chunk of the chunked tier (0 = 1024)        */
     int32_t agg_lds_max;     /* summed-degree bound of LDS aggregation (0 = auto)     */
     int32_t part_target;     /* pairs per merge partition of the chunked tier (0=1024) */
@@ -105,7 +105,9 @@ typedef struct {
 #define LV_KD_MODULARITY_ROWS 11 /* modularity row pass (12 B per edge it reads + 16 V)  */
 #define LV_KD_FRONTIER 12        /* frontier model inference (43 bytes per vertex)       */
 #define LV_KD_MODULARITY_XCD 13  /* heavy-row modularity pass by column block (12 B/edge)  */
-#define LV_KD_COUNT 14
+#define LV_KD_MOVE_WG2 14        /* move kernel of execution bucket 8 (one 1024-lane
+                                    workgroup per vertex, degrees 2049..4096)           */
+#define LV_KD_COUNT 15
 
 /* ---- graph handle (utils.cu:64-106 copyStructures + initM mo.cu:420-422) ---- */
 /* Uploads the CSR (rowptr[V+1], col[E], w[E]; host pointers) and sets up the
