@@ -58,6 +58,13 @@ struct MoveArgs {
     const int32_t *gid;
     const float *gM32;
     const float *gden;
+    // bounds: candidates are evaluated best-first and skipped when a bound
+    // shows they cannot change the decision (set only when every weight is
+    // >= 0).  obs: the margin observation must be exact too (sweeps whose
+    // observations a frontier reads, lv_decide); otherwise candidates that
+    // cannot pass the move test are skipped as well.
+    int32_t bounds;
+    int32_t obs;
 };
 
 // M32 and den of vertex v's graph (.x, .y).
@@ -132,6 +139,37 @@ __device__ __forceinline__ void consider(uint64_t &best, int32_t c, double Kc, u
     best = key > best ? key : best;
 }
 
+// The float just below x (toward -inf; finite x).
+__device__ __forceinline__ float float_below(float x) {
+    const uint32_t b = __float_as_uint(x);
+    if (x > 0.0f) return __uint_as_float(b - 1u);
+    if (x == 0.0f) return -__uint_as_float(1u);
+    return __uint_as_float(b + 1u);
+}
+
+// Upper bound of the fp32 gain of a candidate with weight Kc whose total S_c
+// has not been read.  With weights >= 0, S_c >= K_c (each member's strength
+// counts its edges to v); their FP64 sums may round apart by far less than an
+// fp32 ulp, so fp32(S_c) >= float_below(fp32(K_c)).  gain_of falls as S_c
+// grows (each fp32 step is monotone and k32 >= 0), so the gain at that lower
+// bound bounds the candidate's gain from above.
+__device__ __forceinline__ float gain_bound(double Kc, float M32, float den, float k32,
+                                            float scur) {
+    const float kf = (float)Kc;
+    return gain_of(kf, M32, den, k32, scur, float_below(kf));
+}
+
+// Whether a candidate whose gain is at most U can change the decision: its
+// key must beat best0 (a candidate already evaluated, or the start key), and
+// when the margin need not be exact it must also pass decide_from's move test
+// U - kcur/M > 0 (t = (float)kcur / M32, as decide_from computes it).
+__device__ __forceinline__ bool may_win(float U, int32_t c, uint64_t best0, bool obs, float t) {
+#pragma clang fp contract(off)
+    if (pack_best(U, c) <= best0) return false;
+    if (!obs && !(U - t > 0.0f)) return false;
+    return true;
+}
+
 __device__ __forceinline__ uint32_t hash_u32(int32_t c) {
     return (uint32_t)c * 2654435761u;
 }
@@ -167,6 +205,19 @@ __device__ __forceinline__ int lds_insert(int32_t *keys, double *vals, uint32_t 
                 return (int)h;
             }
         }
+        h = (h + 1) & mask;
+    }
+}
+
+// Value under key c in an LDS table after its inserts (0 when absent); the
+// table keeps at least one empty slot.
+__device__ __forceinline__ double lds_value(const int32_t *keys, const double *vals,
+                                            uint32_t shift, uint32_t mask, int32_t c) {
+    uint32_t h = hash_u32(c) >> shift;
+    for (;;) {
+        const int32_t k = keys[h];
+        if (k == c) return vals[h];
+        if (k == -1) return 0.0;
         h = (h + 1) & mask;
     }
 }

@@ -67,6 +67,15 @@ __global__ void k_zero_u64_n(unsigned long long *p, int64_t n) {
     if (i < n) p[i] = 0ull;
 }
 
+// flag = 1 when some weight is negative or NaN (grid-stride).
+__global__ void k_weight_sign(const float *__restrict__ w, int64_t n, int32_t *flag) {
+    bool bad = false;
+    for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n;
+         i += (int64_t)gridDim.x * blockDim.x)
+        bad |= !(w[i] >= 0.0f);
+    if (__ballot(bad) && (threadIdx.x & 63) == 0) atomicOr(flag, 1);
+}
+
 // Row extents of the heavy vertices (host builds their work tables).
 __global__ void k_heavy_rows(const int32_t *__restrict__ rowptr, const int32_t *__restrict__ list,
                              int32_t n, int64_t *out) {
@@ -197,7 +206,8 @@ constexpr int kAggChunk = 4096;   // flat member edges per aggregation chunk
 constexpr int32_t kWgSplit = 1024;  // degree bound of the lower one-workgroup bucket
 constexpr int32_t kWgMid = 2048;    // ... of the middle one (256 lanes, 4096 slots)
 constexpr int32_t kWgMax = 4096;    // ... of the upper one (1024 lanes, 8192 slots)
-constexpr int32_t kXcdMinVertices = 1 << 21;  // XCD-local row passes from this level size
+constexpr int32_t kXcdMinVertices = 1 << 21;
+constexpr int32_t kBoundsMinVertices = 1 << 20;  // candidate bounds from this level size  // XCD-local row passes from this level size
 // A phase with a frontier sweeps only full after this many sweeps (bounds a
 // phase whose pruned and full sweeps keep undoing each other).
 constexpr int64_t kMaxPrunedSweeps = 4096;
@@ -297,6 +307,7 @@ struct lv_graph {
     int32_t bit_words = 0;
     uint64_t step = 0;           // group steps applied with marks
     bool own_valid = false;      // d_q[1] holds the weight inside communities
+    bool weights_nonneg = false; // every weight >= 0: candidate bounds are valid
     int64_t last_moved = -1;     // movers of the previous sweep (-1: unknown)
     int64_t last_moved_edges = -1;  // row entries of those movers
     int32_t incr_den = 2;        // incremental modularity when their rows hold <= E/incr_den entries
@@ -485,6 +496,12 @@ struct lv_graph {
         a.gid = G > 0 ? gid : nullptr;
         a.gM32 = G > 0 ? gM32 : nullptr;
         a.gden = G > 0 ? gden : nullptr;
+        // bounds pay where the records' gathers miss L2 (large levels)
+        a.bounds = weights_nonneg && (opt.candidate_bounds > 0 ||
+                                      (opt.candidate_bounds == 0 && V >= kBoundsMinVertices))
+                       ? 1
+                       : 0;
+        a.obs = 1;
         return a;
     }
 
@@ -738,6 +755,19 @@ struct lv_graph {
             LV_HIP(hipMemcpyAsync(w0, w_h, sizeof(float) * nE, hipMemcpyHostToDevice, stream));
         }
         Vref0 = nV;
+        {
+            int32_t *flag = a_scalars + 12;
+            lv::fill_i32(flag, 0, 1, stream);
+            if (nE > 0)
+                hipLaunchKernelGGL(k_weight_sign, dim3((unsigned)std::min<int64_t>(blocks_for(nE, 256), 4096)),
+                                   dim3(256), 0, stream, w0, (int64_t)nE, flag);
+            LV_LAUNCHED("weight signs");
+            LV_HIP(hipMemcpyAsync(h_stage, flag, sizeof(int32_t), hipMemcpyDeviceToHost, stream));
+            LV_HIP(hipStreamSynchronize(stream));
+            int32_t bad = 1;
+            std::memcpy(&bad, h_stage, sizeof bad);
+            weights_nonneg = bad == 0;
+        }
         LV_TRY(compact_level0(rowptr_h));
         LV_HIP(hipEventRecord(e1, stream));
         LV_HIP(hipStreamSynchronize(stream));
@@ -1139,9 +1169,13 @@ struct lv_graph {
     // ---------------------------------------------------------------- phase 1
     // Move kernels of group g; pruned: only the frontier lists (forder by
     // h_foff), heavy rows by their active flags.
-    lv_status run_group(int g, bool pruned = false) {
+    // obs: the evaluations' margins must be exact (a frontier or the caller
+    // reads them); otherwise candidates that cannot move the vertex are
+    // skipped without reading their records.
+    lv_status run_group(int g, bool pruned = false, bool obs = true) {
         hipStream_t s = stream;
         lv::MoveArgs a = move_args();
+        a.obs = obs ? 1 : 0;
         if (pruned) a.active = active;
         for (int b = 0; b < lv::kNumBuckets; ++b) {
             if (lv::kBucketGroup[b] != g) continue;
@@ -1324,7 +1358,7 @@ struct lv_graph {
             int32_t lo, hi;
             group_range(g, lo, hi);
             if (lo < 0 || hi <= lo) continue;
-            LV_TRY(run_group(g, pruned));
+            LV_TRY(run_group(g, pruned, track));
             hipEvent_t t0 = tbegin();
             lv::GroupMarks gm = marks();
             gm.rounds = rounds;

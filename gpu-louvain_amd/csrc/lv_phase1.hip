@@ -65,6 +65,22 @@ __device__ __forceinline__ void lanes_max_pair(uint64_t &best, double &bestK) {
     }
 }
 
+// Lane (of WIDTH) holding the largest k; ties to the lowest lane.
+template <int WIDTH>
+__device__ __forceinline__ int lanes_argmax(double k, int lane) {
+    int src = lane;
+#pragma unroll
+    for (int off = WIDTH / 2; off > 0; off >>= 1) {
+        const double ok = __shfl_xor(k, off, WIDTH);
+        const int os = __shfl_xor(src, off, WIDTH);
+        if (ok > k || (ok == k && os < src)) {
+            k = ok;
+            src = os;
+        }
+    }
+    return src;
+}
+
 // Decision plus dk = K[new] - K[cur] for a mover (0 otherwise), and the
 // observations the frontier model reads (move margin, own-community weight).
 __device__ __forceinline__ void write_decision(const MoveArgs &a, int32_t v, int32_t cur,
@@ -123,7 +139,24 @@ __global__ __launch_bounds__(256) void k_move_sub(MoveArgs a, const int32_t *__r
 
     uint64_t best = pack_best(0.0f, cur);
     double bestK = 0.0;
-    if (owner && c != cur) {
+    const bool cand = owner && c != cur;
+    if (a.bounds) {
+        // the heaviest candidate first; the others are read only if their
+        // bound can still change the decision
+        const int src = lanes_argmax<X>(cand ? K : -1.0, lane);
+        const uint64_t rcur = a.crec[cur];
+        const float2 md = vertex_mden(a, v);
+        const float vM32 = md.x, vden = md.y;
+        const float k32 = (float)a.k[v], scur = rec_sigma(rcur);
+        const int32_t size_cur = rec_size(rcur);
+        if (cand && lane == src)
+            consider_k(best, bestK, c, K, a.crec[c], cur, size_cur, k32, scur, vM32, vden);
+        best = __shfl(best, src, X);
+        bestK = __shfl(bestK, src, X);
+        if (cand && lane != src &&
+            may_win(gain_bound(K, vM32, vden, k32, scur), c, best, a.obs != 0, (float)kc / vM32))
+            consider_k(best, bestK, c, K, a.crec[c], cur, size_cur, k32, scur, vM32, vden);
+    } else if (cand) {
         const uint64_t rcur = a.crec[cur];
         const uint64_t rc = a.crec[c];
         const float2 md = vertex_mden(a, v);
@@ -190,21 +223,51 @@ __global__ __launch_bounds__(256) void k_move_wave(MoveArgs a, const int32_t *__
         kk[j] = K[lane + 64 * j];
         vv[j] = W[lane + 64 * j];
     }
+    double kc = 0.0;
 #pragma unroll
-    for (int j = 0; j < S; ++j) rr[j] = (kk[j] >= 0 && kk[j] != cur) ? a.crec[kk[j]] : 0ull;
+    for (int j = 0; j < S; ++j)
+        if (kk[j] == cur) kc = vv[j];
+    kc = wave_sum_f64(kc);
     uint64_t best = pack_best(0.0f, cur);
-    double bestK = 0.0, kc = 0.0;
+    double bestK = 0.0;
+    if (a.bounds) {
+        // the heaviest candidate first; the others are read only if their
+        // bound can still change the decision
+        double km = -1.0;
+        int js = 0;
 #pragma unroll
-    for (int j = 0; j < S; ++j) {
-        if (kk[j] < 0) continue;
-        if (kk[j] == cur) {
-            kc = vv[j];
-            continue;
+        for (int j = 0; j < S; ++j)
+            if (kk[j] >= 0 && kk[j] != cur && vv[j] > km) {
+                km = vv[j];
+                js = j;
+            }
+        const int src = lanes_argmax<64>(km, lane);
+        if (lane == src && km >= 0.0)
+            consider_k(best, bestK, kk[js], vv[js], a.crec[kk[js]], cur, size_cur, k32, scur, vM32,
+                       vden);
+        best = __shfl(best, src, 64);
+        bestK = __shfl(bestK, src, 64);
+        const float t = (float)kc / vM32;
+        bool ev[S];
+#pragma unroll
+        for (int j = 0; j < S; ++j)
+            ev[j] = kk[j] >= 0 && kk[j] != cur && !(lane == src && j == js) &&
+                    may_win(gain_bound(vv[j], vM32, vden, k32, scur), kk[j], best, a.obs != 0, t);
+#pragma unroll
+        for (int j = 0; j < S; ++j) rr[j] = ev[j] ? a.crec[kk[j]] : 0ull;
+#pragma unroll
+        for (int j = 0; j < S; ++j)
+            if (ev[j]) consider_k(best, bestK, kk[j], vv[j], rr[j], cur, size_cur, k32, scur, vM32, vden);
+    } else {
+#pragma unroll
+        for (int j = 0; j < S; ++j) rr[j] = (kk[j] >= 0 && kk[j] != cur) ? a.crec[kk[j]] : 0ull;
+#pragma unroll
+        for (int j = 0; j < S; ++j) {
+            if (kk[j] < 0 || kk[j] == cur) continue;
+            consider_k(best, bestK, kk[j], vv[j], rr[j], cur, size_cur, k32, scur, vM32, vden);
         }
-        consider_k(best, bestK, kk[j], vv[j], rr[j], cur, size_cur, k32, scur, vM32, vden);
     }
     lanes_max_pair<64>(best, bestK);
-    kc = wave_sum_f64(kc);
     if (lane == 0) write_decision(a, v, cur, best, bestK, kc);
 }
 
@@ -257,6 +320,71 @@ __global__ __launch_bounds__(T) void k_move_wg(MoveArgs a, const int32_t *__rest
         const uint64_t rcur = a.crec[cur];
         const float scur = rec_sigma(rcur);
         const int32_t size_cur = rec_size(rcur);
+        if (a.bounds) {
+            // kc from the own community's slot.  Each wave evaluates the
+            // heaviest candidate among its own slots first; its other slots
+            // are read only if their bound can still change the decision.
+            const double kc = lds_value(K, W, shift, mask, cur);
+            double km = -1.0;
+            int32_t ks = -1;
+            for (int i = threadIdx.x; i < Cv; i += T) {
+                const int32_t kx = K[i];
+                if (kx >= 0 && kx != cur && W[i] > km) {
+                    km = W[i];
+                    ks = i;
+                }
+            }
+            const int src = lanes_argmax<64>(km, lane);
+            uint64_t best = pack_best(0.0f, cur);
+            double bestK = 0.0;
+            if (lane == src && ks >= 0)
+                consider_k(best, bestK, K[ks], W[ks], a.crec[K[ks]], cur, size_cur, k32, scur, vM32,
+                           vden);
+            best = __shfl(best, src, 64);
+            bestK = __shfl(bestK, src, 64);
+            const int32_t slot0 = __shfl(ks, src, 64);
+            const float t = (float)kc / vM32;
+            for (int i0 = threadIdx.x; i0 < Cv; i0 += 4 * T) {
+                int32_t kk[4];
+                double vv[4];
+                bool ev[4];
+                uint64_t rr[4];
+#pragma unroll
+                for (int j = 0; j < 4; ++j) {
+                    const int i = i0 + j * T;
+                    kk[j] = i < Cv ? K[i] : -1;
+                    vv[j] = i < Cv ? W[i] : 0.0;
+                    ev[j] = kk[j] >= 0 && kk[j] != cur && i != slot0 &&
+                            may_win(gain_bound(vv[j], vM32, vden, k32, scur), kk[j], best,
+                                    a.obs != 0, t);
+                }
+#pragma unroll
+                for (int j = 0; j < 4; ++j) rr[j] = ev[j] ? a.crec[kk[j]] : 0ull;
+#pragma unroll
+                for (int j = 0; j < 4; ++j)
+                    if (ev[j])
+                        consider_k(best, bestK, kk[j], vv[j], rr[j], cur, size_cur, k32, scur,
+                                   vM32, vden);
+            }
+            lanes_max_pair<64>(best, bestK);
+            if (lane == 0) {
+                red_best[wid] = best;
+                red_K[wid] = bestK;
+            }
+            __syncthreads();
+            if (threadIdx.x == 0) {
+                uint64_t b = red_best[0];
+                double bK = red_K[0];
+                for (int w2 = 1; w2 < T / 64; ++w2)
+                    if (red_best[w2] > b) {
+                        b = red_best[w2];
+                        bK = red_K[w2];
+                    }
+                write_decision(a, v, cur, b, bK, kc);
+            }
+            __syncthreads();
+            continue;
+        }
         uint64_t best = pack_best(0.0f, cur);
         double bestK = 0.0, kc = 0.0;
         for (int i0 = threadIdx.x; i0 < Cv; i0 += 4 * T) {

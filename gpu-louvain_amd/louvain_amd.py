@@ -74,7 +74,8 @@ class Options(ctypes.Structure):
         ("xcd_rows", ctypes.c_int32),
         ("agg_mode", ctypes.c_int32),
         ("compact", ctypes.c_int32),
-        ("reserved", ctypes.c_int32 * 5),
+        ("candidate_bounds", ctypes.c_int32),
+        ("reserved", ctypes.c_int32 * 4),
     ]
 
 
@@ -329,7 +330,8 @@ class Graph:
 
     def __init__(self, g: CSR, device: int = 0, timing: bool = False, heavy_wg_max: int = 0,
                  chunk_edges: int = 0, part_target: int = 0, merge_limit: int = 0, agg_lds_max: int = 0,
-                 q_incremental: int = 0, xcd_rows: int = 0, agg_mode: int = 0, compact: int = 0):
+                 q_incremental: int = 0, xcd_rows: int = 0, agg_mode: int = 0, compact: int = 0,
+                 candidate_bounds: int = 0):
         lib = load()
         self._lib = lib
         self._rowptr = np.ascontiguousarray(g.rowptr, dtype=np.int32)
@@ -349,6 +351,7 @@ class Graph:
         opt.xcd_rows = xcd_rows
         opt.agg_mode = agg_mode
         opt.compact = compact
+        opt.candidate_bounds = candidate_bounds
         h = ctypes.c_void_p()
         col = self._col if self._col.size else np.zeros(1, np.int32)
         w = self._w if self._w.size else np.zeros(1, np.float32)

@@ -57,7 +57,12 @@ typedef struct {
                                 degree 0 (results and every returned array stay in
                                 the reference's numbering; batches always store
                                 every vertex); -1: store every vertex               */
-    int32_t reserved[5];
+    int32_t candidate_bounds; /* with weights >= 0 the move kernels may read a
+                                candidate community's record only when a bound on
+                                its gain shows it can change the decision (same
+                                decisions): 0 = on levels of >= 2^20 stored
+                                vertices, 1 = on every level, -1 = off           */
+    int32_t reserved[4];
 } lv_options;
 
 /* Run statistics (lv_run).  Times are device times from HIP events. */

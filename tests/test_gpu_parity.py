@@ -54,6 +54,12 @@ def _cases(lvlib):
     out.append(("rmat13_merge_overflow_xcd", lvlib.gen_rmat(13, 16, seed=43),
                 {"heavy_wg_max": 320, "chunk_edges": 128, "part_target": 100000, "merge_limit": 16,
                  "xcd_rows": 2}))
+    # candidate bounds forced on (by default only levels of >= 2^20 vertices)
+    out.append(("rmat14_bounds", lvlib.gen_rmat(14, 16, seed=48), {"candidate_bounds": 1}))
+    out.append(("planted_bounds", lvlib.gen_planted(8000, 40, 0.02, 0.0008, seed=49),
+                {"candidate_bounds": 1}))
+    out.append(("rmat14_tiers_bounds", lvlib.gen_rmat(14, 16, seed=41),
+                {"heavy_wg_max": 512, "chunk_edges": 128, "candidate_bounds": 1}))
     return out
 
 
@@ -67,7 +73,8 @@ def _names():
             "random_selfloops_real", "star3000", "star20000", "rmat14_chunked_tier",
             "rmat13_merge_overflow", "rmat14_incremental_q", "planted_incremental_q",
             "rmat14_xcd_q", "rmat14_chunked_tier_xcd", "rmat13_merge_overflow_xcd",
-            "rmat14_hash_agg", "planted_hash_agg"]
+            "rmat14_hash_agg", "planted_hash_agg", "rmat14_bounds", "planted_bounds",
+            "rmat14_tiers_bounds"]
 
 
 def _get(cases, name):
