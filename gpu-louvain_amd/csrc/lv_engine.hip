@@ -287,6 +287,7 @@ struct lv_graph {
     int32_t glist_n = 0;
     bool glist_ok = false;
     float *gvscale = nullptr;  // per graph: V_g / (2 M_g) at the current level (features)
+    int32_t gmaxV = 0;         // vertices of the batch's largest graph (level 0)
     std::vector<double> h_gsums;     // per graph: own, sq, moved of the last sweep
     // pinned staging of per-graph flag uploads, one slot per destination: an
     // upload may still be pending when the next one (to another array) is
@@ -496,9 +497,11 @@ struct lv_graph {
         a.gid = G > 0 ? gid : nullptr;
         a.gM32 = G > 0 ? gM32 : nullptr;
         a.gden = G > 0 ? gden : nullptr;
-        // bounds pay where the records' gathers miss L2 (large levels)
+        // bounds pay where the records' gathers miss L2: large levels (a
+        // batch's gathers stay inside one graph's vertex range)
+        const int32_t span = G > 0 ? gmaxV : V;
         a.bounds = weights_nonneg && (opt.candidate_bounds > 0 ||
-                                      (opt.candidate_bounds == 0 && V >= kBoundsMinVertices))
+                                      (opt.candidate_bounds == 0 && span >= kBoundsMinVertices))
                        ? 1
                        : 0;
         a.obs = 1;
@@ -1888,6 +1891,8 @@ struct lv_graph {
 
     lv_status setup_batch(int32_t nG, const int32_t *vp, const int32_t *rowptr_h, const float *w_h) {
         G = nG;
+        gmaxV = 0;
+        for (int32_t g = 0; g < G; ++g) gmaxV = std::max(gmaxV, vp[g + 1] - vp[g]);
         h_vptr0.assign(vp, vp + G + 1);
         gM.assign(G, 0.0);
         std::vector<float> m32(G), dn(G);

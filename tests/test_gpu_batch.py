@@ -102,3 +102,15 @@ def test_batch_pruned_needs_the_model(lvlib):
         with pytest.raises(lvlib.LouvainError) as ei:
             B.run_pruned(1e-6, lvlib.frontier(kind=lvlib.FRONTIER_NEIGHBOR))
         assert ei.value.status == lvlib.LV_ERR_ARG
+
+
+def test_batch_with_candidate_bounds_matches(lvlib):
+    """Bounds forced on in a batch (per-vertex M in the bound) decide as without."""
+    gs = _mixed(lvlib, 12, 7)
+    with lvlib.Batch(gs, candidate_bounds=-1) as A, lvlib.Batch(gs, candidate_bounds=1) as B:
+        pa, ta = A.run(1e-6)
+        pb, tb = B.run(1e-6)
+        assert pa == pb
+        ca, cb = A.assignment(), B.assignment()
+        for i in range(len(gs)):
+            assert np.array_equal(ca[i], cb[i]), i
