@@ -274,7 +274,11 @@ lv_status lv_trace_phase(lv_graph *g, float min_gain, float *feat, uint8_t *move
 lv_status lv_features(lv_graph *g, float *feat);
 /* Raw frontier observations of the current level (any may be null):
  * margin[V], kown[V] (last evaluation), hist[V] (bit 0 = moved in the last
- * sweep), nmv[V] (row entries whose neighbour moved in the last sweep). */
+ * sweep), nmv[V] (row entries whose neighbour moved in the last sweep).
+ * margin is exact after evaluations that record observations (lv_decide,
+ * tracked pruned sweeps, the sweeps of pruned runs and traces that a frontier
+ * reads); after other sweeps on levels using candidate bounds it may be lower
+ * for a vertex that stayed (a candidate that could not move it is not read). */
 lv_status lv_move_state(lv_graph *g, float *margin, float *kown, uint8_t *hist, int32_t *nmv);
 /* Frontier model of the handle (LV_MLP_PARAMS floats). */
 lv_status lv_set_model(lv_graph *g, const float *params);
