@@ -795,15 +795,17 @@ __device__ __forceinline__ bool test_bit(const uint32_t *bits, int32_t idx) {
     return (bits[idx >> 5] >> (idx & 31)) & 1u;
 }
 
+// Movers join their new community and leave the old one: sizes and totals
+// by atomics (the lanes of a wave that share the leading lane's community
+// are combined first), the move history, the mover bitmap and counts.
 __global__ __launch_bounds__(256) void k_apply_join(const int32_t *__restrict__ list, int32_t n,
                                                     const int32_t *__restrict__ rowptr,
                                                     int32_t *newc, int32_t *comm, int32_t *size,
                                                     double *sigma, const double *__restrict__ k,
-                                                    const uint64_t *__restrict__ crec,
                                                     uint32_t *movers, uint8_t *hist,
                                                     unsigned long long *moved) {
     const int32_t i = blockIdx.x * 256 + threadIdx.x;
-    bool mv = false, shared_old = false;
+    bool mv = false;
     int32_t v = 0, nc = 0, oc = 0;
     double kv = 0.0;
     if (i < n) {
@@ -812,7 +814,6 @@ __global__ __launch_bounds__(256) void k_apply_join(const int32_t *__restrict__ 
         oc = comm[v];
         if (nc != oc) {
             mv = true;
-            shared_old = rec_size(crec[oc]) > 1;
             kv = k[v];
             comm[v] = nc;
             newc[v] = oc;  // from here on newc holds the community left
@@ -820,7 +821,7 @@ __global__ __launch_bounds__(256) void k_apply_join(const int32_t *__restrict__ 
         }
     }
     wave_peel_update(mv, nc, kv, 1, size, sigma, 3);
-    wave_peel_update(mv && shared_old, oc, -kv, -1, size, sigma, 1);
+    wave_peel_update(mv, oc, -kv, -1, size, sigma, 1);
     if (movers) wave_set_bit(mv, v, movers);
     const uint64_t m = __ballot(mv);
     // moved[0] += movers, moved[1] += their row entries
@@ -833,22 +834,6 @@ __global__ __launch_bounds__(256) void k_apply_join(const int32_t *__restrict__ 
         atomicAdd(slot, (unsigned long long)__popcll(m));
         atomicAdd(slot + 1, (unsigned long long)wdeg);
     }
-}
-
-__global__ __launch_bounds__(256) void k_apply_leave(const int32_t *__restrict__ list, int32_t n,
-                                                     const int32_t *__restrict__ newc,
-                                                     const int32_t *__restrict__ comm,
-                                                     int32_t *size, double *sigma,
-                                                     const double *__restrict__ k,
-                                                     const uint64_t *__restrict__ crec) {
-    const int32_t i = blockIdx.x * 256 + threadIdx.x;
-    if (i >= n) return;
-    const int32_t v = list[i];
-    const int32_t oc = newc[v];
-    if (oc == comm[v]) return;
-    if (rec_size(crec[oc]) > 1) return;  // shared community: updated by atomics
-    size[oc] -= 1;
-    sigma[oc] -= k[v];
 }
 
 constexpr int kClearWords = 4096;
@@ -1423,10 +1408,8 @@ void launch_apply(const MoveArgs &a, const int32_t *list, int32_t n, const int32
     if (n <= 0 && !gm) return;
     if (n > 0) {
         hipLaunchKernelGGL(k_apply_join, dim3(blocks_for(n, 256)), dim3(256), 0, s, list, n,
-                           a.rowptr, a.newc, comm, size, sigma, a.k, crec,
-                           gm ? gm->movers : nullptr, hist, moved);
-        hipLaunchKernelGGL(k_apply_leave, dim3(blocks_for(n, 256)), dim3(256), 0, s, list, n,
-                           a.newc, comm, size, sigma, a.k, crec);
+                           a.rowptr, a.newc, comm, size, sigma, a.k, gm ? gm->movers : nullptr,
+                           hist, moved);
     }
     const int32_t corr = gm ? gst->total_blocks : 0;
     const int32_t fix = n > 0 ? (int32_t)blocks_for(n, kRowThreads) : 0;
