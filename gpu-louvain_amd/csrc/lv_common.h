@@ -17,9 +17,9 @@ namespace lv {
 // LDS).  Splitting a group changes only the kernel mapping: decisions inside
 // a group are simultaneous.
 // ---------------------------------------------------------------------
-constexpr int kNumBuckets = 10;
+constexpr int kNumBuckets = 12;
 constexpr int kNumGroups = 7;
-constexpr int kBucketGroup[] = {0, 1, 2, 3, 4, 5, 6, 6, 6, 6};
+constexpr int kBucketGroup[] = {0, 1, 2, 3, 4, 5, 5, 6, 6, 6, 6, 6};
 static_assert(sizeof(kBucketGroup) / sizeof(kBucketGroup[0]) == kNumBuckets,
               "one group per bucket");
 constexpr int kGroupBound[kNumGroups + 1] = {0, 4, 8, 16, 32, 84, 319, 0x7fffffff};

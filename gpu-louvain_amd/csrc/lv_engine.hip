@@ -203,9 +203,13 @@ unsigned blocks_for(int64_t n, int per) { return (unsigned)((n + per - 1) / per)
 
 constexpr int kSlotChunk = 4096;  // hash slots per evaluation chunk
 constexpr int kAggChunk = 4096;   // flat member edges per aggregation chunk
-constexpr int32_t kWgSplit = 1024;  // degree bound of the lower one-workgroup bucket
-constexpr int32_t kWgMid = 2048;    // ... of the middle one (256 lanes, 4096 slots)
-constexpr int32_t kWgMax = 4096;    // ... of the upper one (1024 lanes, 8192 slots)
+// Degree bounds of the one-workgroup buckets (tables of twice the bound):
+constexpr int32_t kWgA = 512;       // 256 lanes, 1024 slots
+constexpr int32_t kWgSplit = 1024;  // 256 lanes, 2048 slots
+constexpr int32_t kWgMid = 2048;    // 256 lanes, 4096 slots
+constexpr int32_t kWgMax = 4096;    // 1024 lanes, 8192 slots
+// Group 5 (degrees 85..319) runs 256-slot wave tables up to this degree.
+constexpr int32_t kWave256Hi = 128;
 constexpr int32_t kXcdMinVertices = 1 << 21;
 constexpr int32_t kBoundsMinVertices = 1 << 20;  // candidate bounds from this level size  // XCD-local row passes from this level size
 // A phase with a frontier sweeps only full after this many sweeps (bounds a
@@ -315,9 +319,8 @@ struct lv_graph {
 
     // bucketing
     lv::BucketBounds bb{};
-    int wg_lg = 12;              // slots allocated per launch: 2 * the tier's degree bound
-    int wg_lg_lo = 11;           // ... of the lower one-workgroup bucket
-    int wg_lg_hi = 13;           // ... of the upper one
+    // slots (log2) allocated per launch of each one-workgroup bucket
+    int wg_lg_a = 10, wg_lg_lo = 11, wg_lg = 12, wg_lg_hi = 13;
     int chunk_edges = lv::kChunkEdges, part_target = lv::kPartTarget, merge_limit = lv::kMergeLimit;
     lv::AggBounds agg_bounds{};
     int8_t *keys8 = nullptr;
@@ -629,9 +632,11 @@ struct lv_graph {
         const int32_t cap = (12 << 13) <= max_lds ? kWgMax : kWgMid;
         int wg_max = opt.heavy_wg_max > 0 ? opt.heavy_wg_max : cap;
         wg_max = std::max(320, std::min(wg_max, cap));
+        const int32_t wg_a = std::min(wg_max, kWgA);
         const int32_t wg_lo = std::min(wg_max, kWgSplit);
         const int32_t wg_mid = std::min(wg_max, kWgMid);
-        const int32_t his[] = {4, 8, 16, 32, 84, 319, wg_lo, wg_mid, wg_max, 0x7fffffff};
+        const int32_t his[] = {4,   8,    16,    32,     84,     kWave256Hi,
+                               319, wg_a, wg_lo, wg_mid, wg_max, 0x7fffffff};
         static_assert(sizeof(his) / sizeof(his[0]) == lv::kNumBuckets, "one bound per bucket");
         for (int b = 0; b < lv::kNumBuckets; ++b) bb.hi[b] = his[b];
         // tables of 2 * the bucket's degree bound: the lower bucket runs more
@@ -641,6 +646,7 @@ struct lv_graph {
             while ((1 << lg) < 2 * bound) ++lg;
             return lg;
         };
+        wg_lg_a = lg_for(wg_a);
         wg_lg_lo = lg_for(wg_lo);
         wg_lg = lg_for(wg_mid);
         wg_lg_hi = lg_for(wg_max);
@@ -1013,7 +1019,7 @@ struct lv_graph {
         }
 
         // row-reduction segments: lanes per vertex by bucket
-        static const int lanes_lg[] = {2, 3, 4, 5, 6, 6, 8, 8, 8};
+        static const int lanes_lg[] = {2, 3, 4, 5, 6, 6, 6, 8, 8, 8, 8};
         static_assert(sizeof(lanes_lg) / sizeof(lanes_lg[0]) == lv::kNumBuckets - 1,
                       "one entry per bucket below the chunked tier");
         st = make_segs(h_off, nins);
@@ -1120,7 +1126,7 @@ struct lv_graph {
         blk += (int32_t)((V + lv::kSqItems - 1) / lv::kSqItems);
         st_light.total_blocks = blk;
         // heavy rows: a column-block segment of a row has about deg/8 entries
-        static const int lanes_heavy[] = {3, 4, 6, 8, 8, 8};
+        static const int lanes_heavy[] = {3, 4, 4, 6, 6, 8, 8, 8};
         static_assert(sizeof(lanes_heavy) / sizeof(lanes_heavy[0]) == lv::kNumBuckets - XB,
                       "one entry per heavy bucket");
         int32_t j = 0;
@@ -1144,7 +1150,7 @@ struct lv_graph {
     // (buckets below the chunked tier), then the chunk blocks and the
     // sum-of-squares blocks.
     lv::SegTable make_segs(const int32_t *off, int32_t nchunks) const {
-        static const int lanes[] = {2, 3, 4, 5, 6, 6, 8, 8, 8};
+        static const int lanes[] = {2, 3, 4, 5, 6, 6, 6, 8, 8, 8, 8};
         static_assert(sizeof(lanes) / sizeof(lanes[0]) == lv::kNumBuckets - 1,
                       "one entry per bucket below the chunked tier");
         lv::SegTable t{};
@@ -1196,17 +1202,20 @@ struct lv_graph {
                 lv::launch_move_sub(4 << b, a, list, n, s);
                 tend(t0, LV_K_MOVE_SUB, bytes, LV_KD_MOVE_BUCKET0 + b);
                 break;
-            case 4: case 5:
+            case 4: case 5: case 6:
                 t0 = tbegin();
-                lv::launch_move_wave(b == 4 ? 256 : 512, a, list, n, s);
+                lv::launch_move_wave(b == 6 ? 512 : 256, b - 4, a, list, n, s);
                 tend(t0, LV_K_MOVE_WAVE, bytes, LV_KD_MOVE_BUCKET0 + b);
                 break;
-            case 6: case 7: case 8:
+            case 7: case 8: case 9: case 10: {
+                static const int kd[4] = {LV_KD_MOVE_BUCKET0 + 7, LV_KD_MOVE_BUCKET8,
+                                          LV_KD_MOVE_BUCKET8 + 1, LV_KD_MOVE_BUCKET8 + 2};
+                const int lg[4] = {wg_lg_a, wg_lg_lo, wg_lg, wg_lg_hi};
                 t0 = tbegin();
-                lv::launch_move_wg(b - 6, b == 6 ? wg_lg_lo : (b == 7 ? wg_lg : wg_lg_hi), a, list,
-                                   n, s);
-                tend(t0, LV_K_MOVE_WG, bytes, b == 8 ? LV_KD_MOVE_WG2 : LV_KD_MOVE_BUCKET0 + b);
+                lv::launch_move_wg(b - 7, lg[b - 7], a, list, n, s);
+                tend(t0, LV_K_MOVE_WG, bytes, kd[b - 7]);
                 break;
+            }
             default: {
                 // chunked tier: the family and each of its three kernels;
                 // stage A streams the heavy rows (col, w, comm[col])

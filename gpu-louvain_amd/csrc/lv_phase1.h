@@ -103,9 +103,11 @@ void launch_records_all(int32_t V, const double *sigma, const int32_t *size, uin
 
 // --- moves ---
 void launch_move_sub(int X, const MoveArgs &a, const int32_t *list, int32_t n, hipStream_t s);
-void launch_move_wave(int cap, const MoveArgs &a, const int32_t *list, int32_t n, hipStream_t s);
-// tier 0 / 1 / 2: the lower / middle / upper workgroup bucket (separate kernel
-// instances; tier 2 runs 1024 lanes per vertex).
+// cap: LDS slots per wave (256 or 512); inst names the 256-slot instance.
+void launch_move_wave(int cap, int inst, const MoveArgs &a, const int32_t *list, int32_t n,
+                      hipStream_t s);
+// tier 0..3: the workgroup buckets in degree order (separate kernel
+// instances; tier 3 runs 1024 lanes per vertex).
 void launch_move_wg(int tier, int cap_log2, const MoveArgs &a, const int32_t *list, int32_t n,
                     hipStream_t s);
 void set_wg_lds_limits(int bytes);

@@ -14,10 +14,11 @@
 //                   per-community sums by an in-register all-pairs pass over
 //                   the X lanes (row order, so sums are bitwise the
 //                   sequential ones), argmax by xor-shuffle of packed keys.
-//   k_move_wave<C>  one wavefront per vertex, C-slot LDS hash per wave.
+//   k_move_wave<C,G> one wavefront per vertex, C-slot LDS hash per wave
+//                   (G names the bucket's instance).
 //   k_move_wg<T,i>  one T-lane workgroup per vertex, LDS hash sized per launch
-//                   (i = 0, 1: the two workgroup buckets).
-//   k_chunk_*       rows beyond one workgroup: stage A aggregates 2048-edge
+//                   (i = 0..3: the workgroup buckets in degree order).
+//   k_chunk_*       rows beyond one workgroup: stage A aggregates 1024-edge
 //                   chunks in LDS and buckets the distinct pairs by hash
 //                   partition; stage B merges each (vertex, partition) in
 //                   LDS, evaluates gains and combines partitions through a
@@ -168,7 +169,8 @@ __global__ __launch_bounds__(256) void k_move_sub(MoveArgs a, const int32_t *__r
     if (lane == 0) write_decision(a, v, cur, best, bestK, kc);
 }
 
-template <int C>
+// C: LDS slots per wave; G only names the instance (one per bucket).
+template <int C, int G>
 __global__ __launch_bounds__(256) void k_move_wave(MoveArgs a, const int32_t *__restrict__ list,
                                                    int32_t n) {
     constexpr int LOG2C = ilog2_c(C);
@@ -272,9 +274,12 @@ __global__ __launch_bounds__(256) void k_move_wave(MoveArgs a, const int32_t *__
 }
 
 // TIER only names the instance (one per workgroup bucket) for profiles.
+// 256-lane instances keep to 64 VGPRs (8 waves per SIMD): with their
+// smaller tables LDS admits that many.
 template <int T, int TIER>
-__global__ __launch_bounds__(T) void k_move_wg(MoveArgs a, const int32_t *__restrict__ list,
-                                               int32_t n, int cap_log2) {
+__global__ __launch_bounds__(T, T == 256 ? 8 : 4) void k_move_wg(MoveArgs a,
+                                                                const int32_t *__restrict__ list,
+                                                                int32_t n, int cap_log2) {
     extern __shared__ double dyn_lds[];
     const int C = 1 << cap_log2;  // allocation: W[C] then K[C]
     double *W = dyn_lds;
@@ -1344,13 +1349,16 @@ void launch_move_sub(int X, const MoveArgs &a, const int32_t *list, int32_t n, h
     }
 }
 
-void launch_move_wave(int cap, const MoveArgs &a, const int32_t *list, int32_t n, hipStream_t s) {
+void launch_move_wave(int cap, int inst, const MoveArgs &a, const int32_t *list, int32_t n,
+                      hipStream_t s) {
     if (n <= 0) return;
     const unsigned g = blocks_for(n, 4);
-    if (cap <= 256)
-        hipLaunchKernelGGL(k_move_wave<256>, dim3(g), dim3(256), 0, s, a, list, n);
+    if (cap > 256)
+        hipLaunchKernelGGL((k_move_wave<512, 0>), dim3(g), dim3(256), 0, s, a, list, n);
+    else if (inst == 0)
+        hipLaunchKernelGGL((k_move_wave<256, 0>), dim3(g), dim3(256), 0, s, a, list, n);
     else
-        hipLaunchKernelGGL(k_move_wave<512>, dim3(g), dim3(256), 0, s, a, list, n);
+        hipLaunchKernelGGL((k_move_wave<256, 1>), dim3(g), dim3(256), 0, s, a, list, n);
 }
 
 void launch_move_wg(int tier, int cap_log2, const MoveArgs &a, const int32_t *list, int32_t n,
@@ -1362,8 +1370,10 @@ void launch_move_wg(int tier, int cap_log2, const MoveArgs &a, const int32_t *li
         hipLaunchKernelGGL((k_move_wg<256, 0>), dim3(g), dim3(256), lds, s, a, list, n, cap_log2);
     else if (tier == 1)
         hipLaunchKernelGGL((k_move_wg<256, 1>), dim3(g), dim3(256), lds, s, a, list, n, cap_log2);
+    else if (tier == 2)
+        hipLaunchKernelGGL((k_move_wg<256, 2>), dim3(g), dim3(256), lds, s, a, list, n, cap_log2);
     else
-        hipLaunchKernelGGL((k_move_wg<1024, 2>), dim3(g), dim3(1024), lds, s, a, list, n, cap_log2);
+        hipLaunchKernelGGL((k_move_wg<1024, 3>), dim3(g), dim3(1024), lds, s, a, list, n, cap_log2);
 }
 
 void launch_chunk_aggregate(const MoveArgs &a, const ChunkedTier &t, int32_t nchunks,
@@ -1491,7 +1501,9 @@ void set_wg_lds_limits(int bytes) {
                               hipFuncAttributeMaxDynamicSharedMemorySize, bytes);
     (void)hipFuncSetAttribute(reinterpret_cast<const void *>(&k_move_wg<256, 1>),
                               hipFuncAttributeMaxDynamicSharedMemorySize, bytes);
-    (void)hipFuncSetAttribute(reinterpret_cast<const void *>(&k_move_wg<1024, 2>),
+    (void)hipFuncSetAttribute(reinterpret_cast<const void *>(&k_move_wg<256, 2>),
+                              hipFuncAttributeMaxDynamicSharedMemorySize, bytes);
+    (void)hipFuncSetAttribute(reinterpret_cast<const void *>(&k_move_wg<1024, 3>),
                               hipFuncAttributeMaxDynamicSharedMemorySize, bytes);
 }
 

@@ -110,9 +110,8 @@ typedef struct {
 #define LV_KD_MODULARITY_ROWS 11 /* modularity row pass (12 B per edge it reads + 16 V)  */
 #define LV_KD_FRONTIER 12        /* frontier model inference (43 bytes per vertex)       */
 #define LV_KD_MODULARITY_XCD 13  /* heavy-row modularity pass by column block (12 B/edge)  */
-#define LV_KD_MOVE_WG2 14        /* move kernel of execution bucket 8 (one 1024-lane
-                                    workgroup per vertex, degrees 2049..4096)           */
-#define LV_KD_COUNT 15
+#define LV_KD_MOVE_BUCKET8 14    /* +b-8, b = 8..10: move kernel of execution bucket b  */
+#define LV_KD_COUNT 17
 
 /* ---- graph handle (utils.cu:64-106 copyStructures + initM mo.cu:420-422) ---- */
 /* Uploads the CSR (rowptr[V+1], col[E], w[E]; host pointers) and sets up the
