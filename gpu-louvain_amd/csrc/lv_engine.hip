@@ -599,6 +599,9 @@ struct lv_graph {
         t.n_chunk_order = n_corder;
         t.n_item_order = n_iorder;
         t.range_blk = tier_xcd ? (V + 7) / 8 : 0;
+        int clg = 6;  // table_log2's floor
+        while ((1 << clg) < 2 * chunk_edges && clg < lv::kChunkCapLog2) ++clg;
+        t.chunk_cap_log2 = clg;
         return t;
     }
 

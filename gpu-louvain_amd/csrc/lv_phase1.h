@@ -69,6 +69,7 @@ struct ChunkedTier {
     const int32_t *item_order;
     int32_t n_chunk_order, n_item_order;
     int32_t range_blk;    // > 0: partitions by community-id range (see tier_part)
+    int32_t chunk_cap_log2;  // stage A table slots (log2): twice the chunk size
 };
 
 // Segment of the vertex list processed with a fixed lane count per vertex

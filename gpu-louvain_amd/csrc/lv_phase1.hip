@@ -464,16 +464,18 @@ __device__ __forceinline__ int table_log2(int64_t n, int cap_lg) {
 }
 
 __global__ __launch_bounds__(256) void k_chunk_aggregate(MoveArgs a, ChunkedTier t) {
-    constexpr int C = 1 << kChunkCapLog2;
-    __shared__ int32_t keys[C];
-    __shared__ double vals[C];
-    __shared__ int32_t pcnt[kMaxParts];
+    // dynamic LDS sized by the chunk size: vals[C], keys[C], pcnt[kMaxParts]
+    extern __shared__ double agg_lds[];
+    const int C = 1 << t.chunk_cap_log2;
+    double *vals = agg_lds;
+    int32_t *keys = reinterpret_cast<int32_t *>(agg_lds + C);
+    int32_t *pcnt = keys + C;
     const int32_t ci = t.chunk_order ? t.chunk_order[blockIdx.x] : (int32_t)blockIdx.x;
     if (ci < 0) return;  // padding of the XCD dispatch order
     const Chunk ch = t.chunks[ci];
     if (a.active && !a.active[ch.v]) return;  // not in this sweep's frontier
     const int P = ch.P;
-    const int cl = table_log2(ch.hi - ch.lo, kChunkCapLog2);
+    const int cl = table_log2(ch.hi - ch.lo, t.chunk_cap_log2);
     const int Cv = 1 << cl;
     for (int i = threadIdx.x; i < Cv; i += 256) {
         keys[i] = -1;
@@ -1380,7 +1382,8 @@ void launch_chunk_aggregate(const MoveArgs &a, const ChunkedTier &t, int32_t nch
                             hipStream_t s) {
     const int32_t g = t.chunk_order ? t.n_chunk_order : nchunks;
     if (g <= 0) return;
-    hipLaunchKernelGGL(k_chunk_aggregate, dim3(g), dim3(256), 0, s, a, t);
+    const size_t lds = ((size_t)12 << t.chunk_cap_log2) + sizeof(int32_t) * kMaxParts;
+    hipLaunchKernelGGL(k_chunk_aggregate, dim3(g), dim3(256), lds, s, a, t);
 }
 
 void launch_chunk_merge(const MoveArgs &a, const ChunkedTier &t, int32_t nitems, hipStream_t s) {
