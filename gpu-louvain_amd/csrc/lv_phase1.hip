@@ -277,7 +277,7 @@ __global__ __launch_bounds__(256) void k_move_wave(MoveArgs a, const int32_t *__
 // 256-lane instances keep to 64 VGPRs (8 waves per SIMD): with their
 // smaller tables LDS admits that many.
 template <int T, int TIER>
-__global__ __launch_bounds__(T, T == 256 ? 8 : 4) void k_move_wg(MoveArgs a,
+__global__ __launch_bounds__(T, T <= 256 ? 8 : 4) void k_move_wg(MoveArgs a,
                                                                 const int32_t *__restrict__ list,
                                                                 int32_t n, int cap_log2) {
     extern __shared__ double dyn_lds[];
@@ -1369,7 +1369,7 @@ void launch_move_wg(int tier, int cap_log2, const MoveArgs &a, const int32_t *li
     const size_t lds = (size_t)12 << cap_log2;
     const unsigned g = (unsigned)(n < 1048576 ? n : 1048576);
     if (tier == 0)
-        hipLaunchKernelGGL((k_move_wg<256, 0>), dim3(g), dim3(256), lds, s, a, list, n, cap_log2);
+        hipLaunchKernelGGL((k_move_wg<128, 0>), dim3(g), dim3(128), lds, s, a, list, n, cap_log2);
     else if (tier == 1)
         hipLaunchKernelGGL((k_move_wg<256, 1>), dim3(g), dim3(256), lds, s, a, list, n, cap_log2);
     else if (tier == 2)
@@ -1500,7 +1500,7 @@ void launch_compose(int32_t *orig2comm, int32_t V0, const int32_t *map, hipStrea
 }
 
 void set_wg_lds_limits(int bytes) {
-    (void)hipFuncSetAttribute(reinterpret_cast<const void *>(&k_move_wg<256, 0>),
+    (void)hipFuncSetAttribute(reinterpret_cast<const void *>(&k_move_wg<128, 0>),
                               hipFuncAttributeMaxDynamicSharedMemorySize, bytes);
     (void)hipFuncSetAttribute(reinterpret_cast<const void *>(&k_move_wg<256, 1>),
                               hipFuncAttributeMaxDynamicSharedMemorySize, bytes);
