@@ -209,6 +209,41 @@ __device__ __forceinline__ int lds_insert(int32_t *keys, double *vals, uint32_t 
     }
 }
 
+// Tables of any size n: c's first slot is the high word of hash * n (for a
+// power of two n the same slot as hash >> (32 - log2 n)); linear probing.
+__device__ __forceinline__ uint32_t slot_of(int32_t c, uint32_t n) {
+    return (uint32_t)(((uint64_t)hash_u32(c) * n) >> 32);
+}
+__device__ __forceinline__ void lds_insert_n(int32_t *keys, double *vals, uint32_t n, int32_t c,
+                                             double w) {
+    uint32_t h = slot_of(c, n);
+    for (;;) {
+        int32_t k = __hip_atomic_load(&keys[h], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
+        if (k == c) {
+            atomicAdd(&vals[h], w);
+            return;
+        }
+        if (k == -1) {
+            int32_t old = atomicCAS(&keys[h], -1, c);
+            if (old == -1 || old == c) {
+                atomicAdd(&vals[h], w);
+                return;
+            }
+        }
+        h = h + 1 == n ? 0u : h + 1;
+    }
+}
+__device__ __forceinline__ double lds_value_n(const int32_t *keys, const double *vals, uint32_t n,
+                                              int32_t c) {
+    uint32_t h = slot_of(c, n);
+    for (;;) {
+        const int32_t k = keys[h];
+        if (k == c) return vals[h];
+        if (k == -1) return 0.0;
+        h = h + 1 == n ? 0u : h + 1;
+    }
+}
+
 // Value under key c in an LDS table after its inserts (0 when absent); the
 // table keeps at least one empty slot.
 __device__ __forceinline__ double lds_value(const int32_t *keys, const double *vals,

@@ -203,11 +203,11 @@ unsigned blocks_for(int64_t n, int per) { return (unsigned)((n + per - 1) / per)
 
 constexpr int kSlotChunk = 4096;  // hash slots per evaluation chunk
 constexpr int kAggChunk = 4096;   // flat member edges per aggregation chunk
-// Degree bounds of the one-workgroup buckets (tables of twice the bound):
-constexpr int32_t kWgA = 512;       // 256 lanes, 1024 slots
-constexpr int32_t kWgSplit = 1024;  // 256 lanes, 2048 slots
-constexpr int32_t kWgMid = 2048;    // 256 lanes, 4096 slots
-constexpr int32_t kWgMax = 4096;    // 1024 lanes, 8192 slots
+// Degree bounds of the one-workgroup buckets (tables of 1.5 x the bound):
+constexpr int32_t kWgA = 512;       // 128 lanes, 768 slots
+constexpr int32_t kWgSplit = 1024;  // 256 lanes, 1536 slots
+constexpr int32_t kWgMid = 2048;    // 256 lanes, 3072 slots
+constexpr int32_t kWgMax = 4096;    // 1024 lanes, 8192 slots (2 x)
 // Group 5 (degrees 85..319) runs 256-slot wave tables up to this degree.
 constexpr int32_t kWave256Hi = 128;
 constexpr int32_t kXcdMinVertices = 1 << 21;
@@ -319,8 +319,8 @@ struct lv_graph {
 
     // bucketing
     lv::BucketBounds bb{};
-    // slots (log2) allocated per launch of each one-workgroup bucket
-    int wg_lg_a = 10, wg_lg_lo = 11, wg_lg = 12, wg_lg_hi = 13;
+    // LDS slots allocated per launch of each one-workgroup bucket
+    int wg_slots[4] = {768, 1536, 3072, 8192};
     int chunk_edges = lv::kChunkEdges, part_target = lv::kPartTarget, merge_limit = lv::kMergeLimit;
     lv::AggBounds agg_bounds{};
     int8_t *keys8 = nullptr;
@@ -632,7 +632,7 @@ struct lv_graph {
         // three one-workgroup tiers (LDS tables of twice the bound, load
         // <= 1/2) and the chunked tier.  The upper tier's 96 KB table needs
         // that much LDS per workgroup.
-        const int32_t cap = (12 << 13) <= max_lds ? kWgMax : kWgMid;
+        const int32_t cap = 12 * 2 * kWgMax <= max_lds ? kWgMax : kWgMid;
         int wg_max = opt.heavy_wg_max > 0 ? opt.heavy_wg_max : cap;
         wg_max = std::max(320, std::min(wg_max, cap));
         const int32_t wg_a = std::min(wg_max, kWgA);
@@ -642,17 +642,15 @@ struct lv_graph {
                                319, wg_a, wg_lo, wg_mid, wg_max, 0x7fffffff};
         static_assert(sizeof(his) / sizeof(his[0]) == lv::kNumBuckets, "one bound per bucket");
         for (int b = 0; b < lv::kNumBuckets; ++b) bb.hi[b] = his[b];
-        // tables of 2 * the bucket's degree bound: the lower bucket runs more
-        // workgroups per CU
-        auto lg_for = [](int32_t bound) {
-            int lg = 9;
-            while ((1 << lg) < 2 * bound) ++lg;
-            return lg;
+        // tables of 1.5 x the bucket's degree bound (load <= 2/3), in 64-slot
+        // steps and at least 512: smaller tables, more rows in flight per CU
+        auto slots_for = [](int32_t bound) {
+            return std::max(512, ((bound + bound / 2) + 63) & ~63);
         };
-        wg_lg_a = lg_for(wg_a);
-        wg_lg_lo = lg_for(wg_lo);
-        wg_lg = lg_for(wg_mid);
-        wg_lg_hi = lg_for(wg_max);
+        wg_slots[0] = slots_for(wg_a);
+        wg_slots[1] = slots_for(wg_lo);
+        wg_slots[2] = slots_for(wg_mid);
+        wg_slots[3] = std::max(512, 2 * wg_max);
         chunk_edges = opt.chunk_edges > 0 ? std::max(16, std::min(opt.chunk_edges, lv::kChunkEdges))
                                           : lv::kChunkEdges;
         part_target = opt.part_target > 0 ? std::max(16, opt.part_target) : lv::kPartTarget;
@@ -1213,9 +1211,8 @@ struct lv_graph {
             case 7: case 8: case 9: case 10: {
                 static const int kd[4] = {LV_KD_MOVE_BUCKET0 + 7, LV_KD_MOVE_BUCKET8,
                                           LV_KD_MOVE_BUCKET8 + 1, LV_KD_MOVE_BUCKET8 + 2};
-                const int lg[4] = {wg_lg_a, wg_lg_lo, wg_lg, wg_lg_hi};
                 t0 = tbegin();
-                lv::launch_move_wg(b - 7, lg[b - 7], a, list, n, s);
+                lv::launch_move_wg(b - 7, wg_slots[b - 7], a, list, n, s);
                 tend(t0, LV_K_MOVE_WG, bytes, kd[b - 7]);
                 break;
             }

@@ -109,7 +109,8 @@ void launch_move_wave(int cap, int inst, const MoveArgs &a, const int32_t *list,
                       hipStream_t s);
 // tier 0..3: the workgroup buckets in degree order (separate kernel
 // instances; tier 3 runs 1024 lanes per vertex).
-void launch_move_wg(int tier, int cap_log2, const MoveArgs &a, const int32_t *list, int32_t n,
+// slots: LDS table entries allocated per workgroup (12 bytes each).
+void launch_move_wg(int tier, int slots, const MoveArgs &a, const int32_t *list, int32_t n,
                     hipStream_t s);
 void set_wg_lds_limits(int bytes);
 void launch_move_chunked(const MoveArgs &a, const ChunkedTier &t, int32_t nchunks,

@@ -279,9 +279,9 @@ __global__ __launch_bounds__(256) void k_move_wave(MoveArgs a, const int32_t *__
 template <int T, int TIER>
 __global__ __launch_bounds__(T, T <= 256 ? 8 : 4) void k_move_wg(MoveArgs a,
                                                                 const int32_t *__restrict__ list,
-                                                                int32_t n, int cap_log2) {
+                                                                int32_t n, int cap) {
     extern __shared__ double dyn_lds[];
-    const int C = 1 << cap_log2;  // allocation: W[C] then K[C]
+    const int C = cap;  // allocation (slots): W[C] then K[C]
     double *W = dyn_lds;
     int32_t *K = reinterpret_cast<int32_t *>(dyn_lds + C);
     __shared__ uint64_t red_best[T / 64];
@@ -291,11 +291,12 @@ __global__ __launch_bounds__(T, T <= 256 ? 8 : 4) void k_move_wg(MoveArgs a,
     for (int32_t vi = blockIdx.x; vi < n; vi += gridDim.x) {
         const int32_t v = list[vi];
         const int32_t rs = a.rowptr[v], re = a.rowptr[v + 1];
-        // table of the vertex: the smallest power of two >= 2*degree (load
-        // <= 1/2), at least 512 slots and at most the allocation
-        const int cl = min(cap_log2, max(9, 32 - __clz(2 * (re - rs) - 1)));
-        const int Cv = 1 << cl;
-        const uint32_t shift = 32 - cl, mask = (uint32_t)Cv - 1;
+        // table of the vertex: 1.5 x its degree (load <= 2/3; 2 x in the
+        // 1024-lane instance, which holds one workgroup per CU anyway), in
+        // 64-slot steps, at least 512 slots and at most the allocation
+        const int32_t deg = re - rs;
+        const int32_t want = TIER == 3 ? 2 * deg : deg + deg / 2;
+        const int Cv = min(C, max(512, (want + 63) & ~63));
         for (int i = threadIdx.x; i < Cv; i += T) {
             K[i] = -1;
             W[i] = 0.0;
@@ -316,7 +317,7 @@ __global__ __launch_bounds__(T, T <= 256 ? 8 : 4) void k_move_wg(MoveArgs a,
             for (int j = 0; j < 4; ++j) cc[j] = (u[j] >= 0 && u[j] != v) ? a.comm[u[j]] : -1;
 #pragma unroll
             for (int j = 0; j < 4; ++j)
-                if (cc[j] >= 0) lds_insert(K, W, shift, mask, cc[j], (double)wv[j]);
+                if (cc[j] >= 0) lds_insert_n(K, W, (uint32_t)Cv, cc[j], (double)wv[j]);
         }
         __syncthreads();
         const float k32 = (float)a.k[v];
@@ -329,7 +330,7 @@ __global__ __launch_bounds__(T, T <= 256 ? 8 : 4) void k_move_wg(MoveArgs a,
             // kc from the own community's slot.  Each wave evaluates the
             // heaviest candidate among its own slots first; its other slots
             // are read only if their bound can still change the decision.
-            const double kc = lds_value(K, W, shift, mask, cur);
+            const double kc = lds_value_n(K, W, (uint32_t)Cv, cur);
             double km = -1.0;
             int32_t ks = -1;
             for (int i = threadIdx.x; i < Cv; i += T) {
@@ -1363,19 +1364,19 @@ void launch_move_wave(int cap, int inst, const MoveArgs &a, const int32_t *list,
         hipLaunchKernelGGL((k_move_wave<256, 1>), dim3(g), dim3(256), 0, s, a, list, n);
 }
 
-void launch_move_wg(int tier, int cap_log2, const MoveArgs &a, const int32_t *list, int32_t n,
+void launch_move_wg(int tier, int slots, const MoveArgs &a, const int32_t *list, int32_t n,
                     hipStream_t s) {
     if (n <= 0) return;
-    const size_t lds = (size_t)12 << cap_log2;
+    const size_t lds = (size_t)12 * (size_t)slots;
     const unsigned g = (unsigned)(n < 1048576 ? n : 1048576);
     if (tier == 0)
-        hipLaunchKernelGGL((k_move_wg<128, 0>), dim3(g), dim3(128), lds, s, a, list, n, cap_log2);
+        hipLaunchKernelGGL((k_move_wg<128, 0>), dim3(g), dim3(128), lds, s, a, list, n, slots);
     else if (tier == 1)
-        hipLaunchKernelGGL((k_move_wg<256, 1>), dim3(g), dim3(256), lds, s, a, list, n, cap_log2);
+        hipLaunchKernelGGL((k_move_wg<256, 1>), dim3(g), dim3(256), lds, s, a, list, n, slots);
     else if (tier == 2)
-        hipLaunchKernelGGL((k_move_wg<256, 2>), dim3(g), dim3(256), lds, s, a, list, n, cap_log2);
+        hipLaunchKernelGGL((k_move_wg<256, 2>), dim3(g), dim3(256), lds, s, a, list, n, slots);
     else
-        hipLaunchKernelGGL((k_move_wg<1024, 3>), dim3(g), dim3(1024), lds, s, a, list, n, cap_log2);
+        hipLaunchKernelGGL((k_move_wg<1024, 3>), dim3(g), dim3(1024), lds, s, a, list, n, slots);
 }
 
 void launch_chunk_aggregate(const MoveArgs &a, const ChunkedTier &t, int32_t nchunks,
