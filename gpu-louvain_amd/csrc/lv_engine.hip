@@ -207,7 +207,7 @@ constexpr int kAggChunk = 4096;   // flat member edges per aggregation chunk
 constexpr int32_t kWgA = 512;       // 128 lanes, 768 slots
 constexpr int32_t kWgSplit = 1024;  // 256 lanes, 1536 slots
 constexpr int32_t kWgMid = 2048;    // 256 lanes, 3072 slots
-constexpr int32_t kWgMax = 4096;    // 1024 lanes, 8192 slots (2 x)
+constexpr int32_t kWgMax = 4096;    // 1024 lanes, 6144 slots
 // Group 5 (degrees 85..319) runs 256-slot wave tables up to this degree.
 constexpr int32_t kWave256Hi = 128;
 constexpr int32_t kXcdMinVertices = 1 << 21;
@@ -320,7 +320,7 @@ struct lv_graph {
     // bucketing
     lv::BucketBounds bb{};
     // LDS slots allocated per launch of each one-workgroup bucket
-    int wg_slots[4] = {768, 1536, 3072, 8192};
+    int wg_slots[4] = {768, 1536, 3072, 6144};
     int chunk_edges = lv::kChunkEdges, part_target = lv::kPartTarget, merge_limit = lv::kMergeLimit;
     lv::AggBounds agg_bounds{};
     int8_t *keys8 = nullptr;
@@ -629,10 +629,9 @@ struct lv_graph {
             for (int t = 0; t < lv::kAggTiers - 1; ++t)
                 agg_bounds.hi[t] = std::min(agg_bounds.hi[t], opt.agg_lds_max);
         // Bucket bounds: 0..5 are the reference groups; group 6 splits into
-        // three one-workgroup tiers (LDS tables of twice the bound, load
-        // <= 1/2) and the chunked tier.  The upper tier's 96 KB table needs
-        // that much LDS per workgroup.
-        const int32_t cap = 12 * 2 * kWgMax <= max_lds ? kWgMax : kWgMid;
+        // four one-workgroup tiers and the chunked tier.  The upper tier's
+        // 72 KB table needs that much LDS per workgroup.
+        const int32_t cap = 12 * (kWgMax + kWgMax / 2) <= max_lds ? kWgMax : kWgMid;
         int wg_max = opt.heavy_wg_max > 0 ? opt.heavy_wg_max : cap;
         wg_max = std::max(320, std::min(wg_max, cap));
         const int32_t wg_a = std::min(wg_max, kWgA);
@@ -650,7 +649,7 @@ struct lv_graph {
         wg_slots[0] = slots_for(wg_a);
         wg_slots[1] = slots_for(wg_lo);
         wg_slots[2] = slots_for(wg_mid);
-        wg_slots[3] = std::max(512, 2 * wg_max);
+        wg_slots[3] = slots_for(wg_max);
         chunk_edges = opt.chunk_edges > 0 ? std::max(16, std::min(opt.chunk_edges, lv::kChunkEdges))
                                           : lv::kChunkEdges;
         part_target = opt.part_target > 0 ? std::max(16, opt.part_target) : lv::kPartTarget;

@@ -274,10 +274,10 @@ __global__ __launch_bounds__(256) void k_move_wave(MoveArgs a, const int32_t *__
 }
 
 // TIER only names the instance (one per workgroup bucket) for profiles.
-// 256-lane instances keep to 64 VGPRs (8 waves per SIMD): with their
-// smaller tables LDS admits that many.
+// Every instance keeps to 64 VGPRs (8 waves per SIMD): with tables of 1.5 x
+// the degree, LDS admits that many (two 1024-lane workgroups per CU).
 template <int T, int TIER>
-__global__ __launch_bounds__(T, T <= 256 ? 8 : 4) void k_move_wg(MoveArgs a,
+__global__ __launch_bounds__(T, 8) void k_move_wg(MoveArgs a,
                                                                 const int32_t *__restrict__ list,
                                                                 int32_t n, int cap) {
     extern __shared__ double dyn_lds[];
@@ -291,12 +291,10 @@ __global__ __launch_bounds__(T, T <= 256 ? 8 : 4) void k_move_wg(MoveArgs a,
     for (int32_t vi = blockIdx.x; vi < n; vi += gridDim.x) {
         const int32_t v = list[vi];
         const int32_t rs = a.rowptr[v], re = a.rowptr[v + 1];
-        // table of the vertex: 1.5 x its degree (load <= 2/3; 2 x in the
-        // 1024-lane instance, which holds one workgroup per CU anyway), in
-        // 64-slot steps, at least 512 slots and at most the allocation
+        // table of the vertex: 1.5 x its degree (load <= 2/3), in 64-slot
+        // steps, at least 512 slots and at most the allocation
         const int32_t deg = re - rs;
-        const int32_t want = TIER == 3 ? 2 * deg : deg + deg / 2;
-        const int Cv = min(C, max(512, (want + 63) & ~63));
+        const int Cv = min(C, max(512, ((deg + deg / 2) + 63) & ~63));
         for (int i = threadIdx.x; i < Cv; i += T) {
             K[i] = -1;
             W[i] = 0.0;
