@@ -20,9 +20,14 @@ int64_t radix_scratch_elems(int64_t n);
 
 // Stable LSD sort of (keys, vals) by the bits [shift, shift+8) of each listed
 // digit shift, ping-ponging between (keys, vals) and (tk, tv); returns true
-// when the result ends in (tk, tv).
+// when the result ends in (tk, tv).  Digits are taken from the key with its
+// high word moved down to bit lo_bits, (key >> 32) << lo_bits | (uint32)key:
+// when both words are below 2^lo_bits that is one contiguous 2*lo_bits-bit
+// key with the same order, so no digit is spent on the low word's zero bits
+// (lo_bits = 32 sorts the key as it is).
 bool radix_sort_pairs(uint64_t *keys, float *vals, uint64_t *tk, float *tv, int64_t n,
-                      const int *shifts, int npasses, int32_t *scratch, hipStream_t s);
+                      const int *shifts, int npasses, int lo_bits, int32_t *scratch,
+                      hipStream_t s);
 
 // keys[e] = newid[comm[v]] << 32 | newid[comm[col[e]]], vals[e] = w[e] for
 // every CSR entry e of row v (rows walked by the level's segment table);

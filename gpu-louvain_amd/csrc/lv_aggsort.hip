@@ -12,18 +12,24 @@ inline unsigned blocks_for(int64_t n, int per) { return (unsigned)((n + per - 1)
 
 // ------------------------------------------------------------ radix sort
 
+// Digit of a key whose high word is moved down to bit lo_bits.
+__device__ __forceinline__ int radix_digit(uint64_t k, int shift, int lo_bits) {
+    const uint64_t vk = ((k >> 32) << lo_bits) | (k & 0xffffffffull);
+    return (int)((vk >> shift) & 255u);
+}
+
 // Per tile: digit counts, written digit-major (hist[d * ntiles + tile]) so
 // that one exclusive scan gives every (digit, tile) its output offset.
 __global__ __launch_bounds__(kSortThreads) void k_radix_hist(const uint64_t *__restrict__ keys,
-                                                             int64_t n, int shift, int64_t ntiles,
-                                                             int32_t *hist) {
+                                                             int64_t n, int shift, int lo_bits,
+                                                             int64_t ntiles, int32_t *hist) {
     __shared__ int32_t cnt[256];
     cnt[threadIdx.x] = 0;
     __syncthreads();
     const int64_t base = (int64_t)blockIdx.x * kSortTile;
     for (int j = 0; j < kSortItems; ++j) {
         const int64_t i = base + (int64_t)j * kSortThreads + threadIdx.x;
-        if (i < n) atomicAdd(&cnt[(int)((keys[i] >> shift) & 255u)], 1);
+        if (i < n) atomicAdd(&cnt[radix_digit(keys[i], shift, lo_bits)], 1);
     }
     __syncthreads();
     hist[(int64_t)threadIdx.x * ntiles + blockIdx.x] = cnt[threadIdx.x];
@@ -38,7 +44,8 @@ __global__ __launch_bounds__(kSortThreads) void k_radix_hist(const uint64_t *__r
 // histogram).
 __global__ __launch_bounds__(kSortThreads) void k_radix_scatter(
     const uint64_t *__restrict__ keys, const float *__restrict__ vals, int64_t n, int shift,
-    int64_t ntiles, const int32_t *__restrict__ offs, uint64_t *okeys, float *ovals) {
+    int lo_bits, int64_t ntiles, const int32_t *__restrict__ offs, uint64_t *okeys,
+    float *ovals) {
     constexpr int NW = kSortThreads / 64;
     __shared__ int32_t wcnt[NW][256];
     __shared__ int32_t tstart[256];     // first tile-local slot of each digit
@@ -62,7 +69,7 @@ __global__ __launch_bounds__(kSortThreads) void k_radix_scatter(
         const bool ok = i < n;
         k[j] = ok ? keys[i] : 0ull;
         v[j] = ok ? vals[i] : 0.0f;
-        const int d = ok ? (int)((k[j] >> shift) & 255u) : -1;
+        const int d = ok ? radix_digit(k[j], shift, lo_bits) : -1;
         dg[j] = d;
         // lanes of this round with the same digit
         uint64_t same = __ballot(ok);
@@ -124,7 +131,7 @@ __global__ __launch_bounds__(kSortThreads) void k_radix_scatter(
     const int64_t cnt = n - tile0 < kSortTile ? n - tile0 : kSortTile;
     for (int slot = threadIdx.x; slot < cnt; slot += kSortThreads) {
         const uint64_t key = lk[slot];
-        const int d = (int)((key >> shift) & 255u);
+        const int d = radix_digit(key, shift, lo_bits);
         const int64_t o = (int64_t)gbase[d] + slot;
         okeys[o] = key;
         ovals[o] = lv[slot];
@@ -244,7 +251,8 @@ int64_t radix_scratch_elems(int64_t n) {
 }
 
 bool radix_sort_pairs(uint64_t *keys, float *vals, uint64_t *tk, float *tv, int64_t n,
-                      const int *shifts, int npasses, int32_t *scratch, hipStream_t s) {
+                      const int *shifts, int npasses, int lo_bits, int32_t *scratch,
+                      hipStream_t s) {
     if (n <= 0) return false;
     const int64_t nt = (n + kSortTile - 1) / kSortTile;
     const int64_t m = 256 * nt;
@@ -255,10 +263,10 @@ bool radix_sort_pairs(uint64_t *keys, float *vals, uint64_t *tk, float *tv, int6
     float *iv = vals, *ov = tv;
     for (int p = 0; p < npasses; ++p) {
         hipLaunchKernelGGL(k_radix_hist, dim3((unsigned)nt), dim3(kSortThreads), 0, s, ik, n,
-                           shifts[p], nt, hist);
+                           shifts[p], lo_bits, nt, hist);
         exclusive_scan(hist, offs, m, nullptr, sscr, s);
         hipLaunchKernelGGL(k_radix_scatter, dim3((unsigned)nt), dim3(kSortThreads), 0, s, ik, iv,
-                           n, shifts[p], nt, offs, ok, ov);
+                           n, shifts[p], lo_bits, nt, offs, ok, ov);
         std::swap(ik, ok);
         std::swap(iv, ov);
     }

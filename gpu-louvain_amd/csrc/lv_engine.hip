@@ -2156,14 +2156,14 @@ struct lv_graph {
             LV_TRY(sscratch.reserve(std::max(lv::radix_scratch_elems(E), lv::scan_scratch_elems(E) + 8)));
             lv::launch_agg_pairs(rowptr(), col(), w(), comm, a_newid, V, order, st, ins.p, a_mstart,
                                  skA.p, svA.p, s);
-            // digits of the column id, then of the row id (stable LSD)
+            // (row id, column id) as one key of 2*bits bits, column id low:
+            // 8-bit digits over it (stable LSD)
             int bits = 1;
             while ((1ll << bits) < (int64_t)newV) ++bits;
             int shifts[8], np = 0;
-            for (int b = 0; b < bits; b += 8) shifts[np++] = b;
-            for (int b = 0; b < bits; b += 8) shifts[np++] = 32 + b;
+            for (int b = 0; b < 2 * bits; b += 8) shifts[np++] = b;
             const bool in_b = lv::radix_sort_pairs(skA.p, svA.p, skB.p, svB.p, E, shifts, np,
-                                                   sscratch.p, s);
+                                                   bits, sscratch.p, s);
             uint64_t *keys = in_b ? skB.p : skA.p;
             float *vals = in_b ? svB.p : svA.p;
             uint64_t *ukey = in_b ? skA.p : skB.p;  // the free key buffer
