@@ -233,6 +233,27 @@ __device__ __forceinline__ void lds_insert_n(int32_t *keys, double *vals, uint32
         h = h + 1 == n ? 0u : h + 1;
     }
 }
+// lds_insert_n that reports whether it turned an empty slot into c (the
+// table's count of distinct keys grows by one).
+__device__ __forceinline__ bool lds_insert_claim(int32_t *keys, double *vals, uint32_t n,
+                                                 int32_t c, double w) {
+    uint32_t h = slot_of(c, n);
+    for (;;) {
+        int32_t k = __hip_atomic_load(&keys[h], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
+        if (k == c) {
+            atomicAdd(&vals[h], w);
+            return false;
+        }
+        if (k == -1) {
+            int32_t old = atomicCAS(&keys[h], -1, c);
+            if (old == -1 || old == c) {
+                atomicAdd(&vals[h], w);
+                return old == -1;
+            }
+        }
+        h = h + 1 == n ? 0u : h + 1;
+    }
+}
 __device__ __forceinline__ double lds_value_n(const int32_t *keys, const double *vals, uint32_t n,
                                               int32_t c) {
     uint32_t h = slot_of(c, n);

@@ -208,6 +208,9 @@ constexpr int32_t kWgA = 512;       // 128 lanes, 768 slots
 constexpr int32_t kWgSplit = 1024;  // 256 lanes, 1536 slots
 constexpr int32_t kWgMid = 2048;    // 256 lanes, 3072 slots
 constexpr int32_t kWgMax = 4096;    // 1024 lanes, 6144 slots
+// LDS attempt of the chunked tier's rows: 1024 lanes, 72 KB table
+constexpr int kHeavyTrySlots = 6144;
+static_assert(kHeavyTrySlots > lv::kHeavyTryLimit + 1024, "the attempt's table never fills");
 // Group 5 (degrees 85..319) runs 256-slot wave tables up to this degree.
 constexpr int32_t kWave256Hi = 128;
 constexpr int32_t kXcdMinVertices = 1 << 21;
@@ -321,6 +324,8 @@ struct lv_graph {
     lv::BucketBounds bb{};
     // LDS slots allocated per launch of each one-workgroup bucket
     int wg_slots[4] = {768, 1536, 3072, 6144};
+    bool heavy_try = false;  // LDS attempt of the chunked tier's rows
+    int32_t heavy_limit = lv::kHeavyTryLimit;  // ... given up at this many communities
     int chunk_edges = lv::kChunkEdges, part_target = lv::kPartTarget, merge_limit = lv::kMergeLimit;
     lv::AggBounds agg_bounds{};
     int8_t *keys8 = nullptr;
@@ -344,6 +349,7 @@ struct lv_graph {
     DevVec<uint64_t> hbest;
     DevVec<double> hkcur;
     DevVec<int32_t> hovf;
+    DevVec<int32_t> hdone, hhint, hdist;  // LDS attempt of the heavy rows
     DevVec<int32_t> corder, iorder;  // XCD-aligned dispatch orders of the chunked tier
     int32_t n_corder = 0, n_iorder = 0;
     bool tier_xcd = false;
@@ -523,6 +529,9 @@ struct lv_graph {
         lv::fill_i32(reinterpret_cast<int32_t *>(kown), 0, Vp, stream);
         lv::fill_i32(nmv, 0, Vp, stream);
         lv::fill_i32(reinterpret_cast<int32_t *>(hist), 0, (Vp + 3) / 4, stream);
+        // heavy rows: no distinct count known for the new state, so the next
+        // sweep takes them through the chunked tier
+        if (nh > 0) lv::fill_i32(hhint.p, 0x7fffffff, nh, stream);
     }
 
     lv::FeatureArgs feature_args() const {
@@ -594,6 +603,9 @@ struct lv_graph {
         t.kcur = hkcur.p;
         t.merge_limit = merge_limit;
         t.ovf = hovf.p;
+        t.done = hdone.p;
+        t.hint = hhint.p;
+        t.hdist = hdist.p;
         t.chunk_order = tier_xcd ? corder.p : nullptr;
         t.item_order = tier_xcd ? iorder.p : nullptr;
         t.n_chunk_order = n_corder;
@@ -650,6 +662,9 @@ struct lv_graph {
         wg_slots[1] = slots_for(wg_lo);
         wg_slots[2] = slots_for(wg_mid);
         wg_slots[3] = slots_for(wg_max);
+        heavy_try = opt.heavy_lds >= 0 && 12 * kHeavyTrySlots <= max_lds;
+        heavy_limit = opt.heavy_lds > 0 ? std::min(opt.heavy_lds, lv::kHeavyTryLimit)
+                                        : lv::kHeavyTryLimit;
         chunk_edges = opt.chunk_edges > 0 ? std::max(16, std::min(opt.chunk_edges, lv::kChunkEdges))
                                           : lv::kChunkEdges;
         part_target = opt.part_target > 0 ? std::max(16, opt.part_target) : lv::kPartTarget;
@@ -1006,6 +1021,9 @@ struct lv_graph {
             LV_TRY(hbest.reserve(nh));
             LV_TRY(hkcur.reserve(nh));
             LV_TRY(hovf.reserve(nh));
+            LV_TRY(hdone.reserve(nh));
+            LV_TRY(hhint.reserve(nh));
+            LV_TRY(hdist.reserve(nh));
             LV_HIP(hipMemcpy(ins.p, hin.data(), sizeof(lv::Chunk) * nins, hipMemcpyHostToDevice));
             LV_HIP(hipMemcpy(hvs.p, hh.data(), sizeof(lv::HeavyVertex) * nh, hipMemcpyHostToDevice));
             LV_HIP(hipMemcpy(items.p, hit.data(), sizeof(lv::MergeItem) * nitems,
@@ -1013,6 +1031,8 @@ struct lv_graph {
             lv::fill_i32(ptot.p, 0, tbase, s);
             lv::fill_f64(hkcur.p, 0.0, nh, s);
             lv::fill_i32(hovf.p, 0, nh, s);
+            lv::fill_i32(hdone.p, 0, nh, s);
+            lv::fill_i32(hdist.p, 0, nh, s);
             hipLaunchKernelGGL(k_zero_u64_n, dim3(blocks_for(nh, 256)), dim3(256), 0, s,
                                reinterpret_cast<unsigned long long *>(hbest.p), (int64_t)nh);
             LV_LAUNCHED("chunked tier tables");
@@ -1220,6 +1240,13 @@ struct lv_graph {
                 // stage A streams the heavy rows (col, w, comm[col])
                 const lv::ChunkedTier t = chunked_tier();
                 t0 = tbegin();
+                if (!pruned && heavy_try) {
+                    // rows whose last evaluation saw few distinct communities
+                    // are decided in LDS; the stages below skip them
+                    hipEvent_t tt = tbegin();
+                    lv::launch_heavy_try(kHeavyTrySlots, heavy_limit, a, list, n, t, s);
+                    tend(tt, -1, 0.0, LV_KD_HEAVY_LDS);
+                }
                 hipEvent_t ta = tbegin();
                 lv::launch_chunk_aggregate(a, t, nins, s);
                 tend(ta, -1, 12.0 * (double)h_bedges[b] * ((double)n / (double)nfull),
@@ -2319,6 +2346,9 @@ struct lv_graph {
         hbest.release();
         hkcur.release();
         hovf.release();
+        hdone.release();
+        hhint.release();
+        hdist.release();
         corder.release();
         iorder.release();
         gkeys.release();

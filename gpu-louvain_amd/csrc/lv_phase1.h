@@ -70,7 +70,17 @@ struct ChunkedTier {
     int32_t n_chunk_order, n_item_order;
     int32_t range_blk;    // > 0: partitions by community-id range (see tier_part)
     int32_t chunk_cap_log2;  // stage A table slots (log2): twice the chunk size
+    // LDS attempt (launch_heavy_try), per heavy vertex: decided in LDS this
+    // sweep step; distinct communities of its last evaluation (the attempt
+    // is made when they fit); their count summed by stage B.
+    int32_t *done;
+    int32_t *hint;
+    int32_t *hdist;
 };
+
+// Distinct communities up to which a heavy row is tried in LDS; the table
+// (launch_heavy_try's slots) needs more than this plus 1024 slots.
+constexpr int32_t kHeavyTryLimit = 4096;
 
 // Segment of the vertex list processed with a fixed lane count per vertex
 // (row reductions: strength and modularity).
@@ -113,6 +123,11 @@ void launch_move_wave(int cap, int inst, const MoveArgs &a, const int32_t *list,
 void launch_move_wg(int tier, int slots, const MoveArgs &a, const int32_t *list, int32_t n,
                     hipStream_t s);
 void set_wg_lds_limits(int bytes);
+// LDS attempt of the chunked tier's rows (list = its heavy vertices, in the
+// order of t.hvs), before its stages; fills t.done and t.hint.  A row is
+// given up at `limit` distinct communities (at most kHeavyTryLimit).
+void launch_heavy_try(int slots, int32_t limit, const MoveArgs &a, const int32_t *list,
+                      int32_t n, const ChunkedTier &t, hipStream_t s);
 void launch_move_chunked(const MoveArgs &a, const ChunkedTier &t, int32_t nchunks,
                          int32_t nitems, const int32_t *heavy_list, int32_t nh, hipStream_t s);
 // The three kernels of launch_move_chunked, for callers that time them apart.

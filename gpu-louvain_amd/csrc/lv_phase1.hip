@@ -276,10 +276,19 @@ __global__ __launch_bounds__(256) void k_move_wave(MoveArgs a, const int32_t *__
 // TIER only names the instance (one per workgroup bucket) for profiles.
 // Every instance keeps to 64 VGPRs (8 waves per SIMD): with tables of 1.5 x
 // the degree, LDS admits that many (two 1024-lane workgroups per CU).
+//
+// TIER 4 tries the rows of the chunked tier (list = its heavy vertices, in
+// their order) in one table of the whole allocation.  Row vi is tried only
+// when its last evaluation saw at most kHeavyTryLimit distinct communities
+// (hint[vi]).  Each lane reads the table's key count before every insert and
+// gives up at `limit` (<= kHeavyTryLimit), so at most T keys land beyond it:
+// with cap > kHeavyTryLimit + T the table never fills and every probe ends.
+// A row whose inserts all landed is decided here (done[vi] = 1, hint[vi] =
+// its count); the chunked tier takes the others.
 template <int T, int TIER>
-__global__ __launch_bounds__(T, 8) void k_move_wg(MoveArgs a,
-                                                                const int32_t *__restrict__ list,
-                                                                int32_t n, int cap) {
+__global__ __launch_bounds__(T, 8) void k_move_wg(MoveArgs a, const int32_t *__restrict__ list,
+                                                  int32_t n, int cap, int32_t *hint,
+                                                  int32_t *done, int32_t limit) {
     extern __shared__ double dyn_lds[];
     const int C = cap;  // allocation (slots): W[C] then K[C]
     double *W = dyn_lds;
@@ -287,20 +296,64 @@ __global__ __launch_bounds__(T, 8) void k_move_wg(MoveArgs a,
     __shared__ uint64_t red_best[T / 64];
     __shared__ double red_K[T / 64];
     __shared__ double red_kc[T / 64];
+    __shared__ int32_t fill_s, stop_s;  // TIER 4: keys in the table, an insert was dropped
     const int lane = threadIdx.x & 63, wid = threadIdx.x >> 6;
     for (int32_t vi = blockIdx.x; vi < n; vi += gridDim.x) {
+        if constexpr (TIER == 4) {
+            if (hint[vi] > kHeavyTryLimit) continue;  // the chunked tier takes it
+        }
         const int32_t v = list[vi];
         const int32_t rs = a.rowptr[v], re = a.rowptr[v + 1];
         // table of the vertex: 1.5 x its degree (load <= 2/3), in 64-slot
         // steps, at least 512 slots and at most the allocation
         const int32_t deg = re - rs;
-        const int Cv = min(C, max(512, ((deg + deg / 2) + 63) & ~63));
+        const int Cv = TIER == 4 ? C : min(C, max(512, ((deg + deg / 2) + 63) & ~63));
         for (int i = threadIdx.x; i < Cv; i += T) {
             K[i] = -1;
             W[i] = 0.0;
         }
+        if constexpr (TIER == 4) {
+            if (threadIdx.x == 0) {
+                fill_s = 0;
+                stop_s = 0;
+            }
+        }
         __syncthreads();
         const int32_t cur = a.comm[v];
+        if constexpr (TIER == 4) {
+            volatile int32_t *fill = &fill_s, *stop = &stop_s;
+            bool dropped = false;
+            for (int32_t e0 = rs + threadIdx.x; e0 < re && !dropped; e0 += 4 * T) {
+                int32_t u[4];
+                float wv[4];
+#pragma unroll
+                for (int j = 0; j < 4; ++j) {
+                    const int32_t e = e0 + j * T;
+                    u[j] = e < re ? a.col[e] : -1;
+                    wv[j] = e < re ? a.w[e] : 0.0f;
+                }
+                int32_t cc[4];
+#pragma unroll
+                for (int j = 0; j < 4; ++j) cc[j] = (u[j] >= 0 && u[j] != v) ? a.comm[u[j]] : -1;
+#pragma unroll
+                for (int j = 0; j < 4; ++j) {
+                    if (cc[j] < 0 || dropped) continue;
+                    if (*fill >= limit) {
+                        dropped = true;
+                        *stop = 1;
+                    } else if (lds_insert_claim(K, W, (uint32_t)Cv, cc[j], (double)wv[j])) {
+                        atomicAdd(&fill_s, 1);
+                    }
+                }
+                if (*stop) dropped = true;  // another lane gave up: so does this one
+            }
+            __syncthreads();
+            const bool gave_up = stop_s != 0;
+            if (gave_up) {
+                __syncthreads();  // every lane has read stop_s before the next row resets it
+                continue;
+            }
+        } else {
         for (int32_t e0 = rs + threadIdx.x; e0 < re; e0 += 4 * T) {
             int32_t u[4];
             float wv[4];
@@ -318,6 +371,7 @@ __global__ __launch_bounds__(T, 8) void k_move_wg(MoveArgs a,
                 if (cc[j] >= 0) lds_insert_n(K, W, (uint32_t)Cv, cc[j], (double)wv[j]);
         }
         __syncthreads();
+        }
         const float k32 = (float)a.k[v];
         const float2 md = vertex_mden(a, v);
         const float vM32 = md.x, vden = md.y;
@@ -385,6 +439,10 @@ __global__ __launch_bounds__(T, 8) void k_move_wg(MoveArgs a,
                         bK = red_K[w2];
                     }
                 write_decision(a, v, cur, b, bK, kc);
+                if constexpr (TIER == 4) {
+                    done[vi] = 1;
+                    hint[vi] = fill_s;
+                }
             }
             __syncthreads();
             continue;
@@ -433,6 +491,10 @@ __global__ __launch_bounds__(T, 8) void k_move_wg(MoveArgs a,
                 kk += red_kc[w2];
             }
             write_decision(a, v, cur, b, bK, kk);
+            if constexpr (TIER == 4) {
+                done[vi] = 1;
+                hint[vi] = fill_s;
+            }
         }
         __syncthreads();
     }
@@ -473,6 +535,7 @@ __global__ __launch_bounds__(256) void k_chunk_aggregate(MoveArgs a, ChunkedTier
     if (ci < 0) return;  // padding of the XCD dispatch order
     const Chunk ch = t.chunks[ci];
     if (a.active && !a.active[ch.v]) return;  // not in this sweep's frontier
+    if (t.done[ch.hv]) return;                // decided in LDS
     const int P = ch.P;
     const int cl = table_log2(ch.hi - ch.lo, t.chunk_cap_log2);
     const int Cv = 1 << cl;
@@ -597,12 +660,14 @@ __global__ __launch_bounds__(kMergeThreads) void k_chunk_merge(MoveArgs a, Chunk
     __shared__ uint64_t red_best[kMergeThreads / 64];
     __shared__ double red_K[kMergeThreads / 64];
     __shared__ double red_kc[kMergeThreads / 64];
+    __shared__ int32_t red_n[kMergeThreads / 64];
     const int32_t item = t.item_order ? t.item_order[blockIdx.x] : (int32_t)blockIdx.x;
     if (item < 0) return;  // padding of the XCD dispatch order
     const MergeItem it = t.items[item];
     const HeavyVertex h = t.hvs[it.hv];
     const int32_t v = h.v;
     if (a.active && !a.active[v]) return;  // not in this sweep's frontier
+    if (t.done[it.hv]) return;             // decided in LDS
     const int32_t cur = a.comm[v];
     const float k32 = (float)a.k[v];
     const float2 md = vertex_mden(a, v);
@@ -613,6 +678,7 @@ __global__ __launch_bounds__(kMergeThreads) void k_chunk_merge(MoveArgs a, Chunk
     const int32_t n = t.ptot[h.ptot_base + it.p];
     uint64_t best = 0;
     double bestK = 0.0, kc = 0.0;
+    int32_t nd = 0;  // distinct communities of the partition
     if (n <= t.merge_limit) {
         const int cl = table_log2(n, kMergeCapLog2);
         const int Cv = 1 << cl;
@@ -639,6 +705,7 @@ __global__ __launch_bounds__(kMergeThreads) void k_chunk_merge(MoveArgs a, Chunk
 #pragma unroll
             for (int j = 0; j < 4; ++j) {
                 if (kk[j] < 0) continue;
+                ++nd;
                 if (kk[j] == cur) {
                     kc = vv[j];
                     continue;
@@ -660,6 +727,7 @@ __global__ __launch_bounds__(kMergeThreads) void k_chunk_merge(MoveArgs a, Chunk
         for (uint32_t i = threadIdx.x; i < cap; i += kMergeThreads) {
             const int32_t c = __hip_atomic_load(&K[i], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
             if (c < 0 || tier_part(t, c, (uint32_t)h.P) != (uint32_t)it.p) continue;
+            ++nd;
             const double Kc = __hip_atomic_load(&W[i], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
             // the table stays as it is: other partitions of the vertex may
             // still be inserting; k_clear_overflow empties it afterwards
@@ -672,23 +740,29 @@ __global__ __launch_bounds__(kMergeThreads) void k_chunk_merge(MoveArgs a, Chunk
     }
     lanes_max_pair<64>(best, bestK);
     kc = wave_sum_f64(kc);
+#pragma unroll
+    for (int off = 32; off > 0; off >>= 1) nd += __shfl_xor(nd, off, 64);
     const int lane = threadIdx.x & 63, wid = threadIdx.x >> 6;
     if (lane == 0) {
         red_best[wid] = best;
         red_K[wid] = bestK;
         red_kc[wid] = kc;
+        red_n[wid] = nd;
     }
     __syncthreads();
     if (threadIdx.x == 0) {
         uint64_t b = red_best[0];
         double bK = red_K[0], kk = red_kc[0];
+        int32_t nn = red_n[0];
         for (int w2 = 1; w2 < kMergeThreads / 64; ++w2) {
             if (red_best[w2] > b) {
                 b = red_best[w2];
                 bK = red_K[w2];
             }
             kk += red_kc[w2];
+            nn += red_n[w2];
         }
+        atomicAdd(&t.hdist[it.hv], nn);
         t.item_best[item] = b;
         t.item_K[item] = bK;
         if (b != 0)
@@ -702,6 +776,10 @@ __global__ __launch_bounds__(kMergeThreads) void k_chunk_merge(MoveArgs a, Chunk
 __global__ void k_heavy_final(MoveArgs a, int32_t nh, ChunkedTier t) {
     const int32_t hv = blockIdx.x * blockDim.x + threadIdx.x;
     if (hv >= nh) return;
+    if (t.done[hv]) {  // decided in LDS
+        t.done[hv] = 0;
+        return;
+    }
     const HeavyVertex h = t.hvs[hv];
     const int32_t v = h.v;
     const int32_t cur = a.comm[v];
@@ -719,6 +797,10 @@ __global__ void k_heavy_final(MoveArgs a, int32_t nh, ChunkedTier t) {
     write_decision(a, v, cur, best, bestK, t.kcur[hv]);
     t.best[hv] = 0;
     t.kcur[hv] = 0.0;
+    // distinct communities of this evaluation: the next sweep tries the row
+    // in LDS when they fit
+    t.hint[hv] = t.hdist[hv];
+    t.hdist[hv] = 0;
 }
 
 // Empties the overflow tables the merge items used (one workgroup per heavy
@@ -1367,14 +1449,29 @@ void launch_move_wg(int tier, int slots, const MoveArgs &a, const int32_t *list,
     if (n <= 0) return;
     const size_t lds = (size_t)12 * (size_t)slots;
     const unsigned g = (unsigned)(n < 1048576 ? n : 1048576);
+    int32_t *no = nullptr;
     if (tier == 0)
-        hipLaunchKernelGGL((k_move_wg<128, 0>), dim3(g), dim3(128), lds, s, a, list, n, slots);
+        hipLaunchKernelGGL((k_move_wg<128, 0>), dim3(g), dim3(128), lds, s, a, list, n, slots, no,
+                           no, 0);
     else if (tier == 1)
-        hipLaunchKernelGGL((k_move_wg<256, 1>), dim3(g), dim3(256), lds, s, a, list, n, slots);
+        hipLaunchKernelGGL((k_move_wg<256, 1>), dim3(g), dim3(256), lds, s, a, list, n, slots, no,
+                           no, 0);
     else if (tier == 2)
-        hipLaunchKernelGGL((k_move_wg<256, 2>), dim3(g), dim3(256), lds, s, a, list, n, slots);
+        hipLaunchKernelGGL((k_move_wg<256, 2>), dim3(g), dim3(256), lds, s, a, list, n, slots, no,
+                           no, 0);
     else
-        hipLaunchKernelGGL((k_move_wg<1024, 3>), dim3(g), dim3(1024), lds, s, a, list, n, slots);
+        hipLaunchKernelGGL((k_move_wg<1024, 3>), dim3(g), dim3(1024), lds, s, a, list, n, slots,
+                           no, no, 0);
+}
+
+void launch_heavy_try(int slots, int32_t limit, const MoveArgs &a, const int32_t *list,
+                      int32_t n, const ChunkedTier &t, hipStream_t s) {
+    if (n <= 0) return;
+    if (limit > kHeavyTryLimit) limit = kHeavyTryLimit;
+    const size_t lds = (size_t)12 * (size_t)slots;
+    const unsigned g = (unsigned)(n < 1048576 ? n : 1048576);
+    hipLaunchKernelGGL((k_move_wg<1024, 4>), dim3(g), dim3(1024), lds, s, a, list, n, slots,
+                       t.hint, t.done, limit);
 }
 
 void launch_chunk_aggregate(const MoveArgs &a, const ChunkedTier &t, int32_t nchunks,
@@ -1506,6 +1603,8 @@ void set_wg_lds_limits(int bytes) {
     (void)hipFuncSetAttribute(reinterpret_cast<const void *>(&k_move_wg<256, 2>),
                               hipFuncAttributeMaxDynamicSharedMemorySize, bytes);
     (void)hipFuncSetAttribute(reinterpret_cast<const void *>(&k_move_wg<1024, 3>),
+                              hipFuncAttributeMaxDynamicSharedMemorySize, bytes);
+    (void)hipFuncSetAttribute(reinterpret_cast<const void *>(&k_move_wg<1024, 4>),
                               hipFuncAttributeMaxDynamicSharedMemorySize, bytes);
 }
 

@@ -42,12 +42,13 @@ KERNEL_NAMES = ["move_sub", "move_wave", "move_wg", "move_chunked", "apply", "mo
 # single HIP kernels (lv_kernel_timing_detail) and their device function names
 DETAIL_NAMES = ["move_b0", "move_b1", "move_b2", "move_b3", "move_b4", "move_b5", "move_b6", "move_b7",
                 "chunk_aggregate", "chunk_merge", "chunk_final", "modularity_rows", "frontier",
-                "modularity_xcd", "move_b8", "move_b9", "move_b10"]
+                "modularity_xcd", "move_b8", "move_b9", "move_b10", "heavy_lds"]
 DETAIL_KERNELS = ["k_move_sub<4>", "k_move_sub<8>", "k_move_sub<16>", "k_move_sub<32>",
                   "k_move_wave<256, 0>", "k_move_wave<256, 1>", "k_move_wave<512, 0>",
                   "k_move_wg<128, 0>",
                   "k_chunk_aggregate", "k_chunk_merge", "k_heavy_final", "k_modularity", "k_mlp<false>",
-                  "k_modularity_xcd", "k_move_wg<256, 1>", "k_move_wg<256, 2>", "k_move_wg<1024, 3>"]
+                  "k_modularity_xcd", "k_move_wg<256, 1>", "k_move_wg<256, 2>", "k_move_wg<1024, 3>",
+                  "k_move_wg<1024, 4>"]
 
 I32P = ctypes.POINTER(ctypes.c_int32)
 I64P = ctypes.POINTER(ctypes.c_int64)
@@ -76,7 +77,8 @@ class Options(ctypes.Structure):
         ("agg_mode", ctypes.c_int32),
         ("compact", ctypes.c_int32),
         ("candidate_bounds", ctypes.c_int32),
-        ("reserved", ctypes.c_int32 * 4),
+        ("heavy_lds", ctypes.c_int32),
+        ("reserved", ctypes.c_int32 * 3),
     ]
 
 
@@ -332,7 +334,7 @@ class Graph:
     def __init__(self, g: CSR, device: int = 0, timing: bool = False, heavy_wg_max: int = 0,
                  chunk_edges: int = 0, part_target: int = 0, merge_limit: int = 0, agg_lds_max: int = 0,
                  q_incremental: int = 0, xcd_rows: int = 0, agg_mode: int = 0, compact: int = 0,
-                 candidate_bounds: int = 0):
+                 candidate_bounds: int = 0, heavy_lds: int = 0):
         lib = load()
         self._lib = lib
         self._rowptr = np.ascontiguousarray(g.rowptr, dtype=np.int32)
@@ -353,6 +355,7 @@ class Graph:
         opt.agg_mode = agg_mode
         opt.compact = compact
         opt.candidate_bounds = candidate_bounds
+        opt.heavy_lds = heavy_lds
         h = ctypes.c_void_p()
         col = self._col if self._col.size else np.zeros(1, np.int32)
         w = self._w if self._w.size else np.zeros(1, np.float32)
@@ -475,7 +478,7 @@ class Graph:
 
     def set_options(self, heavy_wg_max: int = 0, chunk_edges: int = 0, part_target: int = 0,
                     merge_limit: int = 0, agg_lds_max: int = 0, q_incremental: int = 0,
-                    xcd_rows: int = 0, agg_mode: int = 0) -> None:
+                    xcd_rows: int = 0, agg_mode: int = 0, heavy_lds: int = 0) -> None:
         """New tier options for this handle (results do not depend on them)."""
         opt = Options()
         opt.device = self.device
@@ -487,6 +490,7 @@ class Graph:
         opt.q_incremental = q_incremental
         opt.xcd_rows = xcd_rows
         opt.agg_mode = agg_mode
+        opt.heavy_lds = heavy_lds
         _check(self._lib.lv_set_options(self._h, ctypes.byref(opt)))
 
     def kernel_timing_detail(self, kernel: int) -> dict:

@@ -62,7 +62,12 @@ typedef struct {
                                 its gain shows it can change the decision (same
                                 decisions): 0 = on levels of >= 2^20 stored
                                 vertices, 1 = on every level, -1 = off           */
-    int32_t reserved[4];
+    int32_t heavy_lds;        /* rows above the workgroup tiers are first tried in
+                                one LDS table when their last evaluation saw at
+                                most 4096 distinct communities (same decisions):
+                                0 = on, -1 = off, n > 0 = on, but a row gives up
+                                at n communities (exercises the fallback)       */
+    int32_t reserved[3];
 } lv_options;
 
 /* Run statistics (lv_run).  Times are device times from HIP events. */
@@ -111,7 +116,9 @@ typedef struct {
 #define LV_KD_FRONTIER 12        /* frontier model inference (43 bytes per vertex)       */
 #define LV_KD_MODULARITY_XCD 13  /* heavy-row modularity pass by column block (12 B/edge)  */
 #define LV_KD_MOVE_BUCKET8 14    /* +b-8, b = 8..10: move kernel of execution bucket b  */
-#define LV_KD_COUNT 17
+#define LV_KD_HEAVY_LDS 17       /* LDS attempt of the chunked tier's rows (bytes not
+                                    counted)                                           */
+#define LV_KD_COUNT 18
 
 /* ---- graph handle (utils.cu:64-106 copyStructures + initM mo.cu:420-422) ---- */
 /* Uploads the CSR (rowptr[V+1], col[E], w[E]; host pointers) and sets up the
