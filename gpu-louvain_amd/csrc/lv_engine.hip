@@ -1240,11 +1240,13 @@ struct lv_graph {
                 // stage A streams the heavy rows (col, w, comm[col])
                 const lv::ChunkedTier t = chunked_tier();
                 t0 = tbegin();
-                if (!pruned && heavy_try) {
+                if (heavy_try) {
                     // rows whose last evaluation saw few distinct communities
-                    // are decided in LDS; the stages below skip them
+                    // are decided in LDS; the stages below skip them.  The
+                    // full heavy list (index = heavy vertex), frontier by flags.
                     hipEvent_t tt = tbegin();
-                    lv::launch_heavy_try(kHeavyTrySlots, heavy_limit, a, list, n, t, s);
+                    lv::launch_heavy_try(kHeavyTrySlots, heavy_limit, a, order + h_off[b], nfull,
+                                         t, s);
                     tend(tt, -1, 0.0, LV_KD_HEAVY_LDS);
                 }
                 hipEvent_t ta = tbegin();

@@ -284,7 +284,8 @@ __global__ __launch_bounds__(256) void k_move_wave(MoveArgs a, const int32_t *__
 // gives up at `limit` (<= kHeavyTryLimit), so at most T keys land beyond it:
 // with cap > kHeavyTryLimit + T the table never fills and every probe ends.
 // A row whose inserts all landed is decided here (done[vi] = 1, hint[vi] =
-// its count); the chunked tier takes the others.
+// its count); the chunked tier takes the others.  Rows outside a pruned
+// sweep's frontier are skipped, as the chunked tier skips them.
 template <int T, int TIER>
 __global__ __launch_bounds__(T, 8) void k_move_wg(MoveArgs a, const int32_t *__restrict__ list,
                                                   int32_t n, int cap, int32_t *hint,
@@ -303,6 +304,9 @@ __global__ __launch_bounds__(T, 8) void k_move_wg(MoveArgs a, const int32_t *__r
             if (hint[vi] > kHeavyTryLimit) continue;  // the chunked tier takes it
         }
         const int32_t v = list[vi];
+        if constexpr (TIER == 4) {
+            if (a.active && !a.active[v]) continue;  // not in this sweep's frontier
+        }
         const int32_t rs = a.rowptr[v], re = a.rowptr[v + 1];
         // table of the vertex: 1.5 x its degree (load <= 2/3), in 64-slot
         // steps, at least 512 slots and at most the allocation

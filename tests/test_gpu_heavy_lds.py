@@ -113,3 +113,34 @@ def test_heavy_lds_run_matches_oracle(lvlib, oracle, mode):
     assert lv == lv_o
     assert np.array_equal(o2c, o2c_o)
     assert abs(st["modularity"] - st_o["modularity"]) <= 1e-9
+
+
+@pytest.mark.parametrize("i", [0, 1])
+def test_heavy_lds_in_pruned_sweeps(lvlib, i):
+    """Pruned sweeps: the attempt skips the rows outside the frontier, as the
+    chunked tier does; sweeps and pruned runs equal those without it."""
+    name, g, opts = _cases(lvlib)[i]
+    print(f"case {name}", flush=True)
+    hs = [lvlib.Graph(g, heavy_lds=m, **opts) for m in MODES]
+    try:
+        for h in hs:
+            h.sweep()  # plain sweep first: the heavy rows get their hints
+        for s in range(4):
+            fr = lvlib.frontier(kind=lvlib.FRONTIER_RANDOM, random_p=0.5, seed=100 + s)
+            r = [h.sweep_pruned(fr, track=True) for h in hs]
+            assert r[1] == r[0] and r[2] == r[0], (s, r)
+            c0 = hs[0].communities()
+            for h in hs[1:]:
+                assert np.array_equal(h.communities(), c0), s
+        fr = lvlib.frontier(kind=lvlib.FRONTIER_NEIGHBOR, full_every=4)
+        for h in hs:
+            h.reset()
+        st = [h.run_pruned(1e-6, fr) for h in hs]
+        for k in ("levels", "sweeps", "final_V", "modularity"):
+            assert st[1][k] == st[0][k] and st[2][k] == st[0][k], (k, [x[k] for x in st])
+        a0, _ = hs[0].assignment()
+        for h in hs[1:]:
+            assert np.array_equal(h.assignment()[0], a0)
+    finally:
+        for h in hs:
+            h.close()
