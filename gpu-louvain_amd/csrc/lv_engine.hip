@@ -1224,7 +1224,7 @@ struct lv_graph {
                 break;
             case 4: case 5: case 6:
                 t0 = tbegin();
-                lv::launch_move_wave(b == 6 ? 512 : 256, b - 4, a, list, n, s);
+                lv::launch_move_wave(b == 6 ? 448 : 256, b - 4, a, list, n, s);
                 tend(t0, LV_K_MOVE_WAVE, bytes, LV_KD_MOVE_BUCKET0 + b);
                 break;
             case 7: case 8: case 9: case 10: {
