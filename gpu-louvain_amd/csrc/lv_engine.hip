@@ -660,7 +660,8 @@ struct lv_graph {
         };
         wg_slots[0] = slots_for(wg_a);
         wg_slots[1] = slots_for(wg_lo);
-        wg_slots[2] = slots_for(wg_mid);
+        // 1025..2048: 1.25 x (load <= 0.8), five workgroups per CU instead of four
+        wg_slots[2] = std::max(512, ((wg_mid + wg_mid / 4) + 63) & ~63);
         wg_slots[3] = slots_for(wg_max);
         heavy_try = opt.heavy_lds >= 0 && 12 * kHeavyTrySlots <= max_lds;
         heavy_limit = opt.heavy_lds > 0 ? std::min(opt.heavy_lds, lv::kHeavyTryLimit)
