@@ -206,7 +206,7 @@ constexpr int kAggChunk = 4096;   // flat member edges per aggregation chunk
 // Degree bounds of the one-workgroup buckets (tables of 1.5 x the bound):
 constexpr int32_t kWgA = 512;       // 128 lanes, 768 slots
 constexpr int32_t kWgSplit = 1024;  // 256 lanes, 1536 slots
-constexpr int32_t kWgMid = 2048;    // 256 lanes, 3072 slots
+constexpr int32_t kWgMid = 2048;    // 256 lanes, 2560 slots (1.25 x)
 constexpr int32_t kWgMax = 4096;    // 1024 lanes, 6144 slots
 // LDS attempt of the chunked tier's rows: 1024 lanes, 72 KB table
 constexpr int kHeavyTrySlots = 6144;
@@ -323,7 +323,7 @@ struct lv_graph {
     // bucketing
     lv::BucketBounds bb{};
     // LDS slots allocated per launch of each one-workgroup bucket
-    int wg_slots[4] = {768, 1536, 3072, 6144};
+    int wg_slots[4] = {768, 1536, 2560, 6144};
     bool heavy_try = false;  // LDS attempt of the chunked tier's rows
     int32_t heavy_limit = lv::kHeavyTryLimit;  // ... given up at this many communities
     int chunk_edges = lv::kChunkEdges, part_target = lv::kPartTarget, merge_limit = lv::kMergeLimit;
