@@ -367,23 +367,24 @@ __global__ __launch_bounds__(T, 8) void k_move_wg(MoveArgs a, const int32_t *__r
                 continue;
             }
         } else {
-        for (int32_t e0 = rs + threadIdx.x; e0 < re; e0 += 4 * T) {
-            int32_t u[4];
-            float wv[4];
+            for (int32_t e0 = rs + threadIdx.x; e0 < re; e0 += 4 * T) {
+                int32_t u[4];
+                float wv[4];
 #pragma unroll
-            for (int j = 0; j < 4; ++j) {
-                const int32_t e = e0 + j * T;
-                u[j] = e < re ? a.col[e] : -1;
-                wv[j] = e < re ? a.w[e] : 0.0f;
+                for (int j = 0; j < 4; ++j) {
+                    const int32_t e = e0 + j * T;
+                    u[j] = e < re ? a.col[e] : -1;
+                    wv[j] = e < re ? a.w[e] : 0.0f;
+                }
+                int32_t cc[4];
+#pragma unroll
+                for (int j = 0; j < 4; ++j)
+                    cc[j] = (u[j] >= 0 && u[j] != v) ? a.comm[u[j]] : -1;
+#pragma unroll
+                for (int j = 0; j < 4; ++j)
+                    if (cc[j] >= 0) lds_insert_n(K, W, (uint32_t)Cv, cc[j], (double)wv[j]);
             }
-            int32_t cc[4];
-#pragma unroll
-            for (int j = 0; j < 4; ++j) cc[j] = (u[j] >= 0 && u[j] != v) ? a.comm[u[j]] : -1;
-#pragma unroll
-            for (int j = 0; j < 4; ++j)
-                if (cc[j] >= 0) lds_insert_n(K, W, (uint32_t)Cv, cc[j], (double)wv[j]);
-        }
-        __syncthreads();
+            __syncthreads();
         }
         const float k32 = (float)a.k[v];
         const float2 md = vertex_mden(a, v);
