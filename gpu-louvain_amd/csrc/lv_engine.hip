@@ -29,6 +29,7 @@
 #include "lv_phase1.h"
 #include "lv_phase2.h"
 #include "lv_prims.h"
+#include "lv_weights.h"
 
 namespace {
 
@@ -200,6 +201,22 @@ struct DevVec {
 };
 
 unsigned blocks_for(int64_t n, int per) { return (unsigned)((n + per - 1) / per); }
+
+// Two HIP events, destroyed on every return path.
+struct EventPair {
+    hipEvent_t a = nullptr, b = nullptr;
+    EventPair() = default;
+    EventPair(const EventPair &) = delete;
+    EventPair &operator=(const EventPair &) = delete;
+    ~EventPair() {
+        if (a) (void)hipEventDestroy(a);
+        if (b) (void)hipEventDestroy(b);
+    }
+    hipError_t create() {
+        const hipError_t e = hipEventCreate(&a);
+        return e != hipSuccess ? e : hipEventCreate(&b);
+    }
+};
 
 constexpr int kSlotChunk = 4096;  // hash slots per evaluation chunk
 constexpr int kAggChunk = 4096;   // flat member edges per aggregation chunk
@@ -770,9 +787,9 @@ struct lv_graph {
         LV_TRY(dev_alloc(&tmp_w, nE));
         LV_TRY(stage(1 << 20));
 
-        hipEvent_t e0 = nullptr, e1 = nullptr;
-        LV_HIP(hipEventCreate(&e0));
-        LV_HIP(hipEventCreate(&e1));
+        EventPair ev;
+        LV_HIP(ev.create());
+        hipEvent_t e0 = ev.a, e1 = ev.b;
         LV_HIP(hipEventRecord(e0, stream));
         LV_HIP(hipMemcpyAsync(rp0, rowptr_h, sizeof(int32_t) * Vp, hipMemcpyHostToDevice, stream));
         if (nE > 0) {
@@ -799,8 +816,6 @@ struct lv_graph {
         float ms = 0.0f;
         LV_HIP(hipEventElapsedTime(&ms, e0, e1));
         memory_ms = ms;  // upload, and the compaction of level 0
-        (void)hipEventDestroy(e0);
-        (void)hipEventDestroy(e1);
         return reset();
     }
 
@@ -1981,9 +1996,9 @@ struct lv_graph {
     lv_status run_batch(float min_gain, lv_batch_result *res, lv_stats *tot,
                         const lv_frontier *mfr = nullptr) {
         hipStream_t s = stream;
-        hipEvent_t e0, e1;
-        LV_HIP(hipEventCreate(&e0));
-        LV_HIP(hipEventCreate(&e1));
+        EventPair ev;
+        LV_HIP(ev.create());
+        hipEvent_t e0 = ev.a, e1 = ev.b;
         LV_HIP(hipEventRecord(e0, s));
         LV_TRY(reset());
         LV_HIP(hipMemcpyAsync(vptr, vptr0, sizeof(int32_t) * (G + 1), hipMemcpyDeviceToDevice, s));
@@ -2137,8 +2152,6 @@ struct lv_graph {
         LV_HIP(hipEventSynchronize(e1));
         float ms = 0.0f;
         LV_HIP(hipEventElapsedTime(&ms, e0, e1));
-        (void)hipEventDestroy(e0);
-        (void)hipEventDestroy(e1);
         double qsum = 0.0;
         for (int32_t g = 0; g < G; ++g) qsum += r[g].modularity;
         st.modularity = G > 0 ? qsum / G : 0.0;
@@ -2396,10 +2409,13 @@ extern "C" {
 
 const char *lv_last_error(void) { return g_last_error.c_str(); }
 
-lv_status lv_create(const int32_t *rowptr, const int32_t *col, const float *w, int32_t V,
-                    int64_t E, const lv_options *opt, lv_graph **out) {
-    if (!out) return fail(LV_ERR_ARG, "out is null");
-    *out = nullptr;
+}  // extern "C"
+
+namespace {
+
+// Shape checks of a CSR handed to lv_create / lv_batch_create.
+lv_status check_csr(const int32_t *rowptr, const int32_t *col, const float *w, int32_t V,
+                    int64_t E) {
     if (V < 0 || E < 0 || E > 0x7fffffffLL) return fail(LV_ERR_ARG, "V/E out of range");
     if (!rowptr || (E > 0 && (!col || !w))) return fail(LV_ERR_ARG, "null CSR array");
     if (rowptr[0] != 0 || (int64_t)rowptr[V] != E) return fail(LV_ERR_ARG, "rowptr[0]/rowptr[V] mismatch");
@@ -2407,6 +2423,26 @@ lv_status lv_create(const int32_t *rowptr, const int32_t *col, const float *w, i
         if (rowptr[v + 1] < rowptr[v]) return fail(LV_ERR_ARG, "rowptr not monotone");
     for (int64_t e = 0; e < E; ++e)
         if (col[e] < 0 || col[e] >= V) return fail(LV_ERR_ARG, "column index out of range");
+    if (E > 0 && !lv::csr_symmetric(rowptr, col, w, V))
+        return fail(LV_ERR_ARG, "CSR is not symmetric: an entry (v, u, w) has no matching (u, v, w)");
+    return LV_OK;
+}
+
+// Weights of entries [lo, hi) rounded to their own grid into wq (allocated
+// on the first change; untouched entries are copied then).
+lv_status grid_round(const float *w, int64_t E, int64_t lo, int64_t hi, std::vector<float> &wq) {
+    int32_t shift = 0;
+    int64_t changed = 0;
+    LV_TRY(lv_weight_grid(w + lo, hi - lo, &shift, &changed));
+    if (changed == 0) return LV_OK;
+    if (wq.empty()) wq.assign(w, w + E);
+    LV_TRY(lv_quantize_weights(w + lo, hi - lo, shift, wq.data() + lo));
+    return LV_OK;
+}
+
+// A handle over a CSR already checked and rounded.
+lv_status create_handle(const int32_t *rowptr, const int32_t *col, const float *w, int32_t V,
+                        int64_t E, const lv_options *opt, lv_graph **out) {
     lv_graph *g = new (std::nothrow) lv_graph();
     if (!g) return fail(LV_ERR_OOM, "host allocation failed");
     lv_status st = g->create(rowptr, col, w, V, E, opt);
@@ -2419,6 +2455,20 @@ lv_status lv_create(const int32_t *rowptr, const int32_t *col, const float *w, i
     }
     *out = g;
     return LV_OK;
+}
+
+}  // namespace
+
+extern "C" {
+
+lv_status lv_create(const int32_t *rowptr, const int32_t *col, const float *w, int32_t V,
+                    int64_t E, const lv_options *opt, lv_graph **out) {
+    if (!out) return fail(LV_ERR_ARG, "out is null");
+    *out = nullptr;
+    LV_TRY(check_csr(rowptr, col, w, V, E));
+    std::vector<float> wq;
+    if (!opt || opt->exact_sums >= 0) LV_TRY(grid_round(w, E, 0, E, wq));
+    return create_handle(rowptr, col, wq.empty() ? w : wq.data(), V, E, opt, out);
 }
 
 void lv_destroy(lv_graph *g) {
@@ -2471,9 +2521,10 @@ namespace {
 // changed the communities; fr selects pruned phases.
 lv_status run_levels(lv_graph *g, float min_gain, const lv_frontier *fr, lv_stats *stats) {
     LV_HIP(hipSetDevice(g->device));
-    hipEvent_t e0, e1;
-    LV_HIP(hipEventCreate(&e0));
-    LV_HIP(hipEventCreate(&e1));
+    EventPair ev, aev;
+    LV_HIP(ev.create());
+    LV_HIP(aev.create());
+    hipEvent_t e0 = ev.a, e1 = ev.b;
     LV_HIP(hipEventRecord(e0, g->stream));
     lv_stats st{};
     double agg_ms = 0.0;
@@ -2493,9 +2544,7 @@ lv_status run_levels(lv_graph *g, float min_gain, const lv_frontier *fr, lv_stat
         st.sweep_ms += ms;
         st.edges_swept += edges;
         if (!changed) break;
-        hipEvent_t a0, a1;
-        LV_HIP(hipEventCreate(&a0));
-        LV_HIP(hipEventCreate(&a1));
+        hipEvent_t a0 = aev.a, a1 = aev.b;
         LV_HIP(hipEventRecord(a0, g->stream));
         LV_TRY(g->aggregate());
         LV_HIP(hipEventRecord(a1, g->stream));
@@ -2503,8 +2552,6 @@ lv_status run_levels(lv_graph *g, float min_gain, const lv_frontier *fr, lv_stat
         float ams = 0.0f;
         LV_HIP(hipEventElapsedTime(&ams, a0, a1));
         agg_ms += ams;
-        (void)hipEventDestroy(a0);
-        (void)hipEventDestroy(a1);
     }
     double q = 0.0;
     LV_TRY(g->modularity(&q));
@@ -2512,8 +2559,6 @@ lv_status run_levels(lv_graph *g, float min_gain, const lv_frontier *fr, lv_stat
     LV_HIP(hipEventSynchronize(e1));
     float ms = 0.0f;
     LV_HIP(hipEventElapsedTime(&ms, e0, e1));
-    (void)hipEventDestroy(e0);
-    (void)hipEventDestroy(e1);
     st.modularity = q;
     st.algo_ms = ms;
     st.aggregate_ms = agg_ms;
@@ -2974,19 +3019,25 @@ lv_status lv_batch_create(int32_t G, const int32_t *vptr, const int32_t *rowptr,
     for (int32_t g = 0; g < G; ++g)
         if (vptr[g + 1] < vptr[g]) return fail(LV_ERR_ARG, "vptr not monotone");
     const int32_t V = vptr[G];
-    if (!rowptr || (E > 0 && (!col || !w)) || rowptr[0] != 0 || (int64_t)rowptr[V] != E)
-        return fail(LV_ERR_ARG, "bad CSR");
+    // shape first (rowptr monotone, E in int32, columns in range), then ranges
+    LV_TRY(check_csr(rowptr, col, w, V, E));
     for (int32_t g = 0; g < G; ++g)
         for (int32_t v = vptr[g]; v < vptr[g + 1]; ++v)
             for (int32_t e = rowptr[v]; e < rowptr[v + 1]; ++e)
                 if (col[e] < vptr[g] || col[e] >= vptr[g + 1])
                     return fail(LV_ERR_ARG, "an edge leaves its graph's vertex range");
+    // each graph on its own grid, as lv_create would round it alone
+    std::vector<float> wq;
+    if (!opt || opt->exact_sums >= 0)
+        for (int32_t g = 0; g < G; ++g)
+            LV_TRY(grid_round(w, E, rowptr[vptr[g]], rowptr[vptr[g + 1]], wq));
+    const float *wu = wq.empty() ? w : wq.data();
     lv_graph *h = nullptr;
     // graph ranges stay in one numbering: batches store every vertex
     lv_options o = opt ? *opt : lv_options{};
     o.compact = -1;
-    LV_TRY(lv_create(rowptr, col, w, V, E, &o, &h));
-    lv_status st = h->setup_batch(G, vptr, rowptr, w);
+    LV_TRY(create_handle(rowptr, col, wu, V, E, &o, &h));
+    lv_status st = h->setup_batch(G, vptr, rowptr, wu);
     if (st != LV_OK) {
         std::string msg = g_last_error;
         lv_destroy(h);

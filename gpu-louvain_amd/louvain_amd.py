@@ -78,7 +78,8 @@ class Options(ctypes.Structure):
         ("compact", ctypes.c_int32),
         ("candidate_bounds", ctypes.c_int32),
         ("heavy_lds", ctypes.c_int32),
-        ("reserved", ctypes.c_int32 * 3),
+        ("exact_sums", ctypes.c_int32),
+        ("reserved", ctypes.c_int32 * 2),
     ]
 
 
@@ -201,6 +202,8 @@ def load() -> ctypes.CDLL:
                                             ctypes.c_uint64, ctypes.c_int32, ctypes.POINTER(I32P),
                                             ctypes.POINTER(I32P), ctypes.POINTER(F32P), I32P, I64P]),
         "lv_free_host": (None, [ctypes.c_void_p]),
+        "lv_weight_grid": (ctypes.c_int32, [F32P, ctypes.c_int64, I32P, I64P]),
+        "lv_quantize_weights": (ctypes.c_int32, [F32P, ctypes.c_int64, ctypes.c_int32, F32P]),
         "lv_run_pruned": (ctypes.c_int32, [ctypes.c_void_p, ctypes.c_float, ctypes.POINTER(Frontier),
                                            ctypes.POINTER(Stats)]),
         "lv_optimise_pruned": (ctypes.c_int32, [ctypes.c_void_p, ctypes.c_float, ctypes.POINTER(Frontier),
@@ -308,6 +311,27 @@ def write_mtx(path: str, g: CSR) -> None:
                             _p(w, ctypes.c_float), g.V))
 
 
+def weight_grid(w: np.ndarray) -> tuple[int, int]:
+    """(shift, changed): the 2^-shift grid every weight is rounded to before a
+    run (lv_weight_grid; host only) and how many weights it changes."""
+    lib = load()
+    w = np.ascontiguousarray(w, dtype=np.float32)
+    shift, changed = ctypes.c_int32(), ctypes.c_int64()
+    _check(lib.lv_weight_grid(_p(w, ctypes.c_float), len(w), ctypes.byref(shift), ctypes.byref(changed)))
+    return shift.value, changed.value
+
+
+def quantize_weights(w: np.ndarray, shift: int | None = None) -> np.ndarray:
+    """Weights rounded to the grid (lv_quantize_weights; host only)."""
+    lib = load()
+    w = np.ascontiguousarray(w, dtype=np.float32)
+    if shift is None:
+        shift = weight_grid(w)[0]
+    out = np.empty_like(w)
+    _check(lib.lv_quantize_weights(_p(w, ctypes.c_float), len(w), shift, _p(out, ctypes.c_float)))
+    return out
+
+
 def gen_rmat(scale: int, edge_factor: int = 16, a: float = 0.57, b: float = 0.19, c: float = 0.19,
              seed: int = 1, unit_weights: bool = False) -> CSR:
     lib = load()
@@ -334,7 +358,7 @@ class Graph:
     def __init__(self, g: CSR, device: int = 0, timing: bool = False, heavy_wg_max: int = 0,
                  chunk_edges: int = 0, part_target: int = 0, merge_limit: int = 0, agg_lds_max: int = 0,
                  q_incremental: int = 0, xcd_rows: int = 0, agg_mode: int = 0, compact: int = 0,
-                 candidate_bounds: int = 0, heavy_lds: int = 0):
+                 candidate_bounds: int = 0, heavy_lds: int = 0, exact_sums: int = 0):
         lib = load()
         self._lib = lib
         self._rowptr = np.ascontiguousarray(g.rowptr, dtype=np.int32)
@@ -356,6 +380,7 @@ class Graph:
         opt.compact = compact
         opt.candidate_bounds = candidate_bounds
         opt.heavy_lds = heavy_lds
+        opt.exact_sums = exact_sums
         h = ctypes.c_void_p()
         col = self._col if self._col.size else np.zeros(1, np.int32)
         w = self._w if self._w.size else np.zeros(1, np.float32)

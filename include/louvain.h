@@ -14,7 +14,10 @@
  * Numerics: M, vertex strengths, community totals, per-candidate weight
  * sums, merged aggregation weights and modularity are FP64; gains are fp32
  * in the reference's operation order (modularity_optimisation.cu:96-102);
- * aggregated weights are stored fp32 (the reference's CSR type).
+ * aggregated weights are stored fp32 (the reference's CSR type).  Weights
+ * are rounded to a grid first (lv_weight_grid) on which every FP64 sum of
+ * weights is exact, so results do not depend on the order of atomics and
+ * repeat bit for bit.
  */
 #ifndef LOUVAIN_H
 #define LOUVAIN_H
@@ -67,7 +70,14 @@ typedef struct {
                                 most 4096 distinct communities (same decisions):
                                 0 = on, -1 = off, n > 0 = on, but a row gives up
                                 at n communities (exercises the fallback)       */
-    int32_t reserved[3];
+    int32_t exact_sums;       /* 0: every weight is first rounded to the grid of
+                                lv_weight_grid, so that every FP64 sum of weights
+                                is exact and no result depends on the order of
+                                atomics (weights already on the grid, e.g.
+                                integers, are unchanged); -1: weights are used as
+                                given (sums of weights off the grid may then
+                                round differently from run to run)             */
+    int32_t reserved[2];
 } lv_options;
 
 /* Run statistics (lv_run).  Times are device times from HIP events. */
@@ -122,7 +132,12 @@ typedef struct {
 
 /* ---- graph handle (utils.cu:64-106 copyStructures + initM mo.cu:420-422) ---- */
 /* Uploads the CSR (rowptr[V+1], col[E], w[E]; host pointers) and sets up the
- * singleton state comm = iota, size = 1 (utils.cu:84-89).  M = (sum w)/2. */
+ * singleton state comm = iota, size = 1 (utils.cu:84-89).  M = (sum w)/2 of
+ * the weights after the grid rounding (lv_options.exact_sums).
+ * Precondition (the reference's reader guarantees it, utils.cu:32-45): the
+ * CSR is symmetric, every entry (v, u, w) matched by an entry (u, v, w) with
+ * the same weight bits (duplicates counted as a multiset).  An asymmetric CSR
+ * or a non-finite weight is rejected with LV_ERR_ARG. */
 lv_status lv_create(const int32_t *rowptr, const int32_t *col, const float *w, int32_t V,
                     int64_t E, const lv_options *opt, lv_graph **out);
 /* utils.cu:108-141 deleteStructures. */
@@ -191,7 +206,9 @@ const char *lv_last_error(void);
  * vptr[g] .. vptr[g+1] and its rows may only reference them.  lv_batch_run
  * solves each graph exactly as lv_run would alone (its own M, phases,
  * termination and levels); all graphs share every launch (one kernel per
- * degree bucket covers the vertices of all graphs in that bucket). */
+ * degree bucket covers the vertices of all graphs in that bucket).  Each
+ * graph's weights are rounded to its own grid, as lv_create would round that
+ * graph alone; the union CSR must be symmetric. */
 typedef struct lv_batch lv_batch;
 typedef struct {
     double modularity;       /* final Q of the graph (FP64)                          */
@@ -300,6 +317,19 @@ lv_status lv_mlp_grad(int32_t device, const float *params, const float *feat, co
                       int64_t n, float pos_weight, float *grad, float *loss);
 lv_status lv_mlp_train(int32_t device, const float *feat, const float *labels, int64_t n,
                        const lv_train_options *opt, float *params, float *loss_per_epoch);
+
+/* ---- weight grid (no counterpart in the reference, whose fp32 atomic sums
+ * depend on thread order: modularity_optimisation.cu:144-153) ----
+ * T = sum of |w| (FP64, in chunks of 65536 entries added in order),
+ * c = the smallest integer with 2^c >= T * (1 + 2^-20), shift = 50 - c
+ * (0 when T = 0); a weight w is rounded to q(w) = rint(w * 2^shift) *
+ * 2^-shift (ties to even), which fp32 holds exactly.  Every partial sum the
+ * engine forms stays below 8T, i.e. below 2^53 grid steps, so every FP64 sum
+ * of rounded weights is exact.  These run on the host (no device needed). */
+/* shift for w[n]; *changed (may be null) = how many weights the rounding changes. */
+lv_status lv_weight_grid(const float *w, int64_t n, int32_t *shift, int64_t *changed);
+/* out[i] = q(w[i]) for the given shift (out may alias w). */
+lv_status lv_quantize_weights(const float *w, int64_t n, int32_t shift, float *out);
 
 /* ---- host-side input (utils.cu:9-62 readInputData) and synthetic graphs ---- */
 /* MatrixMarket coordinate reader with the reference's conventions (skip '%'

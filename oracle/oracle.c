@@ -162,6 +162,56 @@ double orc_total_weight(const orc_csr *g) {
     return s / 2.0;
 }
 
+/* Weight grid.  No reference counterpart: the reference sums fp32 weights
+ * with fp32 atomics in thread order (modularity_optimisation.cu:144-153,
+ * community_aggregation.cu:146-159).  The HIP library rounds every weight to
+ * a grid of 2^-s before a run, so that every FP64 sum of weights is exact;
+ * the oracle applies the same rounding to its input:
+ *   T = sum of |w|, FP64, in chunks of 65536 entries (in order inside a chunk,
+ *       chunk sums added in order);
+ *   s = 50 - c with c the smallest integer such that 2^c >= T * (1 + 2^-20)
+ *       (s = 0 when T = 0);
+ *   q(w) = rint(w * 2^s) * 2^-s, round half to even. */
+int32_t orc_weight_shift(const float *w, int64_t n) {
+    const int64_t chunk = 65536;
+    double T = 0.0;
+    for (int64_t lo = 0; lo < n; lo += chunk) {
+        double part = 0.0;
+        const int64_t hi = lo + chunk < n ? lo + chunk : n;
+        for (int64_t e = lo; e < hi; ++e) part += fabs((double)w[e]);
+        T += part;
+    }
+    if (!(T > 0.0)) return 0;
+    const double target = T * (1.0 + 1.0 / 1048576.0);
+    int c = 0;
+    double p = 1.0;
+    while (p < target) {
+        p *= 2.0;
+        ++c;
+    }
+    while (p / 2.0 >= target) {
+        p /= 2.0;
+        --c;
+    }
+    return 50 - c;
+}
+
+float orc_quantize_weight(float w, int32_t shift) {
+    volatile double x = scalbn((double)w, shift);
+    return (float)scalbn(rint(x), -shift);
+}
+
+int64_t orc_quantize(orc_csr *g) {
+    const int32_t shift = orc_weight_shift(g->w, g->E);
+    int64_t changed = 0;
+    for (int64_t e = 0; e < g->E; ++e) {
+        const float q = orc_quantize_weight(g->w[e], shift);
+        if (memcmp(&q, &g->w[e], sizeof q) != 0) ++changed;
+        g->w[e] = q;
+    }
+    return changed;
+}
+
 /* k_v = sum of row weights, self-loop counted once (modularity_optimisation.cu:23-40). */
 void orc_strength(const orc_csr *g, double *k) {
     for (int32_t v = 0; v < g->V; ++v) {
@@ -221,7 +271,7 @@ static void scratch_free(orc_scratch *s) {
  *   move iff bestGain - K[cur]/M > 0                            (:192-196) */
 static int32_t decide_vertex(const orc_csr *g, float M32, int32_t v, const int32_t *comm,
                              const int32_t *size, const double *k, const double *sigma,
-                             orc_scratch *s, float *margin, float *kown) {
+                             orc_scratch *s, float *margin, float *kown, double *gap) {
     const int32_t cur = comm[v];
     int32_t nt = 0;
     for (int32_t e = g->rowptr[v]; e < g->rowptr[v + 1]; ++e) {
@@ -239,6 +289,9 @@ static int32_t decide_vertex(const orc_csr *g, float M32, int32_t v, const int32
     const float scur = (float)sigma[cur];
     float best_gain = 0.0f;
     int32_t best = cur;
+    /* the two largest of {K[cur]/M} and the eligible candidates' gains */
+    const float stay = (float)kcur64 / M32;
+    double top1 = stay, top2 = -INFINITY;
     for (int32_t i = 0; i < nt; ++i) {
         int32_t c = s->touched[i];
         if (c == cur) continue;
@@ -248,6 +301,16 @@ static int32_t decide_vertex(const orc_csr *g, float M32, int32_t v, const int32
             best_gain = gc;
             best = c;
         }
+        if ((double)gc > top1) {
+            top2 = top1;
+            top1 = gc;
+        } else if ((double)gc > top2) {
+            top2 = gc;
+        }
+    }
+    if (gap) {
+        const double mag = fmax(fabs(top1), fabs(top2));
+        gap[v] = top2 == -INFINITY ? INFINITY : (mag > 0.0 ? (top1 - top2) / mag : 0.0);
     }
     for (int32_t i = 0; i < nt; ++i) {
         s->acc[s->touched[i]] = 0.0;
@@ -273,7 +336,7 @@ static int nthreads_eff(int nthreads) {
 static void decide_list(const orc_csr *g, float M32, const int32_t *list, int32_t n,
                         const int32_t *comm, const int32_t *size, const double *k,
                         const double *sigma, int32_t *new_comm, orc_scratch *scr, int nthr,
-                        const uint8_t *active, float *margin, float *kown) {
+                        const uint8_t *active, float *margin, float *kown, double *gap) {
 #ifdef _OPENMP
     if (nthr > 1 && n > 256) {
 #pragma omp parallel num_threads(nthr)
@@ -284,7 +347,7 @@ static void decide_list(const orc_csr *g, float M32, const int32_t *list, int32_
                 new_comm[list[i]] = (active && !active[list[i]])
                                         ? comm[list[i]]
                                         : decide_vertex(g, M32, list[i], comm, size, k, sigma, s,
-                                                        margin, kown);
+                                                        margin, kown, gap);
         }
         return;
     }
@@ -295,7 +358,7 @@ static void decide_list(const orc_csr *g, float M32, const int32_t *list, int32_
         new_comm[list[i]] = (active && !active[list[i]])
                                 ? comm[list[i]]
                                 : decide_vertex(g, M32, list[i], comm, size, k, sigma, &scr[0],
-                                                margin, kown);
+                                                margin, kown, gap);
 }
 
 /* Vertex lists per degree group, ascending vertex id. */
@@ -328,6 +391,12 @@ void orc_decide_group(const orc_csr *g, double M, const int32_t *comm, int group
 
 void orc_decide_group_obs(const orc_csr *g, double M, const int32_t *comm, int group,
                           int32_t *new_comm, float *margin, float *kown, int nthreads) {
+    orc_decide_group_gap(g, M, comm, group, new_comm, margin, kown, NULL, nthreads);
+}
+
+void orc_decide_group_gap(const orc_csr *g, double M, const int32_t *comm, int group,
+                          int32_t *new_comm, float *margin, float *kown, double *gap,
+                          int nthreads) {
     const int32_t V = g->V;
     int nthr = nthreads_eff(nthreads);
     double *k = malloc(sizeof(double) * ((size_t)V + 1));
@@ -345,7 +414,7 @@ void orc_decide_group_obs(const orc_csr *g, double M, const int32_t *comm, int g
     int lo = group < 0 ? 0 : group, hi = group < 0 ? ORC_NGROUPS - 1 : group;
     for (int b = lo; b <= hi; ++b)
         decide_list(g, M32, gr.ids + gr.off[b], gr.off[b + 1] - gr.off[b], comm, size, k, sigma,
-                    new_comm, scr, nthr, NULL, margin, kown);
+                    new_comm, scr, nthr, NULL, margin, kown, gap);
     for (int t = 0; t < nthr; ++t) scratch_free(&scr[t]);
     free(scr);
     free(gr.ids);
@@ -388,7 +457,7 @@ static int64_t sweep_impl(const orc_csr *g, float M32, int32_t *comm, int32_t *s
         const int32_t n = gr->off[b + 1] - gr->off[b];
         if (n == 0) continue;
         decide_list(g, M32, list, n, comm, size, k, sigma, new_comm, scr, nthr, active, NULL,
-                    NULL);
+                    NULL, NULL);
         for (int32_t i = 0; i < n; ++i) {
             int32_t v = list[i];
             if (new_comm[v] != comm[v]) {

@@ -13,7 +13,8 @@
  *   - M, vertex strengths k, community totals S, per-candidate weight sums K,
  *     merged aggregation weights and modularity are accumulated in FP64.
  *     With weights on a 2^-s grid and totals below 2^(53-s) these sums are
- *     exact, hence independent of summation order.
+ *     exact, hence independent of summation order.  orc_quantize rounds a
+ *     graph's weights to such a grid, as the HIP library does before a run.
  *   - The gain of a candidate community is evaluated in fp32 with the
  *     reference's operation order (modularity_optimisation.cu:96-102) from
  *     fp32 roundings of the FP64 quantities; build with -ffp-contract=off.
@@ -52,6 +53,16 @@ int orc_read_mtx(const char *path, orc_csr *out);
 void orc_free_csr(orc_csr *g);
 const char *orc_last_error(void);
 
+/* --- weight grid (no reference counterpart; the HIP library's canonical
+ * input rounding, under which every FP64 sum of weights is exact) ------ */
+/* s = 50 - c, c the smallest integer with 2^c >= T*(1+2^-20), T = sum |w|
+ * in FP64 over chunks of 65536 entries added in order (0 when T = 0). */
+int32_t orc_weight_shift(const float *w, int64_t n);
+/* rint(w * 2^shift) * 2^-shift (ties to even); exact in fp32. */
+float orc_quantize_weight(float w, int32_t shift);
+/* Rounds g's weights in place with the shift of g; returns how many changed. */
+int64_t orc_quantize(orc_csr *g);
+
 /* --- per-level quantities ------------------------------------------- */
 double orc_total_weight(const orc_csr *g);                 /* M = (sum w)/2 */
 void orc_strength(const orc_csr *g, double *k);            /* k_v = row sum */
@@ -70,6 +81,15 @@ void orc_decide_group(const orc_csr *g, double M, const int32_t *comm, int group
  * computeMove) and K[cur] as fp32 (margin/kown may be NULL). */
 void orc_decide_group_obs(const orc_csr *g, double M, const int32_t *comm, int group,
                           int32_t *new_comm, float *margin, float *kown, int nthreads);
+
+/* As orc_decide_group_obs, also recording for each decided vertex how close
+ * its decision was: with v1 >= v2 the two largest of {K[cur]/M (fp32)} and
+ * the fp32 gains of its eligible candidates, gap[v] = (v1 - v2) /
+ * max(|v1|, |v2|) (0 when both are 0, +inf without candidates).  A decision
+ * can change under a perturbation of the sums only where gap is small. */
+void orc_decide_group_gap(const orc_csr *g, double M, const int32_t *comm, int group,
+                          int32_t *new_comm, float *margin, float *kown, double *gap,
+                          int nthreads);
 
 /* Gain of moving v to community c (fp32, reference operation order). */
 float orc_gain(float K_c, float M32, float k_v, float sigma_cur, float sigma_c);

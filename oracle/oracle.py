@@ -63,6 +63,11 @@ def load() -> ctypes.CDLL:
                                      ctypes.c_int]),
         "orc_aggregate": (ctypes.c_int, [P, I32P, P, I32P, ctypes.c_int32, I32P]),
         "orc_run": (ctypes.c_int, [P, ctypes.c_float, I32P, I32P, ctypes.POINTER(OrcStats), ctypes.c_int]),
+        "orc_weight_shift": (ctypes.c_int32, [F32P, ctypes.c_int64]),
+        "orc_quantize_weight": (ctypes.c_float, [ctypes.c_float, ctypes.c_int32]),
+        "orc_quantize": (ctypes.c_int64, [P]),
+        "orc_decide_group_gap": (None, [P, ctypes.c_double, I32P, ctypes.c_int, I32P, F32P, F32P, F64P,
+                                        ctypes.c_int]),
     }
     for name, (res, args) in sig.items():
         fn = getattr(lib, name)
@@ -77,20 +82,25 @@ def _p(a, t):
 
 
 class Graph:
-    """Holds numpy CSR arrays and an OrcCSR view of them."""
+    """Holds numpy CSR arrays and an OrcCSR view of them.  quantize=True
+    rounds the weights to the grid the HIP library rounds them to before a
+    run (orc_quantize); the arrays are copied first."""
 
-    def __init__(self, rowptr, col, w):
+    def __init__(self, rowptr, col, w, quantize: bool = False):
         self.rowptr = np.ascontiguousarray(rowptr, dtype=np.int32)
         self.col = np.ascontiguousarray(col if len(col) else np.zeros(1, np.int32), dtype=np.int32)
-        self.w = np.ascontiguousarray(w if len(w) else np.zeros(1, np.float32), dtype=np.float32)
+        self.w = np.array(w if len(w) else np.zeros(1, np.float32), dtype=np.float32, copy=True)
         self.E = int(len(col))
         self.V = int(len(rowptr) - 1)
         self.c = OrcCSR(self.V, self.E, _p(self.rowptr, ctypes.c_int32), _p(self.col, ctypes.c_int32),
                         _p(self.w, ctypes.c_float))
+        self.changed = int(load().orc_quantize(ctypes.byref(self.c))) if quantize else 0
 
     @classmethod
-    def from_csr(cls, g):
-        return cls(g.rowptr, g.col, g.w)
+    def from_csr(cls, g, quantize: bool = True):
+        """The oracle's copy of a library CSR, on the library's weight grid
+        (quantize=False keeps the weights as given)."""
+        return cls(g.rowptr, g.col, g.w, quantize=quantize)
 
     def csr_arrays(self):
         return self.rowptr, self.col[: self.E], self.w[: self.E]
@@ -100,7 +110,29 @@ class Graph:
         return ctypes.byref(self.c)
 
 
-def read_mtx(path: str) -> Graph:
+def weight_shift(w: np.ndarray) -> int:
+    w = np.ascontiguousarray(w, dtype=np.float32)
+    return int(load().orc_weight_shift(_p(w, ctypes.c_float), len(w)))
+
+
+def quantize_weight(w: float, shift: int) -> float:
+    return float(load().orc_quantize_weight(w, shift))
+
+
+def decide_gap(g: Graph, comm: np.ndarray, group: int, M: float | None = None, threads: int = 1):
+    """decide() plus each decided vertex's relative gap between its two best
+    options (orc_decide_group_gap; NaN for vertices outside the group)."""
+    lib = load()
+    M = total_weight(g) if M is None else M
+    comm = np.ascontiguousarray(comm, dtype=np.int32)
+    out = np.zeros(max(g.V, 1), np.int32)
+    gap = np.full(max(g.V, 1), np.nan, np.float64)
+    lib.orc_decide_group_gap(g.ref, M, _p(comm, ctypes.c_int32), group, _p(out, ctypes.c_int32), None, None,
+                             _p(gap, ctypes.c_double), threads)
+    return out[: g.V], gap[: g.V]
+
+
+def read_mtx(path: str, quantize: bool = True) -> Graph:
     lib = load()
     c = OrcCSR()
     rc = lib.orc_read_mtx(path.encode(), ctypes.byref(c))
@@ -111,7 +143,7 @@ def read_mtx(path: str) -> Graph:
     col = np.ctypeslib.as_array(c.col, shape=(max(E, 1),))[:E].copy()
     w = np.ctypeslib.as_array(c.w, shape=(max(E, 1),))[:E].copy()
     lib.orc_free_csr(ctypes.byref(c))
-    return Graph(rowptr, col, w)
+    return Graph(rowptr, col, w, quantize=quantize)
 
 
 def total_weight(g: Graph) -> float:

@@ -38,6 +38,7 @@ int main(int argc, char **argv) {
         fprintf(stderr, "error: %s\n", orc_last_error());
         return 2;
     }
+    orc_quantize(&g); /* the library's weight grid (oracle.h) */
     int32_t *o2c = malloc(sizeof(int32_t) * ((size_t)g.V + 1));
     int32_t level_V = 0;
     orc_stats st;

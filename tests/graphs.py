@@ -53,3 +53,35 @@ def star(leaves, extra_ring=True):
     if extra_ring:
         edges += [(i, i + 1) for i in range(1, leaves)]
     return csr_from_edges(leaves + 1, edges)
+
+
+def _mix64(x):
+    """SplitMix64 finaliser over a uint64 numpy array."""
+    x = (x + np.uint64(0x9E3779B97F4A7C15)) & np.uint64(0xFFFFFFFFFFFFFFFF)
+    x = ((x ^ (x >> np.uint64(30))) * np.uint64(0xBF58476D1CE4E5B9)) & np.uint64(0xFFFFFFFFFFFFFFFF)
+    x = ((x ^ (x >> np.uint64(27))) * np.uint64(0x94D049BB133111EB)) & np.uint64(0xFFFFFFFFFFFFFFFF)
+    return x ^ (x >> np.uint64(31))
+
+
+def real_weights(rowptr, col, seed, low=0.0, power=1.0):
+    """Symmetric weights off every dyadic grid the generators use: for each
+    undirected pair {u, v}, float32 of x**power with x a float64 uniform in
+    (low, 1] drawn from a hash of (min, max, seed), so (u, v) and (v, u)
+    agree.  power > 1 skews them toward small values (mixed magnitudes)."""
+    rowptr = np.asarray(rowptr, np.int64)
+    col = np.asarray(col, np.int64)
+    row = np.repeat(np.arange(len(rowptr) - 1, dtype=np.int64), np.diff(rowptr))
+    a, b = np.minimum(row, col).astype(np.uint64), np.maximum(row, col).astype(np.uint64)
+    with np.errstate(over="ignore"):
+        h = _mix64((a << np.uint64(32)) ^ b ^ _mix64(np.full(a.shape, seed, np.uint64)))
+    u = (h >> np.uint64(11)).astype(np.float64) * 2.0**-53  # [0, 1)
+    return ((1.0 - u * (1.0 - low)) ** power).astype(np.float32)
+
+
+def write_decimal_mtx(path, n, edges, weights):
+    """Symmetric MatrixMarket file whose weights are written as given (strings)."""
+    with open(path, "w") as f:
+        f.write("%%MatrixMarket matrix coordinate real symmetric\n")
+        f.write(f"{n} {n} {len(edges)}\n")
+        for (i, j), w in zip(edges, weights):
+            f.write(f"{i + 1} {j + 1} {w}\n")

@@ -44,7 +44,7 @@ def test_create_without_device_fails_cleanly(lvlib):
 def test_reader_matches_oracle_reader(lvlib, oracle, name):
     path = os.path.join(GOLDEN, name)
     a = lvlib.read_mtx(path)
-    b = oracle.read_mtx(path)
+    b = oracle.read_mtx(path, quantize=False)
     rp, cl, w = b.csr_arrays()
     assert np.array_equal(a.rowptr, rp)
     assert np.array_equal(a.col, cl)
@@ -108,7 +108,7 @@ def test_reader_chunk_boundaries_match_oracle(lvlib, oracle, tmp_path, monkeypat
                                                 ("karate.mtx", "reader_conventions.mtx")]
     for path in paths:
         a = lvlib.read_mtx(path)
-        b = oracle.read_mtx(path)
+        b = oracle.read_mtx(path, quantize=False)
         rp, cl, w = b.csr_arrays()
         assert np.array_equal(a.rowptr, rp), path
         assert np.array_equal(a.col, cl), path
@@ -142,7 +142,7 @@ def test_reader_weights_round_like_strtof(lvlib, oracle, tmp_path):
     f = tmp_path / "w.mtx"
     f.write_text(f"%%MatrixMarket matrix coordinate real general\n50 50 {n}\n" + body)
     a = lvlib.read_mtx(str(f))
-    b = oracle.read_mtx(str(f))
+    b = oracle.read_mtx(str(f), quantize=False)
     rp, cl, w = b.csr_arrays()
     assert np.array_equal(a.col, cl)
     assert np.array_equal(a.w.view(np.uint32), w.view(np.uint32))

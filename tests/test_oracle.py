@@ -251,3 +251,56 @@ def test_group_bounds(oracle):
               320: 6, 100000: 6}
     for d, b in expect.items():
         assert oracle.group_of_degree(d) == b
+
+
+@pytest.mark.parametrize("seed", range(4))
+def test_decide_gap_matches_candidate_enumeration(oracle, seed):
+    """orc_decide_group_gap: the relative gap between the two best options of
+    each vertex ({K[cur]/M} and every eligible candidate's fp32 gain),
+    recomputed here by enumerating the candidates with oracle.gain.  A
+    planted mistake (dropping the stay option) is caught."""
+    n = 14
+    rowptr, col, w = graphs.random_graph(n, 30, seed, weighted=True, self_loops=2)
+    g = og_from(oracle, rowptr, col, w)
+    M = oracle.total_weight(g)
+    M32 = np.float32(M)
+    k = oracle.strength(g)
+    rng = np.random.default_rng(200 + seed)
+    comm = rng.integers(0, 5, n).astype(np.int32)
+    size = np.bincount(comm, minlength=n)
+    sigma = np.zeros(n)
+    np.add.at(sigma, comm, k)
+    got, gap = oracle.decide_gap(g, comm, -1)
+    assert np.array_equal(got, oracle.decide(g, comm, -1))
+
+    def options(v, with_stay=True):
+        cur = comm[v]
+        K = {}
+        for e in range(rowptr[v], rowptr[v + 1]):
+            u = col[e]
+            if u != v:
+                K[comm[u]] = K.get(comm[u], 0.0) + float(w[e])
+        vals = [float(np.float32(K.get(cur, 0.0)) / M32)] if with_stay else []
+        for c, Kc in K.items():
+            if c == cur or not (c < cur or size[c] > 1 or size[cur] > 1):
+                continue
+            vals.append(oracle.gain(np.float32(Kc), M32, np.float32(k[v]), np.float32(sigma[cur]),
+                                    np.float32(sigma[c])))
+        return sorted(vals, reverse=True)
+
+    def rel(vals):
+        if len(vals) < 2:
+            return np.inf
+        mag = max(abs(vals[0]), abs(vals[1]))
+        return 0.0 if mag == 0 else (vals[0] - vals[1]) / mag
+
+    checked = planted_differs = 0
+    for v in range(n):
+        if rowptr[v + 1] == rowptr[v]:
+            assert np.isnan(gap[v])
+            continue
+        want = rel(options(v))
+        assert gap[v] == pytest.approx(want, rel=1e-12, abs=0) or (np.isinf(gap[v]) and np.isinf(want))
+        checked += 1
+        planted_differs += rel(options(v, with_stay=False)) != want
+    assert checked > 5 and planted_differs > 0
