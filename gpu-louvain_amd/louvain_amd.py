@@ -36,6 +36,7 @@ LV_ERR_HIP = -3
 LV_ERR_IO = -4
 LV_ERR_STATE = -5
 LV_ERR_NODEVICE = -6
+LV_NOALLOC_MAX_V = 1 << 20  # louvain.h LV_NOALLOC_MAX_V
 
 K_MOVE_SUB, K_MOVE_WAVE, K_MOVE_WG, K_MOVE_GLOBAL, K_APPLY, K_MODULARITY, K_AGGREGATE = range(7)
 KERNEL_NAMES = ["move_sub", "move_wave", "move_wg", "move_chunked", "apply", "modularity", "aggregate"]

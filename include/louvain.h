@@ -161,6 +161,25 @@ lv_status lv_modularity(lv_graph *g, double *q);
 /* Whole multi-level run (gpulouvain.cu:30-44). */
 lv_status lv_run(lv_graph *g, float min_gain, lv_stats *stats);
 
+/* Modularity without any allocation.  No counterpart in the reference: a
+ * variant of lv_modularity for callers that cannot allocate device memory
+ * (code under HIP graph capture, tight serving loops, harnesses that forbid
+ * hipMalloc).  Every array is a caller-owned DEVICE pointer on `device`:
+ * rowptr[V+1], col[E], w[E], comm[V] (community of each vertex, in [0, V)),
+ * and q (one double, written by the last kernel).  Work is enqueued on
+ * `stream` (a hipStream_t; null = the default stream) and not waited for.
+ * Q = own/2M - sum_c S_c^2/(2M)^2 in FP64 with 2M = the sum of w (the weights
+ * as given; lv_modularity uses the handle's rounded weights and run M).  A
+ * row, column or community outside its range makes *q NaN (nothing outside
+ * the arrays is touched).  Scratch is one static device array per GPU (about
+ * 8.4 MB for LV_NOALLOC_MAX_V), held until the process ends: V above
+ * LV_NOALLOC_MAX_V is rejected with LV_ERR_ARG, and calls that overlap on one
+ * GPU race on it. */
+#define LV_NOALLOC_MAX_V (1 << 20)
+lv_status lv_modularity_noalloc(int32_t device, void *stream, const int32_t *rowptr,
+                                const int32_t *col, const float *w, const int32_t *comm,
+                                int32_t V, int64_t E, double *q);
+
 /* ---- state access (for -v output and parity checks) ---- */
 /* Final/current original-vertex -> community map in the current level's ids
  * (data printed by printOriginalToCommunity, modularity_optimisation.cu:403-418). */
