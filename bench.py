@@ -12,8 +12,9 @@ level-0 phase from singletons (4 sweeps on this graph) and start it again
 when the phase ends; a restart (community state reset and the modularity
 before the first sweep) is inside the timed step.  value = CSR entries per
 second over the timed level-0 sweeps.  Also reported: time to converge (full
-multi-level run) with per-level sweep times, and the CPU oracle timed on a
-bounded sample (one sweep, same graph).  The steps and the run are timed on
+multi-level run) with per-level sweep times, and the CPU oracle timed on the
+same level-0 phase on 1 core and on all host cores, with its time to converge
+on RMAT-20.  The steps and the run are timed on
 a handle without per-kernel events; the same warmup and timed steps are then
 repeated on a handle that records HIP events around every kernel on its
 stream, which give the per-kernel timings and the roofline of the dominant
@@ -42,21 +43,81 @@ def log(msg):
     print(msg, file=sys.stderr, flush=True)
 
 
-def cpu_baseline(csr, threads):
-    """One oracle sweep from the singleton state on the host (bounded sample)."""
+def cpu_model():
+    """The host CPU's model name (as lscpu prints it) and logical CPU count."""
+    name = None
+    try:
+        with open("/proc/cpuinfo") as f:
+            for line in f:
+                if line.startswith("model name"):
+                    name = line.split(":", 1)[1].strip()
+                    break
+    except OSError:
+        pass
+    return name, os.cpu_count()
+
+
+def host_threads():
+    """Host threads for the all-cores oracle run: OMP_NUM_THREADS when set
+    (the GPU box's share), else every CPU."""
+    env = os.environ.get("OMP_NUM_THREADS")
+    return max(1, int(env)) if env and env.isdigit() else (os.cpu_count() or 1)
+
+
+def cpu_baseline(L, csr, threads, converge_scale=20):
+    """The oracle on the host, timed on the GPU step's own work: the level-0
+    phase from singletons (its sweeps with the modularity after each, the
+    termination test of the reference's loop), once on 1 core and once on
+    `threads` cores (same results: decisions inside a degree group are
+    independent); then the oracle's whole multi-level run of RMAT-`converge_scale`
+    (seed = scale) on `threads` cores, time to converge."""
     sys.path.insert(0, os.path.join(ROOT, "oracle"))
     import numpy as np
 
     import oracle as O
 
     og = O.Graph.from_csr(csr)
-    comm = np.arange(csr.V, dtype=np.int32)
-    t0 = time.perf_counter()
-    moved = O.sweep(og, comm, threads=threads)
-    dt = time.perf_counter() - t0
-    return {"value": csr.E / dt, "unit": "edges/s", "cores": threads, "kind": "port",
-            "sample": f"one oracle sweep (all 7 degree groups) over the same RMAT-{csr_scale(csr)} graph "
-                      f"from singletons, {moved} moves, {dt:.2f} s"}
+    M = O.total_weight(og)
+    legs = {}
+    comms = {}
+    for t in (1, threads):
+        comm = np.arange(csr.V, dtype=np.int32)
+        o2c = np.arange(csr.V, dtype=np.int32)
+        t0 = time.perf_counter()
+        _, sweeps = O.phase(og, comm, 1e-6, o2c, M, threads=t)
+        dt = time.perf_counter() - t0
+        legs[t] = {"value": csr.E * sweeps / dt, "seconds": dt, "sweeps": sweeps, "threads": t}
+        comms[t] = comm
+        log(f"cpu baseline: level-0 phase on {t} thread(s): {sweeps} sweeps in {dt:.2f} s, "
+            f"{legs[t]['value']:.3e} edges/s")
+    same = bool(np.array_equal(comms[1], comms[threads]))
+    g2 = L.gen_rmat(converge_scale, 16, 0.57, 0.19, 0.19, seed=converge_scale)
+    st, _, _ = O.run(O.Graph.from_csr(g2), 1e-6, threads=threads)
+    log(f"cpu baseline: RMAT-{converge_scale} converged in {st['seconds']:.2f} s on {threads} threads "
+        f"({st['levels']} levels, {st['sweeps']} sweeps, Q={st['modularity']:.6f})")
+    model, ncpu = cpu_model()
+    top = legs[threads]
+    return {"value": top["value"], "unit": "edges/s", "cores": threads, "kind": "port",
+            "sample": f"the oracle's level-0 phase of the same RMAT-{csr_scale(csr)} graph from singletons "
+                      f"({top['sweeps']} sweeps with the modularity after each), the GPU step's work",
+            "cores_1": legs[1], "cores_all": top, "identical_results": same,
+            "cpu_model": model, "host_cpus": ncpu,
+            "time_to_converge_ms": st["seconds"] * 1e3,
+            "converge_workload": f"RMAT-{converge_scale} seed {converge_scale}, {threads} threads, "
+                                 f"{st['levels']} levels, {st['sweeps']} sweeps, Q={st['modularity']:.6f}",
+            "converge_graph": {"V": g2.V, "E": g2.E}}
+
+
+def gpu_converge(L, scale):
+    """The library's time to converge on RMAT-`scale` (seed = scale), the
+    second of two runs."""
+    g = L.gen_rmat(scale, 16, 0.57, 0.19, 0.19, seed=scale)
+    with L.Graph(g) as G:
+        G.run(1e-6)
+        G.reset()
+        st = G.run(1e-6)
+    return {"time_to_converge_ms": st["algo_ms"], "modularity": st["modularity"], "levels": st["levels"],
+            "sweeps": st["sweeps"], "V": g.V, "E": g.E}
 
 
 def calibrate_copy():
@@ -322,7 +383,8 @@ def main():
     ap.add_argument("--warmup", type=int, default=3)
     ap.add_argument("--scale", type=int, default=22)
     ap.add_argument("--seed", type=int, default=None)
-    ap.add_argument("--cpu-threads", type=int, default=1)
+    ap.add_argument("--cpu-threads", type=int, default=0,
+                    help="threads of the all-cores oracle run (0 = OMP_NUM_THREADS or every CPU)")
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--no-converge", action="store_true")
     ap.add_argument("--no-pruned", action="store_true")
@@ -470,6 +532,14 @@ def main():
                     "traffic": traffic_for(d["kernel"], args.scale) if args.level == 0 else None,
                     "bytes_per_launch": d["bytes_per_launch"], "avg_launch_ms": d["avg_ms"],
                     "share_of_sweep": d["ms_per_sweep"] / ms_instrumented}
+    if roofline is not None and args.level == 0:
+        # the whole plain sweep against the same peak: 24 B per CSR entry and
+        # 64 B per vertex of algorithmic bytes (moves 12E + 32V, apply 16V,
+        # modularity 12E + 16V) over the uninstrumented step time
+        sweep_bytes = 24.0 * csr.E + 64.0 * csr.V
+        roofline["sweep_bytes"] = sweep_bytes
+        roofline["sweep_achieved"] = sweep_bytes / (ms_per_step * 1e-3) / 1e9
+        roofline["sweep_frac"] = roofline["sweep_achieved"] / HBM_PEAK_GBS
 
     G.close()
 
@@ -484,9 +554,10 @@ def main():
 
     cpu = None
     if not args.no_cpu_baseline:
-        log("cpu baseline: one oracle sweep ...")
-        cpu = cpu_baseline(csr, args.cpu_threads)
-        log(f"cpu baseline: {cpu['value']:.3e} edges/s")
+        threads = args.cpu_threads if args.cpu_threads > 0 else host_threads()
+        cpu = cpu_baseline(L, csr, threads)
+        cpu["gpu_time_to_converge_ms_same_graph"] = gpu_converge(L, 20)["time_to_converge_ms"]
+        log(f"cpu baseline: {cpu['value']:.3e} edges/s on {threads} threads ({cpu['cpu_model']})")
 
     value = E_step / (ms_per_step * 1e-3)
     out = {
