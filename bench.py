@@ -185,7 +185,7 @@ def pruned_comparison(L, csr, min_gain=1e-6):
     return out, (params, thr)
 
 
-def batch_comparison(L, n_graphs=4096, seed=4, sample=64, min_gain=1e-6, model=None):
+def batch_comparison(L, n_graphs=4096, seed=4, sample=64, min_gain=1e-6, model=None, tune=True):
     """4,096 independent planted-partition graphs solved as one batch: sizes
     log-uniform in [200, 50000], blocks uniform in 4..64, p_in for mean
     in-block degree 12, p_out for mean cross-block degree 2, unit weights.
@@ -208,10 +208,28 @@ def batch_comparison(L, n_graphs=4096, seed=4, sample=64, min_gain=1e-6, model=N
     V = sum(g.V for g in gs)
     E = sum(g.E for g in gs)
     log(f"batch: {n_graphs} planted graphs, V={V} E={E}, generated in {gen_s:.1f}s")
+    tuned = None
+    if tune:
+        import autotune as T
+
+        # the tuner times whole batched runs of a sample of the batch's graphs
+        bt = T.tune_batch(gs[:256], cache_path=None)
+        log(f"batch autotune: {bt['tried']} candidates on 256 graphs, {bt['default_ms']:.2f} -> "
+            f"{bt['ms']:.2f} ms with {bt['options']}")
     with L.Batch(gs) as B:
         B.run(min_gain)  # warm
         per, tot = B.run(min_gain)
         comms = B.assignment()
+        if tune:
+            B.set_options(**bt["options"])
+            B.run(min_gain)  # warm
+            _, ttot = B.run(min_gain)
+            tcomms = B.assignment()
+            same = all(np.array_equal(a, b) for a, b in zip(comms, tcomms))
+            tuned = {"options": bt["options"], "sample_default_ms": bt["default_ms"], "sample_tuned_ms": bt["ms"],
+                     "candidates": bt["tried"], "default_ms": tot["algo_ms"], "tuned_ms": ttot["algo_ms"],
+                     "gain": 1.0 - ttot["algo_ms"] / tot["algo_ms"], "identical_results": bool(same)}
+            log(f"batch: tuned options {ttot['algo_ms']:.1f} ms vs {tot['algo_ms']:.1f} ms, identical {same}")
     log(f"batch: solved in {tot['algo_ms']:.1f} ms ({tot['levels']} levels, {tot['sweeps']} union sweeps)")
     single_ms = 0.0
     mismatches = 0
@@ -260,7 +278,7 @@ def batch_comparison(L, n_graphs=4096, seed=4, sample=64, min_gain=1e-6, model=N
         log(f"batch learned-pruned: {ltot['algo_ms']:.1f} ms ({learned['time_ratio_vs_plain_batch']:.3f}x plain "
             f"batch), mean dQ_rel {learned['mean_dQ_rel']:.4f}, {ltot['evaluated']} evaluations vs "
             f"{tot['evaluated']}; {lmis} mismatches vs one-by-one")
-    return {"learned_pruned": learned, "graphs": n_graphs, "V": V, "E": E, "time_ms": tot["algo_ms"],
+    return {"learned_pruned": learned, "autotune": tuned, "graphs": n_graphs, "V": V, "E": E, "time_ms": tot["algo_ms"],
             "graphs_per_s": n_graphs / (tot["algo_ms"] * 1e-3), "edges_per_s": E / (tot["algo_ms"] * 1e-3),
             "union_levels": tot["levels"], "union_sweeps": tot["sweeps"],
             "aggregate_ms": tot["aggregate_ms"], "upload_ms": tot["memory_ms"],
@@ -548,7 +566,7 @@ def main():
         pruned, model = pruned_comparison(L, csr)
     batch = None
     if not args.no_batch:
-        batch = batch_comparison(L, model=model)
+        batch = batch_comparison(L, model=model, tune=not args.no_tune)
 
     io = io_comparison(L, csr) if args.io else None
 

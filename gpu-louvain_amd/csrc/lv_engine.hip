@@ -341,6 +341,8 @@ struct lv_graph {
     lv::BucketBounds bb{};
     // LDS slots allocated per launch of each one-workgroup bucket
     int wg_slots[4] = {768, 1536, 2560, 6144};
+    int wg_lanes[4] = {128, 256, 256, 1024};  // lanes per row of each tier
+    int table_pct = 0;                        // per-row table size, percent (0 = 150)
     bool heavy_try = false;  // LDS attempt of the chunked tier's rows
     int32_t heavy_limit = lv::kHeavyTryLimit;  // ... given up at this many communities
     int chunk_edges = lv::kChunkEdges, part_target = lv::kPartTarget, merge_limit = lv::kMergeLimit;
@@ -660,26 +662,44 @@ struct lv_graph {
         // Bucket bounds: 0..5 are the reference groups; group 6 splits into
         // four one-workgroup tiers and the chunked tier.  The upper tier's
         // 72 KB table needs that much LDS per workgroup.
-        const int32_t cap = 12 * (kWgMax + kWgMax / 2) <= max_lds ? kWgMax : kWgMid;
+        // the largest one-workgroup bound whose table (up to 2 x the bound,
+        // 12 bytes a slot) fits a workgroup's LDS
+        int32_t cap = kWgMax;
+        while (cap > 320 && 24 * cap > max_lds) cap /= 2;
         int wg_max = opt.heavy_wg_max > 0 ? opt.heavy_wg_max : cap;
         wg_max = std::max(320, std::min(wg_max, cap));
-        const int32_t wg_a = std::min(wg_max, kWgA);
-        const int32_t wg_lo = std::min(wg_max, kWgSplit);
-        const int32_t wg_mid = std::min(wg_max, kWgMid);
-        const int32_t his[] = {4,   8,    16,    32,     84,     kWave256Hi,
+        // group 6's tier bounds, ascending and at most wg_max (empty tiers
+        // are allowed); group 5's split between the two wave tables
+        auto bound = [&](int32_t v, int32_t def, int32_t lo) {
+            return std::max(lo, std::min(wg_max, v > 0 ? v : def));
+        };
+        const int32_t wg_a = bound(opt.wg_bound_a, kWgA, 320);
+        const int32_t wg_lo = bound(opt.wg_bound_b, kWgSplit, wg_a);
+        const int32_t wg_mid = bound(opt.wg_bound_c, kWgMid, wg_lo);
+        const int32_t wsplit =
+            opt.wave_split > 0 ? std::max(85, std::min(255, opt.wave_split)) : kWave256Hi;
+        const int32_t his[] = {4,   8,    16,    32,     84,     wsplit,
                                319, wg_a, wg_lo, wg_mid, wg_max, 0x7fffffff};
         static_assert(sizeof(his) / sizeof(his[0]) == lv::kNumBuckets, "one bound per bucket");
         for (int b = 0; b < lv::kNumBuckets; ++b) bb.hi[b] = his[b];
-        // tables of 1.5 x the bucket's degree bound (load <= 2/3), in 64-slot
-        // steps and at least 512: smaller tables, more rows in flight per CU
-        auto slots_for = [](int32_t bound) {
-            return std::max(512, ((bound + bound / 2) + 63) & ~63);
+        // tables of 1.5 x the bucket's degree bound (load <= 2/3) by default,
+        // 1.25 x for 1025..2048 (five workgroups per CU instead of four), in
+        // 64-slot steps and at least 512: smaller tables, more rows in flight
+        // per CU.  Every table holds more slots than its bound (>= 1.2 x).
+        table_pct = opt.table_pct > 0 ? std::max(120, std::min(200, opt.table_pct)) : 0;
+        auto slots_for = [&](int32_t b, int def_pct) {
+            const int64_t p = table_pct ? table_pct : def_pct;
+            return (int)std::max<int64_t>(512, ((int64_t)b * p / 100 + 63) & ~(int64_t)63);
         };
-        wg_slots[0] = slots_for(wg_a);
-        wg_slots[1] = slots_for(wg_lo);
-        // 1025..2048: 1.25 x (load <= 0.8), five workgroups per CU instead of four
-        wg_slots[2] = std::max(512, ((wg_mid + wg_mid / 4) + 63) & ~63);
-        wg_slots[3] = slots_for(wg_max);
+        wg_slots[0] = slots_for(wg_a, 150);
+        wg_slots[1] = slots_for(wg_lo, 150);
+        wg_slots[2] = slots_for(wg_mid, 125);
+        wg_slots[3] = slots_for(wg_max, 150);
+        static const int kDefLanes[4] = {128, 256, 256, 1024};
+        for (int t = 0; t < 4; ++t) {
+            const int l = opt.wg_lanes[t];
+            wg_lanes[t] = (l == 128 || l == 256 || l == 512 || l == 1024) ? l : kDefLanes[t];
+        }
         heavy_try = opt.heavy_lds >= 0 && 12 * kHeavyTrySlots <= max_lds;
         heavy_limit = opt.heavy_lds > 0 ? std::min(opt.heavy_lds, lv::kHeavyTryLimit)
                                         : lv::kHeavyTryLimit;
@@ -1240,14 +1260,16 @@ struct lv_graph {
                 break;
             case 4: case 5: case 6:
                 t0 = tbegin();
-                lv::launch_move_wave(b == 6 ? 448 : 256, b - 4, a, list, n, s);
+                lv::launch_move_wave(b == 6 ? 448 : 256, b == 5 && bb.hi[5] > 128 ? 2 : b - 4,
+                                     a, list, n, s);
                 tend(t0, LV_K_MOVE_WAVE, bytes, LV_KD_MOVE_BUCKET0 + b);
                 break;
             case 7: case 8: case 9: case 10: {
                 static const int kd[4] = {LV_KD_MOVE_BUCKET0 + 7, LV_KD_MOVE_BUCKET8,
                                           LV_KD_MOVE_BUCKET8 + 1, LV_KD_MOVE_BUCKET8 + 2};
                 t0 = tbegin();
-                lv::launch_move_wg(b - 7, wg_slots[b - 7], a, list, n, s);
+                lv::launch_move_wg(b - 7, wg_lanes[b - 7], wg_slots[b - 7],
+                                   table_pct ? table_pct : 150, a, list, n, s);
                 tend(t0, LV_K_MOVE_WG, bytes, kd[b - 7]);
                 break;
             }
@@ -3091,6 +3113,23 @@ lv_status lv_batch_assignment(lv_batch *b, int32_t *comm) {
     return g->sync();
 }
 
+lv_status lv_batch_set_options(lv_batch *b, const lv_options *opt) {
+    if (!b || !opt) return fail(LV_ERR_ARG, "null argument");
+    lv_graph *g = b->g;
+    if (opt->device != g->device) return fail(LV_ERR_ARG, "the device of a batch cannot change");
+    LV_HIP(hipSetDevice(g->device));
+    LV_TRY(g->sync());
+    const lv_options was = g->opt;
+    g->opt = *opt;
+    g->opt.timing = was.timing;
+    g->opt.compact = was.compact;  // batches store every vertex
+    g->opt.exact_sums = was.exact_sums;
+    g->apply_options();
+    // every run starts from level 0 (reset), which is bucketed again
+    g->level_ready = false;
+    return LV_OK;
+}
+
 lv_status lv_batch_kernel_timing(lv_batch *b, int32_t kernel, lv_kernel_time *out) {
     if (!b) return fail(LV_ERR_ARG, "null batch");
     return lv_kernel_timing(b->g, kernel, out);
@@ -3101,9 +3140,12 @@ lv_status lv_set_options(lv_graph *g, const lv_options *opt) {
     if (opt->device != g->device) return fail(LV_ERR_ARG, "the device of a handle cannot change");
     LV_HIP(hipSetDevice(g->device));
     LV_TRY(g->sync());
-    const int32_t timing = g->opt.timing;
+    const lv_options was = g->opt;
     g->opt = *opt;
-    g->opt.timing = timing;
+    // fixed at creation: the timing flag, the level storage and the weights
+    g->opt.timing = was.timing;
+    g->opt.compact = was.compact;
+    g->opt.exact_sums = was.exact_sums;
     g->apply_options();
     // the current level is bucketed again before its next use; its
     // community state is kept

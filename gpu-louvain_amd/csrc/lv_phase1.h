@@ -114,14 +114,17 @@ void launch_records_all(int32_t V, const double *sigma, const int32_t *size, uin
 
 // --- moves ---
 void launch_move_sub(int X, const MoveArgs &a, const int32_t *list, int32_t n, hipStream_t s);
-// cap: LDS slots per wave (256 or 512); inst names the 256-slot instance.
+// cap: LDS slots per wave (256 or 448); inst names the 256-slot instance:
+// 0 for degrees <= 84, 1 for degrees <= 128, 2 for degrees <= 255.
 void launch_move_wave(int cap, int inst, const MoveArgs &a, const int32_t *list, int32_t n,
                       hipStream_t s);
 // tier 0..3: the workgroup buckets in degree order (separate kernel
-// instances; tier 3 runs 1024 lanes per vertex).
-// slots: LDS table entries allocated per workgroup (12 bytes each).
-void launch_move_wg(int tier, int slots, const MoveArgs &a, const int32_t *list, int32_t n,
-                    hipStream_t s);
+// instances), lanes (128, 256, 512 or 1024) per vertex.
+// slots: LDS table entries allocated per workgroup (12 bytes each, more than
+// the tier's degree bound); pct: a row's table is pct % of its degree
+// (at least 512 slots, at most the allocation).
+void launch_move_wg(int tier, int lanes, int slots, int pct, const MoveArgs &a,
+                    const int32_t *list, int32_t n, hipStream_t s);
 void set_wg_lds_limits(int bytes);
 // LDS attempt of the chunked tier's rows (list = its heavy vertices, in the
 // order of t.hvs), before its stages; fills t.done and t.hint.  A row is

@@ -169,16 +169,20 @@ __global__ __launch_bounds__(256) void k_move_sub(MoveArgs a, const int32_t *__r
     if (lane == 0) write_decision(a, v, cur, best, bestK, kc);
 }
 
-// C: LDS slots per wave, a multiple of 64; G only names the instance (one
-// per bucket).  A power of two hashes by shift, other sizes by multiply-shift
-// (448 slots for degrees <= 319: load <= 0.71 and 7 blocks per CU).
+// C: LDS slots per wave, a multiple of 64; G names the instance (one per
+// bucket).  A power of two hashes by shift, other sizes by multiply-shift
+// (448 slots for degrees <= 319: load <= 0.71 and 7 blocks per CU).  Each
+// lane holds MAXE row entries: 256-slot tables take degrees <= 128 (2 per
+// lane), or up to 255 (4 per lane) in instance G = 2, for a group-5 split
+// above 128.
 template <int C, int G>
 __global__ __launch_bounds__(256) void k_move_wave(MoveArgs a, const int32_t *__restrict__ list,
                                                    int32_t n) {
     static_assert(C % 64 == 0, "whole slots per lane");
     constexpr bool POW2 = (C & (C - 1)) == 0;
     constexpr int LOG2C = ilog2_c(C);
-    constexpr int MAXE = C == 256 ? 2 : 5;  // edges per lane: deg <= 84 / <= 319
+    constexpr int MAXE = C == 256 ? (G == 2 ? 4 : 2) : 5;  // edges per lane
+    static_assert(64 * MAXE >= (C == 256 ? (G == 2 ? 255 : 128) : 319), "a row fits the lanes");
     constexpr int S = C / 64;               // slots per lane
     __shared__ int32_t keys_s[4 * C];
     __shared__ double vals_s[4 * C];
@@ -297,7 +301,7 @@ __global__ __launch_bounds__(256) void k_move_wave(MoveArgs a, const int32_t *__
 // sweep's frontier are skipped, as the chunked tier skips them.
 template <int T, int TIER>
 __global__ __launch_bounds__(T, 8) void k_move_wg(MoveArgs a, const int32_t *__restrict__ list,
-                                                  int32_t n, int cap, int32_t *hint,
+                                                  int32_t n, int cap, int pct, int32_t *hint,
                                                   int32_t *done, int32_t limit) {
     extern __shared__ double dyn_lds[];
     const int C = cap;  // allocation (slots): W[C] then K[C]
@@ -317,10 +321,12 @@ __global__ __launch_bounds__(T, 8) void k_move_wg(MoveArgs a, const int32_t *__r
             if (a.active && !a.active[v]) continue;  // not in this sweep's frontier
         }
         const int32_t rs = a.rowptr[v], re = a.rowptr[v + 1];
-        // table of the vertex: 1.5 x its degree (load <= 2/3), in 64-slot
-        // steps, at least 512 slots and at most the allocation
+        // table of the vertex: pct % of its degree (150: load <= 2/3), in
+        // 64-slot steps, at least 512 slots and at most the allocation (which
+        // holds more slots than the tier's degree bound)
         const int32_t deg = re - rs;
-        const int Cv = TIER == 4 ? C : min(C, max(512, ((deg + deg / 2) + 63) & ~63));
+        const int Cv =
+            TIER == 4 ? C : min(C, max(512, (int)(((int64_t)deg * pct / 100 + 63) & ~(int64_t)63)));
         for (int i = threadIdx.x; i < Cv; i += T) {
             K[i] = -1;
             W[i] = 0.0;
@@ -1454,28 +1460,51 @@ void launch_move_wave(int cap, int inst, const MoveArgs &a, const int32_t *list,
         hipLaunchKernelGGL((k_move_wave<448, 0>), dim3(g), dim3(256), 0, s, a, list, n);
     else if (inst == 0)
         hipLaunchKernelGGL((k_move_wave<256, 0>), dim3(g), dim3(256), 0, s, a, list, n);
-    else
+    else if (inst == 1)
         hipLaunchKernelGGL((k_move_wave<256, 1>), dim3(g), dim3(256), 0, s, a, list, n);
+    else
+        hipLaunchKernelGGL((k_move_wave<256, 2>), dim3(g), dim3(256), 0, s, a, list, n);
 }
 
-void launch_move_wg(int tier, int slots, const MoveArgs &a, const int32_t *list, int32_t n,
-                    hipStream_t s) {
+namespace {
+// One workgroup tier at `lanes` lanes per row (instances per tier, so that
+// each tier is its own kernel in traces).
+template <int TIER>
+void launch_wg_tier(int lanes, unsigned g, size_t lds, hipStream_t s, const MoveArgs &a,
+                    const int32_t *list, int32_t n, int slots, int pct) {
+    int32_t *no = nullptr;
+    switch (lanes) {
+    case 128:
+        hipLaunchKernelGGL((k_move_wg<128, TIER>), dim3(g), dim3(128), lds, s, a, list, n, slots,
+                           pct, no, no, 0);
+        break;
+    case 256:
+        hipLaunchKernelGGL((k_move_wg<256, TIER>), dim3(g), dim3(256), lds, s, a, list, n, slots,
+                           pct, no, no, 0);
+        break;
+    case 512:
+        hipLaunchKernelGGL((k_move_wg<512, TIER>), dim3(g), dim3(512), lds, s, a, list, n, slots,
+                           pct, no, no, 0);
+        break;
+    default:
+        hipLaunchKernelGGL((k_move_wg<1024, TIER>), dim3(g), dim3(1024), lds, s, a, list, n, slots,
+                           pct, no, no, 0);
+        break;
+    }
+}
+}  // namespace
+
+void launch_move_wg(int tier, int lanes, int slots, int pct, const MoveArgs &a,
+                    const int32_t *list, int32_t n, hipStream_t s) {
     if (n <= 0) return;
     const size_t lds = (size_t)12 * (size_t)slots;
     const unsigned g = (unsigned)(n < 1048576 ? n : 1048576);
-    int32_t *no = nullptr;
-    if (tier == 0)
-        hipLaunchKernelGGL((k_move_wg<128, 0>), dim3(g), dim3(128), lds, s, a, list, n, slots, no,
-                           no, 0);
-    else if (tier == 1)
-        hipLaunchKernelGGL((k_move_wg<256, 1>), dim3(g), dim3(256), lds, s, a, list, n, slots, no,
-                           no, 0);
-    else if (tier == 2)
-        hipLaunchKernelGGL((k_move_wg<256, 2>), dim3(g), dim3(256), lds, s, a, list, n, slots, no,
-                           no, 0);
-    else
-        hipLaunchKernelGGL((k_move_wg<1024, 3>), dim3(g), dim3(1024), lds, s, a, list, n, slots,
-                           no, no, 0);
+    switch (tier) {
+    case 0: launch_wg_tier<0>(lanes, g, lds, s, a, list, n, slots, pct); break;
+    case 1: launch_wg_tier<1>(lanes, g, lds, s, a, list, n, slots, pct); break;
+    case 2: launch_wg_tier<2>(lanes, g, lds, s, a, list, n, slots, pct); break;
+    default: launch_wg_tier<3>(lanes, g, lds, s, a, list, n, slots, pct); break;
+    }
 }
 
 void launch_heavy_try(int slots, int32_t limit, const MoveArgs &a, const int32_t *list,
@@ -1485,7 +1514,7 @@ void launch_heavy_try(int slots, int32_t limit, const MoveArgs &a, const int32_t
     const size_t lds = (size_t)12 * (size_t)slots;
     const unsigned g = (unsigned)(n < 1048576 ? n : 1048576);
     hipLaunchKernelGGL((k_move_wg<1024, 4>), dim3(g), dim3(1024), lds, s, a, list, n, slots,
-                       t.hint, t.done, limit);
+                       100, t.hint, t.done, limit);
 }
 
 void launch_chunk_aggregate(const MoveArgs &a, const ChunkedTier &t, int32_t nchunks,
@@ -1609,15 +1638,25 @@ void launch_compose(int32_t *orig2comm, int32_t V0, const int32_t *map, hipStrea
     hipLaunchKernelGGL(k_compose, dim3(blocks_for(V0, 256)), dim3(256), 0, s, orig2comm, V0, map);
 }
 
+namespace {
+template <int TIER>
+void set_tier_lds_limits(int bytes) {
+    (void)hipFuncSetAttribute(reinterpret_cast<const void *>(&k_move_wg<128, TIER>),
+                              hipFuncAttributeMaxDynamicSharedMemorySize, bytes);
+    (void)hipFuncSetAttribute(reinterpret_cast<const void *>(&k_move_wg<256, TIER>),
+                              hipFuncAttributeMaxDynamicSharedMemorySize, bytes);
+    (void)hipFuncSetAttribute(reinterpret_cast<const void *>(&k_move_wg<512, TIER>),
+                              hipFuncAttributeMaxDynamicSharedMemorySize, bytes);
+    (void)hipFuncSetAttribute(reinterpret_cast<const void *>(&k_move_wg<1024, TIER>),
+                              hipFuncAttributeMaxDynamicSharedMemorySize, bytes);
+}
+}  // namespace
+
 void set_wg_lds_limits(int bytes) {
-    (void)hipFuncSetAttribute(reinterpret_cast<const void *>(&k_move_wg<128, 0>),
-                              hipFuncAttributeMaxDynamicSharedMemorySize, bytes);
-    (void)hipFuncSetAttribute(reinterpret_cast<const void *>(&k_move_wg<256, 1>),
-                              hipFuncAttributeMaxDynamicSharedMemorySize, bytes);
-    (void)hipFuncSetAttribute(reinterpret_cast<const void *>(&k_move_wg<256, 2>),
-                              hipFuncAttributeMaxDynamicSharedMemorySize, bytes);
-    (void)hipFuncSetAttribute(reinterpret_cast<const void *>(&k_move_wg<1024, 3>),
-                              hipFuncAttributeMaxDynamicSharedMemorySize, bytes);
+    set_tier_lds_limits<0>(bytes);
+    set_tier_lds_limits<1>(bytes);
+    set_tier_lds_limits<2>(bytes);
+    set_tier_lds_limits<3>(bytes);
     (void)hipFuncSetAttribute(reinterpret_cast<const void *>(&k_move_wg<1024, 4>),
                               hipFuncAttributeMaxDynamicSharedMemorySize, bytes);
 }

@@ -80,6 +80,12 @@ class Options(ctypes.Structure):
         ("candidate_bounds", ctypes.c_int32),
         ("heavy_lds", ctypes.c_int32),
         ("exact_sums", ctypes.c_int32),
+        ("wave_split", ctypes.c_int32),
+        ("wg_bound_a", ctypes.c_int32),
+        ("wg_bound_b", ctypes.c_int32),
+        ("wg_bound_c", ctypes.c_int32),
+        ("wg_lanes", ctypes.c_int32 * 4),
+        ("table_pct", ctypes.c_int32),
         ("reserved", ctypes.c_int32 * 2),
     ]
 
@@ -356,10 +362,9 @@ def gen_planted(n: int, blocks: int, p_in: float, p_out: float, seed: int = 1,
 class Graph:
     """A graph resident on one GPU with its community state (lv_graph)."""
 
-    def __init__(self, g: CSR, device: int = 0, timing: bool = False, heavy_wg_max: int = 0,
-                 chunk_edges: int = 0, part_target: int = 0, merge_limit: int = 0, agg_lds_max: int = 0,
-                 q_incremental: int = 0, xcd_rows: int = 0, agg_mode: int = 0, compact: int = 0,
-                 candidate_bounds: int = 0, heavy_lds: int = 0, exact_sums: int = 0):
+    def __init__(self, g: CSR, device: int = 0, timing: bool = False, **options):
+        """options: lv_options fields (heavy_wg_max, chunk_edges, q_incremental,
+        exact_sums, wg_lanes=(a, b, c, d), ...); 0 = the library default."""
         lib = load()
         self._lib = lib
         self._rowptr = np.ascontiguousarray(g.rowptr, dtype=np.int32)
@@ -367,21 +372,7 @@ class Graph:
         self._w = np.ascontiguousarray(g.w, dtype=np.float32)
         self.V0 = int(self._rowptr.shape[0] - 1)
         self.device = device
-        opt = Options()
-        opt.device = device
-        opt.timing = int(timing)
-        opt.heavy_wg_max = heavy_wg_max
-        opt.chunk_edges = chunk_edges
-        opt.part_target = part_target
-        opt.merge_limit = merge_limit
-        opt.agg_lds_max = agg_lds_max
-        opt.q_incremental = q_incremental
-        opt.xcd_rows = xcd_rows
-        opt.agg_mode = agg_mode
-        opt.compact = compact
-        opt.candidate_bounds = candidate_bounds
-        opt.heavy_lds = heavy_lds
-        opt.exact_sums = exact_sums
+        opt = options_struct(device, timing, **options)
         h = ctypes.c_void_p()
         col = self._col if self._col.size else np.zeros(1, np.int32)
         w = self._w if self._w.size else np.zeros(1, np.float32)
@@ -502,21 +493,10 @@ class Graph:
         _check(self._lib.lv_kernel_timing(self._h, kernel, ctypes.byref(kt)))
         return {"launches": kt.launches, "total_ms": kt.total_ms, "bytes": kt.bytes}
 
-    def set_options(self, heavy_wg_max: int = 0, chunk_edges: int = 0, part_target: int = 0,
-                    merge_limit: int = 0, agg_lds_max: int = 0, q_incremental: int = 0,
-                    xcd_rows: int = 0, agg_mode: int = 0, heavy_lds: int = 0) -> None:
-        """New tier options for this handle (results do not depend on them)."""
-        opt = Options()
-        opt.device = self.device
-        opt.heavy_wg_max = heavy_wg_max
-        opt.chunk_edges = chunk_edges
-        opt.part_target = part_target
-        opt.merge_limit = merge_limit
-        opt.agg_lds_max = agg_lds_max
-        opt.q_incremental = q_incremental
-        opt.xcd_rows = xcd_rows
-        opt.agg_mode = agg_mode
-        opt.heavy_lds = heavy_lds
+    def set_options(self, **options) -> None:
+        """New tier options for this handle (results do not depend on them);
+        fields left out take the library default."""
+        opt = options_struct(self.device, False, **options)
         _check(self._lib.lv_set_options(self._h, ctypes.byref(opt)))
 
     def kernel_timing_detail(self, kernel: int) -> dict:
@@ -643,6 +623,23 @@ class BatchResult(ctypes.Structure):
                 ("final_V", ctypes.c_int32), ("reserved", ctypes.c_int32)]
 
 
+def options_struct(device: int = 0, timing: bool = False, **options) -> Options:
+    """An lv_options with the given fields (wg_lanes takes a sequence of 4)."""
+    opt = Options()
+    opt.device = device
+    opt.timing = int(timing)
+    fields = dict(Options._fields_)
+    for k, v in options.items():
+        if k not in fields or k in ("device", "timing", "reserved"):
+            raise TypeError(f"unknown option {k}")
+        if k == "wg_lanes":
+            v = list(v) + [0] * (4 - len(v))
+            opt.wg_lanes = (ctypes.c_int32 * 4)(*[int(x) for x in v[:4]])
+        else:
+            setattr(opt, k, int(v))
+    return opt
+
+
 def union(graphs) -> tuple[np.ndarray, CSR]:
     """Block-diagonal union of CSRs: (vptr [G+1], union CSR with global ids)."""
     vptr = np.zeros(len(graphs) + 1, np.int64)
@@ -675,19 +672,19 @@ class Batch:
         self.G = len(graphs)
         self._keep = (u.rowptr, u.col if u.col.size else np.zeros(1, np.int32),
                       u.w if u.w.size else np.zeros(1, np.float32))
-        opt = Options()
-        opt.device = device
-        opt.timing = int(timing)
-        for k, v in options.items():
-            if k not in dict(Options._fields_) or k in ("device", "timing", "reserved"):
-                raise TypeError(f"unknown option {k}")
-            setattr(opt, k, int(v))
+        self.device = device
+        opt = options_struct(device, timing, **options)
         h = ctypes.c_void_p()
         _check(lib.lv_batch_create(self.G, _p(self.vptr, ctypes.c_int32), _p(self._keep[0], ctypes.c_int32),
                                    _p(self._keep[1], ctypes.c_int32), _p(self._keep[2], ctypes.c_float),
                                    int(u.col.size), ctypes.byref(opt), ctypes.byref(h)))
         self._h = h
         self.V = int(self.vptr[-1])
+
+    def set_options(self, **options) -> None:
+        """New tier options between runs (lv_batch_set_options)."""
+        opt = options_struct(self.device, False, **options)
+        _check(self._lib.lv_batch_set_options(self._h, ctypes.byref(opt)))
 
     def run(self, min_gain: float) -> tuple[list[dict], dict]:
         res = (BatchResult * self.G)()
@@ -758,6 +755,7 @@ def _batch_sigs(lib) -> None:
         "lv_batch_run_pruned": (ctypes.c_int32, [ctypes.c_void_p, ctypes.c_float, ctypes.POINTER(Frontier),
                                                  ctypes.POINTER(BatchResult), ctypes.POINTER(Stats)]),
         "lv_batch_kernel_timing": (ctypes.c_int32, [ctypes.c_void_p, ctypes.c_int32, ctypes.POINTER(KernelTime)]),
+        "lv_batch_set_options": (ctypes.c_int32, [ctypes.c_void_p, ctypes.POINTER(Options)]),
     }
     for name, (res, args) in sig.items():
         fn = getattr(lib, name)

@@ -77,6 +77,19 @@ typedef struct {
                                 integers, are unchanged); -1: weights are used as
                                 given (sums of weights off the grid may then
                                 round differently from run to run)             */
+    /* Degree cuts and shapes of the tiers inside degree groups 5 and 6
+     * (autotuning; results do not depend on them): */
+    int32_t wave_split;       /* group 5 (85..319): degrees up to this run in
+                                256-slot wave tables, above in 448-slot ones
+                                (85..255; 0 = 128)                              */
+    int32_t wg_bound_a;       /* group 6 one-workgroup tiers: degree bounds of   */
+    int32_t wg_bound_b;       /* the first three (ascending, 320..heavy_wg_max;  */
+    int32_t wg_bound_c;       /* 0 = 512, 1024, 2048); the fourth ends at
+                                heavy_wg_max                                    */
+    int32_t wg_lanes[4];      /* lanes per row of the four one-workgroup tiers:
+                                128, 256, 512 or 1024 (0 = 128, 256, 256, 1024) */
+    int32_t table_pct;        /* LDS hash slots of those tiers per degree bound,
+                                percent (120..200; 0 = 150, 125 for the third)  */
     int32_t reserved[2];
 } lv_options;
 
@@ -208,9 +221,9 @@ lv_status lv_decide(lv_graph *g, int32_t group, int32_t *new_comm);
  * loop at modularity_optimisation.cu:308-354); *moved = vertices moved. */
 lv_status lv_sweep(lv_graph *g, double *q_after, int64_t *moved);
 /* New tier options for an existing handle (no counterpart in the reference;
- * for autotuning).  The device and the timing flag are kept; the current
- * level is re-bucketed and keeps its community state.  Results do not
- * depend on these options. */
+ * for autotuning).  The device, the timing flag, compact and exact_sums are
+ * kept from creation; the current level is re-bucketed and keeps its
+ * community state.  Results do not depend on these options. */
 lv_status lv_set_options(lv_graph *g, const lv_options *opt);
 /* Per-kernel timing, id in [0, LV_K_COUNT). */
 lv_status lv_kernel_timing(lv_graph *g, int32_t kernel, lv_kernel_time *out);
@@ -245,6 +258,9 @@ lv_status lv_batch_run(lv_batch *b, float min_gain, lv_batch_result *per_graph, 
  * lv_assignment numbers a single graph's communities. */
 lv_status lv_batch_assignment(lv_batch *b, int32_t *comm);
 lv_status lv_batch_kernel_timing(lv_batch *b, int32_t kernel, lv_kernel_time *out);
+/* New tier options for a batch between runs (as lv_set_options; for tuning
+ * the batch's launches; results do not depend on them). */
+lv_status lv_batch_set_options(lv_batch *b, const lv_options *opt);
 
 /* ---- learned frontier (no counterpart in the reference) ----
  * A move predictor (MLP 8 -> 32 -> 32 -> 1, ReLU, sigmoid; 1377 fp32
