@@ -65,7 +65,13 @@ struct MoveArgs {
     // cannot pass the move test are skipped as well.
     int32_t bounds;
     int32_t obs;
+    // Sweeps enqueued ahead of the termination test (a device-resident
+    // phase): null, or the phase's end flag; kernels return at once when it
+    // is set, so a sweep enqueued after the last one does nothing.
+    const int32_t *gate;
 };
+
+__device__ __forceinline__ bool gated_off(const int32_t *gate) { return gate && *gate != 0; }
 
 // M32 and den of vertex v's graph (.x, .y).
 __device__ __forceinline__ float2 vertex_mden(const MoveArgs &a, int32_t v) {

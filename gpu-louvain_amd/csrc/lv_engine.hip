@@ -202,6 +202,56 @@ struct DevVec {
 
 unsigned blocks_for(int64_t n, int per) { return (unsigned)((n + per - 1) / per); }
 
+// Control block of a device-resident phase (lv_graph::optimise_resident):
+// the last kernel of every sweep applies the reference's termination test
+// (modularity_optimisation.cu:308-354) to it, and the sweeps enqueued after
+// the phase has ended find `done` set and do nothing.
+struct PhaseCtl {
+    int32_t done;     // the phase has ended (the gate of its sweeps' kernels)
+    int32_t sweeps;   // sweeps run
+    int32_t changed;  // some sweep had a gain > 0
+    int32_t pad;
+    double q_prev;    // modularity before the next sweep
+    double q;         // after the last sweep run
+    unsigned long long moved, moved_edges;  // movers of the last sweep run, their row entries
+};
+constexpr int kCtlRing = 4;  // pinned copies of the block, one per sweep in flight
+
+__global__ void k_phase_start(PhaseCtl *c, const double *__restrict__ q) {
+    c->done = 0;
+    c->sweeps = 0;
+    c->changed = 0;
+    c->pad = 0;
+    c->q_prev = q[0];
+    c->q = q[0];
+    c->moved = 0;
+    c->moved_edges = 0;
+}
+
+// After sweep i: gain = Q after - Q before; a gain > 0 marks the phase as
+// changing the state; the phase ends when nothing moved, the gain is not
+// positive, or it is below min_gain (for min_gain > 0 the second test adds
+// nothing; it ends the phase for min_gain <= 0, where the reference can loop
+// forever).  The moves of the last sweep are kept, as the reference keeps them.
+__global__ void k_phase_step(PhaseCtl *c, const double *__restrict__ q,
+                             const unsigned long long *__restrict__ slots, float min_gain) {
+    if (c->done) return;
+    unsigned long long mv = 0, me = 0;
+    for (int i = 0; i < lv::kMoveSlots; ++i) {
+        mv += slots[(int64_t)i * lv::kMoveSlotStride];
+        me += slots[(int64_t)i * lv::kMoveSlotStride + 1];
+    }
+    const double q1 = q[0];
+    const double gain = q1 - c->q_prev;
+    if (gain > 0.0) c->changed = 1;
+    c->sweeps += 1;
+    c->moved = mv;
+    c->moved_edges = me;
+    c->q = q1;
+    c->q_prev = q1;
+    if (mv == 0 || !(gain > 0.0) || !(gain >= (double)min_gain)) c->done = 1;
+}
+
 // Two HIP events, destroyed on every return path.
 struct EventPair {
     hipEvent_t a = nullptr, b = nullptr;
@@ -388,6 +438,11 @@ struct lv_graph {
     double *d_q = nullptr;
     unsigned long long *d_moved = nullptr;
     double *d_own_slots = nullptr;   // the applies' modularity corrections (kept zero between sweeps)
+    // device-resident phases: the control block, its pinned host copies, and
+    // the gate the sweeps' kernels read (null outside such a phase)
+    PhaseCtl *d_ctl = nullptr;
+    PhaseCtl *h_ctl = nullptr;
+    const int32_t *gate_ptr = nullptr;
 
     // Levels after the first may keep only their vertices of positive degree
     // (compact ids, ascending in the reference's ids).  ref_of/cmp_of map
@@ -533,6 +588,7 @@ struct lv_graph {
                        ? 1
                        : 0;
         a.obs = 1;
+        a.gate = gate_ptr;
         return a;
     }
 
@@ -588,8 +644,10 @@ struct lv_graph {
 
     // The group's correction table for `rounds` (block starts by
     // lv::corr_items).
-    lv::SegTable corr_table(int g, int32_t rounds) const {
-        lv::SegTable t = gst[g];
+    lv::SegTable corr_table(int g, int32_t rounds) const { return corr_of(gst[g], rounds); }
+    // The correction blocks over a row table (a group's, or the level's st).
+    static lv::SegTable corr_of(const lv::SegTable &base, int32_t rounds) {
+        lv::SegTable t = base;
         int32_t gb = 0;
         for (int i = 0; i < t.nseg; ++i) {
             const int lg = t.s[i].lanes_log2;
@@ -785,6 +843,9 @@ struct lv_graph {
         LV_TRY(dev_alloc(&d_q, 8));
         LV_TRY(dev_alloc(&d_moved, lv::kMoveSlots * lv::kMoveSlotStride));
         LV_TRY(dev_alloc(&d_own_slots, lv::kMoveSlots * lv::kMoveSlotStride));
+        LV_TRY(dev_alloc(&d_ctl, 1));
+        LV_HIP(hipHostMalloc(reinterpret_cast<void **>(&h_ctl), sizeof(PhaseCtl) * kCtlRing,
+                             hipHostMallocDefault));
         lv::fill_f64(d_own_slots, 0.0, lv::kMoveSlots * lv::kMoveSlotStride, stream);
         LV_TRY(dev_alloc(&a_newid, Vp));
         LV_TRY(dev_alloc(&a_keep, Vp));
@@ -1331,8 +1392,10 @@ struct lv_graph {
             lv::launch_modularity_xcd(col(), w(), comm, order + xh_lo, xst, xsplit.p, qpart.p,
                                       st_light.total_blocks, stream);
             tend(tx, -1, 12.0 * e_heavy, LV_KD_MODULARITY_XCD);
-            lv::launch_modularity_reduce(st_light.total_blocks + 8 * xst.total_blocks, M, qpart.p,
-                                         d_q, stream);
+            lv::launch_modularity_reduce(st_light.total_blocks + 8 * xst.total_blocks,
+                                         st_light.sq_block_start,
+                                         st_light.total_blocks - st_light.sq_block_start, M,
+                                         qpart.p, d_q, stream);
             tend(t0, LV_K_MODULARITY, 12.0 * (double)E + 16.0 * (double)V);
             LV_LAUNCHED("modularity");
             own_valid = true;
@@ -1422,60 +1485,7 @@ struct lv_graph {
         if (pruned) LV_TRY(select_frontier(*fr));
         if (evaluated)
             *evaluated = pruned ? h_foff[lv::kNumBuckets] : h_off[lv::kNumBuckets];
-        sweep_count++;
-        lv::launch_hist_shift(hist, V, s);
-        hipLaunchKernelGGL(k_zero_u64_n, dim3(blocks_for(lv::kMoveSlots * lv::kMoveSlotStride, 256)),
-                           dim3(256), 0, s, d_moved, (int64_t)lv::kMoveSlots * lv::kMoveSlotStride);
-        const lv::MoveArgs a = move_args();
-        // correction rows per lane group: about one mover per block when the
-        // last sweep's share of movers holds
-        const int64_t nact = std::max<int64_t>(h_off[lv::kNumBuckets], 1);
-        const int32_t rounds = (int32_t)std::min<int64_t>(
-            1024, std::max<int64_t>(1, nact / std::max<int64_t>(last_moved, 1)));
-        for (int g = 0; g < lv::kNumGroups; ++g) {
-            int32_t lo, hi;
-            group_range(g, lo, hi);
-            if (lo < 0 || hi <= lo) continue;
-            LV_TRY(run_group(g, pruned, track));
-            hipEvent_t t0 = tbegin();
-            lv::GroupMarks gm = marks();
-            gm.rounds = rounds;
-            if (G > 0) {
-                gm.gid = gid;
-                gm.gin = gin;
-            }
-            const lv::SegTable ct = corr_table(g, rounds);
-            // pruned: only frontier vertices can move
-            int32_t alo = lo, ahi = hi;
-            if (pruned) group_range(g, alo, ahi, h_foff);
-            const int32_t *alist = (pruned ? forder : order) + alo;
-            lv::launch_apply(a, alist, ahi - alo, order, &ct, ins.p, comm, size, sigma, crec,
-                             incremental ? &gm : nullptr, hist, d_moved, d_own_slots, s);
-            tend(t0, LV_K_APPLY, 16.0 * (double)(ahi - alo));
-            LV_LAUNCHED("apply");
-            if (incremental) step++;
-        }
-        if (G > 0) {
-            // a recount walks whole graphs: the graph frontier lists them, the
-            // model's does not (then every row is walked)
-            LV_TRY(enqueue_graph_sums(pruned && fr->kind == kFrontierBatchGraphs, incremental));
-        } else if (incremental) {
-            hipEvent_t tq = tbegin();
-            lv::launch_modularity_from_own(sigma, V, M, qpart.p, d_q, d_own_slots, s);
-            tend(tq, LV_K_MODULARITY, 8.0 * (double)V);
-            LV_LAUNCHED("modularity");
-        } else {
-            LV_TRY(enqueue_modularity());
-        }
-        if (track) {
-            lv::fill_i32(nmv, 0, V, s);
-            // movers' neighbours lie in their own graph: the active graphs'
-            // list covers every count that changes
-            const bool gl = G > 0 && glist_ok;
-            lv::launch_neighbor_moves(rowptr(), col(), gl ? glist : order,
-                                      gl ? glist_n : h_off[lv::kNumBuckets], hist, nmv, s);
-            LV_LAUNCHED("neighbour moves");
-        }
+        LV_TRY(enqueue_sweep(pruned ? fr : nullptr, track, incremental, last_moved));
         if (e1) (void)hipEventRecord(e1, s);
         const size_t mv_bytes = sizeof(unsigned long long) * lv::kMoveSlots * lv::kMoveSlotStride;
         LV_TRY(stage(64 + mv_bytes));
@@ -1505,6 +1515,104 @@ struct lv_graph {
         }
         if (e0) ev_pool.push_back(e0);
         if (e1) ev_pool.push_back(e1);
+        return LV_OK;
+    }
+
+    // Correction rows per lane group: about one mover per block when the
+    // share of movers of an earlier sweep (prev_moved, -1 = unknown) holds.
+    int32_t rounds_for(int64_t prev_moved) const {
+        const int64_t nact = std::max<int64_t>(h_off[lv::kNumBuckets], 1);
+        return (int32_t)std::min<int64_t>(
+            1024, std::max<int64_t>(1, nact / std::max<int64_t>(prev_moved, 1)));
+    }
+
+    // The kernels of one sweep: groups 0..6 (moves, then the apply), then the
+    // modularity, and with track the neighbour moves.  fr: a pruned sweep
+    // over the frontier select_frontier built.  incremental: the modularity
+    // is carried forward from the last one (only the movers' rows are
+    // walked) instead of a full pass; both give the same value bit for bit.
+    lv_status enqueue_sweep(const lv_frontier *fr, bool track, bool incremental,
+                            int64_t prev_moved) {
+        const int32_t rounds = rounds_for(prev_moved);
+        LV_TRY(enqueue_groups(fr, track, incremental, rounds));
+        return enqueue_tail(fr, track, incremental, rounds);
+    }
+
+    // Groups 0..6 of a sweep.  Batches with an incremental sweep keep mover
+    // bitmaps and add each group's correction per graph; a single graph's
+    // weight inside communities is carried forward once per sweep
+    // (enqueue_tail).  Under a resident phase's gate these kernels do nothing
+    // once the phase has ended.
+    lv_status enqueue_groups(const lv_frontier *fr, bool track, bool incremental, int32_t rounds) {
+        hipStream_t s = stream;
+        const bool pruned = fr != nullptr;
+        const bool with_marks = incremental && G > 0;
+        // a single graph's movers in one bitmap for the sweep (the first half
+        // of bits), when its modularity may be carried forward
+        const bool sweep_marks = G == 0 && (incremental || gate_ptr != nullptr);
+        uint32_t *smovers = sweep_marks ? bits : nullptr;
+        sweep_count++;
+        if (sweep_marks) lv::fill_i32(reinterpret_cast<int32_t *>(bits), 0, bit_words, s);
+        lv::launch_hist_shift(hist, V, s, gate_ptr);
+        hipLaunchKernelGGL(k_zero_u64_n, dim3(blocks_for(lv::kMoveSlots * lv::kMoveSlotStride, 256)),
+                           dim3(256), 0, s, d_moved, (int64_t)lv::kMoveSlots * lv::kMoveSlotStride);
+        const lv::MoveArgs a = move_args();
+        for (int g = 0; g < lv::kNumGroups; ++g) {
+            int32_t lo, hi;
+            group_range(g, lo, hi);
+            if (lo < 0 || hi <= lo) continue;
+            LV_TRY(run_group(g, pruned, track));
+            hipEvent_t t0 = tbegin();
+            lv::GroupMarks gm = marks();
+            gm.rounds = rounds;
+            if (G > 0) {
+                gm.gid = gid;
+                gm.gin = gin;
+            }
+            const lv::SegTable ct = corr_table(g, rounds);
+            // pruned: only frontier vertices can move
+            int32_t alo = lo, ahi = hi;
+            if (pruned) group_range(g, alo, ahi, h_foff);
+            const int32_t *alist = (pruned ? forder : order) + alo;
+            lv::launch_apply(a, alist, ahi - alo, order, &ct, ins.p, comm, size, sigma, crec,
+                             with_marks ? &gm : nullptr, hist, d_moved, d_own_slots, s, smovers);
+            tend(t0, LV_K_APPLY, 16.0 * (double)(ahi - alo));
+            LV_LAUNCHED("apply");
+            if (with_marks) step++;
+        }
+        return LV_OK;
+    }
+
+    // The modularity after a sweep's groups (and the neighbour moves).  A
+    // single graph's incremental sweep adds the change of the weight inside
+    // communities over the movers' rows (newc holds, after a sweep, each
+    // vertex's community before it) and recomputes the sum of S_c^2.
+    lv_status enqueue_tail(const lv_frontier *fr, bool track, bool incremental, int32_t rounds) {
+        hipStream_t s = stream;
+        const bool pruned = fr != nullptr;
+        if (G > 0) {
+            // a recount walks whole graphs: the graph frontier lists them, the
+            // model's does not (then every row is walked)
+            LV_TRY(enqueue_graph_sums(pruned && fr->kind == kFrontierBatchGraphs, incremental));
+        } else if (incremental) {
+            hipEvent_t tq = tbegin();
+            lv::launch_sweep_delta(rowptr(), col(), w(), comm, newc, dk, bits, order,
+                                   corr_of(st, rounds), ins.p, rounds, d_own_slots, s);
+            lv::launch_modularity_from_own(sigma, V, M, qpart.p, d_q, d_own_slots, s);
+            tend(tq, LV_K_MODULARITY, 8.0 * (double)V);
+            LV_LAUNCHED("modularity");
+        } else {
+            LV_TRY(enqueue_modularity());
+        }
+        if (track) {
+            lv::fill_i32(nmv, 0, V, s);
+            // movers' neighbours lie in their own graph: the active graphs'
+            // list covers every count that changes
+            const bool gl = G > 0 && glist_ok;
+            lv::launch_neighbor_moves(rowptr(), col(), gl ? glist : order,
+                                      gl ? glist_n : h_off[lv::kNumBuckets], hist, nmv, s);
+            LV_LAUNCHED("neighbour moves");
+        }
         return LV_OK;
     }
 
@@ -1659,6 +1767,14 @@ struct lv_graph {
                        int64_t *evaluated_out = nullptr, int32_t *full_out = nullptr,
                        TraceSink *trace = nullptr) {
         if (!level_ready) LV_TRY(setup_level());
+        if (fr == nullptr && trace == nullptr && G == 0) {
+            int32_t sw = 0;
+            if (!sweeps_out) sweeps_out = &sw;
+            LV_TRY(optimise_resident(min_gain, changed, sweeps_out, ms_out, edges_out));
+            if (evaluated_out) *evaluated_out = (int64_t)*sweeps_out * h_off[lv::kNumBuckets];
+            if (full_out) *full_out = *sweeps_out;
+            return LV_OK;
+        }
         double q0 = 0.0;
         LV_TRY(modularity(&q0));
         const int k_full = fr ? (fr->full_every > 0 ? fr->full_every : 4) : 1;
@@ -1712,6 +1828,75 @@ struct lv_graph {
         if (edges_out) *edges_out = edges;
         if (evaluated_out) *evaluated_out = evaluated;
         if (full_out) *full_out = fulls;
+        return LV_OK;
+    }
+
+    // A plain phase resident on the device: the termination test runs on the
+    // device after every sweep (k_phase_step), and the host keeps the next
+    // sweep's groups enqueued while the current one runs instead of waiting
+    // for each sweep's modularity.  Iteration i enqueues sweep i's groups,
+    // then waits for sweep i-1's control block: if sweep i-1 ended the phase,
+    // sweep i's groups find the gate set and do nothing; otherwise it
+    // enqueues sweep i's modularity, chosen (full pass or incremental) from
+    // sweep i-1's movers exactly as the synchronous loop chooses it.  Results
+    // are those of the synchronous loop: the same sweeps, judged by the same
+    // rule on the same modularity values.
+    lv_status optimise_resident(float min_gain, int32_t *changed, int32_t *sweeps_out,
+                                double *ms_out, int64_t *edges_out) {
+        hipStream_t s = stream;
+        LV_TRY(enqueue_modularity());  // Q before the first sweep (and the weight inside communities)
+        hipLaunchKernelGGL(k_phase_start, dim3(1), dim3(1), 0, s, d_ctl, d_q);
+        LV_LAUNCHED("phase start");
+        EventPair span;
+        LV_HIP(span.create());
+        hipEvent_t ev[kCtlRing];
+        for (auto &e : ev) {
+            e = take_event();
+            if (!e) return fail(LV_ERR_HIP, "hipEventCreate failed");
+        }
+        struct Release {
+            lv_graph *g;
+            hipEvent_t *ev;
+            ~Release() {
+                g->gate_ptr = nullptr;
+                for (int i = 0; i < kCtlRing; ++i) g->ev_pool.push_back(ev[i]);
+            }
+        } release{this, ev};
+        LV_HIP(hipEventRecord(span.a, s));
+        gate_ptr = &d_ctl->done;
+        int64_t pm = -1, pme = -1;  // movers of sweep i-1 and their row entries (-1: none yet)
+        int64_t i = 0;
+        for (;; ++i) {
+            LV_TRY(enqueue_groups(nullptr, false, false, 1));
+            if (i >= 1) {
+                LV_HIP(hipEventSynchronize(ev[(i - 1) % kCtlRing]));
+                const PhaseCtl &c1 = h_ctl[(i - 1) % kCtlRing];
+                if (c1.done) break;
+                pm = (int64_t)c1.moved;
+                pme = (int64_t)c1.moved_edges;
+            }
+            const bool incremental = incr_den > 0 && pme >= 0 && pme * (int64_t)incr_den <= E;
+            LV_TRY(enqueue_tail(nullptr, false, incremental, rounds_for(pm)));
+            hipLaunchKernelGGL(k_phase_step, dim3(1), dim3(1), 0, s, d_ctl, d_q, d_moved, min_gain);
+            LV_LAUNCHED("phase step");
+            LV_HIP(hipMemcpyAsync(&h_ctl[i % kCtlRing], d_ctl, sizeof(PhaseCtl),
+                                  hipMemcpyDeviceToHost, s));
+            LV_HIP(hipEventRecord(ev[i % kCtlRing], s));
+        }
+        LV_HIP(hipEventRecord(span.b, s));
+        LV_TRY(sync());
+        const PhaseCtl c = h_ctl[(i - 1) % kCtlRing];  // after the last sweep run
+        float ms = 0.0f;
+        LV_HIP(hipEventElapsedTime(&ms, span.a, span.b));
+        own_valid = true;
+        last_moved = (int64_t)c.moved;
+        last_moved_edges = (int64_t)c.moved_edges;
+        compose_phase();
+        LV_LAUNCHED("compose");
+        if (changed) *changed = c.changed;
+        if (sweeps_out) *sweeps_out = c.sweeps;
+        if (ms_out) *ms_out = ms;
+        if (edges_out) *edges_out = (int64_t)c.sweeps * E;
         return LV_OK;
     }
 
@@ -2354,6 +2539,9 @@ struct lv_graph {
         dev_free(d_q);
         dev_free(d_moved);
         dev_free(d_own_slots);
+        dev_free(d_ctl);
+        if (h_ctl) (void)hipHostFree(h_ctl);
+        h_ctl = nullptr;
         dev_free(a_newid);
         dev_free(a_keep);
         dev_free(newid_ref);

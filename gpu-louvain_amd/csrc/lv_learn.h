@@ -50,7 +50,7 @@ enum FrontierKind { kFrontierModel = 0, kFrontierNeighbor = 1, kFrontierRandom =
 constexpr int32_t kNmvMaxDegree = 1024;
 
 // hist[v] <<= 1 for every vertex (start of a sweep).
-void launch_hist_shift(uint8_t *hist, int32_t V, hipStream_t s);
+void launch_hist_shift(uint8_t *hist, int32_t V, hipStream_t s, const int32_t *gate = nullptr);
 // nmv[u] += 1 for every row entry (u, v) of a vertex v that moved in the last
 // sweep (hist bit 0) and has degree <= kNmvMaxDegree; nmv must be zero
 // before.  The n vertices of list are scanned 64 per wave; the wave then

@@ -81,7 +81,8 @@ __global__ void k_isolated_features(float M, float vscale, float *out) {
     }
 }
 
-__global__ void k_hist_shift(uint8_t *hist, int32_t V) {
+__global__ void k_hist_shift(uint8_t *hist, int32_t V, const int32_t *__restrict__ gate) {
+    if (gated_off(gate)) return;
     const int32_t v = blockIdx.x * blockDim.x + threadIdx.x;
     if (v < V) hist[v] = (uint8_t)(hist[v] << 1);
 }
@@ -419,9 +420,9 @@ void launch_accumulate_loss(const float *grads, double weight, double *acc, hipS
     hipLaunchKernelGGL(k_accumulate_loss, dim3(1), dim3(64), 0, s, grads, weight, acc);
 }
 
-void launch_hist_shift(uint8_t *hist, int32_t V, hipStream_t s) {
+void launch_hist_shift(uint8_t *hist, int32_t V, hipStream_t s, const int32_t *gate) {
     if (V <= 0) return;
-    hipLaunchKernelGGL(k_hist_shift, dim3(blocks_for(V, 256)), dim3(256), 0, s, hist, V);
+    hipLaunchKernelGGL(k_hist_shift, dim3(blocks_for(V, 256)), dim3(256), 0, s, hist, V, gate);
 }
 
 void launch_neighbor_moves(const int32_t *rowptr, const int32_t *col, const int32_t *list,

@@ -173,13 +173,16 @@ __host__ __device__ inline int32_t corr_items(int X, int32_t rounds) {
 // is added to own (kMoveSlots slots kMoveSlotStride apart; gst: the group's
 // segments over order, then its heavy-row chunks, corr_items(X, gm->rounds)
 // items per block).
+// Without gm, sweep_movers (may be null) is a bitmap over vertices in which
+// the movers are marked (the sweep's movers, for launch_sweep_delta).
 // hist[v] gets bit 0 set for a mover (the move history of the frontier model);
 // moved holds kMoveSlots counter pairs kMoveSlotStride apart: summed over
 // the slots, [0] counts the movers and [1] their row entries.
 void launch_apply(const MoveArgs &a, const int32_t *list, int32_t n, const int32_t *order,
                   const SegTable *gst, const Chunk *chunks, int32_t *comm, int32_t *size,
                   double *sigma, uint64_t *crec, const GroupMarks *gm, uint8_t *hist,
-                  unsigned long long *moved, double *own, hipStream_t s);
+                  unsigned long long *moved, double *own, hipStream_t s,
+                  uint32_t *sweep_movers = nullptr);
 
 // --- modularity ---
 // Full pass: per-block partials (own, sq), then q[0] = Q, q[1] = own, q[2] = sq.
@@ -208,8 +211,21 @@ void launch_xcd_splits(const int32_t *rowptr, const int32_t *col, const int32_t 
 void launch_modularity_xcd(const int32_t *col, const float *w, const int32_t *comm,
                            const int32_t *list, const SegTable &xst, const int32_t *split,
                            double *partials, int32_t base, hipStream_t s);
-// Fixed-order reduction of nb partial pairs into q (as launch_modularity_finish).
-void launch_modularity_reduce(int32_t nb, double M, double *partials, double *q, hipStream_t s);
+// Fixed-order reduction of nb partial pairs into q (as launch_modularity_finish);
+// S_c^2 is summed over the nb_sq pairs from sq_lo.
+void launch_modularity_reduce(int32_t nb, int32_t sq_lo, int32_t nb_sq, double M,
+                              double *partials, double *q, hipStream_t s);
+
+// Change over one sweep of the weight inside communities, added to the
+// kMoveSlots slots of own_slots: the rows of the vertices whose community
+// changed (comm0 = communities before the sweep; movers = the sweep's mover
+// bitmap; dk = each mover's weight change with its neighbours as its
+// decision saw them) walked by the blocks of the correction table ct (the
+// level's row table with corr_items block starts for rounds).
+void launch_sweep_delta(const int32_t *rowptr, const int32_t *col, const float *w,
+                        const int32_t *comm, const int32_t *comm0, const double *dk,
+                        const uint32_t *movers, const int32_t *order, const SegTable &ct,
+                        const Chunk *chunks, int32_t rounds, double *own_slots, hipStream_t s);
 
 // Incremental: one pass for sum of S_c^2; q[0] = Q from q[1] plus the
 // applies' corrections in own_slots (kMoveSlots slots kMoveSlotStride apart,

@@ -105,6 +105,7 @@ __device__ __forceinline__ void write_decision(const MoveArgs &a, int32_t v, int
 template <int X>
 __global__ __launch_bounds__(256) void k_move_sub(MoveArgs a, const int32_t *__restrict__ list,
                                                   int32_t n) {
+    if (gated_off(a.gate)) return;
     const int lane = threadIdx.x & (X - 1);
     const int32_t slot = (int32_t)((blockIdx.x * 256u + threadIdx.x) / X);
     if (slot >= n) return;  // whole X-lane segment leaves together
@@ -178,6 +179,7 @@ __global__ __launch_bounds__(256) void k_move_sub(MoveArgs a, const int32_t *__r
 template <int C, int G>
 __global__ __launch_bounds__(256) void k_move_wave(MoveArgs a, const int32_t *__restrict__ list,
                                                    int32_t n) {
+    if (gated_off(a.gate)) return;
     static_assert(C % 64 == 0, "whole slots per lane");
     constexpr bool POW2 = (C & (C - 1)) == 0;
     constexpr int LOG2C = ilog2_c(C);
@@ -303,6 +305,7 @@ template <int T, int TIER>
 __global__ __launch_bounds__(T, 8) void k_move_wg(MoveArgs a, const int32_t *__restrict__ list,
                                                   int32_t n, int cap, int pct, int32_t *hint,
                                                   int32_t *done, int32_t limit) {
+    if (gated_off(a.gate)) return;
     extern __shared__ double dyn_lds[];
     const int C = cap;  // allocation (slots): W[C] then K[C]
     double *W = dyn_lds;
@@ -545,6 +548,7 @@ __device__ __forceinline__ int table_log2(int64_t n, int cap_lg) {
 }
 
 __global__ __launch_bounds__(256) void k_chunk_aggregate(MoveArgs a, ChunkedTier t) {
+    if (gated_off(a.gate)) return;
     // dynamic LDS sized by the chunk size: vals[C], keys[C], pcnt[kMaxParts]
     extern __shared__ double agg_lds[];
     const int C = 1 << t.chunk_cap_log2;
@@ -672,6 +676,7 @@ __device__ __forceinline__ void for_partition_pairs(const ChunkedTier &t, const 
 }
 
 __global__ __launch_bounds__(kMergeThreads) void k_chunk_merge(MoveArgs a, ChunkedTier t) {
+    if (gated_off(a.gate)) return;
     constexpr int C = 1 << kMergeCapLog2;
     __shared__ int32_t keys[C];
     __shared__ double vals[C];
@@ -794,6 +799,7 @@ __global__ __launch_bounds__(kMergeThreads) void k_chunk_merge(MoveArgs a, Chunk
 }
 
 __global__ void k_heavy_final(MoveArgs a, int32_t nh, ChunkedTier t) {
+    if (gated_off(a.gate)) return;
     const int32_t hv = blockIdx.x * blockDim.x + threadIdx.x;
     if (hv >= nh) return;
     if (t.done[hv]) {  // decided in LDS
@@ -911,7 +917,9 @@ __global__ __launch_bounds__(256) void k_apply_join(const int32_t *__restrict__ 
                                                     int32_t *newc, int32_t *comm, int32_t *size,
                                                     double *sigma, const double *__restrict__ k,
                                                     uint32_t *movers, uint8_t *hist,
-                                                    unsigned long long *moved) {
+                                                    unsigned long long *moved,
+                                                    const int32_t *__restrict__ gate) {
+    if (gated_off(gate)) return;
     const int32_t i = blockIdx.x * 256 + threadIdx.x;
     bool mv = false;
     int32_t v = 0, nc = 0, oc = 0;
@@ -957,6 +965,7 @@ __global__ __launch_bounds__(kRowThreads) void k_apply_finish(
     const uint32_t *__restrict__ movers, uint32_t *clear_bits, int32_t words, double *own,
     int32_t fix_start, int32_t clear_start, int32_t rounds, const int32_t *__restrict__ gid,
     double *gin) {
+    if (gated_off(a.gate)) return;
     __shared__ double lds[kRowThreads / 64];
     const int b = blockIdx.x;
     if (b >= clear_start) {
@@ -1073,6 +1082,102 @@ __global__ __launch_bounds__(kRowThreads) void k_apply_finish(
         if (threadIdx.x == 0 && acc != 0.0) atomicAdd(&gin[gblock], acc);
         return;
     }
+    if (threadIdx.x == 0 && acc != 0.0)
+        atomicAdd(&own[(int64_t)(blockIdx.x % kMoveSlots) * kMoveSlotStride], acc);
+}
+
+// Degree group of a vertex (modularity_optimisation.cuh:7-19): the order in
+// which a sweep decides and applies the groups.
+__device__ __forceinline__ int group_of_degree(int32_t deg) {
+    int g = 0;
+#pragma unroll
+    for (int i = 1; i < kNumGroups; ++i) g += deg > kGroupBound[i] ? 1 : 0;
+    return g;
+}
+
+// Change over one sweep of the weight inside communities.  Per entry
+// (v, u, w) of a mover v's row it is e = w ([c1(u) = c1(v)] - [c0(u) = c0(v)])
+// (c0, c1: communities before and after the sweep); the entry's mirror in
+// u's row has the same w and e.  A mover's 2 dk[v] (dk = K[new] - K[old]
+// with its neighbours in the communities its decision saw) covers both
+// entries of every neighbour u that did not move exactly.  For a neighbour
+// that moved, dk counted 2 w ([cu = c1(v)] - [cu = c0(v)]) with cu = u's
+// community at v's decision: c1(u) when u's degree group came earlier, c0(u)
+// when it came later or is v's own (its decisions are simultaneous); the
+// entry (v, u) replaces that by e, and (u, v) is seen from u's row.  Each
+// vertex moves at most once per sweep.  Blocks as k_apply_finish's
+// correction: corr_items rows per X-lane group, the movers compacted in LDS,
+// only their rows walked, the others' bits tested in the sweep's mover
+// bitmap; the sum goes to the kMoveSlots slots.
+__global__ __launch_bounds__(kRowThreads) void k_sweep_delta(
+    const int32_t *__restrict__ rowptr, const int32_t *__restrict__ col,
+    const float *__restrict__ w, const int32_t *__restrict__ comm,
+    const int32_t *__restrict__ comm0, const double *__restrict__ dk,
+    const uint32_t *__restrict__ movers, const int32_t *__restrict__ order, SegTable st,
+    const Chunk *__restrict__ chunks, int32_t rounds, double *own) {
+    __shared__ double lds[kRowThreads / 64];
+    __shared__ int32_t items[kCorrItems];
+    __shared__ int32_t nitems;
+    const int b = blockIdx.x;
+    const bool chunked = b >= st.chunk_block_start;
+    int X = kRowThreads;
+    int32_t first = 0, cnt = 0, list_off = 0;
+    if (chunked) {
+        const int32_t per = corr_items(kRowThreads, rounds);
+        first = (b - st.chunk_block_start) * per;
+        cnt = min(per, st.nchunks - first);
+    } else {
+        int si = 0;
+        while (si + 1 < st.nseg && b >= st.s[si + 1].block_start) ++si;
+        const Seg sg = st.s[si];
+        X = 1 << (sg.lanes_log2 < 8 ? sg.lanes_log2 : 8);
+        const int32_t per = corr_items(X, rounds);
+        first = (b - sg.block_start) * per;
+        cnt = min(per, sg.count - first);
+        list_off = sg.list_off;
+    }
+    if (threadIdx.x == 0) nitems = 0;
+    __syncthreads();
+    for (int32_t t = (int32_t)threadIdx.x; t < cnt; t += kRowThreads) {
+        const int32_t v = chunked ? chunks[first + t].v : order[list_off + first + t];
+        if (test_bit(movers, v)) items[atomicAdd(&nitems, 1)] = chunked ? first + t : v;
+    }
+    __syncthreads();
+    const int32_t nm = nitems;
+    if (nm == 0) return;  // block-uniform
+    const int lane = (int)threadIdx.x & (X - 1);
+    double acc = 0.0;
+    for (int32_t m = (int32_t)threadIdx.x / X; m < nm; m += kRowThreads / X) {
+        int32_t v;
+        int64_t lo, hi;
+        bool first_piece;
+        if (chunked) {
+            const Chunk ch = chunks[items[m]];
+            v = ch.v;
+            lo = ch.lo;
+            hi = ch.hi;
+            first_piece = ch.first != 0;
+        } else {
+            v = items[m];
+            lo = rowptr[v];
+            hi = rowptr[v + 1];
+            first_piece = true;
+        }
+        if (lane == 0 && first_piece) acc += 2.0 * dk[v];
+        const int32_t c1v = comm[v], c0v = comm0[v];
+        const int gv = group_of_degree(rowptr[v + 1] - rowptr[v]);
+        for (int64_t e = lo + lane; e < hi; e += X) {
+            const int32_t u = col[e];
+            if (u == v || !test_bit(movers, u)) continue;
+            const int32_t c1u = comm[u], c0u = comm0[u];
+            const int gu = group_of_degree(rowptr[u + 1] - rowptr[u]);
+            const int32_t cut = gu < gv ? c1u : c0u;
+            const int d = (int)(c1u == c1v) - (int)(c0u == c0v) -
+                          2 * ((int)(cut == c1v) - (int)(cut == c0v));
+            if (d != 0) acc += (double)d * (double)w[e];
+        }
+    }
+    acc = block_sum_f64(acc, lds);
     if (threadIdx.x == 0 && acc != 0.0)
         atomicAdd(&own[(int64_t)(blockIdx.x % kMoveSlots) * kMoveSlotStride], acc);
 }
@@ -1358,19 +1463,25 @@ __global__ __launch_bounds__(kRowThreads) void k_sum_squares(const double *__res
 }
 
 // Two-level fixed-order reduction of the per-block partials: 256 blocks
-// each add a contiguous range, then one block adds the 256 results.
+// each add a contiguous range, then one block adds the 256 results.  The
+// weight inside communities is summed over all nb pairs (its sums are exact,
+// so their order does not matter); the sum of S_c^2 only over the nb_sq
+// pairs of the S_c^2 blocks from sq_lo, split by nb_sq alone, so that a full
+// pass and an incremental sweep (k_sum_squares) add S_c^2 in the same order
+// and give the same modularity bit for bit.
 constexpr int kQReduceBlocks = 256;
 
 __global__ __launch_bounds__(256) void k_modularity_reduce(const double *__restrict__ partials,
-                                                           int32_t nb, double *partials2) {
+                                                           int32_t nb, int32_t sq_lo, int32_t nb_sq,
+                                                           double *partials2) {
     __shared__ double lds[4];
     const int64_t lo = (int64_t)nb * blockIdx.x / kQReduceBlocks;
     const int64_t hi = (int64_t)nb * (blockIdx.x + 1) / kQReduceBlocks;
+    const int64_t slo = sq_lo + (int64_t)nb_sq * blockIdx.x / kQReduceBlocks;
+    const int64_t shi = sq_lo + (int64_t)nb_sq * (blockIdx.x + 1) / kQReduceBlocks;
     double own = 0.0, sq = 0.0;
-    for (int64_t i = lo + threadIdx.x; i < hi; i += 256) {
-        own += partials[2 * i];
-        sq += partials[2 * i + 1];
-    }
+    for (int64_t i = lo + threadIdx.x; i < hi; i += 256) own += partials[2 * i];
+    for (int64_t i = slo + threadIdx.x; i < shi; i += 256) sq += partials[2 * i + 1];
     own = block_sum_f64(own, lds);
     sq = block_sum_f64(sq, lds);
     if (threadIdx.x == 0) {
@@ -1556,12 +1667,12 @@ void launch_clear_tables(int32_t *keys, double *vals, int64_t n, hipStream_t s) 
 void launch_apply(const MoveArgs &a, const int32_t *list, int32_t n, const int32_t *order,
                   const SegTable *gst, const Chunk *chunks, int32_t *comm, int32_t *size,
                   double *sigma, uint64_t *crec, const GroupMarks *gm, uint8_t *hist,
-                  unsigned long long *moved, double *own, hipStream_t s) {
+                  unsigned long long *moved, double *own, hipStream_t s, uint32_t *sweep_movers) {
     if (n <= 0 && !gm) return;
     if (n > 0) {
         hipLaunchKernelGGL(k_apply_join, dim3(blocks_for(n, 256)), dim3(256), 0, s, list, n,
-                           a.rowptr, a.newc, comm, size, sigma, a.k, gm ? gm->movers : nullptr,
-                           hist, moved);
+                           a.rowptr, a.newc, comm, size, sigma, a.k, gm ? gm->movers : sweep_movers,
+                           hist, moved, a.gate);
     }
     const int32_t corr = gm ? gst->total_blocks : 0;
     const int32_t fix = n > 0 ? (int32_t)blocks_for(n, kRowThreads) : 0;
@@ -1573,6 +1684,15 @@ void launch_apply(const MoveArgs &a, const int32_t *list, int32_t n, const int32
                        gm ? gm->movers : nullptr, gm ? gm->clear_movers : nullptr,
                        gm ? gm->words : 0, own, corr, corr + fix, gm ? gm->rounds : 1,
                        gm ? gm->gid : nullptr, gm ? gm->gin : nullptr);
+}
+
+void launch_sweep_delta(const int32_t *rowptr, const int32_t *col, const float *w,
+                        const int32_t *comm, const int32_t *comm0, const double *dk,
+                        const uint32_t *movers, const int32_t *order, const SegTable &ct,
+                        const Chunk *chunks, int32_t rounds, double *own_slots, hipStream_t s) {
+    if (ct.total_blocks <= 0) return;
+    hipLaunchKernelGGL(k_sweep_delta, dim3(ct.total_blocks), dim3(kRowThreads), 0, s, rowptr, col, w,
+                       comm, comm0, dk, movers, order, ct, chunks, rounds, own_slots);
 }
 
 void launch_modularity_rows(const int32_t *rowptr, const int32_t *col, const float *w,
@@ -1587,7 +1707,8 @@ void launch_modularity_finish(const SegTable &st, double M, double *partials, do
                               hipStream_t s) {
     double *partials2 = partials + 2 * (int64_t)st.total_blocks;
     hipLaunchKernelGGL(k_modularity_reduce, dim3(kQReduceBlocks), dim3(256), 0, s, partials,
-                       st.total_blocks, partials2);
+                       st.total_blocks, st.sq_block_start, st.total_blocks - st.sq_block_start,
+                       partials2);
     hipLaunchKernelGGL(k_modularity_final, dim3(1), dim3(256), 0, s, partials2, M, q, nullptr);
 }
 
@@ -1616,10 +1737,11 @@ void launch_modularity_xcd(const int32_t *col, const float *w, const int32_t *co
                        w, comm, list, xst, split, partials, base);
 }
 
-void launch_modularity_reduce(int32_t nb, double M, double *partials, double *q, hipStream_t s) {
+void launch_modularity_reduce(int32_t nb, int32_t sq_lo, int32_t nb_sq, double M,
+                              double *partials, double *q, hipStream_t s) {
     double *partials2 = partials + 2 * (int64_t)nb;
     hipLaunchKernelGGL(k_modularity_reduce, dim3(kQReduceBlocks), dim3(256), 0, s, partials, nb,
-                       partials2);
+                       sq_lo, nb_sq, partials2);
     hipLaunchKernelGGL(k_modularity_final, dim3(1), dim3(256), 0, s, partials2, M, q, nullptr);
 }
 
@@ -1628,8 +1750,8 @@ void launch_modularity_from_own(const double *sigma, int32_t V, double M, double
     const int32_t nb = (int32_t)(V > 0 ? blocks_for(V, kSqItems) : 1);
     hipLaunchKernelGGL(k_sum_squares, dim3(nb), dim3(kRowThreads), 0, s, sigma, V, partials);
     double *partials2 = partials + 2 * (int64_t)nb;
-    hipLaunchKernelGGL(k_modularity_reduce, dim3(kQReduceBlocks), dim3(256), 0, s, partials, nb,
-                       partials2);
+    hipLaunchKernelGGL(k_modularity_reduce, dim3(kQReduceBlocks), dim3(256), 0, s, partials, nb, 0,
+                       nb, partials2);
     hipLaunchKernelGGL(k_modularity_final, dim3(1), dim3(256), 0, s, partials2, M, q, own_slots);
 }
 
