@@ -182,7 +182,87 @@ def pruned_comparison(L, csr, min_gain=1e-6):
         out.update(res)
         out["target"] = {"dQ_rel_max": 0.005, "time_ratio_max": 0.5,
                          "met": bool(res["learned"]["dQ_rel"] <= 0.005 and res["learned"]["time_ratio"] <= 0.5)}
+    out["selection"] = selection_quality(L, csr, params, thr)
+    log(f"selection on the level-0 phase: model {out['selection']['model_selected']:.3f} of vertices, recall "
+        f"{out['selection']['model_recall']:.3f}; rule {out['selection']['rule_selected']:.3f}, recall "
+        f"{out['selection']['rule_recall']:.3f}")
     return out, (params, thr)
+
+
+def selection_quality(L, csr, params, thr, sweeps=8):
+    """The model's frontier against the neighbour rule on the same states:
+    over the sweeps of the plain level-0 phase after the first, the share of
+    vertices each selects and the share of that sweep's movers it catches."""
+    import numpy as np
+
+    ALL = L.frontier(kind=L.FRONTIER_ALL)
+    tot = dict(moved=0, n=0, m_sel=0, m_hit=0, r_sel=0, r_hit=0)
+    with L.Graph(csr) as G:
+        G.sweep_pruned(ALL, True)
+        for _ in range(sweeps):
+            ms = G.move_state()
+            feat = G.features()
+            c0 = G.communities().copy()
+            _, mv, _ = G.sweep_pruned(ALL, True)
+            moved = G.communities() != c0
+            p = L.mlp_forward(params, feat)
+            has_edges = feat[:, 0] > 0
+            msel = (p > thr) & has_edges
+            rsel = (((ms["hist"] & 1) > 0) | (ms["nmv"] > 0)) & has_edges
+            tot["moved"] += int(moved.sum())
+            tot["n"] += int(has_edges.sum())
+            tot["m_sel"] += int(msel.sum())
+            tot["m_hit"] += int((msel & moved).sum())
+            tot["r_sel"] += int(rsel.sum())
+            tot["r_hit"] += int((rsel & moved).sum())
+            if mv == 0:
+                break
+    n, m = max(tot["n"], 1), max(tot["moved"], 1)
+    return {"model_selected": tot["m_sel"] / n, "model_recall": tot["m_hit"] / m,
+            "rule_selected": tot["r_sel"] / n, "rule_recall": tot["r_hit"] / m, "movers": tot["moved"]}
+
+
+def c5_comparison(L, min_gain=1e-6):
+    """Plain vs learned-pruned on RMAT-24 (seed 24) with the model retrained
+    on plain-run traces of 8 RMAT-20 graphs (seeds 2001-2008), threshold for
+    97% recall on seed 2009; the neighbour rule beside them."""
+    import frontier as F
+
+    t0 = time.perf_counter()
+    train = [L.gen_rmat(20, 16, 0.57, 0.19, 0.19, seed=s) for s in range(2001, 2009)]
+    X, y, tinfo = F.collect_traces(train, min_gain=min_gain, per_sweep=4096)
+    del train
+    params, losses, info = F.train(X, y)
+    Xv, yv, _ = F.collect_traces([L.gen_rmat(20, 16, 0.57, 0.19, 0.19, seed=2009)], min_gain=min_gain,
+                                 per_sweep=1 << 16, seed=1)
+    thr, frac = F.recall_threshold(params, Xv, yv, recall=0.97)
+    train_s = time.perf_counter() - t0
+    log(f"c5: model trained on RMAT-20 traces in {train_s:.1f}s ({tinfo['samples']} samples), threshold {thr:.3f}")
+    g = L.gen_rmat(24, 16, 0.57, 0.19, 0.19, seed=24)
+    out = {"workload": f"RMAT-24 seed 24 (V={g.V}, E={g.E})", "training": {
+        "graphs": "RMAT-20 seeds 2001-2008", "samples": tinfo["samples"], "threshold": thr,
+        "val_selected": frac, "total_s": train_s}}
+    with L.Graph(g) as G:
+        G.set_model(params)
+        G.run(min_gain)  # warm
+        runs = {"plain": None,
+                "learned": L.frontier(kind=L.FRONTIER_MODEL, threshold=thr, full_every=4),
+                "neighbor": L.frontier(kind=L.FRONTIER_NEIGHBOR, full_every=4)}
+        for name, fr in runs.items():
+            G.reset()
+            st = G.run(min_gain) if fr is None else G.run_pruned(min_gain, fr)
+            out[name] = {"modularity": st["modularity"], "time_ms": st["algo_ms"], "sweeps": st["sweeps"],
+                         "full_sweeps": st["full_sweeps"], "evaluated": st["evaluated"], "levels": st["levels"]}
+            log(f"c5 {name}: Q={st['modularity']:.6f} {st['algo_ms']:.1f} ms, {st['sweeps']} sweeps, "
+                f"{st['evaluated']} evaluations")
+    qp, tp, ep = out["plain"]["modularity"], out["plain"]["time_ms"], out["plain"]["evaluated"]
+    for name in ("learned", "neighbor"):
+        r = out[name]
+        r["dQ_rel"] = (qp - r["modularity"]) / qp
+        r["time_ratio"] = r["time_ms"] / tp
+        r["evaluated_ratio"] = r["evaluated"] / max(ep, 1)
+    out["selection"] = selection_quality(L, g, params, thr)
+    return out
 
 
 def batch_comparison(L, n_graphs=4096, seed=4, sample=64, min_gain=1e-6, model=None, tune=True):
@@ -410,6 +490,8 @@ def main():
     ap.add_argument("--no-tune", action="store_true")
     ap.add_argument("--level", type=int, default=0,
                     help="time sweeps of this level (reached by plain phases) from its singleton state")
+    ap.add_argument("--c5", action="store_true",
+                    help="also run plain vs learned-pruned on RMAT-24 with the model retrained on RMAT-20 traces")
     ap.add_argument("--io", action="store_true",
                     help="also time reading the graph from MatrixMarket and binary CSR files")
     ap.add_argument("--calibrate-copy", action="store_true",
@@ -569,6 +651,7 @@ def main():
         batch = batch_comparison(L, model=model, tune=not args.no_tune)
 
     io = io_comparison(L, csr) if args.io else None
+    c5 = c5_comparison(L) if args.c5 else None
 
     cpu = None
     if not args.no_cpu_baseline:
@@ -604,6 +687,7 @@ def main():
         "batch": batch,
         "autotune": tuned,
         "io": io,
+        "c5": c5,
     }
     print(json.dumps(out))
 

@@ -1437,7 +1437,7 @@ struct lv_graph {
             // the list's build
             if (!glist_ok) return fail(LV_ERR_STATE, "graph list not built");
             lv::launch_frontier(fa, glist, glist_n, mlp, fr.threshold, fr.heavy_threshold,
-                                fr.heavy_degree, bb, fkeys, active, nullptr, s);
+                                fr.heavy_degree, bb, fkeys, active, nullptr, s, fr.and_neighbor);
             lv::launch_graph_model_frontier(rowptr(), glist, glist_n, gid, gact, gfull, bb, fkeys,
                                             active, s);
         } else if (fr.kind == LV_FRONTIER_MODEL) {
@@ -1445,7 +1445,7 @@ struct lv_graph {
             hipEvent_t t0 = tbegin();
             lv::launch_frontier(fa, order, h_off[lv::kNumBuckets], mlp, fr.threshold,
                                 fr.heavy_threshold, fr.heavy_degree, bb, fkeys, active, nullptr,
-                                s);
+                                s, fr.and_neighbor);
             // per vertex: rowptr 8, k 8, comm 4, record 8, margin 4, kown 4,
             // hist 1, nmv 4 read; keys 1, active 1 written
             tend(t0, -1, 43.0 * (double)V, LV_KD_FRONTIER);

@@ -68,9 +68,12 @@ void launch_features(const FeatureArgs &f, float *feat, hipStream_t s);
 // heavy_degree (when positive) use heavy_threshold instead.  Over the n
 // vertices of list, or over all f.V vertices when list is null (entries of
 // vertices outside the list are left as they are).
+// and_neighbor: a vertex is active only when the neighbour rule also selects
+// it (it or a neighbour moved in the last sweep).
 void launch_frontier(const FeatureArgs &f, const int32_t *list, int32_t n, const float *params,
                      float threshold, float heavy_threshold, int32_t heavy_degree,
-                     BucketBounds bb, int8_t *keys, uint8_t *active, float *prob, hipStream_t s);
+                     BucketBounds bb, int8_t *keys, uint8_t *active, float *prob, hipStream_t s,
+                     int32_t and_neighbor = 0);
 // Batches: the vertices of list[0..n) of the active graphs (gact) keep the
 // frontier flags the model set when their graph's sweep is pruned, and are
 // all in it when their graph's sweep is full (gfull); the others leave it.

@@ -215,7 +215,8 @@ __global__ __launch_bounds__(256) void k_mlp(FeatureArgs f, const int32_t *__res
                                              const float *__restrict__ feat,
                                              int64_t n, const float *__restrict__ P,
                                              float threshold, float heavy_threshold,
-                                             int32_t heavy_degree, BucketBounds bb, int8_t *keys,
+                                             int32_t heavy_degree, int32_t and_nbr,
+                                             BucketBounds bb, int8_t *keys,
                                              uint8_t *active, float *prob) {
     __shared__ float xs[4][64][kFeat + 1];
     __shared__ float h1s[4][16][kHid + 1];
@@ -259,7 +260,8 @@ __global__ __launch_bounds__(256) void k_mlp(FeatureArgs f, const int32_t *__res
                         const float thr = heavy_degree > 0 && deg > heavy_degree
                                               ? heavy_threshold
                                               : threshold;
-                        const bool act = deg > 0 && p > thr;
+                        const bool act = deg > 0 && p > thr &&
+                                         (!and_nbr || (f.hist[v] & 1u) || f.nmv[v] > 0);
                         active[v] = act ? 1 : 0;
                         keys[v] = act ? (int8_t)bucket_of(deg, bb) : (int8_t)-1;
                     }
@@ -443,14 +445,15 @@ void launch_features(const FeatureArgs &f, float *feat, hipStream_t s) {
 
 void launch_frontier(const FeatureArgs &f, const int32_t *list, int32_t n, const float *params,
                      float threshold, float heavy_threshold, int32_t heavy_degree,
-                     BucketBounds bb, int8_t *keys, uint8_t *active, float *prob, hipStream_t s) {
+                     BucketBounds bb, int8_t *keys, uint8_t *active, float *prob, hipStream_t s,
+                     int32_t and_neighbor) {
     if (!list) n = f.V;
     if (n <= 0) return;
     int64_t g = blocks_for(n, 256);
     if (g > 8192) g = 8192;
     hipLaunchKernelGGL(k_mlp<false>, dim3((unsigned)g), dim3(256), 0, s, f, list, nullptr,
-                       (int64_t)n, params, threshold, heavy_threshold, heavy_degree, bb, keys,
-                       active, prob);
+                       (int64_t)n, params, threshold, heavy_threshold, heavy_degree, and_neighbor,
+                       bb, keys, active, prob);
 }
 
 void launch_graph_model_frontier(const int32_t *rowptr, const int32_t *list, int32_t n,
@@ -477,7 +480,7 @@ void launch_mlp_forward(const float *feat, int64_t n, const float *params, float
     if (g > 8192) g = 8192;
     FeatureArgs f{};
     hipLaunchKernelGGL(k_mlp<true>, dim3((unsigned)g), dim3(256), 0, s, f, nullptr, feat, n, params,
-                       0.0f, 0.0f, 0, BucketBounds{}, nullptr, nullptr, prob);
+                       0.0f, 0.0f, 0, 0, BucketBounds{}, nullptr, nullptr, prob);
 }
 
 int64_t grad_blocks(int64_t n) { return (n + kTrainBlock - 1) / kTrainBlock; }

@@ -130,7 +130,8 @@ class Frontier(ctypes.Structure):
         ("seed", ctypes.c_uint64),
         ("heavy_threshold", ctypes.c_float),
         ("heavy_degree", ctypes.c_int32),
-        ("reserved", ctypes.c_int32 * 6),
+        ("and_neighbor", ctypes.c_int32),
+        ("reserved", ctypes.c_int32 * 5),
     ]
 
 
@@ -149,9 +150,10 @@ class TrainOptions(ctypes.Structure):
 
 def frontier(kind: int = FRONTIER_MODEL, full_every: int = 4, threshold: float = 0.5,
              random_p: float = 0.25, seed: int = 1, heavy_threshold: float = 0.0,
-             heavy_degree: int = 0) -> Frontier:
+             heavy_degree: int = 0, and_neighbor: bool = False) -> Frontier:
     return Frontier(kind=kind, full_every=full_every, threshold=threshold, random_p=random_p,
-                    seed=seed, heavy_threshold=heavy_threshold, heavy_degree=heavy_degree)
+                    seed=seed, heavy_threshold=heavy_threshold, heavy_degree=heavy_degree,
+                    and_neighbor=int(and_neighbor))
 
 
 class KernelTime(ctypes.Structure):

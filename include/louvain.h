@@ -287,7 +287,11 @@ typedef struct {
     uint64_t seed;           /* random: seed                                         */
     float heavy_threshold;   /* model: threshold for vertices of degree above        */
     int32_t heavy_degree;    /*   heavy_degree (0 = one threshold for every degree)  */
-    int32_t reserved[6];
+    int32_t and_neighbor;    /* model: 1 = a vertex is evaluated only when the model
+                                selects it and it or a neighbour moved in the last
+                                sweep (the model prunes the neighbour rule's
+                                frontier)                                           */
+    int32_t reserved[5];
 } lv_frontier;
 
 typedef struct {
