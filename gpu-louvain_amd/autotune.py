@@ -1,5 +1,6 @@
-"""Autotuner of the engine's tier options: the degree cuts inside degree
-groups 5 and 6, the lanes per row and the LDS table sizes of the
+"""Autotuner of the engine's tier options: the lanes per vertex of the
+sub-wave buckets (degrees up to 32), the degree cuts inside degree groups 5
+and 6, the lanes per row and the LDS table sizes of the
 one-workgroup tiers, the chunked tier's chunk, partition and merge sizes, and
 the column-blocked passes.
 
@@ -34,6 +35,7 @@ DEFAULT_CACHE = os.path.join(tempfile.gettempdir(), "gpulouvain_tune.json")
 # candidate values per option; a value of the library's default is listed
 # first, so that the search starts at the defaults
 SPACE = {
+    "sub_lanes": [(4, 4, 4), (4, 4, 8), (4, 8, 8), (8, 16, 32), (8, 8, 16), (4, 8, 16)],
     "heavy_wg_max": [4096, 2048, 1024],
     "wg_bound_a": [512, 384, 640],
     "wg_bound_b": [1024, 768, 1536],
@@ -49,8 +51,8 @@ SPACE = {
 }
 # the options that shape a batch's launches (its graphs are small: the
 # chunked tier and the column-blocked passes rarely apply)
-BATCH_SPACE = {k: SPACE[k] for k in ("heavy_wg_max", "wg_bound_a", "wg_bound_b", "wave_split",
-                                     "wg_lanes", "table_pct")}
+BATCH_SPACE = {k: SPACE[k] for k in ("sub_lanes", "heavy_wg_max", "wg_bound_a", "wg_bound_b",
+                                     "wave_split", "wg_lanes", "table_pct")}
 EXHAUSTIVE_MAX = 64
 _BOUNDS = [4, 8, 16, 32, 84, 319, 2048]
 

@@ -392,6 +392,7 @@ struct lv_graph {
     // LDS slots allocated per launch of each one-workgroup bucket
     int wg_slots[4] = {768, 1536, 2560, 6144};
     int wg_lanes[4] = {128, 256, 256, 1024};  // lanes per row of each tier
+    int sub_x[4] = {4, 4, 4, 4};              // lanes per vertex of buckets 0..3
     int table_pct = 0;                        // per-row table size, percent (0 = 150)
     bool heavy_try = false;  // LDS attempt of the chunked tier's rows
     int32_t heavy_limit = lv::kHeavyTryLimit;  // ... given up at this many communities
@@ -757,6 +758,16 @@ struct lv_graph {
         for (int t = 0; t < 4; ++t) {
             const int l = opt.wg_lanes[t];
             wg_lanes[t] = (l == 128 || l == 256 || l == 512 || l == 1024) ? l : kDefLanes[t];
+        }
+        // sub-wave buckets 1..3 (degree bounds 8, 16, 32): lanes per vertex.
+        // Four lanes (a quad: every exchange is DPP) with 2, 4 or 8 entries
+        // each put 16 vertices in flight per wave.
+        static const int kDefSub[3] = {4, 4, 4};
+        for (int b = 1; b < 4; ++b) {
+            const int l = opt.sub_lanes[b - 1];
+            const int bound = 4 << b;
+            sub_x[b] = (l >= 4 && l <= bound && (l & (l - 1)) == 0 &&
+                        lv::move_sub_shape(l, bound / l)) ? l : kDefSub[b - 1];
         }
         heavy_try = opt.heavy_lds >= 0 && 12 * kHeavyTrySlots <= max_lds;
         heavy_limit = opt.heavy_lds > 0 ? std::min(opt.heavy_lds, lv::kHeavyTryLimit)
@@ -1316,7 +1327,7 @@ struct lv_graph {
             switch (b) {
             case 0: case 1: case 2: case 3:
                 t0 = tbegin();
-                lv::launch_move_sub(4 << b, a, list, n, s);
+                lv::launch_move_sub(sub_x[b], (4 << b) / sub_x[b], a, list, n, s);
                 tend(t0, LV_K_MOVE_SUB, bytes, LV_KD_MOVE_BUCKET0 + b);
                 break;
             case 4: case 5: case 6:

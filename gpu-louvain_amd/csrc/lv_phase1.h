@@ -113,7 +113,12 @@ void launch_records_all(int32_t V, const double *sigma, const int32_t *size, uin
                         hipStream_t s);
 
 // --- moves ---
-void launch_move_sub(int X, const MoveArgs &a, const int32_t *list, int32_t n, hipStream_t s);
+// X lanes per vertex, E row entries per lane (degrees <= X * E); the shapes
+// move_sub_shape accepts: X * E = 4 (4x1), 8 (8x1, 4x2), 16 (16x1, 8x2, 4x4)
+// or 32 (32x1, 16x2, 8x4, 4x8).
+bool move_sub_shape(int X, int E);
+void launch_move_sub(int X, int E, const MoveArgs &a, const int32_t *list, int32_t n,
+                     hipStream_t s);
 // cap: LDS slots per wave (256 or 448); inst names the 256-slot instance:
 // 0 for degrees <= 84, 1 for degrees <= 128, 2 for degrees <= 255.
 void launch_move_wave(int cap, int inst, const MoveArgs &a, const int32_t *list, int32_t n,

@@ -10,10 +10,11 @@
 // maximum for non-negative weights), and K is accumulated in FP64.
 //
 // Kernel shapes (64-lane wavefronts):
-//   k_move_sub<X>   X = 4,8,16,32 lanes per vertex, one edge per lane;
+//   k_move_sub<X,E> X = 4..32 lanes per vertex, E row entries per lane;
 //                   per-community sums by an in-register all-pairs pass over
-//                   the X lanes (row order, so sums are bitwise the
-//                   sequential ones), argmax by xor-shuffle of packed keys.
+//                   the segment (row order, so sums are bitwise the
+//                   sequential ones), argmax of packed keys by DPP
+//                   butterflies.
 //   k_move_wave<C,G> one wavefront per vertex, C-slot LDS hash per wave
 //                   (G names the bucket's instance).
 //   k_move_wg<T,i>  one T-lane workgroup per vertex, LDS hash sized per launch
@@ -52,34 +53,123 @@ __device__ __forceinline__ void consider_k(uint64_t &best, double &bestK, int32_
     if (best != before) bestK = Kc;
 }
 
-// Max of packed keys over WIDTH lanes (xor butterfly), carrying the weight.
+// ------------------------------------------------------------ lane exchange
+// Partner value for a butterfly all-reduction over aligned groups of lanes.
+// Steps 1 and 2 swap lanes inside quads (DPP quad_perm); step 4 mirrors the
+// two quads of each 8-lane half-row and step 8 the two halves of each 16-lane
+// row (DPP row_half_mirror / row_mirror).  The mirrors do not pair lane i
+// with lane i ^ step, but after the earlier steps every lane of a block holds
+// that block's value, so any lane of the other block serves.  Steps 16 and
+// 32 are shuffles.  DPP is VALU work: no LDS traffic.  Every lane of a group
+// must be active.
+template <int STEP>
+__device__ __forceinline__ int32_t xchg_i32(int32_t x) {
+    if constexpr (STEP == 1) return __builtin_amdgcn_update_dpp(0, x, 0xB1, 0xF, 0xF, false);
+    else if constexpr (STEP == 2) return __builtin_amdgcn_update_dpp(0, x, 0x4E, 0xF, 0xF, false);
+    else if constexpr (STEP == 4) return __builtin_amdgcn_update_dpp(0, x, 0x141, 0xF, 0xF, false);
+    else if constexpr (STEP == 8) return __builtin_amdgcn_update_dpp(0, x, 0x140, 0xF, 0xF, false);
+    else return __shfl_xor(x, STEP, 64);
+}
+template <int STEP>
+__device__ __forceinline__ uint64_t xchg_u64(uint64_t x) {
+    const int32_t lo = xchg_i32<STEP>((int32_t)(uint32_t)x);
+    const int32_t hi = xchg_i32<STEP>((int32_t)(uint32_t)(x >> 32));
+    return ((uint64_t)(uint32_t)hi << 32) | (uint32_t)lo;
+}
+template <int STEP>
+__device__ __forceinline__ double xchg_f64(double x) {
+    return __longlong_as_double((long long)xchg_u64<STEP>((uint64_t)__double_as_longlong(x)));
+}
+
+// Sum over WIDTH lanes, in every lane (FP64 addition is commutative, so the
+// lanes of a group end with the same value).
+template <int WIDTH>
+__device__ __forceinline__ double lanes_sum(double x) {
+    if constexpr (WIDTH >= 2) x += xchg_f64<1>(x);
+    if constexpr (WIDTH >= 4) x += xchg_f64<2>(x);
+    if constexpr (WIDTH >= 8) x += xchg_f64<4>(x);
+    if constexpr (WIDTH >= 16) x += xchg_f64<8>(x);
+    if constexpr (WIDTH >= 32) x += xchg_f64<16>(x);
+    if constexpr (WIDTH >= 64) x += xchg_f64<32>(x);
+    return x;
+}
+
+template <int STEP>
+__device__ __forceinline__ void max_pair_step(uint64_t &best, double &bestK) {
+    const uint64_t ob = xchg_u64<STEP>(best);
+    const double oK = xchg_f64<STEP>(bestK);
+    if (ob > best) {
+        best = ob;
+        bestK = oK;
+    }
+}
+
+// Max of packed keys over WIDTH lanes, carrying the weight, in every lane.
+// Equal keys name the same community, hence carry the same weight.
 template <int WIDTH>
 __device__ __forceinline__ void lanes_max_pair(uint64_t &best, double &bestK) {
-#pragma unroll
-    for (int off = WIDTH / 2; off > 0; off >>= 1) {
-        const uint64_t ob = __shfl_xor(best, off, WIDTH);
-        const double oK = __shfl_xor(bestK, off, WIDTH);
-        if (ob > best) {
-            best = ob;
-            bestK = oK;
-        }
+    if constexpr (WIDTH >= 2) max_pair_step<1>(best, bestK);
+    if constexpr (WIDTH >= 4) max_pair_step<2>(best, bestK);
+    if constexpr (WIDTH >= 8) max_pair_step<4>(best, bestK);
+    if constexpr (WIDTH >= 16) max_pair_step<8>(best, bestK);
+    if constexpr (WIDTH >= 32) max_pair_step<16>(best, bestK);
+    if constexpr (WIDTH >= 64) max_pair_step<32>(best, bestK);
+}
+
+template <int STEP>
+__device__ __forceinline__ void argmax_step(double &k, int32_t &src) {
+    const double ok = xchg_f64<STEP>(k);
+    const int32_t os = xchg_i32<STEP>(src);
+    if (ok > k || (ok == k && os < src)) {
+        k = ok;
+        src = os;
     }
+}
+
+// Largest k over WIDTH lanes and the index that goes with it (ties to the
+// lowest index), in every lane.
+template <int WIDTH>
+__device__ __forceinline__ void lanes_argmax_idx(double &k, int32_t &idx) {
+    if constexpr (WIDTH >= 2) argmax_step<1>(k, idx);
+    if constexpr (WIDTH >= 4) argmax_step<2>(k, idx);
+    if constexpr (WIDTH >= 8) argmax_step<4>(k, idx);
+    if constexpr (WIDTH >= 16) argmax_step<8>(k, idx);
+    if constexpr (WIDTH >= 32) argmax_step<16>(k, idx);
+    if constexpr (WIDTH >= 64) argmax_step<32>(k, idx);
 }
 
 // Lane (of WIDTH) holding the largest k; ties to the lowest lane.
 template <int WIDTH>
 __device__ __forceinline__ int lanes_argmax(double k, int lane) {
-    int src = lane;
-#pragma unroll
-    for (int off = WIDTH / 2; off > 0; off >>= 1) {
-        const double ok = __shfl_xor(k, off, WIDTH);
-        const int os = __shfl_xor(src, off, WIDTH);
-        if (ok > k || (ok == k && os < src)) {
-            k = ok;
-            src = os;
-        }
-    }
+    int32_t src = lane;
+    lanes_argmax_idx<WIDTH>(k, src);
     return src;
+}
+
+// Lane l of each quad (DPP quad_perm broadcast).
+__device__ __forceinline__ int32_t quad_bcast(int32_t x, int l) {
+    switch (l) {
+    case 0: return __builtin_amdgcn_update_dpp(0, x, 0x00, 0xF, 0xF, false);
+    case 1: return __builtin_amdgcn_update_dpp(0, x, 0x55, 0xF, 0xF, false);
+    case 2: return __builtin_amdgcn_update_dpp(0, x, 0xAA, 0xF, 0xF, false);
+    default: return __builtin_amdgcn_update_dpp(0, x, 0xFF, 0xF, 0xF, false);
+    }
+}
+
+// Lane l of each aligned X-lane segment (lane = this lane's index in its
+// segment): DPP for X = 4 and 8 (the other quad of an 8-lane segment through
+// the half-row mirror), a shuffle above.
+template <int X>
+__device__ __forceinline__ int32_t seg_bcast(int32_t x, int l, int lane) {
+    if constexpr (X == 4) {
+        return quad_bcast(x, l);
+    } else if constexpr (X == 8) {
+        const int32_t y = quad_bcast(x, l & 3);
+        const int32_t z = __builtin_amdgcn_update_dpp(0, y, 0x141, 0xF, 0xF, false);
+        return ((lane ^ l) & 4) ? z : y;
+    } else {
+        return __shfl(x, l, X);
+    }
 }
 
 // Decision plus dk = K[new] - K[cur] for a mover (0 otherwise), and the
@@ -87,8 +177,7 @@ __device__ __forceinline__ int lanes_argmax(double k, int lane) {
 __device__ __forceinline__ void write_decision(const MoveArgs &a, int32_t v, int32_t cur,
                                                uint64_t best, double bestK, double kc) {
 #pragma clang fp contract(off)
-    const float2 md = vertex_mden(a, v);
-    const float vM32 = md.x, vden = md.y;
+    const float vM32 = vertex_mden(a, v).x;
     const int32_t nc = decide_from(best, kc, vM32, cur);
     a.newc[v] = nc;
     a.dk[v] = nc != cur ? bestK - kc : 0.0;
@@ -102,130 +191,164 @@ __device__ __forceinline__ void write_decision(const MoveArgs &a, int32_t v, int
 
 // ------------------------------------------------------------------ moves
 
-template <int X>
+// One X-lane segment per vertex; lane l holds row entries l, l + X, ...,
+// l + (E-1) X (deg <= X * E by bucket).  Narrow segments with several
+// entries per lane put more vertices, hence more independent gathers, in
+// flight per wave; X <= 8 exchanges through DPP only.
+// Decision of vertex v by its X-lane segment (lane = index in the segment;
+// every lane of the segment calls it together).
+template <int X, int E>
+__device__ __forceinline__ void move_seg(const MoveArgs &a, int32_t v, int lane) {
+    const int32_t rs = a.rowptr[v];
+    const int32_t deg = a.rowptr[v + 1] - rs;  // <= X * E by bucket
+    const int32_t cur = a.comm[v];
+    int32_t u[E];
+    float wf[E];
+#pragma unroll
+    for (int t = 0; t < E; ++t) {
+        const int32_t j = t * X + lane;
+        u[t] = j < deg ? a.col[rs + j] : v;
+        wf[t] = j < deg ? a.w[rs + j] : 0.0f;
+    }
+    int32_t c[E];
+#pragma unroll
+    for (int t = 0; t < E; ++t) c[t] = u[t] != v ? a.comm[u[t]] : -1;
+    // All-pairs pass in row order: K = sum of the weights of the entries
+    // sharing c; the entry of lowest index owns the community.
+    double K[E];
+    bool own[E];
+#pragma unroll
+    for (int t = 0; t < E; ++t) {
+        K[t] = 0.0;
+        own[t] = c[t] >= 0;
+    }
+#pragma unroll
+    for (int s = 0; s < E; ++s) {
+#pragma unroll
+        for (int l = 0; l < X; ++l) {
+            const int32_t j = s * X + l;
+            if (j < deg) {  // the same in the whole segment
+                const int32_t cj = seg_bcast<X>(c[s], l, lane);
+                const float wj = __int_as_float(seg_bcast<X>(__float_as_int(wf[s]), l, lane));
+#pragma unroll
+                for (int t = 0; t < E; ++t) {
+                    if (cj == c[t]) {
+                        K[t] += (double)wj;
+                        if (j < t * X + lane) own[t] = false;
+                    }
+                }
+            }
+        }
+    }
+    double kc = 0.0;
+#pragma unroll
+    for (int t = 0; t < E; ++t)
+        if (own[t] && c[t] == cur) kc = K[t];
+    kc = lanes_sum<X>(kc);  // one lane holds it
+
+    const float2 md = vertex_mden(a, v);
+    const float vM32 = md.x, vden = md.y;
+    const uint64_t rcur = a.crec[cur];
+    const float k32 = (float)a.k[v], scur = rec_sigma(rcur);
+    const int32_t size_cur = rec_size(rcur);
+    uint64_t best = pack_best(0.0f, cur);
+    double bestK = 0.0;
+    bool cand[E];
+#pragma unroll
+    for (int t = 0; t < E; ++t) cand[t] = own[t] && c[t] != cur;
+    uint64_t rc[E];
+    if (a.bounds) {
+        // the heaviest candidate first; the others are read only if their
+        // bound can still change the decision
+        double hk = -1.0;
+        int32_t hj = 0x7fffffff;
+#pragma unroll
+        for (int t = 0; t < E; ++t)
+            if (cand[t] && K[t] > hk) {
+                hk = K[t];
+                hj = t * X + lane;
+            }
+        lanes_argmax_idx<X>(hk, hj);
+#pragma unroll
+        for (int t = 0; t < E; ++t)
+            if (t * X + lane == hj)
+                consider_k(best, bestK, c[t], K[t], a.crec[c[t]], cur, size_cur, k32, scur, vM32,
+                           vden);
+        lanes_max_pair<X>(best, bestK);
+        const float tc = (float)kc / vM32;
+#pragma unroll
+        for (int t = 0; t < E; ++t)
+            cand[t] = cand[t] && t * X + lane != hj &&
+                      may_win(gain_bound(K[t], vM32, vden, k32, scur), c[t], best, a.obs != 0, tc);
+    }
+#pragma unroll
+    for (int t = 0; t < E; ++t) rc[t] = cand[t] ? a.crec[c[t]] : 0ull;
+#pragma unroll
+    for (int t = 0; t < E; ++t)
+        if (cand[t])
+            consider_k(best, bestK, c[t], K[t], rc[t], cur, size_cur, k32, scur, vM32, vden);
+    lanes_max_pair<X>(best, bestK);
+    if (lane == 0) write_decision(a, v, cur, best, bestK, kc);
+}
+
+template <int X, int E>
 __global__ __launch_bounds__(256) void k_move_sub(MoveArgs a, const int32_t *__restrict__ list,
                                                   int32_t n) {
     if (gated_off(a.gate)) return;
     const int lane = threadIdx.x & (X - 1);
     const int32_t slot = (int32_t)((blockIdx.x * 256u + threadIdx.x) / X);
     if (slot >= n) return;  // whole X-lane segment leaves together
-    const int32_t v = list[slot];
-    const int32_t rs = a.rowptr[v];
-    const int32_t deg = a.rowptr[v + 1] - rs;  // <= X by bucket
-    const int32_t cur = a.comm[v];
-    int32_t c = -1;
-    double wv = 0.0;
-    if (lane < deg) {
-        const int32_t u = a.col[rs + lane];
-        const float wf = a.w[rs + lane];
-        if (u != v) {
-            c = a.comm[u];
-            wv = (double)wf;
-        }
-    }
-    // All-pairs pass in row order: K = sum of weights of lanes sharing c;
-    // the lowest such lane owns the community.
-    double K = 0.0;
-    bool owner = c >= 0;
-    for (int j = 0; j < deg; ++j) {
-        const int32_t cj = __shfl(c, j, X);
-        const double wj = __shfl(wv, j, X);
-        if (cj == c) {
-            K += wj;
-            if (j < lane) owner = false;
-        }
-    }
-    double kc = (owner && c == cur) ? K : 0.0;
-#pragma unroll
-    for (int off = X / 2; off > 0; off >>= 1) kc += __shfl_xor(kc, off, X);
-
-    uint64_t best = pack_best(0.0f, cur);
-    double bestK = 0.0;
-    const bool cand = owner && c != cur;
-    if (a.bounds) {
-        // the heaviest candidate first; the others are read only if their
-        // bound can still change the decision
-        const int src = lanes_argmax<X>(cand ? K : -1.0, lane);
-        const uint64_t rcur = a.crec[cur];
-        const float2 md = vertex_mden(a, v);
-        const float vM32 = md.x, vden = md.y;
-        const float k32 = (float)a.k[v], scur = rec_sigma(rcur);
-        const int32_t size_cur = rec_size(rcur);
-        if (cand && lane == src)
-            consider_k(best, bestK, c, K, a.crec[c], cur, size_cur, k32, scur, vM32, vden);
-        best = __shfl(best, src, X);
-        bestK = __shfl(bestK, src, X);
-        if (cand && lane != src &&
-            may_win(gain_bound(K, vM32, vden, k32, scur), c, best, a.obs != 0, (float)kc / vM32))
-            consider_k(best, bestK, c, K, a.crec[c], cur, size_cur, k32, scur, vM32, vden);
-    } else if (cand) {
-        const uint64_t rcur = a.crec[cur];
-        const uint64_t rc = a.crec[c];
-        const float2 md = vertex_mden(a, v);
-        const float vM32 = md.x, vden = md.y;
-        consider_k(best, bestK, c, K, rc, cur, rec_size(rcur), (float)a.k[v], rec_sigma(rcur),
-                   vM32, vden);
-    }
-    lanes_max_pair<X>(best, bestK);
-    if (lane == 0) write_decision(a, v, cur, best, bestK, kc);
+    move_seg<X, E>(a, list[slot], lane);
 }
 
-// C: LDS slots per wave, a multiple of 64; G names the instance (one per
-// bucket).  A power of two hashes by shift, other sizes by multiply-shift
-// (448 slots for degrees <= 319: load <= 0.71 and 7 blocks per CU).  Each
-// lane holds MAXE row entries: 256-slot tables take degrees <= 128 (2 per
-// lane), or up to 255 (4 per lane) in instance G = 2, for a group-5 split
-// above 128.
-template <int C, int G>
-__global__ __launch_bounds__(256) void k_move_wave(MoveArgs a, const int32_t *__restrict__ list,
-                                                   int32_t n) {
-    if (gated_off(a.gate)) return;
-    static_assert(C % 64 == 0, "whole slots per lane");
-    constexpr bool POW2 = (C & (C - 1)) == 0;
-    constexpr int LOG2C = ilog2_c(C);
-    constexpr int MAXE = C == 256 ? (G == 2 ? 4 : 2) : 5;  // edges per lane
-    static_assert(64 * MAXE >= (C == 256 ? (G == 2 ? 255 : 128) : 319), "a row fits the lanes");
-    constexpr int S = C / 64;               // slots per lane
-    __shared__ int32_t keys_s[4 * C];
-    __shared__ double vals_s[4 * C];
-    const int lane = threadIdx.x & 63, wid = threadIdx.x >> 6;
-    const int32_t slot = blockIdx.x * 4 + wid;
-    const bool active = slot < n;
-    int32_t *K = keys_s + wid * C;
-    double *W = vals_s + wid * C;
+// One wave's LDS hash table of C slots (keys K, FP64 sums W; C a multiple
+// of 64): a power of two hashes by shift, other sizes by multiply-shift.
+template <int C>
+__device__ __forceinline__ void wave_tab_clear(int32_t *K, double *W, int lane) {
 #pragma unroll
-    for (int j = 0; j < S; ++j) {
+    for (int j = 0; j < C / 64; ++j) {
         K[lane + 64 * j] = -1;
         W[lane + 64 * j] = 0.0;
     }
-    __syncthreads();
-    int32_t v = 0, cur = 0;
-    if (active) {
-        v = list[slot];
-        const int32_t rs = a.rowptr[v], re = a.rowptr[v + 1];
-        cur = a.comm[v];
-        int32_t u[MAXE];
-        float wv[MAXE];
+}
+
+// Inserts vertex v's row (MAXE entries per lane, deg <= 64 * MAXE) into the
+// wave's table; returns v's community.
+template <int C, int MAXE>
+__device__ __forceinline__ int32_t wave_tab_fill(const MoveArgs &a, int32_t v, int32_t *K,
+                                                 double *W, int lane) {
+    constexpr bool POW2 = (C & (C - 1)) == 0;
+    constexpr int LOG2C = ilog2_c(C);
+    const int32_t rs = a.rowptr[v], re = a.rowptr[v + 1];
+    const int32_t cur = a.comm[v];
+    int32_t u[MAXE];
+    float wv[MAXE];
 #pragma unroll
-        for (int j = 0; j < MAXE; ++j) {
-            const int32_t e = rs + lane + 64 * j;
-            u[j] = e < re ? a.col[e] : -1;
-            wv[j] = e < re ? a.w[e] : 0.0f;
-        }
-        int32_t cc[MAXE];
-#pragma unroll
-        for (int j = 0; j < MAXE; ++j) cc[j] = (u[j] >= 0 && u[j] != v) ? a.comm[u[j]] : -1;
-#pragma unroll
-        for (int j = 0; j < MAXE; ++j) {
-            if (cc[j] < 0) continue;
-            if constexpr (POW2)
-                lds_insert(K, W, 32 - LOG2C, C - 1, cc[j], (double)wv[j]);
-            else
-                lds_insert_n(K, W, (uint32_t)C, cc[j], (double)wv[j]);
-        }
+    for (int j = 0; j < MAXE; ++j) {
+        const int32_t e = rs + lane + 64 * j;
+        u[j] = e < re ? a.col[e] : -1;
+        wv[j] = e < re ? a.w[e] : 0.0f;
     }
-    __syncthreads();
-    if (!active) return;  // only wave-level communication below
+    int32_t cc[MAXE];
+#pragma unroll
+    for (int j = 0; j < MAXE; ++j) cc[j] = (u[j] >= 0 && u[j] != v) ? a.comm[u[j]] : -1;
+#pragma unroll
+    for (int j = 0; j < MAXE; ++j) {
+        if (cc[j] < 0) continue;
+        if constexpr (POW2)
+            lds_insert(K, W, 32 - LOG2C, C - 1, cc[j], (double)wv[j]);
+        else
+            lds_insert_n(K, W, (uint32_t)C, cc[j], (double)wv[j]);
+    }
+    return cur;
+}
+
+// Decision of v (in community cur) from its filled table; the whole wave
+// calls it.
+template <int C>
+__device__ __forceinline__ void wave_tab_decide(const MoveArgs &a, int32_t v, int32_t cur,
+                                                const int32_t *K, const double *W, int lane) {
+    constexpr int S = C / 64;  // slots per lane
     const float k32 = (float)a.k[v];
     const float2 md = vertex_mden(a, v);
     const float vM32 = md.x, vden = md.y;
@@ -244,7 +367,7 @@ __global__ __launch_bounds__(256) void k_move_wave(MoveArgs a, const int32_t *__
 #pragma unroll
     for (int j = 0; j < S; ++j)
         if (kk[j] == cur) kc = vv[j];
-    kc = wave_sum_f64(kc);
+    kc = lanes_sum<64>(kc);
     uint64_t best = pack_best(0.0f, cur);
     double bestK = 0.0;
     if (a.bounds) {
@@ -262,8 +385,7 @@ __global__ __launch_bounds__(256) void k_move_wave(MoveArgs a, const int32_t *__
         if (lane == src && km >= 0.0)
             consider_k(best, bestK, kk[js], vv[js], a.crec[kk[js]], cur, size_cur, k32, scur, vM32,
                        vden);
-        best = __shfl(best, src, 64);
-        bestK = __shfl(bestK, src, 64);
+        lanes_max_pair<64>(best, bestK);  // src holds the best, the others the start key
         const float t = (float)kc / vM32;
         bool ev[S];
 #pragma unroll
@@ -286,6 +408,36 @@ __global__ __launch_bounds__(256) void k_move_wave(MoveArgs a, const int32_t *__
     }
     lanes_max_pair<64>(best, bestK);
     if (lane == 0) write_decision(a, v, cur, best, bestK, kc);
+}
+
+// C: LDS slots per wave; G names the instance (one per bucket).  448 slots
+// for degrees <= 319: load <= 0.71 and 7 blocks per CU.  Each lane holds MAXE
+// row entries: 256-slot tables take degrees <= 128 (2 per lane), or up to
+// 255 (4 per lane) in instance G = 2, for a group-5 split above 128.
+template <int C, int G>
+__global__ __launch_bounds__(256) void k_move_wave(MoveArgs a, const int32_t *__restrict__ list,
+                                                   int32_t n) {
+    if (gated_off(a.gate)) return;
+    static_assert(C % 64 == 0, "whole slots per lane");
+    constexpr int MAXE = C == 256 ? (G == 2 ? 4 : 2) : 5;  // edges per lane
+    static_assert(64 * MAXE >= (C == 256 ? (G == 2 ? 255 : 128) : 319), "a row fits the lanes");
+    __shared__ int32_t keys_s[4 * C];
+    __shared__ double vals_s[4 * C];
+    const int lane = threadIdx.x & 63, wid = threadIdx.x >> 6;
+    const int32_t slot = blockIdx.x * 4 + wid;
+    const bool active = slot < n;
+    int32_t *K = keys_s + wid * C;
+    double *W = vals_s + wid * C;
+    wave_tab_clear<C>(K, W, lane);
+    __syncthreads();
+    int32_t v = 0, cur = 0;
+    if (active) {
+        v = list[slot];
+        cur = wave_tab_fill<C, MAXE>(a, v, K, W, lane);
+    }
+    __syncthreads();
+    if (!active) return;  // only wave-level communication below
+    wave_tab_decide<C>(a, v, cur, K, W, lane);
 }
 
 // TIER only names the instance (one per workgroup bucket) for profiles.
@@ -496,7 +648,7 @@ __global__ __launch_bounds__(T, 8) void k_move_wg(MoveArgs a, const int32_t *__r
             }
         }
         lanes_max_pair<64>(best, bestK);
-        kc = wave_sum_f64(kc);
+        kc = lanes_sum<64>(kc);
         if (lane == 0) {
             red_best[wid] = best;
             red_K[wid] = bestK;
@@ -764,7 +916,7 @@ __global__ __launch_bounds__(kMergeThreads) void k_chunk_merge(MoveArgs a, Chunk
         }
     }
     lanes_max_pair<64>(best, bestK);
-    kc = wave_sum_f64(kc);
+    kc = lanes_sum<64>(kc);
 #pragma unroll
     for (int off = 32; off > 0; off >>= 1) nd += __shfl_xor(nd, off, 64);
     const int lane = threadIdx.x & 63, wid = threadIdx.x >> 6;
@@ -856,10 +1008,9 @@ __global__ void k_clear_tables(int32_t *keys, double *vals, int64_t n) {
 // community totals updated by exact FP64 increments (the reference rebuilds
 // them, :328-330).  Many movers of a wave often join the same (hub)
 // community: lanes sharing the wave leader's key are combined into one
-// atomic per field, a few times, before the rest go one by one.  A vertex
-// leaving a singleton community is that community's only writer, so that
-// update is a plain read-modify-write in a second launch, after all joins.
-// A third launch refreshes the records of the touched communities and, when
+// atomic per field, a few times, before the rest go one by one (joins and
+// leaves alike).  A second launch refreshes the records of the touched
+// communities and, when
 // the sweep tracks modularity incrementally, adds the group's change of the
 // weight inside communities:
 //   2*dk[v] per mover v (its own view, neighbours fixed), plus for each
@@ -1552,15 +1703,40 @@ void launch_records_all(int32_t V, const double *sigma, const int32_t *size, uin
                        crec);
 }
 
-void launch_move_sub(int X, const MoveArgs &a, const int32_t *list, int32_t n, hipStream_t s) {
+bool move_sub_shape(int X, int E) {
+    switch (X * 64 + E) {
+    case 4 * 64 + 1: case 8 * 64 + 1: case 16 * 64 + 1: case 32 * 64 + 1:
+    case 4 * 64 + 2: case 4 * 64 + 4: case 4 * 64 + 8:
+    case 8 * 64 + 2: case 8 * 64 + 4: case 16 * 64 + 2:
+        return true;
+    default:
+        return false;
+    }
+}
+
+void launch_move_sub(int X, int E, const MoveArgs &a, const int32_t *list, int32_t n,
+                     hipStream_t s) {
     if (n <= 0) return;
     const unsigned g = blocks_for((int64_t)n * X, 256);
-    switch (X) {
-    case 4: hipLaunchKernelGGL(k_move_sub<4>, dim3(g), dim3(256), 0, s, a, list, n); break;
-    case 8: hipLaunchKernelGGL(k_move_sub<8>, dim3(g), dim3(256), 0, s, a, list, n); break;
-    case 16: hipLaunchKernelGGL(k_move_sub<16>, dim3(g), dim3(256), 0, s, a, list, n); break;
-    default: hipLaunchKernelGGL(k_move_sub<32>, dim3(g), dim3(256), 0, s, a, list, n); break;
+#define LV_SUB(x, e)                                                                   \
+    case x * 64 + e:                                                                   \
+        hipLaunchKernelGGL((k_move_sub<x, e>), dim3(g), dim3(256), 0, s, a, list, n); \
+        break;
+    switch (X * 64 + E) {
+        LV_SUB(4, 1)
+        LV_SUB(8, 1)
+        LV_SUB(16, 1)
+        LV_SUB(4, 2)
+        LV_SUB(4, 4)
+        LV_SUB(4, 8)
+        LV_SUB(8, 2)
+        LV_SUB(8, 4)
+        LV_SUB(16, 2)
+    default:
+        hipLaunchKernelGGL((k_move_sub<32, 1>), dim3(g), dim3(256), 0, s, a, list, n);
+        break;
     }
+#undef LV_SUB
 }
 
 void launch_move_wave(int cap, int inst, const MoveArgs &a, const int32_t *list, int32_t n,

@@ -86,7 +86,8 @@ class Options(ctypes.Structure):
         ("wg_bound_c", ctypes.c_int32),
         ("wg_lanes", ctypes.c_int32 * 4),
         ("table_pct", ctypes.c_int32),
-        ("reserved", ctypes.c_int32 * 2),
+        ("sub_lanes", ctypes.c_int32 * 3),
+        ("reserved", ctypes.c_int32 * 1),
     ]
 
 
@@ -626,7 +627,8 @@ class BatchResult(ctypes.Structure):
 
 
 def options_struct(device: int = 0, timing: bool = False, **options) -> Options:
-    """An lv_options with the given fields (wg_lanes takes a sequence of 4)."""
+    """An lv_options with the given fields (wg_lanes takes a sequence of 4,
+    sub_lanes one of 3)."""
     opt = Options()
     opt.device = device
     opt.timing = int(timing)
@@ -634,9 +636,10 @@ def options_struct(device: int = 0, timing: bool = False, **options) -> Options:
     for k, v in options.items():
         if k not in fields or k in ("device", "timing", "reserved"):
             raise TypeError(f"unknown option {k}")
-        if k == "wg_lanes":
-            v = list(v) + [0] * (4 - len(v))
-            opt.wg_lanes = (ctypes.c_int32 * 4)(*[int(x) for x in v[:4]])
+        if k in ("wg_lanes", "sub_lanes"):
+            n = 4 if k == "wg_lanes" else 3
+            v = list(v) + [0] * (n - len(v))
+            setattr(opt, k, (ctypes.c_int32 * n)(*[int(x) for x in v[:n]]))
         else:
             setattr(opt, k, int(v))
     return opt

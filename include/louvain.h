@@ -90,7 +90,11 @@ typedef struct {
                                 128, 256, 512 or 1024 (0 = 128, 256, 256, 1024) */
     int32_t table_pct;        /* LDS hash slots of those tiers per degree bound,
                                 percent (120..200; 0 = 150, 125 for the third)  */
-    int32_t reserved[2];
+    int32_t sub_lanes[3];     /* lanes per vertex of the sub-wave buckets of
+                                degrees 5..8, 9..16 and 17..32: a power of two
+                                from 4 to the bucket's bound; each lane then
+                                holds bound / lanes row entries (0 = 4, 4, 4)    */
+    int32_t reserved[1];
 } lv_options;
 
 /* Run statistics (lv_run).  Times are device times from HIP events. */

@@ -1,7 +1,8 @@
 """Autotuner: cache, signature and search logic on the CPU; on the GPU, that
-every candidate (bucket cuts inside groups 5 and 6, lanes per row and LDS
-table sizes of the one-workgroup tiers, chunked-tier sizes, column-blocked
-passes) gives the same sweep as the defaults (results do not depend on
+every candidate (lanes per vertex of the sub-wave buckets, bucket cuts inside
+groups 5 and 6, lanes per row and LDS table sizes of the one-workgroup tiers,
+chunked-tier sizes, column-blocked passes) gives the same sweep as the
+defaults (results do not depend on
 tuning), and that tuning single graphs and batches picks one of them."""
 import numpy as np
 import pytest
@@ -34,7 +35,8 @@ def test_candidate_space_covers_every_option():
     cands = list(T.candidates(small))
     assert len(cands) == 8 and all(set(c) == set(small) for c in cands)
     assert T.space_size(T.SPACE) > T.EXHAUSTIVE_MAX
-    for k in ("wave_split", "wg_bound_a", "wg_bound_b", "wg_bound_c", "wg_lanes", "table_pct"):
+    for k in ("wave_split", "wg_bound_a", "wg_bound_b", "wg_bound_c", "wg_lanes", "table_pct",
+              "sub_lanes"):
         assert k in T.SPACE
     assert set(T.BATCH_SPACE) <= set(T.SPACE)
 
@@ -57,8 +59,10 @@ def test_coordinate_search_finds_separable_minimum():
 
 
 def test_options_struct_fields(lvlib):
-    o = lvlib.options_struct(0, False, wg_lanes=(128, 512), table_pct=175, wave_split=200)
+    o = lvlib.options_struct(0, False, wg_lanes=(128, 512), table_pct=175, wave_split=200,
+                             sub_lanes=(4, 8))
     assert list(o.wg_lanes) == [128, 512, 0, 0] and o.table_pct == 175 and o.wave_split == 200
+    assert list(o.sub_lanes) == [4, 8, 0]
     with pytest.raises(TypeError):
         lvlib.options_struct(0, False, no_such_option=1)
 
@@ -76,6 +80,10 @@ SAME_SPACE = [
     {"wg_lanes": (256, 512, 128, 512), "table_pct": 120},
     {"wg_lanes": (512, 128, 1024, 256), "table_pct": 200},
     {"table_pct": 133, "wave_split": 97, "wg_bound_b": 600},
+    {"sub_lanes": (4, 4, 4)},
+    {"sub_lanes": (4, 8, 8)},
+    {"sub_lanes": (8, 8, 16), "wave_split": 200},
+    {"sub_lanes": (4, 16, 16), "candidate_bounds": -1},
 ]
 
 
@@ -113,7 +121,7 @@ def test_tier_options_give_the_same_run_and_decisions(lvlib, oracle):
             assert st["sweeps"] == st_o["sweeps"] and np.array_equal(o2c, o2c_o), opt
             G.reset()
             G.set_communities(comm)
-            for grp in (5, 6):
+            for grp in range(1, 7):
                 assert np.array_equal(G.decide(grp), oracle.decide(og, comm, grp)), (opt, grp)
 
 
