@@ -652,7 +652,7 @@ struct lv_graph {
         int32_t gb = 0;
         for (int i = 0; i < t.nseg; ++i) {
             const int lg = t.s[i].lanes_log2;
-            const int32_t per = lv::corr_items(1 << (lg < 8 ? lg : 8), rounds);
+            const int32_t per = lv::corr_items(lv::corr_lanes(lg), rounds);
             t.s[i].block_start = gb;
             gb += (t.s[i].count + per - 1) / per;
         }

@@ -165,6 +165,12 @@ struct GroupMarks {
 // kCorrItems.  rounds = 1 gives one row per lane group (most vertices
 // moved); late sweeps, where few move, use more so that blocks stay few.
 constexpr int32_t kCorrItems = 1024;
+// Lanes per row of the correction passes: 4 for rows of at most 32 entries
+// (of a row table's segments with lanes_log2 <= 5; their gathers go four a
+// step), else the segment's lanes, at most a workgroup.
+__host__ __device__ inline int corr_lanes(int lanes_log2) {
+    return lanes_log2 <= 5 ? 4 : 1 << (lanes_log2 < 8 ? lanes_log2 : 8);
+}
 __host__ __device__ inline int32_t corr_items(int X, int32_t rounds) {
     const int64_t c = (int64_t)(kRowThreads / X) * rounds;
     return (int32_t)(c < kCorrItems ? c : kCorrItems);

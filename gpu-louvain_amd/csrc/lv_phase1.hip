@@ -1071,6 +1071,9 @@ __global__ __launch_bounds__(256) void k_apply_join(const int32_t *__restrict__ 
                                                     unsigned long long *moved,
                                                     const int32_t *__restrict__ gate) {
     if (gated_off(gate)) return;
+    __shared__ unsigned long long blk_moved[2];
+    if (threadIdx.x < 2) blk_moved[threadIdx.x] = 0ull;
+    __syncthreads();
     const int32_t i = blockIdx.x * 256 + threadIdx.x;
     bool mv = false;
     int32_t v = 0, nc = 0, oc = 0;
@@ -1091,15 +1094,20 @@ __global__ __launch_bounds__(256) void k_apply_join(const int32_t *__restrict__ 
     wave_peel_update(mv, oc, -kv, -1, size, sigma, 1);
     if (movers) wave_set_bit(mv, v, movers);
     const uint64_t m = __ballot(mv);
-    // moved[0] += movers, moved[1] += their row entries
+    // moved[0] += movers, moved[1] += their row entries: summed over the
+    // block in LDS, then one atomic pair per block, spread over kMoveSlots
+    // cache lines (counters shared by many blocks serialise their atomics)
     const double deg = mv ? (double)(rowptr[v + 1] - rowptr[v]) : 0.0;
     const double wdeg = wave_sum_f64(deg);
     if ((threadIdx.x & 63) == 0 && m) {
-        // spread over kMoveSlots cache lines: one counter would serialise
-        // every wave of the launch on one address
+        atomicAdd(&blk_moved[0], (unsigned long long)__popcll(m));
+        atomicAdd(&blk_moved[1], (unsigned long long)wdeg);
+    }
+    __syncthreads();
+    if (threadIdx.x == 0 && blk_moved[0] != 0) {
         unsigned long long *slot = moved + (int64_t)(blockIdx.x % kMoveSlots) * kMoveSlotStride;
-        atomicAdd(slot, (unsigned long long)__popcll(m));
-        atomicAdd(slot + 1, (unsigned long long)wdeg);
+        atomicAdd(slot, blk_moved[0]);
+        atomicAdd(slot + 1, blk_moved[1]);
     }
 }
 
@@ -1150,13 +1158,32 @@ __global__ __launch_bounds__(kRowThreads) void k_apply_finish(
     auto row = [&](int32_t v, int64_t elo, int64_t ehi, int stride, bool first) {
         const int32_t cv_old = a.newc[v], cv_new = comm[v];
         if (first) acc += 2.0 * a.dk[v];
-        for (int64_t e = elo; e < ehi; e += stride) {
-            const int32_t u = a.col[e];
-            if (u == v || !test_bit(movers, u)) continue;
-            const int32_t cu_new = comm[u], cu_old = a.newc[u];
-            const int d = (int)(cu_new == cv_new) - (int)(cu_old == cv_new) -
-                          (int)(cv_old == cu_new) + (int)(cu_old == cv_old);
-            if (d != 0) acc += (double)d * (double)a.w[e];
+        // four entries a step, so that their gathers are in flight together
+        for (int64_t e = elo; e < ehi; e += 4 * (int64_t)stride) {
+            int32_t u[4];
+            float x[4];
+#pragma unroll
+            for (int k = 0; k < 4; ++k) {
+                const int64_t ek = e + k * (int64_t)stride;
+                u[k] = ek < ehi ? a.col[ek] : v;
+                x[k] = ek < ehi ? a.w[ek] : 0.0f;
+            }
+            bool mv[4];
+#pragma unroll
+            for (int k = 0; k < 4; ++k) mv[k] = u[k] != v && test_bit(movers, u[k]);
+            int32_t cn[4], co[4];
+#pragma unroll
+            for (int k = 0; k < 4; ++k) {
+                cn[k] = mv[k] ? comm[u[k]] : 0;
+                co[k] = mv[k] ? a.newc[u[k]] : 0;
+            }
+#pragma unroll
+            for (int k = 0; k < 4; ++k) {
+                if (!mv[k]) continue;
+                const int d = (int)(cn[k] == cv_new) - (int)(co[k] == cv_new) -
+                              (int)(cv_old == cn[k]) + (int)(co[k] == cv_old);
+                if (d != 0) acc += (double)d * (double)x[k];
+            }
         }
     };
     // The block checks its items (rounds x 256/X vertices of one segment, or
@@ -1175,7 +1202,7 @@ __global__ __launch_bounds__(kRowThreads) void k_apply_finish(
         int si = 0;
         while (si + 1 < st.nseg && b >= st.s[si + 1].block_start) ++si;
         const Seg sg = st.s[si];
-        X = 1 << (sg.lanes_log2 < 8 ? sg.lanes_log2 : 8);
+        X = corr_lanes(sg.lanes_log2);
         const int32_t per = corr_items(X, rounds);
         first = (b - sg.block_start) * per;
         cnt = min(per, sg.count - first);
@@ -1281,7 +1308,7 @@ __global__ __launch_bounds__(kRowThreads) void k_sweep_delta(
         int si = 0;
         while (si + 1 < st.nseg && b >= st.s[si + 1].block_start) ++si;
         const Seg sg = st.s[si];
-        X = 1 << (sg.lanes_log2 < 8 ? sg.lanes_log2 : 8);
+        X = corr_lanes(sg.lanes_log2);
         const int32_t per = corr_items(X, rounds);
         first = (b - sg.block_start) * per;
         cnt = min(per, sg.count - first);
@@ -1317,15 +1344,34 @@ __global__ __launch_bounds__(kRowThreads) void k_sweep_delta(
         if (lane == 0 && first_piece) acc += 2.0 * dk[v];
         const int32_t c1v = comm[v], c0v = comm0[v];
         const int gv = group_of_degree(rowptr[v + 1] - rowptr[v]);
-        for (int64_t e = lo + lane; e < hi; e += X) {
-            const int32_t u = col[e];
-            if (u == v || !test_bit(movers, u)) continue;
-            const int32_t c1u = comm[u], c0u = comm0[u];
-            const int gu = group_of_degree(rowptr[u + 1] - rowptr[u]);
-            const int32_t cut = gu < gv ? c1u : c0u;
-            const int d = (int)(c1u == c1v) - (int)(c0u == c0v) -
-                          2 * ((int)(cut == c1v) - (int)(cut == c0v));
-            if (d != 0) acc += (double)d * (double)w[e];
+        // four entries a step, so that their gathers are in flight together
+        for (int64_t e = lo + lane; e < hi; e += 4 * (int64_t)X) {
+            int32_t u[4];
+            float x[4];
+#pragma unroll
+            for (int k = 0; k < 4; ++k) {
+                const int64_t ek = e + k * (int64_t)X;
+                u[k] = ek < hi ? col[ek] : v;
+                x[k] = ek < hi ? w[ek] : 0.0f;
+            }
+            bool mv[4];
+#pragma unroll
+            for (int k = 0; k < 4; ++k) mv[k] = u[k] != v && test_bit(movers, u[k]);
+            int32_t c1u[4], c0u[4], du[4];
+#pragma unroll
+            for (int k = 0; k < 4; ++k) {
+                c1u[k] = mv[k] ? comm[u[k]] : 0;
+                c0u[k] = mv[k] ? comm0[u[k]] : 0;
+                du[k] = mv[k] ? rowptr[u[k] + 1] - rowptr[u[k]] : 0;
+            }
+#pragma unroll
+            for (int k = 0; k < 4; ++k) {
+                if (!mv[k]) continue;
+                const int32_t cut = group_of_degree(du[k]) < gv ? c1u[k] : c0u[k];
+                const int d = (int)(c1u[k] == c1v) - (int)(c0u[k] == c0v) -
+                              2 * ((int)(cut == c1v) - (int)(cut == c0v));
+                if (d != 0) acc += (double)d * (double)x[k];
+            }
         }
     }
     acc = block_sum_f64(acc, lds);
