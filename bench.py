@@ -349,15 +349,36 @@ def batch_comparison(L, n_graphs=4096, seed=4, sample=64, min_gain=1e-6, model=N
                 lmis += 1
         qp = np.array([r["modularity"] for r in per])
         ql = np.array([r["modularity"] for r in lper])
+        dl = (qp - ql) / qp
+        # the plain method's own spread under a change of move order: every
+        # fourth graph renumbered by a seeded permutation, run plainly
+        sys.path.insert(0, os.path.join(ROOT, "tests"))
+        import graphs as TG
+
+        idx = list(range(0, n_graphs, 4))
+        perm = [L.CSR(*TG.relabelled(gs[i].rowptr, gs[i].col, gs[i].w, 500 + i)) for i in idx]
+        with L.Batch(perm) as B:
+            rper, _ = B.run(min_gain)
+        qpi = qp[idx]
+        dp = (qpi - np.array([r["modularity"] for r in rper])) / qpi
+
+        def spread(d):
+            return {"mean": float(d.mean()), "p90_abs": float(np.quantile(np.abs(d), 0.9)),
+                    "p99": float(np.quantile(d, 0.99)), "max": float(d.max()), "min": float(d.min()),
+                    "share_over_5pct": float(np.mean(d > 0.05))}
+
         learned = {"time_ms": ltot["algo_ms"], "time_ratio_vs_plain_batch": ltot["algo_ms"] / tot["algo_ms"],
                    "union_sweeps": ltot["sweeps"], "graph_sweeps": int(sum(r["sweeps"] for r in lper)),
                    "graph_full_sweeps": ltot["full_sweeps"], "evaluated": ltot["evaluated"],
                    "evaluated_plain": tot["evaluated"], "mean_modularity": float(ql.mean()),
-                   "mean_dQ_rel": float(np.mean((qp - ql) / qp)), "max_dQ_rel": float(np.max((qp - ql) / qp)),
+                   "mean_dQ_rel": float(np.mean(dl)), "max_dQ_rel": float(np.max(dl)),
+                   "dQ_rel": spread(dl), "relabelled_plain_dQ_rel": spread(dp),
+                   "relabelled_graphs": len(idx),
                    "one_by_one_ms_per_graph": lsingle_ms / sample, "sample_mismatches": lmis}
         log(f"batch learned-pruned: {ltot['algo_ms']:.1f} ms ({learned['time_ratio_vs_plain_batch']:.3f}x plain "
             f"batch), mean dQ_rel {learned['mean_dQ_rel']:.4f}, {ltot['evaluated']} evaluations vs "
-            f"{tot['evaluated']}; {lmis} mismatches vs one-by-one")
+            f"{tot['evaluated']}; {lmis} mismatches vs one-by-one; per-graph dQ_rel {learned['dQ_rel']}, "
+            f"relabelled plain {learned['relabelled_plain_dQ_rel']}")
     return {"learned_pruned": learned, "autotune": tuned, "graphs": n_graphs, "V": V, "E": E, "time_ms": tot["algo_ms"],
             "graphs_per_s": n_graphs / (tot["algo_ms"] * 1e-3), "edges_per_s": E / (tot["algo_ms"] * 1e-3),
             "union_levels": tot["levels"], "union_sweeps": tot["sweeps"],

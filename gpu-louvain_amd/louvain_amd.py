@@ -41,10 +41,13 @@ LV_NOALLOC_MAX_V = 1 << 20  # louvain.h LV_NOALLOC_MAX_V
 K_MOVE_SUB, K_MOVE_WAVE, K_MOVE_WG, K_MOVE_GLOBAL, K_APPLY, K_MODULARITY, K_AGGREGATE = range(7)
 KERNEL_NAMES = ["move_sub", "move_wave", "move_wg", "move_chunked", "apply", "modularity", "aggregate"]
 # single HIP kernels (lv_kernel_timing_detail) and their device function names
+# at the default options (sub_lanes picks the k_move_sub<lanes, entries>
+# instance of buckets 1..3; wave_split and the workgroup tiers move rows
+# between the others)
 DETAIL_NAMES = ["move_b0", "move_b1", "move_b2", "move_b3", "move_b4", "move_b5", "move_b6", "move_b7",
                 "chunk_aggregate", "chunk_merge", "chunk_final", "modularity_rows", "frontier",
                 "modularity_xcd", "move_b8", "move_b9", "move_b10", "heavy_lds"]
-DETAIL_KERNELS = ["k_move_sub<4>", "k_move_sub<8>", "k_move_sub<16>", "k_move_sub<32>",
+DETAIL_KERNELS = ["k_move_sub<4, 1>", "k_move_sub<4, 2>", "k_move_sub<4, 4>", "k_move_sub<4, 8>",
                   "k_move_wave<256, 0>", "k_move_wave<256, 1>", "k_move_wave<448, 0>",
                   "k_move_wg<128, 0>",
                   "k_chunk_aggregate", "k_chunk_merge", "k_heavy_final", "k_modularity", "k_mlp<false>",

@@ -85,3 +85,17 @@ def write_decimal_mtx(path, n, edges, weights):
         f.write(f"{n} {n} {len(edges)}\n")
         for (i, j), w in zip(edges, weights):
             f.write(f"{i + 1} {j + 1} {w}\n")
+
+
+def relabelled(rowptr, col, w, seed):
+    """The same graph with its vertices renumbered by a seeded random
+    permutation (rows sorted by the new column ids)."""
+    rng = np.random.default_rng(seed)
+    V = len(rowptr) - 1
+    p = rng.permutation(V).astype(np.int64)  # old id -> new id
+    rows = np.repeat(np.arange(V), np.diff(rowptr))
+    nr, nc = p[rows], p[np.asarray(col, np.int64)]
+    o = np.lexsort((nc, nr))
+    rp = np.zeros(V + 1, np.int64)
+    np.add.at(rp, nr + 1, 1)
+    return np.cumsum(rp).astype(np.int32), nc[o].astype(np.int32), np.asarray(w, np.float32)[o]
