@@ -323,9 +323,10 @@ def batch_comparison(L, n_graphs=4096, seed=4, sample=64, min_gain=1e-6, model=N
         if (abs(st["modularity"] - per[i]["modularity"]) > 1e-12 or not np.array_equal(o2c, comms[i])
                 or lv != per[i]["final_V"]):
             mismatches += 1
-    est = single_ms / sample * n_graphs
-    log(f"batch: one-by-one {single_ms / sample:.2f} ms per graph (first {sample}), "
-        f"{est:.0f} ms estimated for all; {mismatches} mismatches")
+    per_graph = single_ms / sample if sample else None
+    est = per_graph * n_graphs if sample else None
+    log(f"batch: one-by-one {per_graph or 0.0:.2f} ms per graph (first {sample}), "
+        f"{est or 0.0:.0f} ms estimated for all; {mismatches} mismatches")
     learned = None
     if model is not None:
         params, thr = model
@@ -374,7 +375,8 @@ def batch_comparison(L, n_graphs=4096, seed=4, sample=64, min_gain=1e-6, model=N
                    "mean_dQ_rel": float(np.mean(dl)), "max_dQ_rel": float(np.max(dl)),
                    "dQ_rel": spread(dl), "relabelled_plain_dQ_rel": spread(dp),
                    "relabelled_graphs": len(idx),
-                   "one_by_one_ms_per_graph": lsingle_ms / sample, "sample_mismatches": lmis}
+                   "one_by_one_ms_per_graph": lsingle_ms / sample if sample else None,
+                   "sample_mismatches": lmis}
         log(f"batch learned-pruned: {ltot['algo_ms']:.1f} ms ({learned['time_ratio_vs_plain_batch']:.3f}x plain "
             f"batch), mean dQ_rel {learned['mean_dQ_rel']:.4f}, {ltot['evaluated']} evaluations vs "
             f"{tot['evaluated']}; {lmis} mismatches vs one-by-one; per-graph dQ_rel {learned['dQ_rel']}, "
@@ -384,8 +386,8 @@ def batch_comparison(L, n_graphs=4096, seed=4, sample=64, min_gain=1e-6, model=N
             "union_levels": tot["levels"], "union_sweeps": tot["sweeps"],
             "aggregate_ms": tot["aggregate_ms"], "upload_ms": tot["memory_ms"],
             "mean_modularity": float(np.mean([r["modularity"] for r in per])),
-            "one_by_one_ms_per_graph": single_ms / sample, "one_by_one_estimate_ms": est,
-            "speedup_vs_one_by_one": est / tot["algo_ms"], "sample_checked": sample,
+            "one_by_one_ms_per_graph": per_graph, "one_by_one_estimate_ms": est,
+            "speedup_vs_one_by_one": est / tot["algo_ms"] if est else None, "sample_checked": sample,
             "sample_mismatches": mismatches}
 
 
@@ -509,6 +511,8 @@ def main():
     ap.add_argument("--no-pruned", action="store_true")
     ap.add_argument("--no-batch", action="store_true")
     ap.add_argument("--no-tune", action="store_true")
+    ap.add_argument("--batch-sample", type=int, default=64,
+                    help="batch graphs also run one by one and compared (0 = none)")
     ap.add_argument("--level", type=int, default=0,
                     help="time sweeps of this level (reached by plain phases) from its singleton state")
     ap.add_argument("--c5", action="store_true",
@@ -669,7 +673,7 @@ def main():
         pruned, model = pruned_comparison(L, csr)
     batch = None
     if not args.no_batch:
-        batch = batch_comparison(L, model=model, tune=not args.no_tune)
+        batch = batch_comparison(L, model=model, tune=not args.no_tune, sample=args.batch_sample)
 
     io = io_comparison(L, csr) if args.io else None
     c5 = c5_comparison(L) if args.c5 else None
