@@ -217,6 +217,14 @@ struct PhaseCtl {
 };
 constexpr int kCtlRing = 4;  // pinned copies of the block, one per sweep in flight
 
+// Weight inside communities of a singleton state: the level's self-loop
+// weight (sown), into q[1]; the correction slots start at zero.
+__global__ void k_singleton_own(double *q, const double *__restrict__ sown, double *own_slots) {
+    const int i = threadIdx.x;
+    if (i == 0) q[1] = *sown;
+    if (i < lv::kMoveSlots) own_slots[(int64_t)i * lv::kMoveSlotStride] = 0.0;
+}
+
 __global__ void k_phase_start(PhaseCtl *c, const double *__restrict__ q) {
     c->done = 0;
     c->sweeps = 0;
@@ -382,6 +390,15 @@ struct lv_graph {
     int32_t bit_words = 0;
     uint64_t step = 0;           // group steps applied with marks
     bool own_valid = false;      // d_q[1] holds the weight inside communities
+    // Singleton state (every vertex alone, after a level's set-up or a
+    // reset): its weight inside communities is the level's self-loop weight,
+    // summed at the level's set-up into d_sown ([0] level 0, [1] the current
+    // aggregated level; then partials), so that Q needs only the sum of
+    // S_c^2.  Only with weights on the grid, where that sum is exact in any
+    // order and so equals the full pass's.  Single graphs only.
+    bool singleton_state = false;
+    bool sown0_ok = false, sown_next_ok = false;
+    double *d_sown = nullptr;
     bool weights_nonneg = false; // every weight >= 0: candidate bounds are valid
     int64_t last_moved = -1;     // movers of the previous sweep (-1: unknown)
     int64_t last_moved_edges = -1;  // row entries of those movers
@@ -595,9 +612,10 @@ struct lv_graph {
 
     // The state changed other than by a sweep: the tracked weight inside
     // communities, the mover bitmaps and the frontier observations start over.
-    void state_changed() {
+    void state_changed(bool singletons = false) {
         lv::fill_i32(reinterpret_cast<int32_t *>(bits), 0, 2 * (int64_t)bit_words, stream);
         own_valid = false;
+        singleton_state = singletons;
         last_moved = -1;
         last_moved_edges = -1;
         const int64_t Vp = (int64_t)V + 1;
@@ -854,6 +872,7 @@ struct lv_graph {
         LV_TRY(dev_alloc(&d_q, 8));
         LV_TRY(dev_alloc(&d_moved, lv::kMoveSlots * lv::kMoveSlotStride));
         LV_TRY(dev_alloc(&d_own_slots, lv::kMoveSlots * lv::kMoveSlotStride));
+        LV_TRY(dev_alloc(&d_sown, 2 + lv::kSelfLoopBlocks));
         LV_TRY(dev_alloc(&d_ctl, 1));
         LV_HIP(hipHostMalloc(reinterpret_cast<void **>(&h_ctl), sizeof(PhaseCtl) * kCtlRing,
                              hipHostMallocDefault));
@@ -951,7 +970,7 @@ struct lv_graph {
             lv::iota_i32(orig2comm, Vorig, stream);
         if (cur_buf < 0 && level_ready) {
             lv::launch_level_state(V, k, comm, size, sigma, crec, stream);
-            state_changed();
+            state_changed(true);
             LV_LAUNCHED("reset");
             return LV_OK;
         }
@@ -1186,7 +1205,13 @@ struct lv_graph {
         lv::fill_f64(k, 0.0, V, s);
         lv::launch_strength(rp, w(), order, st, ins.p, k, s);
         lv::launch_level_state(V, k, comm, size, sigma, crec, s);
-        state_changed();
+        state_changed(true);
+        if (G == 0 && opt.exact_sums >= 0) {
+            const bool lvl0 = cur_buf < 0;
+            lv::launch_selfloop_weight(rp, col(), w(), V, d_sown + 2, d_sown + (lvl0 ? 0 : 1), s);
+            LV_LAUNCHED("self-loop weight");
+            (lvl0 ? sown0_ok : sown_next_ok) = true;
+        }
         LV_LAUNCHED("level state");
         if (G > 0) LV_TRY(update_graph_scales());
         level_ready = true;
@@ -1391,6 +1416,23 @@ struct lv_graph {
     }
 
     lv_status enqueue_modularity() {
+        const bool lvl0 = cur_buf < 0;
+        if (singleton_state && (lvl0 ? sown0_ok : sown_next_ok)) {
+            // singletons: own = the level's self-loop weight; S_c^2 by the
+            // same blocks and order as a full pass, so Q is the same bits
+            hipEvent_t t0 = tbegin();
+            hipLaunchKernelGGL(k_singleton_own, dim3(1), dim3(64), 0, stream, d_q,
+                               d_sown + (lvl0 ? 0 : 1), d_own_slots);
+            lv::launch_modularity_from_own(sigma, V, M, qpart.p, d_q, d_own_slots, stream);
+            tend(t0, LV_K_MODULARITY, 8.0 * (double)V);
+            LV_LAUNCHED("modularity");
+            own_valid = true;
+            return LV_OK;
+        }
+        return enqueue_modularity_full();
+    }
+
+    lv_status enqueue_modularity_full() {
         if (xcd_ok) {
             double e_light = 0.0, e_heavy = 0.0;
             for (int b = 0; b < lv::kNumBuckets; ++b) (b < 4 ? e_light : e_heavy) += (double)h_bedges[b];
@@ -1563,6 +1605,7 @@ struct lv_graph {
         const bool sweep_marks = G == 0 && (incremental || gate_ptr != nullptr);
         uint32_t *smovers = sweep_marks ? bits : nullptr;
         sweep_count++;
+        singleton_state = false;
         if (sweep_marks) lv::fill_i32(reinterpret_cast<int32_t *>(bits), 0, bit_words, s);
         lv::launch_hist_shift(hist, V, s, gate_ptr);
         hipLaunchKernelGGL(k_zero_u64_n, dim3(blocks_for(lv::kMoveSlots * lv::kMoveSlotStride, 256)),

@@ -1379,6 +1379,33 @@ __global__ __launch_bounds__(kRowThreads) void k_sweep_delta(
         atomicAdd(&own[(int64_t)(blockIdx.x % kMoveSlots) * kMoveSlotStride], acc);
 }
 
+// Self-loop weight: one wave per row (grid-stride), lanes over its entries.
+__global__ __launch_bounds__(256) void k_selfloop_partials(const int32_t *__restrict__ rowptr,
+                                                           const int32_t *__restrict__ col,
+                                                           const float *__restrict__ w,
+                                                           int32_t V, double *partials) {
+    __shared__ double lds[4];
+    const int lane = threadIdx.x & 63;
+    const int64_t waves = (int64_t)gridDim.x * 4;
+    double x = 0.0;
+    for (int64_t v = (int64_t)blockIdx.x * 4 + (threadIdx.x >> 6); v < V; v += waves) {
+        const int32_t hi = rowptr[v + 1];
+        for (int32_t e = rowptr[v] + lane; e < hi; e += 64)
+            if (col[e] == (int32_t)v) x += (double)w[e];
+    }
+    x = block_sum_f64(x, lds);
+    if (threadIdx.x == 0) partials[blockIdx.x] = x;
+}
+
+__global__ __launch_bounds__(256) void k_selfloop_final(const double *__restrict__ partials,
+                                                        double *out) {
+    __shared__ double lds[4];
+    double x = 0.0;
+    for (int i = threadIdx.x; i < kSelfLoopBlocks; i += 256) x += partials[i];
+    x = block_sum_f64(x, lds);
+    if (threadIdx.x == 0) *out = x;
+}
+
 __global__ void k_records_all(int32_t V, const double *__restrict__ sigma,
                               const int32_t *__restrict__ size, uint64_t *crec) {
     const int32_t c = blockIdx.x * blockDim.x + threadIdx.x;
@@ -1965,6 +1992,13 @@ void launch_modularity_reduce(int32_t nb, int32_t sq_lo, int32_t nb_sq, double M
     hipLaunchKernelGGL(k_modularity_reduce, dim3(kQReduceBlocks), dim3(256), 0, s, partials, nb,
                        sq_lo, nb_sq, partials2);
     hipLaunchKernelGGL(k_modularity_final, dim3(1), dim3(256), 0, s, partials2, M, q, nullptr);
+}
+
+void launch_selfloop_weight(const int32_t *rowptr, const int32_t *col, const float *w, int32_t V,
+                            double *partials, double *out, hipStream_t s) {
+    hipLaunchKernelGGL(k_selfloop_partials, dim3(kSelfLoopBlocks), dim3(256), 0, s, rowptr, col, w,
+                       V, partials);
+    hipLaunchKernelGGL(k_selfloop_final, dim3(1), dim3(256), 0, s, partials, out);
 }
 
 void launch_modularity_from_own(const double *sigma, int32_t V, double M, double *partials,

@@ -253,3 +253,30 @@ def test_xcd_modularity_pass_matches_plain_pass(lvlib, oracle, scale, seed):
         A.aggregate()
         B.aggregate()
         assert abs(A.modularity() - B.modularity()) == 0.0
+
+
+@pytest.mark.parametrize("kind", ["rmat", "self_loops"])
+def test_singleton_modularity_equals_full_pass(lvlib, oracle, kind):
+    """Q of a singleton state (after a level's set-up or a reset) is computed
+    from the level's self-loop weight and the sum of S_c^2; it equals the
+    full pass's Q (lv_set_communities forces one) bit for bit, at level 0
+    and at aggregated levels, whose rows carry self-loops."""
+    if kind == "rmat":
+        g = lvlib.gen_rmat(14, 16, seed=65)
+    else:
+        g = _csr(lvlib, *graphs.random_graph(3000, 20000, 66, True, 40))
+    og = oracle.Graph.from_csr(g)
+    with lvlib.Graph(g) as G:
+        for level in range(3):
+            V, _ = G.level_size()
+            q_fast = G.modularity()
+            G.set_communities(np.arange(V, dtype=np.int32))
+            q_full = G.modularity()
+            assert q_fast == q_full, level
+            if level == 0:
+                assert abs(q_fast - oracle.modularity(og, np.arange(g.V, dtype=np.int32))) <= 1e-9
+                G.reset()
+                assert G.modularity() == q_full
+            if not G.optimise(1e-6):
+                break
+            G.aggregate()
