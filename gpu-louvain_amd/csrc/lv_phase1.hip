@@ -722,11 +722,23 @@ __global__ __launch_bounds__(256) void k_chunk_aggregate(MoveArgs a, ChunkedTier
     for (int p = threadIdx.x; p < P; p += 256) pcnt[p] = 0;
     __syncthreads();
     const int32_t v = ch.v;
-    for (int64_t e = ch.lo + threadIdx.x; e < ch.hi; e += 256) {
-        const int32_t u = __builtin_nontemporal_load(&a.col[e]);
-        const float wf = __builtin_nontemporal_load(&a.w[e]);
-        if (u == v) continue;
-        lds_insert(keys, vals, 32 - cl, (uint32_t)Cv - 1, a.comm[u], (double)wf);
+    // four entries per lane at a time: their loads, then their gathers, are
+    // in flight together ahead of the LDS inserts
+    for (int64_t e0 = ch.lo + threadIdx.x; e0 < ch.hi; e0 += 4 * 256) {
+        int32_t u[4];
+        float wf[4];
+#pragma unroll
+        for (int j = 0; j < 4; ++j) {
+            const int64_t e = e0 + j * 256;
+            u[j] = e < ch.hi ? __builtin_nontemporal_load(&a.col[e]) : -1;
+            wf[j] = e < ch.hi ? __builtin_nontemporal_load(&a.w[e]) : 0.0f;
+        }
+        int32_t cc[4];
+#pragma unroll
+        for (int j = 0; j < 4; ++j) cc[j] = (u[j] >= 0 && u[j] != v) ? a.comm[u[j]] : -1;
+#pragma unroll
+        for (int j = 0; j < 4; ++j)
+            if (cc[j] >= 0) lds_insert(keys, vals, 32 - cl, (uint32_t)Cv - 1, cc[j], (double)wf[j]);
     }
     __syncthreads();
     for (int s = threadIdx.x; s < Cv; s += 256) {
@@ -813,15 +825,30 @@ __device__ __forceinline__ void for_partition_pairs(const ChunkedTier &t, const 
         if (threadIdx.x == 0) seg_off[nct] = carry;
         __syncthreads();
         const int32_t total = carry;
-        for (int32_t f0 = threadIdx.x; f0 < total; f0 += kMergeThreads) {
-            int32_t a = 0, b = nct - 1;  // last segment with seg_off <= f0
-            while (a < b) {
-                const int32_t mid = (a + b + 1) >> 1;
-                if (seg_off[mid] <= f0) a = mid;
-                else b = mid - 1;
+        // four pairs per lane at a time: their loads are in flight together
+        for (int32_t f0 = threadIdx.x; f0 < total; f0 += 4 * kMergeThreads) {
+            int32_t kk[4];
+            double vv[4];
+#pragma unroll
+            for (int j = 0; j < 4; ++j) {
+                const int32_t fj = f0 + j * kMergeThreads;
+                kk[j] = -1;
+                vv[j] = 0.0;
+                if (fj < total) {
+                    int32_t a = 0, b = nct - 1;  // last segment with seg_off <= fj
+                    while (a < b) {
+                        const int32_t mid = (a + b + 1) >> 1;
+                        if (seg_off[mid] <= fj) a = mid;
+                        else b = mid - 1;
+                    }
+                    const int64_t idx = seg_base[a] + (fj - seg_off[a]);
+                    kk[j] = t.out_key[idx];
+                    vv[j] = t.out_val[idx];
+                }
             }
-            const int64_t idx = seg_base[a] + (f0 - seg_off[a]);
-            f(t.out_key[idx], t.out_val[idx]);
+#pragma unroll
+            for (int j = 0; j < 4; ++j)
+                if (kk[j] >= 0) f(kk[j], vv[j]);
         }
         __syncthreads();
     }
@@ -950,26 +977,41 @@ __global__ __launch_bounds__(kMergeThreads) void k_chunk_merge(MoveArgs a, Chunk
     }
 }
 
-__global__ void k_heavy_final(MoveArgs a, int32_t nh, ChunkedTier t) {
+// One wave per heavy vertex: its lanes look for the merge item that holds
+// the vertex's best candidate (a community lies in one partition, so one
+// item matches), lane 0 writes the decision.
+__global__ __launch_bounds__(256) void k_heavy_final(MoveArgs a, int32_t nh, ChunkedTier t) {
     if (gated_off(a.gate)) return;
-    const int32_t hv = blockIdx.x * blockDim.x + threadIdx.x;
-    if (hv >= nh) return;
-    if (t.done[hv]) {  // decided in LDS
-        t.done[hv] = 0;
+    const int lane = threadIdx.x & 63;
+    const int32_t hv = blockIdx.x * 4 + (threadIdx.x >> 6);
+    if (hv >= nh) return;  // the whole wave
+    if (t.done[hv]) {      // decided in LDS
+        if (lane == 0) t.done[hv] = 0;
         return;
     }
     const HeavyVertex h = t.hvs[hv];
     const int32_t v = h.v;
     const int32_t cur = a.comm[v];
     if (a.active && !a.active[v]) {  // not evaluated: stays
-        a.newc[v] = cur;
-        a.dk[v] = 0.0;
+        if (lane == 0) {
+            a.newc[v] = cur;
+            a.dk[v] = 0.0;
+        }
         return;
     }
     uint64_t best = t.best[hv];
     double bestK = 0.0;
-    for (int32_t p = 0; p < h.P; ++p)
-        if (best != 0 && t.item_best[h.ptot_base + p] == best) bestK = t.item_K[h.ptot_base + p];
+    if (best != 0) {
+        for (int32_t p0 = 0; p0 < h.P; p0 += 64) {
+            const int32_t p = p0 + lane;
+            const uint64_t m = __ballot(p < h.P && t.item_best[h.ptot_base + p] == best);
+            if (m) {
+                bestK = t.item_K[h.ptot_base + p0 + (__ffsll((unsigned long long)m) - 1)];
+                break;
+            }
+        }
+    }
+    if (lane != 0) return;
     const uint64_t init = pack_best(0.0f, cur);
     if (init > best) best = init;
     write_decision(a, v, cur, best, bestK, t.kcur[hv]);
@@ -1893,7 +1935,7 @@ void launch_chunk_merge(const MoveArgs &a, const ChunkedTier &t, int32_t nitems,
 
 void launch_heavy_final(const MoveArgs &a, const ChunkedTier &t, int32_t nh, hipStream_t s) {
     if (nh <= 0) return;
-    hipLaunchKernelGGL(k_heavy_final, dim3(blocks_for(nh, 256)), dim3(256), 0, s, a, nh, t);
+    hipLaunchKernelGGL(k_heavy_final, dim3(blocks_for(nh, 4)), dim3(256), 0, s, a, nh, t);
     hipLaunchKernelGGL(k_clear_overflow, dim3(nh), dim3(256), 0, s, t, nh);
 }
 
