@@ -69,9 +69,23 @@ struct MoveArgs {
     // phase): null, or the phase's end flag; kernels return at once when it
     // is set, so a sweep enqueued after the last one does nothing.
     const int32_t *gate;
+    // Row gathers of the move kernels.  Entry e of the rows names column
+    // ccol[e] and its community is ccomm[ccol[e]]; vertex v's own number
+    // among the columns is cself[v].  Without renumbering: ccol = col, ccomm
+    // = comm, cself = null (identity).  With it, columns are numbered in
+    // bucket order (the rows of high degree, which most entries name, lie
+    // together) and ccomm is comm in that order, kept by the applies.
+    const int32_t *ccol;
+    const int32_t *ccomm;
+    const int32_t *cself;
 };
 
 __device__ __forceinline__ bool gated_off(const int32_t *gate) { return gate && *gate != 0; }
+
+// Vertex v's number among the move kernels' columns (MoveArgs::cself).
+__device__ __forceinline__ int32_t col_self(const MoveArgs &a, int32_t v) {
+    return a.cself ? a.cself[v] : v;
+}
 
 // M32 and den of vertex v's graph (.x, .y).
 __device__ __forceinline__ float2 vertex_mden(const MoveArgs &a, int32_t v) {

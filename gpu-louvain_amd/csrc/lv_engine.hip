@@ -288,6 +288,9 @@ constexpr int kHeavyTrySlots = 6144;
 static_assert(kHeavyTrySlots > lv::kHeavyTryLimit + 1024, "the attempt's table never fills");
 // Group 5 (degrees 85..319) runs 256-slot wave tables up to this degree.
 constexpr int32_t kWave256Hi = 128;
+// levels from which the move kernels read bucket-ordered columns: comm
+// (4 bytes a vertex) well beyond one XCD's 4 MB L2
+constexpr int32_t kColOrderMinVertices = 1 << 20;
 constexpr int32_t kXcdMinVertices = 1 << 21;
 constexpr int32_t kBoundsMinVertices = 1 << 20;  // candidate bounds from this level size  // XCD-local row passes from this level size
 // A phase with a frontier sweeps only full after this many sweeps (bounds a
@@ -442,6 +445,11 @@ struct lv_graph {
     bool tier_xcd = false;
     DevVec<int32_t> gkeys;  // global hash arena, kept all-empty between uses
     DevVec<double> gvals;
+
+    // column numbering of the move kernels (lv::MoveArgs::ccol): positions
+    // in `order`, the rows' columns in that numbering and comm copied into it
+    bool col_shadow = false;
+    DevVec<int32_t> c_pos, c_col, c_comm;
 
     // row-reduction segments
     lv::SegTable st{};
@@ -607,12 +615,16 @@ struct lv_graph {
                        : 0;
         a.obs = 1;
         a.gate = gate_ptr;
+        a.ccol = col_shadow ? c_col.p : a.col;
+        a.ccomm = col_shadow ? c_comm.p : comm;
+        a.cself = col_shadow ? c_pos.p : nullptr;
         return a;
     }
 
     // The state changed other than by a sweep: the tracked weight inside
     // communities, the mover bitmaps and the frontier observations start over.
     void state_changed(bool singletons = false) {
+        if (col_shadow) lv::launch_col_comm(order, V, comm, c_comm.p, stream);
         lv::fill_i32(reinterpret_cast<int32_t *>(bits), 0, 2 * (int64_t)bit_words, stream);
         own_valid = false;
         singleton_state = singletons;
@@ -988,6 +1000,7 @@ struct lv_graph {
         }
         LV_LAUNCHED("reset");
         level_ready = false;
+        col_shadow = false;
         return LV_OK;
     }
 
@@ -1199,6 +1212,19 @@ struct lv_graph {
             gb += t.nchunks;
             t.sq_block_start = gb;
             t.total_blocks = gb;
+        }
+
+        // column numbering of the move kernels: on large single-graph levels
+        // whose vertices all have entries (all of them are in `order`)
+        col_shadow = false;
+        if (G == 0 && opt.col_order >= 0 && V > 0 && h_off[lv::kNumBuckets] == V &&
+            (opt.col_order > 0 || V >= kColOrderMinVertices)) {
+            LV_TRY(c_pos.reserve(V));
+            LV_TRY(c_col.reserve(E));
+            LV_TRY(c_comm.reserve(V));
+            lv::launch_col_numbering(order, V, col(), E, c_pos.p, c_col.p, s);
+            LV_LAUNCHED("column numbering");
+            col_shadow = true;
         }
 
         // strengths and singleton state (utils.cu:84-89; computeEdgesSum)
@@ -2140,6 +2166,7 @@ struct lv_graph {
         Vref = nVref;
         E = newE;
         level_ready = false;
+        col_shadow = false;
         if (G > 0) {
             std::swap(vptr, vptr_next);
             lv::launch_graph_ids(vptr, G, gid, s);
@@ -2496,6 +2523,7 @@ struct lv_graph {
         Vref = nVref;
         E = newE;
         level_ready = false;
+        col_shadow = false;
         if (G > 0) {
             std::swap(vptr, vptr_next);
             lv::launch_graph_ids(vptr, G, gid, s);
@@ -2633,6 +2661,9 @@ struct lv_graph {
         iorder.release();
         gkeys.release();
         gvals.release();
+        c_pos.release();
+        c_col.release();
+        c_comm.release();
         item_best.release();
         item_K.release();
         qpart.release();

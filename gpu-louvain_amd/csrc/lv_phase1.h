@@ -112,6 +112,14 @@ void launch_level_state(int32_t V, const double *k, int32_t *comm, int32_t *size
 void launch_records_all(int32_t V, const double *sigma, const int32_t *size, uint64_t *crec,
                         hipStream_t s);
 
+// Column numbering of the move kernels (MoveArgs::ccol, cself): order lists
+// all V vertices; cpos = its inverse, ccol[e] = cpos[col[e]].  launch_col_comm
+// copies comm into that numbering (after any change of comm but the applies').
+void launch_col_numbering(const int32_t *order, int32_t V, const int32_t *col, int64_t E,
+                          int32_t *cpos, int32_t *ccol, hipStream_t s);
+void launch_col_comm(const int32_t *order, int32_t V, const int32_t *comm, int32_t *ccomm,
+                     hipStream_t s);
+
 // --- moves ---
 // X lanes per vertex, E row entries per lane (degrees <= X * E); the shapes
 // move_sub_shape accepts: X * E = 4 (4x1), 8 (8x1, 4x2), 16 (16x1, 8x2, 4x4)

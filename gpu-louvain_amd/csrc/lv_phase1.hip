@@ -26,6 +26,8 @@
 //                   packed 64-bit atomic max.
 #include "lv_phase1.h"
 
+#include <algorithm>
+
 namespace lv {
 
 namespace {
@@ -202,17 +204,18 @@ __device__ __forceinline__ void move_seg(const MoveArgs &a, int32_t v, int lane)
     const int32_t rs = a.rowptr[v];
     const int32_t deg = a.rowptr[v + 1] - rs;  // <= X * E by bucket
     const int32_t cur = a.comm[v];
+    const int32_t vs = col_self(a, v);
     int32_t u[E];
     float wf[E];
 #pragma unroll
     for (int t = 0; t < E; ++t) {
         const int32_t j = t * X + lane;
-        u[t] = j < deg ? a.col[rs + j] : v;
+        u[t] = j < deg ? a.ccol[rs + j] : vs;
         wf[t] = j < deg ? a.w[rs + j] : 0.0f;
     }
     int32_t c[E];
 #pragma unroll
-    for (int t = 0; t < E; ++t) c[t] = u[t] != v ? a.comm[u[t]] : -1;
+    for (int t = 0; t < E; ++t) c[t] = u[t] != vs ? a.ccomm[u[t]] : -1;
     // All-pairs pass in row order: K = sum of the weights of the entries
     // sharing c; the entry of lowest index owns the community.
     double K[E];
@@ -321,17 +324,18 @@ __device__ __forceinline__ int32_t wave_tab_fill(const MoveArgs &a, int32_t v, i
     constexpr int LOG2C = ilog2_c(C);
     const int32_t rs = a.rowptr[v], re = a.rowptr[v + 1];
     const int32_t cur = a.comm[v];
+    const int32_t vs = col_self(a, v);
     int32_t u[MAXE];
     float wv[MAXE];
 #pragma unroll
     for (int j = 0; j < MAXE; ++j) {
         const int32_t e = rs + lane + 64 * j;
-        u[j] = e < re ? a.col[e] : -1;
+        u[j] = e < re ? a.ccol[e] : -1;
         wv[j] = e < re ? a.w[e] : 0.0f;
     }
     int32_t cc[MAXE];
 #pragma unroll
-    for (int j = 0; j < MAXE; ++j) cc[j] = (u[j] >= 0 && u[j] != v) ? a.comm[u[j]] : -1;
+    for (int j = 0; j < MAXE; ++j) cc[j] = (u[j] >= 0 && u[j] != vs) ? a.ccomm[u[j]] : -1;
 #pragma unroll
     for (int j = 0; j < MAXE; ++j) {
         if (cc[j] < 0) continue;
@@ -494,6 +498,7 @@ __global__ __launch_bounds__(T, 8) void k_move_wg(MoveArgs a, const int32_t *__r
         }
         __syncthreads();
         const int32_t cur = a.comm[v];
+        const int32_t vs = col_self(a, v);
         if constexpr (TIER == 4) {
             volatile int32_t *fill = &fill_s, *stop = &stop_s;
             bool dropped = false;
@@ -503,12 +508,12 @@ __global__ __launch_bounds__(T, 8) void k_move_wg(MoveArgs a, const int32_t *__r
 #pragma unroll
                 for (int j = 0; j < 4; ++j) {
                     const int32_t e = e0 + j * T;
-                    u[j] = e < re ? a.col[e] : -1;
+                    u[j] = e < re ? a.ccol[e] : -1;
                     wv[j] = e < re ? a.w[e] : 0.0f;
                 }
                 int32_t cc[4];
 #pragma unroll
-                for (int j = 0; j < 4; ++j) cc[j] = (u[j] >= 0 && u[j] != v) ? a.comm[u[j]] : -1;
+                for (int j = 0; j < 4; ++j) cc[j] = (u[j] >= 0 && u[j] != vs) ? a.ccomm[u[j]] : -1;
 #pragma unroll
                 for (int j = 0; j < 4; ++j) {
                     if (cc[j] < 0 || dropped) continue;
@@ -534,13 +539,13 @@ __global__ __launch_bounds__(T, 8) void k_move_wg(MoveArgs a, const int32_t *__r
 #pragma unroll
                 for (int j = 0; j < 4; ++j) {
                     const int32_t e = e0 + j * T;
-                    u[j] = e < re ? a.col[e] : -1;
+                    u[j] = e < re ? a.ccol[e] : -1;
                     wv[j] = e < re ? a.w[e] : 0.0f;
                 }
                 int32_t cc[4];
 #pragma unroll
                 for (int j = 0; j < 4; ++j)
-                    cc[j] = (u[j] >= 0 && u[j] != v) ? a.comm[u[j]] : -1;
+                    cc[j] = (u[j] >= 0 && u[j] != vs) ? a.ccomm[u[j]] : -1;
 #pragma unroll
                 for (int j = 0; j < 4; ++j)
                     if (cc[j] >= 0) lds_insert_n(K, W, (uint32_t)Cv, cc[j], (double)wv[j]);
@@ -721,7 +726,7 @@ __global__ __launch_bounds__(256) void k_chunk_aggregate(MoveArgs a, ChunkedTier
     }
     for (int p = threadIdx.x; p < P; p += 256) pcnt[p] = 0;
     __syncthreads();
-    const int32_t v = ch.v;
+    const int32_t vs = col_self(a, ch.v);
     // four entries per lane at a time: their loads, then their gathers, are
     // in flight together ahead of the LDS inserts
     for (int64_t e0 = ch.lo + threadIdx.x; e0 < ch.hi; e0 += 4 * 256) {
@@ -730,12 +735,12 @@ __global__ __launch_bounds__(256) void k_chunk_aggregate(MoveArgs a, ChunkedTier
 #pragma unroll
         for (int j = 0; j < 4; ++j) {
             const int64_t e = e0 + j * 256;
-            u[j] = e < ch.hi ? __builtin_nontemporal_load(&a.col[e]) : -1;
+            u[j] = e < ch.hi ? __builtin_nontemporal_load(&a.ccol[e]) : -1;
             wf[j] = e < ch.hi ? __builtin_nontemporal_load(&a.w[e]) : 0.0f;
         }
         int32_t cc[4];
 #pragma unroll
-        for (int j = 0; j < 4; ++j) cc[j] = (u[j] >= 0 && u[j] != v) ? a.comm[u[j]] : -1;
+        for (int j = 0; j < 4; ++j) cc[j] = (u[j] >= 0 && u[j] != vs) ? a.ccomm[u[j]] : -1;
 #pragma unroll
         for (int j = 0; j < 4; ++j)
             if (cc[j] >= 0) lds_insert(keys, vals, 32 - cl, (uint32_t)Cv - 1, cc[j], (double)wf[j]);
@@ -1111,7 +1116,9 @@ __global__ __launch_bounds__(256) void k_apply_join(const int32_t *__restrict__ 
                                                     double *sigma, const double *__restrict__ k,
                                                     uint32_t *movers, uint8_t *hist,
                                                     unsigned long long *moved,
-                                                    const int32_t *__restrict__ gate) {
+                                                    const int32_t *__restrict__ gate,
+                                                    int32_t *ccomm,
+                                                    const int32_t *__restrict__ cself) {
     if (gated_off(gate)) return;
     __shared__ unsigned long long blk_moved[2];
     if (threadIdx.x < 2) blk_moved[threadIdx.x] = 0ull;
@@ -1128,6 +1135,7 @@ __global__ __launch_bounds__(256) void k_apply_join(const int32_t *__restrict__ 
             mv = true;
             kv = k[v];
             comm[v] = nc;
+            if (cself) ccomm[cself[v]] = nc;  // the move kernels' copy of comm
             newc[v] = oc;  // from here on newc holds the community left
             hist[v] |= 1;  // move history (bit 0 = this sweep)
         }
@@ -1532,6 +1540,27 @@ __global__ __launch_bounds__(kRowThreads) void k_strength(const int32_t *__restr
         s = block_sum_f64(s, lds);
         if (threadIdx.x == 0) k[v] = s;
     }
+}
+
+// Column numbering of the move kernels (MoveArgs::ccol): cpos[order[i]] = i,
+// order listing every vertex once, bucket by bucket; ccol[e] = cpos[col[e]].
+__global__ void k_col_pos(const int32_t *__restrict__ order, int32_t V, int32_t *cpos) {
+    const int32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < V) cpos[order[i]] = i;
+}
+
+__global__ void k_col_renumber(const int32_t *__restrict__ col, int64_t E,
+                               const int32_t *__restrict__ cpos, int32_t *ccol) {
+    for (int64_t e = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; e < E;
+         e += (int64_t)gridDim.x * blockDim.x)
+        ccol[e] = cpos[__builtin_nontemporal_load(&col[e])];
+}
+
+// ccomm[i] = comm[order[i]]: the communities in that numbering.
+__global__ void k_col_comm(const int32_t *__restrict__ order, int32_t V,
+                           const int32_t *__restrict__ comm, int32_t *ccomm) {
+    const int32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < V) ccomm[i] = comm[order[i]];
 }
 
 __global__ void k_level_state(int32_t V, const double *__restrict__ k, int32_t *comm,
@@ -1955,6 +1984,21 @@ void launch_clear_tables(int32_t *keys, double *vals, int64_t n, hipStream_t s) 
     hipLaunchKernelGGL(k_clear_tables, dim3((unsigned)g), dim3(256), 0, s, keys, vals, n);
 }
 
+void launch_col_numbering(const int32_t *order, int32_t V, const int32_t *col, int64_t E,
+                          int32_t *cpos, int32_t *ccol, hipStream_t s) {
+    if (V <= 0) return;
+    hipLaunchKernelGGL(k_col_pos, dim3(blocks_for(V, 256)), dim3(256), 0, s, order, V, cpos);
+    if (E > 0)
+        hipLaunchKernelGGL(k_col_renumber, dim3((unsigned)std::min<int64_t>(blocks_for(E, 256), 65536)),
+                           dim3(256), 0, s, col, E, cpos, ccol);
+}
+
+void launch_col_comm(const int32_t *order, int32_t V, const int32_t *comm, int32_t *ccomm,
+                     hipStream_t s) {
+    if (V <= 0) return;
+    hipLaunchKernelGGL(k_col_comm, dim3(blocks_for(V, 256)), dim3(256), 0, s, order, V, comm, ccomm);
+}
+
 void launch_apply(const MoveArgs &a, const int32_t *list, int32_t n, const int32_t *order,
                   const SegTable *gst, const Chunk *chunks, int32_t *comm, int32_t *size,
                   double *sigma, uint64_t *crec, const GroupMarks *gm, uint8_t *hist,
@@ -1963,7 +2007,7 @@ void launch_apply(const MoveArgs &a, const int32_t *list, int32_t n, const int32
     if (n > 0) {
         hipLaunchKernelGGL(k_apply_join, dim3(blocks_for(n, 256)), dim3(256), 0, s, list, n,
                            a.rowptr, a.newc, comm, size, sigma, a.k, gm ? gm->movers : sweep_movers,
-                           hist, moved, a.gate);
+                           hist, moved, a.gate, const_cast<int32_t *>(a.ccomm), a.cself);
     }
     const int32_t corr = gm ? gst->total_blocks : 0;
     const int32_t fix = n > 0 ? (int32_t)blocks_for(n, kRowThreads) : 0;

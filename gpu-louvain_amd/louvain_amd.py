@@ -90,7 +90,7 @@ class Options(ctypes.Structure):
         ("wg_lanes", ctypes.c_int32 * 4),
         ("table_pct", ctypes.c_int32),
         ("sub_lanes", ctypes.c_int32 * 3),
-        ("reserved", ctypes.c_int32 * 1),
+        ("col_order", ctypes.c_int32),
     ]
 
 

@@ -94,7 +94,12 @@ typedef struct {
                                 degrees 5..8, 9..16 and 17..32: a power of two
                                 from 4 to the bucket's bound; each lane then
                                 holds bound / lanes row entries (0 = 4, 4, 4)    */
-    int32_t reserved[1];
+    int32_t col_order;        /* the move kernels read row columns renumbered in
+                                degree-bucket order, with a copy of the
+                                communities in that order (same decisions;
+                                4 bytes per entry and 8 per vertex more): 0 = on
+                                single-graph levels of >= 2^20 vertices that all
+                                have entries, 1 = on every such level, -1 = off */
 } lv_options;
 
 /* Run statistics (lv_run).  Times are device times from HIP events. */

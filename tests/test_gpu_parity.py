@@ -60,6 +60,20 @@ def _cases(lvlib):
                 {"candidate_bounds": 1}))
     out.append(("rmat14_tiers_bounds", lvlib.gen_rmat(14, 16, seed=41),
                 {"heavy_wg_max": 512, "chunk_edges": 128, "candidate_bounds": 1}))
+    # bucket-ordered columns for the move kernels' gathers (by default only on
+    # levels of >= 2^20 vertices) forced on: every tier, the chunked tier,
+    # self-loops, incremental modularity
+    out.append(("rmat14_col_order", lvlib.gen_rmat(14, 16, seed=50), {"col_order": 1}))
+    out.append(("rmat14_chunked_col_order", lvlib.gen_rmat(14, 16, seed=41),
+                {"heavy_wg_max": 320, "chunk_edges": 64, "part_target": 64, "col_order": 1}))
+    out.append(("planted_col_order", lvlib.gen_planted(8000, 40, 0.02, 0.0008, seed=51),
+                {"col_order": 1}))
+    out.append(("random_selfloops_col_order", _csr(lvlib, *graphs.random_graph(400, 3000, 7, True, 40)),
+                {"col_order": 1}))
+    out.append(("star20000_col_order", _csr(lvlib, *graphs.star(20000)), {"col_order": 1}))
+    out.append(("rmat14_incremental_col_order", lvlib.gen_rmat(14, 16, seed=44),
+                {"q_incremental": 1, "heavy_wg_max": 320, "chunk_edges": 256, "col_order": 1,
+                 "candidate_bounds": 1}))
     return out
 
 
@@ -74,7 +88,9 @@ def _names():
             "rmat13_merge_overflow", "rmat14_incremental_q", "planted_incremental_q",
             "rmat14_xcd_q", "rmat14_chunked_tier_xcd", "rmat13_merge_overflow_xcd",
             "rmat14_hash_agg", "planted_hash_agg", "rmat14_bounds", "planted_bounds",
-            "rmat14_tiers_bounds"]
+            "rmat14_tiers_bounds", "rmat14_col_order", "rmat14_chunked_col_order",
+            "planted_col_order", "random_selfloops_col_order", "star20000_col_order",
+            "rmat14_incremental_col_order"]
 
 
 def _get(cases, name):
@@ -181,6 +197,36 @@ def test_full_run_matches_oracle(lvlib, oracle, cases, name):
     assert lv == lv_o
     assert np.array_equal(o2c, o2c_o)
     assert abs(st["modularity"] - st_o["modularity"]) <= Q_TOL
+
+
+def test_col_order_default_at_scale_is_identical(lvlib):
+    """RMAT-21 (about 1.5M vertices with entries: the bucket-ordered columns
+    are on by default at levels 0 and 1) against the same run with them off:
+    every sweep's modularity and movers, the levels and the assignment agree
+    bit for bit."""
+    g = lvlib.gen_rmat(21, 16, seed=21)
+    print(f"case rmat21 V={g.V} E={g.E}", flush=True)
+    runs = []
+    for opt in ({}, {"col_order": -1}):
+        with lvlib.Graph(g, **opt) as G:
+            trace = []
+            for _ in range(2):
+                q0 = G.modularity()
+                for _ in range(6):
+                    q, moved = G.sweep()
+                    trace.append((q, moved))
+                    if moved == 0 or q - q0 < 1e-6:
+                        break
+                    q0 = q
+                G.aggregate()
+            G.reset()
+            st = G.run(1e-6)
+            o2c, lv = G.assignment()
+            runs.append((trace, st["sweeps"], st["levels"], st["modularity"], lv, o2c))
+    (t1, s1, l1, q1, v1, a1), (t2, s2, l2, q2, v2, a2) = runs
+    assert t1 == t2
+    assert (s1, l1, q1, v1) == (s2, l2, q2, v2)
+    assert np.array_equal(a1, a2)
 
 
 def test_rerun_after_reset_is_identical(lvlib):
