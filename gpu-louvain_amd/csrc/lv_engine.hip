@@ -2,12 +2,13 @@
 //
 // Host orchestration of one graph on one GPU stream.  Per level:
 //   setup   degree buckets (stable partition, no PCIe traffic), strengths,
-//           singleton state, heavy-vertex work tables;
+//           singleton state, heavy-vertex work tables (planned and written on
+//           the device; the host reads back bucket offsets and table totals);
 //   phase   sweeps of 7 degree groups (moves of a group are simultaneous,
 //           groups run in order), each group = move kernels + apply; one
-//           fused modularity reduction per sweep; one host sync per sweep
-//           for the termination test (optimiseModularity,
-//           modularity_optimisation.cu:284-363);
+//           modularity per sweep; the termination test (optimiseModularity,
+//           modularity_optimisation.cu:284-363) runs on the device and the
+//           host enqueues the next sweep ahead of it;
 //   merge   aggregation into the other CSR buffer (aggregateCommunities,
 //           community_aggregation.cu:248-347).
 // The level-0 CSR is kept so lv_reset can restart without re-uploading.
@@ -77,14 +78,176 @@ __global__ void k_weight_sign(const float *__restrict__ w, int64_t n, int32_t *f
     if (__ballot(bad) && (threadIdx.x & 63) == 0) atomicOr(flag, 1);
 }
 
-// Row extents of the heavy vertices (host builds their work tables).
-__global__ void k_heavy_rows(const int32_t *__restrict__ rowptr, const int32_t *__restrict__ list,
-                             int32_t n, int64_t *out) {
-    const int32_t i = blockIdx.x * blockDim.x + threadIdx.x;
-    if (i < n) {
-        const int32_t v = list[i];
-        out[2 * i] = rowptr[v];
-        out[2 * i + 1] = rowptr[v + 1];
+// ------------------------------------------------ chunked tier work tables
+// Built on the device once per level.  k_tier_plan (one workgroup) walks the
+// heavy vertices in list order and gives each its partition count, chunks,
+// merge items and table slots by exclusive scans; k_tier_fill (one workgroup
+// per heavy vertex) writes the chunks and merge items.  The host reads back
+// only the totals (to size the tables).  With xcd = true the chunks of a row
+// are cut at its column-block boundaries (split, 9 per row) and both lists
+// get XCD-aligned dispatch orders: workgroup 8j + x takes the j-th chunk
+// (item) of XCD x's list.
+
+// Per heavy vertex, what k_tier_fill needs beyond its HeavyVertex.
+struct HeavyPlan {
+    int64_t lo, hi;  // row range
+    int64_t out0;    // first pair slot of its first chunk
+    int32_t pbase0;  // partition offsets of its first chunk
+    int32_t xc[8];   // rank of its first chunk of column block x in XCD x's list
+    int32_t xi[8];   // ... of its first merge item of block x
+};
+
+struct TierTotals {
+    int64_t outs, slots, nchunks, nitems, pbase;
+    int64_t xc[8];  // chunks in XCD x's list
+    int64_t xi[8];  // merge items in XCD x's list
+};
+
+constexpr int kPlanThreads = 1024;
+
+// Exclusive scan of x over the workgroup (kPlanThreads threads) plus the
+// running carry; returns the thread's offset and adds the tile total to carry.
+__device__ __forceinline__ int64_t plan_scan(int64_t x, int64_t &carry, int64_t *lds) {
+    const int lane = threadIdx.x & 63, wid = threadIdx.x >> 6;
+    int64_t inc = x;
+#pragma unroll
+    for (int off = 1; off < 64; off <<= 1) {
+        const int64_t y = __shfl_up(inc, off, 64);
+        if (lane >= off) inc += y;
+    }
+    if (lane == 63) lds[wid] = inc;
+    __syncthreads();
+    int64_t woff = 0, tot = 0;
+#pragma unroll
+    for (int w2 = 0; w2 < kPlanThreads / 64; ++w2) {
+        woff += w2 < wid ? lds[w2] : 0;
+        tot += lds[w2];
+    }
+    __syncthreads();
+    const int64_t r = carry + woff + inc - x;
+    carry += tot;
+    return r;
+}
+
+__global__ __launch_bounds__(kPlanThreads) void k_tier_plan(
+    const int32_t *__restrict__ rowptr, const int32_t *__restrict__ list, int32_t nh, int32_t ce,
+    int32_t pt, const int32_t *__restrict__ split, lv::HeavyVertex *hvs, HeavyPlan *plan,
+    TierTotals *tot) {
+    __shared__ int64_t lds[kPlanThreads / 64];
+    const bool xcd = split != nullptr;
+    int64_t c_chunk = 0, c_item = 0, c_pb = 0, c_slot = 0, c_out = 0;
+    int64_t c_xc[8] = {0, 0, 0, 0, 0, 0, 0, 0}, c_xi[8] = {0, 0, 0, 0, 0, 0, 0, 0};
+    for (int32_t base = 0; base < nh; base += kPlanThreads) {
+        const int32_t i = base + (int32_t)threadIdx.x;
+        const bool ok = i < nh;
+        int32_t v = 0;
+        int64_t lo = 0, hi = 0, deg = 0, P = 0, n = 0;
+        int64_t nx[8] = {0, 0, 0, 0, 0, 0, 0, 0};
+        int tab_lg = 6;
+        if (ok) {
+            v = list[i];
+            lo = rowptr[v];
+            hi = rowptr[v + 1];
+            deg = hi - lo;
+            if (xcd) {
+                const int64_t m = max((int64_t)1, (deg + 8 * (int64_t)pt - 1) / (8 * (int64_t)pt));
+                P = min((int64_t)lv::kMaxParts, 8 * m);
+                for (int x = 0; x < 8; ++x) {
+                    const int64_t s0 = split[9 * (int64_t)i + x], s1 = split[9 * (int64_t)i + x + 1];
+                    nx[x] = (s1 - s0 + ce - 1) / ce;
+                    n += nx[x];
+                }
+            } else {
+                P = max((int64_t)1, min((int64_t)lv::kMaxParts, (deg + pt - 1) / pt));
+                n = (deg + ce - 1) / ce;
+            }
+            while (((int64_t)1 << tab_lg) < 2 * deg) ++tab_lg;  // >= 2 deg, at least 2^6
+        }
+        const int64_t chunk0 = plan_scan(n, c_chunk, lds);
+        const int64_t tb = plan_scan(P, c_item, lds);
+        const int64_t pb = plan_scan(n * (P + 1), c_pb, lds);
+        const int64_t toff = plan_scan(ok ? ((int64_t)1 << tab_lg) : 0, c_slot, lds);
+        const int64_t out0 = plan_scan(deg, c_out, lds);
+        HeavyPlan hp{};
+        if (xcd) {
+            for (int x = 0; x < 8; ++x) {
+                hp.xc[x] = (int32_t)plan_scan(nx[x], c_xc[x], lds);
+                hp.xi[x] = (int32_t)plan_scan(P / 8, c_xi[x], lds);
+            }
+        }
+        if (ok) {
+            lv::HeavyVertex h;
+            h.v = v;
+            h.P = (int32_t)P;
+            h.chunk0 = (int32_t)chunk0;
+            h.nchunks = (int32_t)n;
+            h.ptot_base = (int32_t)tb;
+            h.tab_lg = tab_lg;
+            h.tab_off = toff;
+            hvs[i] = h;
+            hp.lo = lo;
+            hp.hi = hi;
+            hp.out0 = out0;
+            hp.pbase0 = (int32_t)pb;
+            plan[i] = hp;
+        }
+    }
+    if (threadIdx.x == 0) {
+        tot->outs = c_out;
+        tot->slots = c_slot;
+        tot->nchunks = c_chunk;
+        tot->nitems = c_item;
+        tot->pbase = c_pb;
+        for (int x = 0; x < 8; ++x) {
+            tot->xc[x] = c_xc[x];
+            tot->xi[x] = c_xi[x];
+        }
+    }
+}
+
+__global__ __launch_bounds__(256) void k_tier_fill(const lv::HeavyVertex *__restrict__ hvs,
+                                                   const HeavyPlan *__restrict__ plan, int32_t ce,
+                                                   const int32_t *__restrict__ split,
+                                                   lv::Chunk *chunks, lv::MergeItem *items,
+                                                   int32_t *corder, int32_t *iorder) {
+    const int32_t i = blockIdx.x;
+    const lv::HeavyVertex h = hvs[i];
+    const HeavyPlan hp = plan[i];
+    auto put = [&](int32_t k, int64_t clo, int64_t chi) {
+        lv::Chunk c;
+        c.hv = i;
+        c.v = h.v;
+        c.lo = clo;
+        c.hi = chi;
+        c.out = hp.out0 + (clo - hp.lo);  // chunks tile the row in order
+        c.pbase = hp.pbase0 + k * (h.P + 1);
+        c.P = h.P;
+        c.ptot_base = h.ptot_base;
+        c.first = k == 0 ? 1 : 0;
+        chunks[h.chunk0 + k] = c;
+    };
+    if (!split) {
+        for (int32_t k = threadIdx.x; k < h.nchunks; k += blockDim.x) {
+            const int64_t clo = hp.lo + (int64_t)k * ce;
+            put(k, clo, min(hp.hi, clo + ce));
+        }
+    } else {
+        int32_t kb = 0;
+        for (int x = 0; x < 8; ++x) {
+            const int64_t s0 = split[9 * (int64_t)i + x], s1 = split[9 * (int64_t)i + x + 1];
+            const int32_t cnt = (int32_t)((s1 - s0 + ce - 1) / ce);
+            for (int32_t j = threadIdx.x; j < cnt; j += blockDim.x) {
+                const int64_t clo = s0 + (int64_t)j * ce;
+                put(kb + j, clo, min(s1, clo + ce));
+                corder[8 * (hp.xc[x] + j) + x] = h.chunk0 + kb + j;
+            }
+            kb += cnt;
+        }
+    }
+    const int32_t m = h.P / 8;
+    for (int32_t p = threadIdx.x; p < h.P; p += blockDim.x) {
+        items[h.ptot_base + p] = lv::MergeItem{i, p};
+        if (split) iorder[8 * (hp.xi[p / m] + p % m) + p / m] = h.ptot_base + p;
     }
 }
 
@@ -153,13 +316,73 @@ __global__ void k_state_from_comm(int32_t V, const int32_t *__restrict__ comm,
 }
 
 // Summed member degree of each global-tier community (host builds its chunks).
-__global__ void k_agg_extent(const int32_t *__restrict__ list, int32_t n,
-                             const int32_t *__restrict__ size, const int32_t *__restrict__ mstart,
-                             const int32_t *__restrict__ mscan, int32_t *D) {
-    const int32_t i = blockIdx.x * blockDim.x + threadIdx.x;
-    if (i < n) {
-        const int32_t c = list[i];
-        D[i] = mscan[mstart[c] + size[c]] - mscan[mstart[c]];
+// Work tables of the aggregation's global hash tier (agg_mode 1), built on
+// the device like the chunked tier's: per listed community i, its summed
+// member degree D (insert chunks of `ic` flat member edges), a table of
+// 2^lg >= 2 min(D, newV) slots (extract chunks of `xc` slots) and its sort
+// range.  k_agg_plan scans them (one workgroup) into off / soff / lg and the
+// first chunk of each kind; k_agg_fill writes the chunks.
+struct AggPlan {
+    int32_t D;
+    int32_t ins0, ext0;
+    int32_t pad;
+};
+
+struct AggTotals {
+    int64_t nins, next, slots, sorts;
+};
+
+__global__ __launch_bounds__(kPlanThreads) void k_agg_plan(
+    const int32_t *__restrict__ list, int32_t n, const int32_t *__restrict__ size,
+    const int32_t *__restrict__ mstart, const int32_t *__restrict__ mscan, int32_t newV, int32_t ic,
+    int32_t xc, int64_t *off, int64_t *soff, int32_t *lg, AggPlan *plan, AggTotals *tot) {
+    __shared__ int64_t lds[kPlanThreads / 64];
+    int64_t c_ins = 0, c_ext = 0, c_slot = 0, c_sort = 0;
+    for (int32_t base = 0; base < n; base += kPlanThreads) {
+        const int32_t i = base + (int32_t)threadIdx.x;
+        const bool ok = i < n;
+        int64_t D = 0, cap = 0;
+        int l = 6;
+        if (ok) {
+            const int32_t c = list[i];
+            D = mscan[mstart[c] + size[c]] - mscan[mstart[c]];
+            const int64_t distinct_max = D < (int64_t)newV ? D : (int64_t)newV;
+            while (((int64_t)1 << l) < 2 * distinct_max) ++l;
+            cap = (int64_t)1 << l;
+        }
+        const int64_t i0 = plan_scan((D + ic - 1) / ic, c_ins, lds);
+        const int64_t x0 = plan_scan((cap + xc - 1) / xc, c_ext, lds);
+        const int64_t t0 = plan_scan(cap, c_slot, lds);
+        const int64_t s0 = plan_scan(D, c_sort, lds);
+        if (ok) {
+            off[i] = t0;
+            soff[i] = s0;
+            lg[i] = l;
+            plan[i] = AggPlan{(int32_t)D, (int32_t)i0, (int32_t)x0, 0};
+        }
+    }
+    if (threadIdx.x == 0) *tot = AggTotals{c_ins, c_ext, c_slot, c_sort};
+}
+
+__global__ __launch_bounds__(256) void k_agg_fill(const int32_t *__restrict__ list,
+                                                  const AggPlan *__restrict__ plan,
+                                                  const int64_t *__restrict__ off,
+                                                  const int32_t *__restrict__ lg, int32_t ic,
+                                                  int32_t xc, lv::AggChunk *ins, lv::AggChunk *ext) {
+    const int32_t i = blockIdx.x;
+    const int32_t c = list[i];
+    const AggPlan p = plan[i];
+    const int64_t D = p.D;
+    const int32_t ni = (int32_t)((D + ic - 1) / ic);
+    for (int32_t j = threadIdx.x; j < ni; j += blockDim.x) {
+        const int64_t f = (int64_t)j * ic;
+        ins[p.ins0 + j] = lv::AggChunk{c, i, f, f + ic < D ? f + ic : D};
+    }
+    const int64_t cap = (int64_t)1 << lg[i];
+    const int32_t nx = (int32_t)((cap + xc - 1) / xc);
+    for (int32_t j = threadIdx.x; j < nx; j += blockDim.x) {
+        const int64_t q = (int64_t)j * xc;
+        ext[p.ext0 + j] = lv::AggChunk{c, i, off[i] + q, off[i] + (q + xc < cap ? q + xc : cap)};
     }
 }
 
@@ -431,7 +654,8 @@ struct lv_graph {
     DevVec<lv::Chunk> ins;
     DevVec<lv::HeavyVertex> hvs;
     DevVec<lv::MergeItem> items;
-    DevVec<int64_t> hrows;
+    DevVec<HeavyPlan> hplan;      // per heavy vertex: k_tier_fill's inputs
+    DevVec<TierTotals> tier_tot;  // totals of the tables (read back to size them)
     DevVec<int32_t> out_key, poff, ptot;
     DevVec<double> out_val;
     DevVec<uint64_t> item_best;
@@ -496,7 +720,9 @@ struct lv_graph {
     float *tmp_w = nullptr;
     DevVec<lv::AggChunk> a_ins, a_ext;
     DevVec<int64_t> a_toff, a_soff;
-    DevVec<int32_t> a_lg, a_D;
+    DevVec<int32_t> a_lg;
+    DevVec<AggPlan> a_plan;
+    DevVec<AggTotals> a_tot;
     DevVec<uint64_t> sortA, sortB;
     // sort-based aggregation: key/value ping-pong, run heads and slots, sums
     DevVec<uint64_t> skA, skB;
@@ -1039,118 +1265,51 @@ struct lv_graph {
         LV_TRY(setup_xcd());
 
         // chunked tier: row chunks (stage A, strength, modularity), per-vertex
-        // partitions (stage B) and overflow tables
+        // partitions (stage B) and overflow tables, planned and written on the
+        // device (k_tier_plan, k_tier_fill); only their totals come back
         const int HB = lv::kNumBuckets - 1;
         nh = h_off[HB + 1] - h_off[HB];
         nins = nitems = 0;
         tier_xcd = false;
         n_corder = n_iorder = 0;
         if (nh > 0) {
-            LV_TRY(hrows.reserve(2 * (int64_t)nh));
-            hipLaunchKernelGGL(k_heavy_rows, dim3(blocks_for(nh, 256)), dim3(256), 0, s, rp,
-                               order + h_off[HB], nh, hrows.p);
-            LV_LAUNCHED("heavy rows");
-            std::vector<int64_t> rows(2 * (size_t)nh);
-            std::vector<int32_t> ids(nh);
-            LV_HIP(hipMemcpyAsync(rows.data(), hrows.p, sizeof(int64_t) * 2 * nh,
-                                  hipMemcpyDeviceToHost, s));
-            LV_HIP(hipMemcpyAsync(ids.data(), order + h_off[HB], sizeof(int32_t) * nh,
-                                  hipMemcpyDeviceToHost, s));
-            LV_TRY(sync());
             // with sorted rows: chunks cut at the column-block boundaries and
             // partitions by id range, both in an XCD-aligned dispatch order
             // (measured slower than the hash partitions on RMAT-22: opt-in)
             tier_xcd = xcd_ok && opt.xcd_rows == 2;
-            std::vector<int32_t> sp;
+            const int32_t *split = tier_xcd ? xsplit.p + 9 * (int64_t)(h_off[HB] - xh_lo) : nullptr;
+            LV_TRY(hvs.reserve(nh));
+            LV_TRY(hplan.reserve(nh));
+            LV_TRY(tier_tot.reserve(1));
+            hipLaunchKernelGGL(k_tier_plan, dim3(1), dim3(kPlanThreads), 0, s, rp, order + h_off[HB],
+                               nh, (int32_t)chunk_edges, (int32_t)part_target, split, hvs.p,
+                               hplan.p, tier_tot.p);
+            LV_LAUNCHED("chunked tier plan");
+            LV_TRY(stage(sizeof(TierTotals)));
+            LV_HIP(hipMemcpyAsync(h_stage, tier_tot.p, sizeof(TierTotals), hipMemcpyDeviceToHost, s));
+            LV_TRY(sync());
+            TierTotals tt;
+            std::memcpy(&tt, h_stage, sizeof tt);
+            if (tt.nchunks > INT32_MAX || tt.nitems > INT32_MAX || tt.pbase > INT32_MAX)
+                return fail(LV_ERR_ARG, "chunked tier tables exceed 2^31 entries (raise chunk_edges)");
+            nins = (int32_t)tt.nchunks;
+            nitems = (int32_t)tt.nitems;
+            const int64_t pbase = tt.pbase, tbase = tt.nitems, slots = tt.slots, outs = tt.outs;
             if (tier_xcd) {
-                sp.resize(9 * (size_t)nh);
-                LV_HIP(hipMemcpy(sp.data(), xsplit.p + 9 * (int64_t)(h_off[HB] - xh_lo),
-                                 sizeof(int32_t) * 9 * nh, hipMemcpyDeviceToHost));
-            }
-            std::vector<std::vector<int32_t>> xl(8), il(8);
-            std::vector<lv::Chunk> hin;
-            std::vector<lv::HeavyVertex> hh(nh);
-            std::vector<lv::MergeItem> hit;
-            int64_t slots = 0, outs = 0;
-            int32_t pbase = 0, tbase = 0;
-            for (int32_t i = 0; i < nh; ++i) {
-                const int64_t lo = rows[2 * i], hi = rows[2 * i + 1], deg = hi - lo;
-                int32_t P;
-                if (tier_xcd) {
-                    const int64_t m = std::max<int64_t>(1, (deg + 8 * (int64_t)part_target - 1) /
-                                                               (8 * (int64_t)part_target));
-                    P = (int32_t)std::min<int64_t>(lv::kMaxParts, 8 * m);
-                } else {
-                    P = (int32_t)std::min<int64_t>(lv::kMaxParts, (deg + part_target - 1) / part_target);
-                    P = std::max(P, 1);
+                int64_t Lc = 0, Li = 0;
+                for (int x = 0; x < 8; ++x) {
+                    Lc = std::max(Lc, tt.xc[x]);
+                    Li = std::max(Li, tt.xi[x]);
                 }
-                lv::HeavyVertex &h = hh[i];
-                h.v = ids[i];
-                h.P = P;
-                h.chunk0 = (int32_t)hin.size();
-                h.ptot_base = tbase;
-                h.tab_lg = std::max(6, ceil_log2(2 * deg));
-                h.tab_off = slots;
-                auto add_chunk = [&](int64_t clo, int64_t chi, int x) {
-                    lv::Chunk c{};
-                    c.hv = i;
-                    c.v = ids[i];
-                    c.lo = clo;
-                    c.hi = chi;
-                    c.out = outs;
-                    c.pbase = pbase;
-                    c.P = P;
-                    c.ptot_base = tbase;
-                    c.first = hin.size() == (size_t)h.chunk0 ? 1 : 0;
-                    outs += chi - clo;
-                    pbase += P + 1;
-                    if (x >= 0) xl[x].push_back((int32_t)hin.size());
-                    hin.push_back(c);
-                };
-                if (tier_xcd) {
-                    for (int x = 0; x < 8; ++x) {
-                        const int64_t s0 = sp[9 * (size_t)i + x], s1 = sp[9 * (size_t)i + x + 1];
-                        for (int64_t e = s0; e < s1; e += chunk_edges)
-                            add_chunk(e, std::min<int64_t>(s1, e + chunk_edges), x);
-                    }
-                } else {
-                    for (int64_t e = lo; e < hi; e += chunk_edges)
-                        add_chunk(e, std::min<int64_t>(hi, e + chunk_edges), -1);
-                }
-                h.nchunks = (int32_t)hin.size() - h.chunk0;
-                for (int32_t p = 0; p < P; ++p) {
-                    if (tier_xcd) il[p / (P / 8)].push_back((int32_t)hit.size());
-                    hit.push_back({i, p});
-                }
-                tbase += P;
-                slots += (int64_t)1 << h.tab_lg;
-            }
-            n_corder = n_iorder = 0;
-            if (tier_xcd) {
-                // workgroup 8j + x takes the j-th entry of XCD x's list
-                auto interleave = [](const std::vector<std::vector<int32_t>> &l,
-                                     std::vector<int32_t> &out) {
-                    size_t L = 0;
-                    for (const auto &q : l) L = std::max(L, q.size());
-                    out.assign(8 * L, -1);
-                    for (int x = 0; x < 8; ++x)
-                        for (size_t j = 0; j < l[x].size(); ++j) out[8 * j + x] = l[x][j];
-                };
-                std::vector<int32_t> co, io;
-                interleave(xl, co);
-                interleave(il, io);
-                n_corder = (int32_t)co.size();
-                n_iorder = (int32_t)io.size();
+                n_corder = (int32_t)(8 * Lc);
+                n_iorder = (int32_t)(8 * Li);
                 LV_TRY(corder.reserve(n_corder));
                 LV_TRY(iorder.reserve(n_iorder));
-                LV_HIP(hipMemcpy(corder.p, co.data(), sizeof(int32_t) * co.size(), hipMemcpyHostToDevice));
-                LV_HIP(hipMemcpy(iorder.p, io.data(), sizeof(int32_t) * io.size(), hipMemcpyHostToDevice));
+                lv::fill_i32(corder.p, -1, n_corder, s);
+                lv::fill_i32(iorder.p, -1, n_iorder, s);
             }
-            nins = (int32_t)hin.size();
-            nitems = (int32_t)hit.size();
             LV_TRY(arena(slots));
             LV_TRY(ins.reserve(nins));
-            LV_TRY(hvs.reserve(nh));
             LV_TRY(items.reserve(nitems));
             LV_TRY(out_key.reserve(outs));
             LV_TRY(out_val.reserve(outs));
@@ -1164,10 +1323,9 @@ struct lv_graph {
             LV_TRY(hdone.reserve(nh));
             LV_TRY(hhint.reserve(nh));
             LV_TRY(hdist.reserve(nh));
-            LV_HIP(hipMemcpy(ins.p, hin.data(), sizeof(lv::Chunk) * nins, hipMemcpyHostToDevice));
-            LV_HIP(hipMemcpy(hvs.p, hh.data(), sizeof(lv::HeavyVertex) * nh, hipMemcpyHostToDevice));
-            LV_HIP(hipMemcpy(items.p, hit.data(), sizeof(lv::MergeItem) * nitems,
-                             hipMemcpyHostToDevice));
+            hipLaunchKernelGGL(k_tier_fill, dim3(nh), dim3(256), 0, s, hvs.p, hplan.p,
+                               (int32_t)chunk_edges, split, ins.p, items.p, corder.p, iorder.p);
+            LV_LAUNCHED("chunked tier fill");
             lv::fill_i32(ptot.p, 0, tbase, s);
             lv::fill_f64(hkcur.p, 0.0, nh, s);
             lv::fill_i32(hovf.p, 0, nh, s);
@@ -2097,49 +2255,31 @@ struct lv_graph {
         const int32_t ng = toff[GT + 1] - toff[GT];
         if (ng > 0) {
             const int32_t *glist = a_order + toff[GT];
-            LV_TRY(a_D.reserve(ng));
-            hipLaunchKernelGGL(k_agg_extent, dim3(blocks_for(ng, 256)), dim3(256), 0, s, glist, ng,
-                               size, a_mstart, a_mscan, a_D.p);
-            LV_LAUNCHED("aggregation extents");
-            std::vector<int32_t> D(ng), ids(ng);
-            LV_HIP(hipMemcpyAsync(D.data(), a_D.p, sizeof(int32_t) * ng, hipMemcpyDeviceToHost, s));
-            LV_HIP(hipMemcpyAsync(ids.data(), glist, sizeof(int32_t) * ng, hipMemcpyDeviceToHost,
-                                  s));
-            LV_TRY(sync());
-            std::vector<lv::AggChunk> hin, hex;
-            std::vector<int64_t> toffs(ng), soffs(ng);
-            std::vector<int32_t> lg(ng);
-            int64_t slots = 0, sorts = 0;
-            for (int32_t i = 0; i < ng; ++i) {
-                const int64_t d = D[i];
-                for (int64_t f = 0; f < d; f += kAggChunk)
-                    hin.push_back({ids[i], i, f, std::min<int64_t>(d, f + kAggChunk)});
-                const int64_t distinct_max = std::min<int64_t>(d, (int64_t)newV);
-                lg[i] = std::max(6, ceil_log2(2 * distinct_max));
-                toffs[i] = slots;
-                const int64_t cap = (int64_t)1 << lg[i];
-                for (int64_t q = 0; q < cap; q += kSlotChunk)
-                    hex.push_back({ids[i], i, slots + q, slots + std::min<int64_t>(cap, q + kSlotChunk)});
-                slots += cap;
-                soffs[i] = sorts;
-                sorts += d;
-            }
-            LV_TRY(arena(slots));
-            LV_TRY(a_ins.reserve((int64_t)hin.size()));
-            LV_TRY(a_ext.reserve((int64_t)hex.size()));
+            LV_TRY(a_plan.reserve(ng));
+            LV_TRY(a_tot.reserve(1));
             LV_TRY(a_toff.reserve(ng));
             LV_TRY(a_soff.reserve(ng));
             LV_TRY(a_lg.reserve(ng));
-            LV_TRY(sortA.reserve(sorts));
-            LV_TRY(sortB.reserve(sorts));
-            LV_HIP(hipMemcpy(a_ins.p, hin.data(), sizeof(lv::AggChunk) * hin.size(),
-                             hipMemcpyHostToDevice));
-            LV_HIP(hipMemcpy(a_ext.p, hex.data(), sizeof(lv::AggChunk) * hex.size(),
-                             hipMemcpyHostToDevice));
-            LV_HIP(hipMemcpy(a_toff.p, toffs.data(), sizeof(int64_t) * ng, hipMemcpyHostToDevice));
-            LV_HIP(hipMemcpy(a_soff.p, soffs.data(), sizeof(int64_t) * ng, hipMemcpyHostToDevice));
-            LV_HIP(hipMemcpy(a_lg.p, lg.data(), sizeof(int32_t) * ng, hipMemcpyHostToDevice));
-            lv::launch_agg_global(a, a_ins.p, (int32_t)hin.size(), a_ext.p, (int32_t)hex.size(),
+            hipLaunchKernelGGL(k_agg_plan, dim3(1), dim3(kPlanThreads), 0, s, glist, ng, size,
+                               a_mstart, a_mscan, newV, kAggChunk, kSlotChunk, a_toff.p, a_soff.p,
+                               a_lg.p, a_plan.p, a_tot.p);
+            LV_LAUNCHED("aggregation plan");
+            LV_TRY(stage(sizeof(AggTotals)));
+            LV_HIP(hipMemcpyAsync(h_stage, a_tot.p, sizeof(AggTotals), hipMemcpyDeviceToHost, s));
+            LV_TRY(sync());
+            AggTotals at;
+            std::memcpy(&at, h_stage, sizeof at);
+            if (at.nins > INT32_MAX || at.next > INT32_MAX)
+                return fail(LV_ERR_ARG, "aggregation tables exceed 2^31 chunks");
+            LV_TRY(arena(at.slots));
+            LV_TRY(a_ins.reserve(at.nins));
+            LV_TRY(a_ext.reserve(at.next));
+            LV_TRY(sortA.reserve(at.sorts));
+            LV_TRY(sortB.reserve(at.sorts));
+            hipLaunchKernelGGL(k_agg_fill, dim3(ng), dim3(256), 0, s, glist, a_plan.p, a_toff.p,
+                               a_lg.p, kAggChunk, kSlotChunk, a_ins.p, a_ext.p);
+            LV_LAUNCHED("aggregation tables");
+            lv::launch_agg_global(a, a_ins.p, (int32_t)at.nins, a_ext.p, (int32_t)at.next,
                                   glist, ng, a_toff.p, a_lg.p, gkeys.p, gvals.p, sortA.p, sortB.p,
                                   a_soff.p, s);
             LV_LAUNCHED("merge (global tier)");
@@ -2650,7 +2790,8 @@ struct lv_graph {
         out_val.release();
         poff.release();
         ptot.release();
-        hrows.release();
+        hplan.release();
+        tier_tot.release();
         hbest.release();
         hkcur.release();
         hovf.release();
@@ -2672,7 +2813,8 @@ struct lv_graph {
         a_toff.release();
         a_soff.release();
         a_lg.release();
-        a_D.release();
+        a_plan.release();
+        a_tot.release();
         sortA.release();
         sortB.release();
         skA.release();
