@@ -625,6 +625,9 @@ struct lv_graph {
     bool singleton_state = false;
     bool sown0_ok = false, sown_next_ok = false;
     double *d_sown = nullptr;
+    // the current aggregated level's rows are sorted by column without
+    // duplicates (the radix aggregation writes them so)
+    bool rows_sorted = false;
     bool weights_nonneg = false; // every weight >= 0: candidate bounds are valid
     int64_t last_moved = -1;     // movers of the previous sweep (-1: unknown)
     int64_t last_moved_edges = -1;  // row entries of those movers
@@ -1213,6 +1216,7 @@ struct lv_graph {
             return LV_OK;
         }
         cur_buf = -1;
+        rows_sorted = false;
         V = V0;
         E = E0;
         map_cur = 0;
@@ -1392,7 +1396,8 @@ struct lv_graph {
         state_changed(true);
         if (G == 0 && opt.exact_sums >= 0) {
             const bool lvl0 = cur_buf < 0;
-            lv::launch_selfloop_weight(rp, col(), w(), V, d_sown + 2, d_sown + (lvl0 ? 0 : 1), s);
+            lv::launch_selfloop_weight(rp, col(), w(), V, rows_sorted && !lvl0, d_sown + 2,
+                                       d_sown + (lvl0 ? 0 : 1), s);
             LV_LAUNCHED("self-loop weight");
             (lvl0 ? sown0_ok : sown_next_ok) = true;
         }
@@ -2307,6 +2312,7 @@ struct lv_graph {
         E = newE;
         level_ready = false;
         col_shadow = false;
+        rows_sorted = false;
         if (G > 0) {
             std::swap(vptr, vptr_next);
             lv::launch_graph_ids(vptr, G, gid, s);
@@ -2664,6 +2670,7 @@ struct lv_graph {
         E = newE;
         level_ready = false;
         col_shadow = false;
+        rows_sorted = true;
         if (G > 0) {
             std::swap(vptr, vptr_next);
             lv::launch_graph_ids(vptr, G, gid, s);

@@ -205,10 +205,11 @@ void launch_apply(const MoveArgs &a, const int32_t *list, int32_t n, const int32
 
 // --- modularity ---
 // *out = the weight of the self-loop entries (col[e] == row), FP64, over a
-// fixed grid of kSelfLoopBlocks blocks with per-block partials.
+// fixed grid of kSelfLoopBlocks blocks with per-block partials.  sorted: the
+// rows are sorted by column without duplicates (a binary search per row).
 constexpr int kSelfLoopBlocks = 1024;
 void launch_selfloop_weight(const int32_t *rowptr, const int32_t *col, const float *w, int32_t V,
-                            double *partials, double *out, hipStream_t s);
+                            bool sorted, double *partials, double *out, hipStream_t s);
 // Full pass: per-block partials (own, sq), then q[0] = Q, q[1] = own, q[2] = sq.
 void launch_modularity(const int32_t *rowptr, const int32_t *col, const float *w,
                        const int32_t *comm, const double *sigma, int32_t V, const int32_t *order,

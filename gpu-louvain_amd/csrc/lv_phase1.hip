@@ -316,10 +316,13 @@ __device__ __forceinline__ void wave_tab_clear(int32_t *K, double *W, int lane) 
 }
 
 // Inserts vertex v's row (MAXE entries per lane, deg <= 64 * MAXE) into the
-// wave's table; returns v's community.
+// wave's table; returns v's community.  Entries of v's own community are
+// summed in registers instead (the key most entries of a settled row name,
+// which would serialise the lanes' LDS atomics on one slot); kc gets that
+// weight, summed over the wave.
 template <int C, int MAXE>
 __device__ __forceinline__ int32_t wave_tab_fill(const MoveArgs &a, int32_t v, int32_t *K,
-                                                 double *W, int lane) {
+                                                 double *W, int lane, double &kc) {
     constexpr bool POW2 = (C & (C - 1)) == 0;
     constexpr int LOG2C = ilog2_c(C);
     const int32_t rs = a.rowptr[v], re = a.rowptr[v + 1];
@@ -336,22 +339,29 @@ __device__ __forceinline__ int32_t wave_tab_fill(const MoveArgs &a, int32_t v, i
     int32_t cc[MAXE];
 #pragma unroll
     for (int j = 0; j < MAXE; ++j) cc[j] = (u[j] >= 0 && u[j] != vs) ? a.ccomm[u[j]] : -1;
+    double own = 0.0;
 #pragma unroll
     for (int j = 0; j < MAXE; ++j) {
         if (cc[j] < 0) continue;
+        if (cc[j] == cur) {
+            own += (double)wv[j];
+            continue;
+        }
         if constexpr (POW2)
             lds_insert(K, W, 32 - LOG2C, C - 1, cc[j], (double)wv[j]);
         else
             lds_insert_n(K, W, (uint32_t)C, cc[j], (double)wv[j]);
     }
+    kc = lanes_sum<64>(own);
     return cur;
 }
 
-// Decision of v (in community cur) from its filled table; the whole wave
-// calls it.
+// Decision of v (in community cur, kc = its weight to cur) from its filled
+// table, which holds the other communities; the whole wave calls it.
 template <int C>
 __device__ __forceinline__ void wave_tab_decide(const MoveArgs &a, int32_t v, int32_t cur,
-                                                const int32_t *K, const double *W, int lane) {
+                                                double kc, const int32_t *K, const double *W,
+                                                int lane) {
     constexpr int S = C / 64;  // slots per lane
     const float k32 = (float)a.k[v];
     const float2 md = vertex_mden(a, v);
@@ -367,11 +377,6 @@ __device__ __forceinline__ void wave_tab_decide(const MoveArgs &a, int32_t v, in
         kk[j] = K[lane + 64 * j];
         vv[j] = W[lane + 64 * j];
     }
-    double kc = 0.0;
-#pragma unroll
-    for (int j = 0; j < S; ++j)
-        if (kk[j] == cur) kc = vv[j];
-    kc = lanes_sum<64>(kc);
     uint64_t best = pack_best(0.0f, cur);
     double bestK = 0.0;
     if (a.bounds) {
@@ -435,13 +440,14 @@ __global__ __launch_bounds__(256) void k_move_wave(MoveArgs a, const int32_t *__
     wave_tab_clear<C>(K, W, lane);
     __syncthreads();
     int32_t v = 0, cur = 0;
+    double kc = 0.0;
     if (active) {
         v = list[slot];
-        cur = wave_tab_fill<C, MAXE>(a, v, K, W, lane);
+        cur = wave_tab_fill<C, MAXE>(a, v, K, W, lane, kc);
     }
     __syncthreads();
     if (!active) return;  // only wave-level communication below
-    wave_tab_decide<C>(a, v, cur, K, W, lane);
+    wave_tab_decide<C>(a, v, cur, kc, K, W, lane);
 }
 
 // TIER only names the instance (one per workgroup bucket) for profiles.
@@ -499,6 +505,10 @@ __global__ __launch_bounds__(T, 8) void k_move_wg(MoveArgs a, const int32_t *__r
         __syncthreads();
         const int32_t cur = a.comm[v];
         const int32_t vs = col_self(a, v);
+        // entries of v's own community are summed in registers (the key most
+        // entries of a settled row name: in the table its slot would
+        // serialise the lanes' atomics), the others go to the table
+        double own = 0.0;
         if constexpr (TIER == 4) {
             volatile int32_t *fill = &fill_s, *stop = &stop_s;
             bool dropped = false;
@@ -517,6 +527,10 @@ __global__ __launch_bounds__(T, 8) void k_move_wg(MoveArgs a, const int32_t *__r
 #pragma unroll
                 for (int j = 0; j < 4; ++j) {
                     if (cc[j] < 0 || dropped) continue;
+                    if (cc[j] == cur) {
+                        own += (double)wv[j];
+                        continue;
+                    }
                     if (*fill >= limit) {
                         dropped = true;
                         *stop = 1;
@@ -526,6 +540,8 @@ __global__ __launch_bounds__(T, 8) void k_move_wg(MoveArgs a, const int32_t *__r
                 }
                 if (*stop) dropped = true;  // another lane gave up: so does this one
             }
+            own = lanes_sum<64>(own);
+            if (lane == 0) red_kc[wid] = own;
             __syncthreads();
             const bool gave_up = stop_s != 0;
             if (gave_up) {
@@ -547,11 +563,21 @@ __global__ __launch_bounds__(T, 8) void k_move_wg(MoveArgs a, const int32_t *__r
                 for (int j = 0; j < 4; ++j)
                     cc[j] = (u[j] >= 0 && u[j] != vs) ? a.ccomm[u[j]] : -1;
 #pragma unroll
-                for (int j = 0; j < 4; ++j)
-                    if (cc[j] >= 0) lds_insert_n(K, W, (uint32_t)Cv, cc[j], (double)wv[j]);
+                for (int j = 0; j < 4; ++j) {
+                    if (cc[j] == cur)
+                        own += (double)wv[j];
+                    else if (cc[j] >= 0)
+                        lds_insert_n(K, W, (uint32_t)Cv, cc[j], (double)wv[j]);
+                }
             }
+            own = lanes_sum<64>(own);
+            if (lane == 0) red_kc[wid] = own;
             __syncthreads();
         }
+        // weight to the own community, summed over the waves in a fixed order
+        double kc = 0.0;
+#pragma unroll
+        for (int w2 = 0; w2 < T / 64; ++w2) kc += red_kc[w2];
         const float k32 = (float)a.k[v];
         const float2 md = vertex_mden(a, v);
         const float vM32 = md.x, vden = md.y;
@@ -559,10 +585,9 @@ __global__ __launch_bounds__(T, 8) void k_move_wg(MoveArgs a, const int32_t *__r
         const float scur = rec_sigma(rcur);
         const int32_t size_cur = rec_size(rcur);
         if (a.bounds) {
-            // kc from the own community's slot.  Each wave evaluates the
-            // heaviest candidate among its own slots first; its other slots
-            // are read only if their bound can still change the decision.
-            const double kc = lds_value_n(K, W, (uint32_t)Cv, cur);
+            // Each wave evaluates the heaviest candidate among its own slots
+            // first; its other slots are read only if their bound can still
+            // change the decision.
             double km = -1.0;
             int32_t ks = -1;
             for (int i = threadIdx.x; i < Cv; i += T) {
@@ -628,7 +653,7 @@ __global__ __launch_bounds__(T, 8) void k_move_wg(MoveArgs a, const int32_t *__r
             continue;
         }
         uint64_t best = pack_best(0.0f, cur);
-        double bestK = 0.0, kc = 0.0;
+        double bestK = 0.0;
         for (int i0 = threadIdx.x; i0 < Cv; i0 += 4 * T) {
             int32_t kk[4];
             double vv[4];
@@ -640,37 +665,30 @@ __global__ __launch_bounds__(T, 8) void k_move_wg(MoveArgs a, const int32_t *__r
                 vv[j] = i < Cv ? W[i] : 0.0;
             }
 #pragma unroll
-            for (int j = 0; j < 4; ++j) rr[j] = (kk[j] >= 0 && kk[j] != cur) ? a.crec[kk[j]] : 0ull;
+            for (int j = 0; j < 4; ++j) rr[j] = kk[j] >= 0 ? a.crec[kk[j]] : 0ull;
 #pragma unroll
             for (int j = 0; j < 4; ++j) {
                 if (kk[j] < 0) continue;
-                if (kk[j] == cur) {
-                    kc = vv[j];
-                    continue;
-                }
                 consider_k(best, bestK, kk[j], vv[j], rr[j], cur, size_cur, k32, scur, vM32,
                            vden);
             }
         }
         lanes_max_pair<64>(best, bestK);
-        kc = lanes_sum<64>(kc);
         if (lane == 0) {
             red_best[wid] = best;
             red_K[wid] = bestK;
-            red_kc[wid] = kc;
         }
         __syncthreads();
         if (threadIdx.x == 0) {
             uint64_t b = red_best[0];
-            double bK = red_K[0], kk = red_kc[0];
+            double bK = red_K[0];
             for (int w2 = 1; w2 < T / 64; ++w2) {
                 if (red_best[w2] > b) {
                     b = red_best[w2];
                     bK = red_K[w2];
                 }
-                kk += red_kc[w2];
             }
-            write_decision(a, v, cur, b, bK, kk);
+            write_decision(a, v, cur, b, bK, kc);
             if constexpr (TIER == 4) {
                 done[vi] = 1;
                 hint[vi] = fill_s;
@@ -727,6 +745,10 @@ __global__ __launch_bounds__(256) void k_chunk_aggregate(MoveArgs a, ChunkedTier
     for (int p = threadIdx.x; p < P; p += 256) pcnt[p] = 0;
     __syncthreads();
     const int32_t vs = col_self(a, ch.v);
+    const int32_t cur = a.comm[ch.v];
+    // entries of the vertex's own community are summed in registers and
+    // added to kcur (one FP64 atomic per chunk), the others go to the table
+    double own = 0.0;
     // four entries per lane at a time: their loads, then their gathers, are
     // in flight together ahead of the LDS inserts
     for (int64_t e0 = ch.lo + threadIdx.x; e0 < ch.hi; e0 += 4 * 256) {
@@ -742,10 +764,23 @@ __global__ __launch_bounds__(256) void k_chunk_aggregate(MoveArgs a, ChunkedTier
 #pragma unroll
         for (int j = 0; j < 4; ++j) cc[j] = (u[j] >= 0 && u[j] != vs) ? a.ccomm[u[j]] : -1;
 #pragma unroll
-        for (int j = 0; j < 4; ++j)
-            if (cc[j] >= 0) lds_insert(keys, vals, 32 - cl, (uint32_t)Cv - 1, cc[j], (double)wf[j]);
+        for (int j = 0; j < 4; ++j) {
+            if (cc[j] == cur)
+                own += (double)wf[j];
+            else if (cc[j] >= 0)
+                lds_insert(keys, vals, 32 - cl, (uint32_t)Cv - 1, cc[j], (double)wf[j]);
+        }
     }
-    __syncthreads();
+    {
+        __shared__ double own_s[4];
+        own = lanes_sum<64>(own);
+        if ((threadIdx.x & 63) == 0) own_s[threadIdx.x >> 6] = own;
+        __syncthreads();
+        if (threadIdx.x == 0) {
+            const double x = own_s[0] + own_s[1] + own_s[2] + own_s[3];
+            if (x != 0.0) atomicAdd(&t.kcur[ch.hv], x);
+        }
+    }
     for (int s = threadIdx.x; s < Cv; s += 256) {
         const int32_t k = keys[s];
         if (k >= 0) atomicAdd(&pcnt[tier_part(t, k, (uint32_t)P)], 1);
@@ -868,7 +903,6 @@ __global__ __launch_bounds__(kMergeThreads) void k_chunk_merge(MoveArgs a, Chunk
     __shared__ int32_t seg_off[kMergeTile + 1];
     __shared__ uint64_t red_best[kMergeThreads / 64];
     __shared__ double red_K[kMergeThreads / 64];
-    __shared__ double red_kc[kMergeThreads / 64];
     __shared__ int32_t red_n[kMergeThreads / 64];
     const int32_t item = t.item_order ? t.item_order[blockIdx.x] : (int32_t)blockIdx.x;
     if (item < 0) return;  // padding of the XCD dispatch order
@@ -886,7 +920,7 @@ __global__ __launch_bounds__(kMergeThreads) void k_chunk_merge(MoveArgs a, Chunk
     const int32_t size_cur = rec_size(rcur);
     const int32_t n = t.ptot[h.ptot_base + it.p];
     uint64_t best = 0;
-    double bestK = 0.0, kc = 0.0;
+    double bestK = 0.0;
     int32_t nd = 0;  // distinct communities of the partition
     if (n <= t.merge_limit) {
         const int cl = table_log2(n, kMergeCapLog2);
@@ -910,15 +944,11 @@ __global__ __launch_bounds__(kMergeThreads) void k_chunk_merge(MoveArgs a, Chunk
                 vv[j] = i < Cv ? vals[i] : 0.0;
             }
 #pragma unroll
-            for (int j = 0; j < 4; ++j) rr[j] = (kk[j] >= 0 && kk[j] != cur) ? a.crec[kk[j]] : 0ull;
+            for (int j = 0; j < 4; ++j) rr[j] = kk[j] >= 0 ? a.crec[kk[j]] : 0ull;
 #pragma unroll
             for (int j = 0; j < 4; ++j) {
                 if (kk[j] < 0) continue;
                 ++nd;
-                if (kk[j] == cur) {
-                    kc = vv[j];
-                    continue;
-                }
                 consider_k(best, bestK, kk[j], vv[j], rr[j], cur, size_cur, k32, scur, vM32,
                            vden);
             }
@@ -940,35 +970,28 @@ __global__ __launch_bounds__(kMergeThreads) void k_chunk_merge(MoveArgs a, Chunk
             const double Kc = __hip_atomic_load(&W[i], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
             // the table stays as it is: other partitions of the vertex may
             // still be inserting; k_clear_overflow empties it afterwards
-            if (c == cur) {
-                kc = Kc;
-                continue;
-            }
             consider_k(best, bestK, c, Kc, a.crec[c], cur, size_cur, k32, scur, vM32, vden);
         }
     }
     lanes_max_pair<64>(best, bestK);
-    kc = lanes_sum<64>(kc);
 #pragma unroll
     for (int off = 32; off > 0; off >>= 1) nd += __shfl_xor(nd, off, 64);
     const int lane = threadIdx.x & 63, wid = threadIdx.x >> 6;
     if (lane == 0) {
         red_best[wid] = best;
         red_K[wid] = bestK;
-        red_kc[wid] = kc;
         red_n[wid] = nd;
     }
     __syncthreads();
     if (threadIdx.x == 0) {
         uint64_t b = red_best[0];
-        double bK = red_K[0], kk = red_kc[0];
+        double bK = red_K[0];
         int32_t nn = red_n[0];
         for (int w2 = 1; w2 < kMergeThreads / 64; ++w2) {
             if (red_best[w2] > b) {
                 b = red_best[w2];
                 bK = red_K[w2];
             }
-            kk += red_kc[w2];
             nn += red_n[w2];
         }
         atomicAdd(&t.hdist[it.hv], nn);
@@ -977,7 +1000,6 @@ __global__ __launch_bounds__(kMergeThreads) void k_chunk_merge(MoveArgs a, Chunk
         if (b != 0)
             atomicMax(reinterpret_cast<unsigned long long *>(&t.best[it.hv]),
                       (unsigned long long)b);
-        if (tier_part(t, cur, (uint32_t)h.P) == (uint32_t)it.p) t.kcur[it.hv] = kk;
         t.ptot[h.ptot_base + it.p] = 0;  // ready for the next sweep
     }
 }
@@ -1442,6 +1464,27 @@ __global__ __launch_bounds__(256) void k_selfloop_partials(const int32_t *__rest
         const int32_t hi = rowptr[v + 1];
         for (int32_t e = rowptr[v] + lane; e < hi; e += 64)
             if (col[e] == (int32_t)v) x += (double)w[e];
+    }
+    x = block_sum_f64(x, lds);
+    if (threadIdx.x == 0) partials[blockIdx.x] = x;
+}
+
+// The same sum over rows sorted by column: one thread per row finds its
+// diagonal entry by binary search (a few reads per row instead of the row).
+__global__ __launch_bounds__(256) void k_selfloop_sorted(const int32_t *__restrict__ rowptr,
+                                                         const int32_t *__restrict__ col,
+                                                         const float *__restrict__ w, int32_t V,
+                                                         double *partials) {
+    __shared__ double lds[4];
+    double x = 0.0;
+    for (int64_t v = (int64_t)blockIdx.x * 256 + threadIdx.x; v < V;
+         v += (int64_t)gridDim.x * 256) {
+        int32_t lo = rowptr[v], hi = rowptr[v + 1];  // first e in [lo, hi) with col[e] >= v
+        while (lo < hi) {
+            const int32_t m = lo + ((hi - lo) >> 1);
+            if (col[m] < (int32_t)v) lo = m + 1; else hi = m;
+        }
+        if (lo < rowptr[v + 1] && col[lo] == (int32_t)v) x += (double)w[lo];
     }
     x = block_sum_f64(x, lds);
     if (threadIdx.x == 0) partials[blockIdx.x] = x;
@@ -2081,9 +2124,13 @@ void launch_modularity_reduce(int32_t nb, int32_t sq_lo, int32_t nb_sq, double M
 }
 
 void launch_selfloop_weight(const int32_t *rowptr, const int32_t *col, const float *w, int32_t V,
-                            double *partials, double *out, hipStream_t s) {
-    hipLaunchKernelGGL(k_selfloop_partials, dim3(kSelfLoopBlocks), dim3(256), 0, s, rowptr, col, w,
-                       V, partials);
+                            bool sorted, double *partials, double *out, hipStream_t s) {
+    if (sorted)
+        hipLaunchKernelGGL(k_selfloop_sorted, dim3(kSelfLoopBlocks), dim3(256), 0, s, rowptr, col,
+                           w, V, partials);
+    else
+        hipLaunchKernelGGL(k_selfloop_partials, dim3(kSelfLoopBlocks), dim3(256), 0, s, rowptr,
+                           col, w, V, partials);
     hipLaunchKernelGGL(k_selfloop_final, dim3(1), dim3(256), 0, s, partials, out);
 }
 
