@@ -332,6 +332,11 @@ __device__ __forceinline__ double wave_sum_f64(double x) {
     for (int off = 32; off > 0; off >>= 1) x += __shfl_xor(x, off, 64);
     return x;
 }
+__device__ __forceinline__ int32_t wave_sum_i32(int32_t x) {
+#pragma unroll
+    for (int off = 32; off > 0; off >>= 1) x += __shfl_xor(x, off, 64);
+    return x;
+}
 __device__ __forceinline__ int64_t wave_sum_i64(int64_t x) {
 #pragma unroll
     for (int off = 32; off > 0; off >>= 1) x += __shfl_xor(x, off, 64);

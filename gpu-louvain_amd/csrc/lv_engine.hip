@@ -640,6 +640,12 @@ struct lv_graph {
     int wg_lanes[4] = {128, 256, 256, 1024};  // lanes per row of each tier
     int sub_x[4] = {4, 4, 4, 4};              // lanes per vertex of buckets 0..3
     int table_pct = 0;                        // per-row table size, percent (0 = 150)
+    // small-table attempt of the one-workgroup tiers' rows (plain sweeps):
+    // per row of buckets 7..10 (from h_off[7]) its distinct communities at
+    // its last evaluation (INT32_MAX: none yet) and whether the attempt
+    // decided it this sweep
+    bool hinted_on = true;
+    DevVec<int32_t> ghint, hdone_w;
     bool heavy_try = false;  // LDS attempt of the chunked tier's rows
     int32_t heavy_limit = lv::kHeavyTryLimit;  // ... given up at this many communities
     int chunk_edges = lv::kChunkEdges, part_target = lv::kPartTarget, merge_limit = lv::kMergeLimit;
@@ -1028,6 +1034,7 @@ struct lv_graph {
             sub_x[b] = (l >= 4 && l <= bound && (l & (l - 1)) == 0 &&
                         lv::move_sub_shape(l, bound / l)) ? l : kDefSub[b - 1];
         }
+        hinted_on = opt.hinted >= 0;
         heavy_try = opt.heavy_lds >= 0 && 12 * kHeavyTrySlots <= max_lds;
         heavy_limit = opt.heavy_lds > 0 ? std::min(opt.heavy_lds, lv::kHeavyTryLimit)
                                         : lv::kHeavyTryLimit;
@@ -1340,6 +1347,17 @@ struct lv_graph {
             LV_LAUNCHED("chunked tier tables");
         }
 
+        // hints of the small-table attempt: none yet at this level
+        {
+            const int32_t n7 = h_off[11] - h_off[7];
+            if (n7 > 0) {
+                LV_TRY(ghint.reserve(n7));
+                LV_TRY(hdone_w.reserve(n7));
+                lv::fill_i32(ghint.p, 0x7fffffff, n7, s);
+                lv::fill_i32(hdone_w.p, 0, n7, s);
+            }
+        }
+
         // row-reduction segments: lanes per vertex by bucket
         static const int lanes_lg[] = {2, 3, 4, 5, 6, 6, 6, 8, 8, 8, 8};
         static_assert(sizeof(lanes_lg) / sizeof(lanes_lg[0]) == lv::kNumBuckets - 1,
@@ -1528,6 +1546,17 @@ struct lv_graph {
         lv::MoveArgs a = move_args();
         a.obs = obs ? 1 : 0;
         if (pruned) a.active = active;
+        if (g == lv::kBucketGroup[7] && hinted_on && !pruned && h_off[12] > h_off[7]) {
+            // plain sweeps: the small-table attempt over the rows of the four
+            // one-workgroup tiers (hints and flags indexed from bucket 7) and
+            // of the chunked tier (its own hints and flags), before their
+            // kernels, which skip the rows it decides
+            hipEvent_t th = tbegin();
+            const int32_t nh2 = heavy_try ? nh : 0;
+            lv::launch_move_hinted(a, order + h_off[7], h_off[11] - h_off[7], ghint.p, hdone_w.p,
+                                   nh2, hhint.p, hdone.p, s);
+            tend(th, LV_K_MOVE_WG, 0.0, LV_KD_HINTED);
+        }
         for (int b = 0; b < lv::kNumBuckets; ++b) {
             if (lv::kBucketGroup[b] != g) continue;
             const int32_t nfull = h_off[b + 1] - h_off[b];
@@ -1553,9 +1582,13 @@ struct lv_graph {
             case 7: case 8: case 9: case 10: {
                 static const int kd[4] = {LV_KD_MOVE_BUCKET0 + 7, LV_KD_MOVE_BUCKET8,
                                           LV_KD_MOVE_BUCKET8 + 1, LV_KD_MOVE_BUCKET8 + 2};
+                // hints and flags of the small-table attempt (run above)
+                const bool hinted = hinted_on && !pruned;
+                int32_t *hp = hinted ? ghint.p + (h_off[b] - h_off[7]) : nullptr;
+                int32_t *dp = hinted ? hdone_w.p + (h_off[b] - h_off[7]) : nullptr;
                 t0 = tbegin();
                 lv::launch_move_wg(b - 7, wg_lanes[b - 7], wg_slots[b - 7],
-                                   table_pct ? table_pct : 150, a, list, n, s);
+                                   table_pct ? table_pct : 150, a, list, n, hp, dp, s);
                 tend(t0, LV_K_MOVE_WG, bytes, kd[b - 7]);
                 break;
             }
@@ -2797,6 +2830,8 @@ struct lv_graph {
         out_val.release();
         poff.release();
         ptot.release();
+        ghint.release();
+        hdone_w.release();
         hplan.release();
         tier_tot.release();
         hbest.release();

@@ -135,9 +135,20 @@ void launch_move_wave(int cap, int inst, const MoveArgs &a, const int32_t *list,
 // instances), lanes (128, 256, 512 or 1024) per vertex.
 // slots: LDS table entries allocated per workgroup (12 bytes each, more than
 // the tier's degree bound); pct: a row's table is pct % of its degree
-// (at least 512 slots, at most the allocation).
+// (at least 512 slots, at most the allocation).  hint/done (both or neither,
+// indexed like list): rows with done[i] set were decided by
+// launch_move_hinted and are skipped; hint[i] gets the distinct communities
+// of each row evaluated.
 void launch_move_wg(int tier, int lanes, int slots, int pct, const MoveArgs &a,
-                    const int32_t *list, int32_t n, hipStream_t s);
+                    const int32_t *list, int32_t n, int32_t *hint, int32_t *done, hipStream_t s);
+// Small-table attempt over the one-workgroup tiers' rows (list, n): rows with
+// hint[i] <= 256 are tried one wave per row in 512 slots; done[i] = 1 for
+// those decided (0 for the others), hint[i] updated.  The next n2 rows of
+// list are the chunked tier's (hints and flags hint2/done2, its own), tried
+// the same way; done2 is only set, for rows decided here.
+void launch_move_hinted(const MoveArgs &a, const int32_t *list, int32_t n, int32_t *hint,
+                        int32_t *done, int32_t n2, int32_t *hint2, int32_t *done2,
+                        hipStream_t s);
 void set_wg_lds_limits(int bytes);
 // LDS attempt of the chunked tier's rows (list = its heavy vertices, in the
 // order of t.hvs), before its stages; fills t.done and t.hint.  A row is

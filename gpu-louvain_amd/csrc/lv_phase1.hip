@@ -450,6 +450,167 @@ __global__ __launch_bounds__(256) void k_move_wave(MoveArgs a, const int32_t *__
     wave_tab_decide<C>(a, v, cur, kc, K, W, lane);
 }
 
+// Small-table attempt for the rows of the one-workgroup tiers (degrees above
+// the wave tiers, up to the chunked tier): one wave per row and a table of
+// kHintSlots slots per wave, so that a CU holds many more rows in flight than
+// degree-sized tables allow.  On aggregated levels a row's distinct
+// communities are soon a small fraction of its degree.  Row i is tried when
+// its last evaluation saw at most kHintLimit distinct communities (hint[i];
+// INT32_MAX before its first); the lanes read the table's key count before
+// every insert and the wave gives up once it reaches kHintLimit, so at most
+// 63 keys land beyond it and the table never fills.  done[i] = 1 for a row
+// decided here (k_move_wg skips it), 0 otherwise; hint[i] = its distinct
+// count.  Decisions are the tier kernels' (the same exact sums, argmax and
+// candidate bounds).
+constexpr int kHintSlots = 512;
+constexpr int kHintLimit = 256;
+constexpr int kHintRound = 8;  // row entries per lane in flight together
+constexpr int32_t kHintMaxHeavy = 16384;  // longest chunked-tier row tried
+static_assert(kHintSlots > kHintLimit + 64, "the attempt's table never fills");
+static_assert(kHintSlots % 256 == 0, "four slots per lane per step");
+
+__device__ __forceinline__ void wave_sync() {
+    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+    __builtin_amdgcn_wave_barrier();
+    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+}
+
+// The list may continue past n with n2 rows of the chunked tier (its heavy
+// vertices in order), whose hints and flags are hint2/done2 (the tier's
+// own): those rows are left alone unless decided here (done2 = 1 then, which
+// k_heavy_final clears); rows longer than kHintMaxHeavy are not tried (one
+// wave would hold the launch up).
+__global__ __launch_bounds__(256, 6) void k_move_hinted(MoveArgs a, const int32_t *__restrict__ list,
+                                                        int32_t n, int32_t *hint, int32_t *done,
+                                                        int32_t n2, int32_t *hint2,
+                                                        int32_t *done2) {
+    if (gated_off(a.gate)) return;
+    constexpr int C = kHintSlots;
+    __shared__ int32_t keys_s[4 * C];
+    __shared__ double vals_s[4 * C];
+    __shared__ int32_t fill_s[4];
+    const int lane = threadIdx.x & 63, wid = threadIdx.x >> 6;
+    int32_t *K = keys_s + wid * C;
+    double *W = vals_s + wid * C;
+    volatile int32_t *fill = &fill_s[wid];
+    for (int32_t i = blockIdx.x * 4 + wid; i < n + n2; i += gridDim.x * 4) {
+        const bool heavy = i >= n;  // a row of the chunked tier
+        int32_t *const hp = heavy ? hint2 + (i - n) : hint + i;
+        int32_t *const dp = heavy ? done2 + (i - n) : done + i;
+        if (*hp > kHintLimit) {  // the tier kernel takes it
+            if (lane == 0 && !heavy) *dp = 0;
+            continue;
+        }
+        const int32_t v = list[i];
+        const int32_t rs = a.rowptr[v], re = a.rowptr[v + 1];
+        if (heavy && re - rs > kHintMaxHeavy) continue;  // too long for one wave
+        wave_tab_clear<C>(K, W, lane);
+        if (lane == 0) *fill = 0;
+        wave_sync();
+        const int32_t cur = a.comm[v];
+        const int32_t vs = col_self(a, v);
+        double own = 0.0;  // the own community's weight, in registers
+        bool dropped = false;
+        for (int32_t e0 = rs + lane; e0 < re; e0 += kHintRound * 64) {
+            int32_t u[kHintRound];
+            float wv[kHintRound];
+#pragma unroll
+            for (int j = 0; j < kHintRound; ++j) {
+                const int32_t e = e0 + j * 64;
+                u[j] = e < re ? a.ccol[e] : -1;
+                wv[j] = e < re ? a.w[e] : 0.0f;
+            }
+            int32_t cc[kHintRound];
+#pragma unroll
+            for (int j = 0; j < kHintRound; ++j)
+                cc[j] = (u[j] >= 0 && u[j] != vs) ? a.ccomm[u[j]] : -1;
+#pragma unroll
+            for (int j = 0; j < kHintRound; ++j) {
+                // every lane reads the count before any lane of this step
+                // inserts: the step adds at most 64 keys
+                const bool full = *fill >= kHintLimit;
+                dropped |= full;
+                if (full || cc[j] < 0) continue;
+                if (cc[j] == cur) {
+                    own += (double)wv[j];
+                    continue;
+                }
+                if (lds_insert_claim(K, W, (uint32_t)C, cc[j], (double)wv[j]))
+                    atomicAdd(&fill_s[wid], 1);
+            }
+            if (__any(dropped)) {
+                dropped = true;
+                break;
+            }
+        }
+        if (dropped) {  // the whole wave: the tier kernel takes the row
+            if (lane == 0 && !heavy) *dp = 0;
+            wave_sync();
+            continue;
+        }
+        const double kc = lanes_sum<64>(own);
+        wave_sync();
+        // decision as in k_move_wg, four slots per lane at a time
+        const float k32 = (float)a.k[v];
+        const float2 md = vertex_mden(a, v);
+        const float vM32 = md.x, vden = md.y;
+        const uint64_t rcur = a.crec[cur];
+        const float scur = rec_sigma(rcur);
+        const int32_t size_cur = rec_size(rcur);
+        uint64_t best = pack_best(0.0f, cur);
+        double bestK = 0.0;
+        int32_t slot0 = -1;
+        if (a.bounds) {
+            // the heaviest candidate first; the others are read only if
+            // their bound can still change the decision
+            double km = -1.0;
+            int32_t ks = -1;
+            for (int s2 = lane; s2 < C; s2 += 64)
+                if (K[s2] >= 0 && W[s2] > km) {
+                    km = W[s2];
+                    ks = s2;
+                }
+            const int src = lanes_argmax<64>(km, lane);
+            if (lane == src && ks >= 0)
+                consider_k(best, bestK, K[ks], W[ks], a.crec[K[ks]], cur, size_cur, k32, scur,
+                           vM32, vden);
+            best = __shfl(best, src, 64);
+            bestK = __shfl(bestK, src, 64);
+            slot0 = __shfl(ks, src, 64);
+        }
+        const float tc = (float)kc / vM32;
+        for (int i0 = lane; i0 < C; i0 += 4 * 64) {
+            int32_t kk[4];
+            double vv[4];
+            bool ev[4];
+            uint64_t rr[4];
+#pragma unroll
+            for (int j = 0; j < 4; ++j) {
+                const int s2 = i0 + j * 64;
+                kk[j] = K[s2];
+                vv[j] = W[s2];
+                ev[j] = kk[j] >= 0 && s2 != slot0 &&
+                        (!a.bounds || may_win(gain_bound(vv[j], vM32, vden, k32, scur), kk[j],
+                                              best, a.obs != 0, tc));
+            }
+#pragma unroll
+            for (int j = 0; j < 4; ++j) rr[j] = ev[j] ? a.crec[kk[j]] : 0ull;
+#pragma unroll
+            for (int j = 0; j < 4; ++j)
+                if (ev[j])
+                    consider_k(best, bestK, kk[j], vv[j], rr[j], cur, size_cur, k32, scur, vM32,
+                               vden);
+        }
+        lanes_max_pair<64>(best, bestK);
+        if (lane == 0) {
+            write_decision(a, v, cur, best, bestK, kc);
+            *dp = 1;
+            *hp = *fill;
+        }
+        wave_sync();  // every lane is done with the table before the next row clears it
+    }
+}
+
 // TIER only names the instance (one per workgroup bucket) for profiles.
 // Every instance keeps to 64 VGPRs (8 waves per SIMD): with tables of 1.5 x
 // the degree, LDS admits that many (two 1024-lane workgroups per CU).
@@ -475,11 +636,16 @@ __global__ __launch_bounds__(T, 8) void k_move_wg(MoveArgs a, const int32_t *__r
     __shared__ uint64_t red_best[T / 64];
     __shared__ double red_K[T / 64];
     __shared__ double red_kc[T / 64];
+    __shared__ int32_t red_n[T / 64];   // distinct keys per wave (hints)
     __shared__ int32_t fill_s, stop_s;  // TIER 4: keys in the table, an insert was dropped
     const int lane = threadIdx.x & 63, wid = threadIdx.x >> 6;
     for (int32_t vi = blockIdx.x; vi < n; vi += gridDim.x) {
         if constexpr (TIER == 4) {
-            if (hint[vi] > kHeavyTryLimit) continue;  // the chunked tier takes it
+            // the chunked tier takes it, or the small-table attempt decided it
+            if (hint[vi] > kHeavyTryLimit || done[vi]) continue;
+        } else {
+            // decided by the small-table attempt (k_move_hinted) this sweep
+            if (done && done[vi]) continue;
         }
         const int32_t v = list[vi];
         if constexpr (TIER == 4) {
@@ -589,9 +755,10 @@ __global__ __launch_bounds__(T, 8) void k_move_wg(MoveArgs a, const int32_t *__r
             // first; its other slots are read only if their bound can still
             // change the decision.
             double km = -1.0;
-            int32_t ks = -1;
+            int32_t ks = -1, nd = 0;
             for (int i = threadIdx.x; i < Cv; i += T) {
                 const int32_t kx = K[i];
+                nd += kx >= 0 ? 1 : 0;
                 if (kx >= 0 && kx != cur && W[i] > km) {
                     km = W[i];
                     ks = i;
@@ -630,9 +797,11 @@ __global__ __launch_bounds__(T, 8) void k_move_wg(MoveArgs a, const int32_t *__r
                                    vM32, vden);
             }
             lanes_max_pair<64>(best, bestK);
+            nd = wave_sum_i32(nd);
             if (lane == 0) {
                 red_best[wid] = best;
                 red_K[wid] = bestK;
+                red_n[wid] = nd;
             }
             __syncthreads();
             if (threadIdx.x == 0) {
@@ -647,6 +816,10 @@ __global__ __launch_bounds__(T, 8) void k_move_wg(MoveArgs a, const int32_t *__r
                 if constexpr (TIER == 4) {
                     done[vi] = 1;
                     hint[vi] = fill_s;
+                } else if (hint) {
+                    int32_t c = 0;
+                    for (int w2 = 0; w2 < T / 64; ++w2) c += red_n[w2];
+                    hint[vi] = c;
                 }
             }
             __syncthreads();
@@ -654,6 +827,7 @@ __global__ __launch_bounds__(T, 8) void k_move_wg(MoveArgs a, const int32_t *__r
         }
         uint64_t best = pack_best(0.0f, cur);
         double bestK = 0.0;
+        int32_t nd = 0;
         for (int i0 = threadIdx.x; i0 < Cv; i0 += 4 * T) {
             int32_t kk[4];
             double vv[4];
@@ -669,14 +843,17 @@ __global__ __launch_bounds__(T, 8) void k_move_wg(MoveArgs a, const int32_t *__r
 #pragma unroll
             for (int j = 0; j < 4; ++j) {
                 if (kk[j] < 0) continue;
+                ++nd;
                 consider_k(best, bestK, kk[j], vv[j], rr[j], cur, size_cur, k32, scur, vM32,
                            vden);
             }
         }
         lanes_max_pair<64>(best, bestK);
+        nd = wave_sum_i32(nd);
         if (lane == 0) {
             red_best[wid] = best;
             red_K[wid] = bestK;
+            red_n[wid] = nd;
         }
         __syncthreads();
         if (threadIdx.x == 0) {
@@ -692,6 +869,10 @@ __global__ __launch_bounds__(T, 8) void k_move_wg(MoveArgs a, const int32_t *__r
             if constexpr (TIER == 4) {
                 done[vi] = 1;
                 hint[vi] = fill_s;
+            } else if (hint) {
+                int32_t c = 0;
+                for (int w2 = 0; w2 < T / 64; ++w2) c += red_n[w2];
+                hint[vi] = c;
             }
         }
         __syncthreads();
@@ -1945,40 +2126,49 @@ namespace {
 // each tier is its own kernel in traces).
 template <int TIER>
 void launch_wg_tier(int lanes, unsigned g, size_t lds, hipStream_t s, const MoveArgs &a,
-                    const int32_t *list, int32_t n, int slots, int pct) {
-    int32_t *no = nullptr;
+                    const int32_t *list, int32_t n, int slots, int pct, int32_t *hint,
+                    int32_t *done) {
     switch (lanes) {
     case 128:
         hipLaunchKernelGGL((k_move_wg<128, TIER>), dim3(g), dim3(128), lds, s, a, list, n, slots,
-                           pct, no, no, 0);
+                           pct, hint, done, 0);
         break;
     case 256:
         hipLaunchKernelGGL((k_move_wg<256, TIER>), dim3(g), dim3(256), lds, s, a, list, n, slots,
-                           pct, no, no, 0);
+                           pct, hint, done, 0);
         break;
     case 512:
         hipLaunchKernelGGL((k_move_wg<512, TIER>), dim3(g), dim3(512), lds, s, a, list, n, slots,
-                           pct, no, no, 0);
+                           pct, hint, done, 0);
         break;
     default:
         hipLaunchKernelGGL((k_move_wg<1024, TIER>), dim3(g), dim3(1024), lds, s, a, list, n, slots,
-                           pct, no, no, 0);
+                           pct, hint, done, 0);
         break;
     }
 }
 }  // namespace
 
 void launch_move_wg(int tier, int lanes, int slots, int pct, const MoveArgs &a,
-                    const int32_t *list, int32_t n, hipStream_t s) {
+                    const int32_t *list, int32_t n, int32_t *hint, int32_t *done, hipStream_t s) {
     if (n <= 0) return;
     const size_t lds = (size_t)12 * (size_t)slots;
     const unsigned g = (unsigned)(n < 1048576 ? n : 1048576);
     switch (tier) {
-    case 0: launch_wg_tier<0>(lanes, g, lds, s, a, list, n, slots, pct); break;
-    case 1: launch_wg_tier<1>(lanes, g, lds, s, a, list, n, slots, pct); break;
-    case 2: launch_wg_tier<2>(lanes, g, lds, s, a, list, n, slots, pct); break;
-    default: launch_wg_tier<3>(lanes, g, lds, s, a, list, n, slots, pct); break;
+    case 0: launch_wg_tier<0>(lanes, g, lds, s, a, list, n, slots, pct, hint, done); break;
+    case 1: launch_wg_tier<1>(lanes, g, lds, s, a, list, n, slots, pct, hint, done); break;
+    case 2: launch_wg_tier<2>(lanes, g, lds, s, a, list, n, slots, pct, hint, done); break;
+    default: launch_wg_tier<3>(lanes, g, lds, s, a, list, n, slots, pct, hint, done); break;
     }
+}
+
+void launch_move_hinted(const MoveArgs &a, const int32_t *list, int32_t n, int32_t *hint,
+                        int32_t *done, int32_t n2, int32_t *hint2, int32_t *done2,
+                        hipStream_t s) {
+    if (n + n2 <= 0) return;
+    const int64_t b = ((int64_t)n + n2 + 3) / 4;
+    hipLaunchKernelGGL(k_move_hinted, dim3((unsigned)(b < 16384 ? b : 16384)), dim3(256), 0, s, a,
+                       list, n, hint, done, n2, hint2, done2);
 }
 
 void launch_heavy_try(int slots, int32_t limit, const MoveArgs &a, const int32_t *list,

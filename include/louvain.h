@@ -100,6 +100,11 @@ typedef struct {
                                 4 bytes per entry and 8 per vertex more): 0 = on
                                 single-graph levels of >= 2^20 vertices that all
                                 have entries, 1 = on every such level, -1 = off */
+    int32_t hinted;           /* rows of the one-workgroup tiers whose last
+                                evaluation saw at most 256 distinct communities
+                                are first tried one wave per row in a 512-slot
+                                table (plain sweeps; same decisions): 0 = on,
+                                -1 = off                                        */
 } lv_options;
 
 /* Run statistics (lv_run).  Times are device times from HIP events. */
@@ -150,7 +155,9 @@ typedef struct {
 #define LV_KD_MOVE_BUCKET8 14    /* +b-8, b = 8..10: move kernel of execution bucket b  */
 #define LV_KD_HEAVY_LDS 17       /* LDS attempt of the chunked tier's rows (bytes not
                                     counted)                                           */
-#define LV_KD_COUNT 18
+#define LV_KD_HINTED 18          /* small-table attempt of the one-workgroup tiers' rows
+                                    (lv_options.hinted; bytes not counted)             */
+#define LV_KD_COUNT 19
 
 /* ---- graph handle (utils.cu:64-106 copyStructures + initM mo.cu:420-422) ---- */
 /* Uploads the CSR (rowptr[V+1], col[E], w[E]; host pointers) and sets up the
