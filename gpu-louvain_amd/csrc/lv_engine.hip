@@ -1612,7 +1612,8 @@ struct lv_graph {
                      LV_KD_CHUNK_AGGREGATE);
                 hipEvent_t tb = tbegin();
                 lv::launch_chunk_merge(a, t, nitems, s);
-                tend(tb, -1, 0.0, LV_KD_CHUNK_MERGE);
+                tend(tb, -1, 12.0 * (double)h_bedges[b] * ((double)n / (double)nfull),
+                     LV_KD_CHUNK_MERGE);
                 hipEvent_t tc = tbegin();
                 lv::launch_heavy_final(a, t, nh, s);
                 tend(tc, -1, 0.0, LV_KD_CHUNK_FINAL);

@@ -147,7 +147,8 @@ typedef struct {
 #define LV_KD_MOVE_BUCKET0 0     /* +b, b = 0..7: move kernel of execution bucket b
                                     (0-3 sub-wave, 4-5 wave, 6-7 workgroup)          */
 #define LV_KD_CHUNK_AGGREGATE 8  /* chunked tier stage A (bytes: 12 per heavy edge)      */
-#define LV_KD_CHUNK_MERGE 9      /* chunked tier stage B (bytes not counted)             */
+#define LV_KD_CHUNK_MERGE 9      /* chunked tier stage B (bytes: 12 per heavy edge, the
+                                    pairs it reads; a row has at most one per edge)    */
 #define LV_KD_CHUNK_FINAL 10     /* chunked tier decisions (bytes not counted)           */
 #define LV_KD_MODULARITY_ROWS 11 /* modularity row pass (12 B per edge it reads + 16 V)  */
 #define LV_KD_FRONTIER 12        /* frontier model inference (43 bytes per vertex)       */
