@@ -103,7 +103,7 @@ struct TierTotals {
     int64_t xi[8];  // merge items in XCD x's list
 };
 
-constexpr int kPlanThreads = 1024;
+constexpr int kPlanThreads = 256;
 
 // Exclusive scan of x over the workgroup (kPlanThreads threads) plus the
 // running carry; returns the thread's offset and adds the tile total to carry.
@@ -129,12 +129,13 @@ __device__ __forceinline__ int64_t plan_scan(int64_t x, int64_t &carry, int64_t 
     return r;
 }
 
+template <bool XCD>
 __global__ __launch_bounds__(kPlanThreads) void k_tier_plan(
     const int32_t *__restrict__ rowptr, const int32_t *__restrict__ list, int32_t nh, int32_t ce,
     int32_t pt, const int32_t *__restrict__ split, lv::HeavyVertex *hvs, HeavyPlan *plan,
     TierTotals *tot) {
     __shared__ int64_t lds[kPlanThreads / 64];
-    const bool xcd = split != nullptr;
+    constexpr bool xcd = XCD;
     int64_t c_chunk = 0, c_item = 0, c_pb = 0, c_slot = 0, c_out = 0;
     int64_t c_xc[8] = {0, 0, 0, 0, 0, 0, 0, 0}, c_xi[8] = {0, 0, 0, 0, 0, 0, 0, 0};
     for (int32_t base = 0; base < nh; base += kPlanThreads) {
@@ -149,9 +150,10 @@ __global__ __launch_bounds__(kPlanThreads) void k_tier_plan(
             lo = rowptr[v];
             hi = rowptr[v + 1];
             deg = hi - lo;
-            if (xcd) {
+            if constexpr (XCD) {
                 const int64_t m = max((int64_t)1, (deg + 8 * (int64_t)pt - 1) / (8 * (int64_t)pt));
                 P = min((int64_t)lv::kMaxParts, 8 * m);
+#pragma unroll
                 for (int x = 0; x < 8; ++x) {
                     const int64_t s0 = split[9 * (int64_t)i + x], s1 = split[9 * (int64_t)i + x + 1];
                     nx[x] = (s1 - s0 + ce - 1) / ce;
@@ -169,7 +171,8 @@ __global__ __launch_bounds__(kPlanThreads) void k_tier_plan(
         const int64_t toff = plan_scan(ok ? ((int64_t)1 << tab_lg) : 0, c_slot, lds);
         const int64_t out0 = plan_scan(deg, c_out, lds);
         HeavyPlan hp{};
-        if (xcd) {
+        if constexpr (XCD) {
+#pragma unroll
             for (int x = 0; x < 8; ++x) {
                 hp.xc[x] = (int32_t)plan_scan(nx[x], c_xc[x], lds);
                 hp.xi[x] = (int32_t)plan_scan(P / 8, c_xi[x], lds);
@@ -198,6 +201,7 @@ __global__ __launch_bounds__(kPlanThreads) void k_tier_plan(
         tot->nchunks = c_chunk;
         tot->nitems = c_item;
         tot->pbase = c_pb;
+#pragma unroll
         for (int x = 0; x < 8; ++x) {
             tot->xc[x] = c_xc[x];
             tot->xi[x] = c_xi[x];
@@ -233,6 +237,7 @@ __global__ __launch_bounds__(256) void k_tier_fill(const lv::HeavyVertex *__rest
         }
     } else {
         int32_t kb = 0;
+#pragma unroll
         for (int x = 0; x < 8; ++x) {
             const int64_t s0 = split[9 * (int64_t)i + x], s1 = split[9 * (int64_t)i + x + 1];
             const int32_t cnt = (int32_t)((s1 - s0 + ce - 1) / ce);
@@ -247,7 +252,7 @@ __global__ __launch_bounds__(256) void k_tier_fill(const lv::HeavyVertex *__rest
     const int32_t m = h.P / 8;
     for (int32_t p = threadIdx.x; p < h.P; p += blockDim.x) {
         items[h.ptot_base + p] = lv::MergeItem{i, p};
-        if (split) iorder[8 * (hp.xi[p / m] + p % m) + p / m] = h.ptot_base + p;
+        if (split) iorder[8 * (plan[i].xi[p / m] + p % m) + p / m] = h.ptot_base + p;
     }
 }
 
@@ -1292,9 +1297,14 @@ struct lv_graph {
             LV_TRY(hvs.reserve(nh));
             LV_TRY(hplan.reserve(nh));
             LV_TRY(tier_tot.reserve(1));
-            hipLaunchKernelGGL(k_tier_plan, dim3(1), dim3(kPlanThreads), 0, s, rp, order + h_off[HB],
-                               nh, (int32_t)chunk_edges, (int32_t)part_target, split, hvs.p,
-                               hplan.p, tier_tot.p);
+            if (split)
+                hipLaunchKernelGGL(k_tier_plan<true>, dim3(1), dim3(kPlanThreads), 0, s, rp,
+                                   order + h_off[HB], nh, (int32_t)chunk_edges,
+                                   (int32_t)part_target, split, hvs.p, hplan.p, tier_tot.p);
+            else
+                hipLaunchKernelGGL(k_tier_plan<false>, dim3(1), dim3(kPlanThreads), 0, s, rp,
+                                   order + h_off[HB], nh, (int32_t)chunk_edges,
+                                   (int32_t)part_target, split, hvs.p, hplan.p, tier_tot.p);
             LV_LAUNCHED("chunked tier plan");
             LV_TRY(stage(sizeof(TierTotals)));
             LV_HIP(hipMemcpyAsync(h_stage, tier_tot.p, sizeof(TierTotals), hipMemcpyDeviceToHost, s));
@@ -1363,8 +1373,12 @@ struct lv_graph {
         static_assert(sizeof(lanes_lg) / sizeof(lanes_lg[0]) == lv::kNumBuckets - 1,
                       "one entry per bucket below the chunked tier");
         st = make_segs(h_off, nins);
-        LV_TRY(qpart.reserve(2 * ((int64_t)st.total_blocks + st_light.total_blocks +
-                                  8 * (int64_t)xst.total_blocks) + 2 * 256));
+        // (also the self-loop partials of the strength pass: one per block
+        // and kSelfLoopBlocks folds)
+        LV_TRY(qpart.reserve(std::max<int64_t>(
+            2 * ((int64_t)st.total_blocks + st_light.total_blocks + 8 * (int64_t)xst.total_blocks) +
+                2 * 256,
+            (int64_t)st.total_blocks + lv::kSelfLoopBlocks)));
         // per-group segments for the modularity correction of the applies
         // (block starts are set per sweep by corr_table)
         for (int g = 0; g < lv::kNumGroups; ++g) {
@@ -1408,17 +1422,25 @@ struct lv_graph {
         }
 
         // strengths and singleton state (utils.cu:84-89; computeEdgesSum)
+        // and the level's self-loop weight (the singleton state's weight
+        // inside communities): summed with the strengths, or by a binary
+        // search per row on levels the aggregation left sorted; level 0's CSR
+        // never changes, so it is summed once
+        const bool lvl0 = cur_buf < 0;
+        const bool loops = G == 0 && opt.exact_sums >= 0 && !(lvl0 && sown0_ok);
+        const bool loops_sorted = loops && rows_sorted && !lvl0;
         lv::fill_f64(k, 0.0, V, s);
-        lv::launch_strength(rp, w(), order, st, ins.p, k, s);
+        if (loops && !loops_sorted)
+            lv::launch_strength(rp, w(), order, st, ins.p, k, s, col(), qpart.p,
+                                d_sown + (lvl0 ? 0 : 1));
+        else
+            lv::launch_strength(rp, w(), order, st, ins.p, k, s);
+        if (loops_sorted)
+            lv::launch_selfloop_weight(rp, col(), w(), V, d_sown + 2, d_sown + 1, s);
+        LV_LAUNCHED("strengths and self-loop weight");
+        if (G == 0 && opt.exact_sums >= 0) (lvl0 ? sown0_ok : sown_next_ok) = true;
         lv::launch_level_state(V, k, comm, size, sigma, crec, s);
         state_changed(true);
-        if (G == 0 && opt.exact_sums >= 0) {
-            const bool lvl0 = cur_buf < 0;
-            lv::launch_selfloop_weight(rp, col(), w(), V, rows_sorted && !lvl0, d_sown + 2,
-                                       d_sown + (lvl0 ? 0 : 1), s);
-            LV_LAUNCHED("self-loop weight");
-            (lvl0 ? sown0_ok : sown_next_ok) = true;
-        }
         LV_LAUNCHED("level state");
         if (G > 0) LV_TRY(update_graph_scales());
         level_ready = true;

@@ -105,8 +105,13 @@ constexpr int kSqItems = 2048;  // communities per block in the S_c^2 pass
 // --- level setup ---
 void launch_degree_keys(const int32_t *rowptr, int32_t V, BucketBounds bb, int8_t *keys,
                         hipStream_t s);
+// With loops_out: also *loops_out = the weight of the self-loop entries
+// (col[e] == row), FP64, from per-block partials (loop_partials: one per
+// block of st, then kSelfLoopBlocks folds) in a fixed order.
 void launch_strength(const int32_t *rowptr, const float *w, const int32_t *order,
-                     const SegTable &st, const Chunk *chunks, double *k, hipStream_t s);
+                     const SegTable &st, const Chunk *chunks, double *k, hipStream_t s,
+                     const int32_t *col = nullptr, double *loop_partials = nullptr,
+                     double *loops_out = nullptr);
 void launch_level_state(int32_t V, const double *k, int32_t *comm, int32_t *size, double *sigma,
                         uint64_t *crec, hipStream_t s);
 void launch_records_all(int32_t V, const double *sigma, const int32_t *size, uint64_t *crec,
@@ -216,11 +221,12 @@ void launch_apply(const MoveArgs &a, const int32_t *list, int32_t n, const int32
 
 // --- modularity ---
 // *out = the weight of the self-loop entries (col[e] == row), FP64, over a
-// fixed grid of kSelfLoopBlocks blocks with per-block partials.  sorted: the
-// rows are sorted by column without duplicates (a binary search per row).
+// fixed grid of kSelfLoopBlocks blocks with per-block partials, for rows
+// sorted by column without duplicates (a binary search per row; unsorted
+// levels sum it with their strengths, launch_strength).
 constexpr int kSelfLoopBlocks = 1024;
 void launch_selfloop_weight(const int32_t *rowptr, const int32_t *col, const float *w, int32_t V,
-                            bool sorted, double *partials, double *out, hipStream_t s);
+                            double *partials, double *out, hipStream_t s);
 // Full pass: per-block partials (own, sq), then q[0] = Q, q[1] = own, q[2] = sq.
 void launch_modularity(const int32_t *rowptr, const int32_t *col, const float *w,
                        const int32_t *comm, const double *sigma, int32_t V, const int32_t *order,

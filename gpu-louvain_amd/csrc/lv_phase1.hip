@@ -25,6 +25,7 @@
 //                   LDS, evaluates gains and combines partitions through a
 //                   packed 64-bit atomic max.
 #include "lv_phase1.h"
+#include "lv_prims.h"
 
 #include <algorithm>
 
@@ -1633,25 +1634,9 @@ __global__ __launch_bounds__(kRowThreads) void k_sweep_delta(
 }
 
 // Self-loop weight: one wave per row (grid-stride), lanes over its entries.
-__global__ __launch_bounds__(256) void k_selfloop_partials(const int32_t *__restrict__ rowptr,
-                                                           const int32_t *__restrict__ col,
-                                                           const float *__restrict__ w,
-                                                           int32_t V, double *partials) {
-    __shared__ double lds[4];
-    const int lane = threadIdx.x & 63;
-    const int64_t waves = (int64_t)gridDim.x * 4;
-    double x = 0.0;
-    for (int64_t v = (int64_t)blockIdx.x * 4 + (threadIdx.x >> 6); v < V; v += waves) {
-        const int32_t hi = rowptr[v + 1];
-        for (int32_t e = rowptr[v] + lane; e < hi; e += 64)
-            if (col[e] == (int32_t)v) x += (double)w[e];
-    }
-    x = block_sum_f64(x, lds);
-    if (threadIdx.x == 0) partials[blockIdx.x] = x;
-}
-
-// The same sum over rows sorted by column: one thread per row finds its
-// diagonal entry by binary search (a few reads per row instead of the row).
+// Weight of the self-loop entries (col[e] == row) of rows sorted by column:
+// one thread per row finds its diagonal entry by binary search (a few reads
+// per row instead of the row); per-block partials.
 __global__ __launch_bounds__(256) void k_selfloop_sorted(const int32_t *__restrict__ rowptr,
                                                          const int32_t *__restrict__ col,
                                                          const float *__restrict__ w, int32_t V,
@@ -1671,11 +1656,22 @@ __global__ __launch_bounds__(256) void k_selfloop_sorted(const int32_t *__restri
     if (threadIdx.x == 0) partials[blockIdx.x] = x;
 }
 
-__global__ __launch_bounds__(256) void k_selfloop_final(const double *__restrict__ partials,
-                                                        double *out) {
+// partials[0, n) folded into kSelfLoopBlocks sums (block b: b, b + grid, ...)
+__global__ __launch_bounds__(256) void k_selfloop_fold(const double *__restrict__ partials,
+                                                       int32_t n, double *folded) {
     __shared__ double lds[4];
     double x = 0.0;
-    for (int i = threadIdx.x; i < kSelfLoopBlocks; i += 256) x += partials[i];
+    for (int32_t i = blockIdx.x + gridDim.x * threadIdx.x; i < n; i += gridDim.x * 256)
+        x += partials[i];
+    x = block_sum_f64(x, lds);
+    if (threadIdx.x == 0) folded[blockIdx.x] = x;
+}
+
+__global__ __launch_bounds__(256) void k_selfloop_final(const double *__restrict__ partials,
+                                                        int32_t n, double *out) {
+    __shared__ double lds[4];
+    double x = 0.0;
+    for (int i = threadIdx.x; i < n; i += 256) x += partials[i];
     x = block_sum_f64(x, lds);
     if (threadIdx.x == 0) *out = x;
 }
@@ -1726,43 +1722,61 @@ __global__ void k_degree_keys(const int32_t *__restrict__ rowptr, int32_t V, Buc
 }
 
 // k_v = sum of row weights (computeEdgesSum, modularity_optimisation.cu:23-40), FP64.
+// Strengths k_v = sum of v's row weights.  With loops (per-block partials),
+// also the weight of the self-loop entries (col[e] == v) per block.
 __global__ __launch_bounds__(kRowThreads) void k_strength(const int32_t *__restrict__ rowptr,
                                                           const float *__restrict__ w,
                                                           const int32_t *__restrict__ order,
                                                           SegTable st,
                                                           const Chunk *__restrict__ chunks,
-                                                          double *k) {
+                                                          double *k,
+                                                          const int32_t *__restrict__ col,
+                                                          double *loops) {
     __shared__ double lds[kRowThreads / 64];
     const int b = blockIdx.x;
+    double sl = 0.0;  // self-loop weight seen by this thread
     if (b >= st.chunk_block_start) {
         const Chunk ch = chunks[b - st.chunk_block_start];
         double s = 0.0;
-        for (int64_t e = ch.lo + threadIdx.x; e < ch.hi; e += kRowThreads) s += (double)w[e];
+        for (int64_t e = ch.lo + threadIdx.x; e < ch.hi; e += kRowThreads) {
+            s += (double)w[e];
+            if (loops && col[e] == ch.v) sl += (double)w[e];
+        }
         s = block_sum_f64(s, lds);
         if (threadIdx.x == 0) atomicAdd(&k[ch.v], s);
-        return;
-    }
-    const Seg sg = st.s[find_seg(st, b)];
-    const int lg = sg.lanes_log2;
-    if (lg <= 6) {
-        const int X = 1 << lg;
-        const int32_t idx = (b - sg.block_start) * (kRowThreads >> lg) + (threadIdx.x >> lg);
-        const int lane = threadIdx.x & (X - 1);
-        double s = 0.0;
-        int32_t v = -1;
-        if (idx < sg.count) {
-            v = order[sg.list_off + idx];
-            for (int32_t e = rowptr[v] + lane; e < rowptr[v + 1]; e += X) s += (double)w[e];
-        }
-        for (int off = X / 2; off > 0; off >>= 1) s += __shfl_xor(s, off, 64);
-        if (lane == 0 && v >= 0) k[v] = s;
     } else {
-        const int32_t v = order[sg.list_off + (b - sg.block_start)];
-        double s = 0.0;
-        for (int32_t e = rowptr[v] + threadIdx.x; e < rowptr[v + 1]; e += kRowThreads)
-            s += (double)w[e];
-        s = block_sum_f64(s, lds);
-        if (threadIdx.x == 0) k[v] = s;
+        const Seg sg = st.s[find_seg(st, b)];
+        const int lg = sg.lanes_log2;
+        if (lg <= 6) {
+            const int X = 1 << lg;
+            const int32_t idx = (b - sg.block_start) * (kRowThreads >> lg) + (threadIdx.x >> lg);
+            const int lane = threadIdx.x & (X - 1);
+            double s = 0.0;
+            int32_t v = -1;
+            if (idx < sg.count) {
+                v = order[sg.list_off + idx];
+                for (int32_t e = rowptr[v] + lane; e < rowptr[v + 1]; e += X) {
+                    s += (double)w[e];
+                    if (loops && col[e] == v) sl += (double)w[e];
+                }
+            }
+            for (int off = X / 2; off > 0; off >>= 1) s += __shfl_xor(s, off, 64);
+            if (lane == 0 && v >= 0) k[v] = s;
+        } else {
+            const int32_t v = order[sg.list_off + (b - sg.block_start)];
+            double s = 0.0;
+            for (int32_t e = rowptr[v] + threadIdx.x; e < rowptr[v + 1]; e += kRowThreads) {
+                s += (double)w[e];
+                if (loops && col[e] == v) sl += (double)w[e];
+            }
+            s = block_sum_f64(s, lds);
+            if (threadIdx.x == 0) k[v] = s;
+        }
+    }
+    if (loops) {
+        __syncthreads();  // lds is reused
+        sl = block_sum_f64(sl, lds);
+        if (threadIdx.x == 0) loops[b] = sl;
     }
 }
 
@@ -2051,10 +2065,23 @@ void launch_degree_keys(const int32_t *rowptr, int32_t V, BucketBounds bb, int8_
 }
 
 void launch_strength(const int32_t *rowptr, const float *w, const int32_t *order,
-                     const SegTable &st, const Chunk *chunks, double *k, hipStream_t s) {
-    if (st.chunk_block_start + st.nchunks <= 0) return;
-    hipLaunchKernelGGL(k_strength, dim3(st.chunk_block_start + st.nchunks), dim3(kRowThreads), 0,
-                       s, rowptr, w, order, st, chunks, k);
+                     const SegTable &st, const Chunk *chunks, double *k, hipStream_t s,
+                     const int32_t *col, double *loop_partials, double *loops_out) {
+    const int32_t nb = st.chunk_block_start + st.nchunks;
+    if (nb <= 0) {
+        if (loops_out) fill_f64(loops_out, 0.0, 1, s);
+        return;
+    }
+    hipLaunchKernelGGL(k_strength, dim3(nb), dim3(kRowThreads), 0, s, rowptr, w, order, st, chunks,
+                       k, loops_out ? col : nullptr, loops_out ? loop_partials : nullptr);
+    if (loops_out) {
+        // partials of the nb blocks, then kSelfLoopBlocks folds, in fixed orders
+        double *folded = loop_partials + nb;
+        hipLaunchKernelGGL(k_selfloop_fold, dim3(kSelfLoopBlocks), dim3(256), 0, s, loop_partials,
+                           nb, folded);
+        hipLaunchKernelGGL(k_selfloop_final, dim3(1), dim3(256), 0, s, folded,
+                           (int32_t)kSelfLoopBlocks, loops_out);
+    }
 }
 
 void launch_level_state(int32_t V, const double *k, int32_t *comm, int32_t *size, double *sigma,
@@ -2314,14 +2341,11 @@ void launch_modularity_reduce(int32_t nb, int32_t sq_lo, int32_t nb_sq, double M
 }
 
 void launch_selfloop_weight(const int32_t *rowptr, const int32_t *col, const float *w, int32_t V,
-                            bool sorted, double *partials, double *out, hipStream_t s) {
-    if (sorted)
-        hipLaunchKernelGGL(k_selfloop_sorted, dim3(kSelfLoopBlocks), dim3(256), 0, s, rowptr, col,
-                           w, V, partials);
-    else
-        hipLaunchKernelGGL(k_selfloop_partials, dim3(kSelfLoopBlocks), dim3(256), 0, s, rowptr,
-                           col, w, V, partials);
-    hipLaunchKernelGGL(k_selfloop_final, dim3(1), dim3(256), 0, s, partials, out);
+                            double *partials, double *out, hipStream_t s) {
+    hipLaunchKernelGGL(k_selfloop_sorted, dim3(kSelfLoopBlocks), dim3(256), 0, s, rowptr, col, w,
+                       V, partials);
+    hipLaunchKernelGGL(k_selfloop_final, dim3(1), dim3(256), 0, s, partials,
+                       (int32_t)kSelfLoopBlocks, out);
 }
 
 void launch_modularity_from_own(const double *sigma, int32_t V, double M, double *partials,
