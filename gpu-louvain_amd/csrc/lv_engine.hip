@@ -519,8 +519,10 @@ constexpr int32_t kWave256Hi = 128;
 // levels from which the move kernels read bucket-ordered columns: comm
 // (4 bytes a vertex) well beyond one XCD's 4 MB L2
 constexpr int32_t kColOrderMinVertices = 1 << 20;
-constexpr int32_t kXcdMinVertices = 1 << 21;
-constexpr int32_t kBoundsMinVertices = 1 << 20;  // candidate bounds from this level size  // XCD-local row passes from this level size
+constexpr int32_t kXcdMinVertices = 1 << 21;  // XCD-local row passes from this level size
+// candidate bounds from this level size; the small-table attempt skips input
+// levels of this size
+constexpr int32_t kBoundsMinVertices = 1 << 20;
 // A phase with a frontier sweeps only full after this many sweeps (bounds a
 // phase whose pruned and full sweeps keep undoing each other).
 constexpr int64_t kMaxPrunedSweeps = 4096;
@@ -1568,7 +1570,10 @@ struct lv_graph {
         lv::MoveArgs a = move_args();
         a.obs = obs ? 1 : 0;
         if (pruned) a.active = active;
-        if (g == lv::kBucketGroup[7] && hinted_on && !pruned && h_off[12] > h_off[7]) {
+        // (not on large input levels, whose rows see nearly as many
+        // communities as entries: no row would qualify)
+        const bool hinted_here = hinted_on && !pruned && !(cur_buf < 0 && V >= kBoundsMinVertices);
+        if (g == lv::kBucketGroup[7] && hinted_here && h_off[12] > h_off[7]) {
             // plain sweeps: the small-table attempt over the rows of the four
             // one-workgroup tiers (hints and flags indexed from bucket 7) and
             // of the chunked tier (its own hints and flags), before their
@@ -1605,7 +1610,7 @@ struct lv_graph {
                 static const int kd[4] = {LV_KD_MOVE_BUCKET0 + 7, LV_KD_MOVE_BUCKET8,
                                           LV_KD_MOVE_BUCKET8 + 1, LV_KD_MOVE_BUCKET8 + 2};
                 // hints and flags of the small-table attempt (run above)
-                const bool hinted = hinted_on && !pruned;
+                const bool hinted = hinted_here;
                 int32_t *hp = hinted ? ghint.p + (h_off[b] - h_off[7]) : nullptr;
                 int32_t *dp = hinted ? hdone_w.p + (h_off[b] - h_off[7]) : nullptr;
                 t0 = tbegin();

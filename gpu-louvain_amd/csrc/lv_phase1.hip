@@ -317,13 +317,10 @@ __device__ __forceinline__ void wave_tab_clear(int32_t *K, double *W, int lane) 
 }
 
 // Inserts vertex v's row (MAXE entries per lane, deg <= 64 * MAXE) into the
-// wave's table; returns v's community.  Entries of v's own community are
-// summed in registers instead (the key most entries of a settled row name,
-// which would serialise the lanes' LDS atomics on one slot); kc gets that
-// weight, summed over the wave.
+// wave's table; returns v's community.
 template <int C, int MAXE>
 __device__ __forceinline__ int32_t wave_tab_fill(const MoveArgs &a, int32_t v, int32_t *K,
-                                                 double *W, int lane, double &kc) {
+                                                 double *W, int lane) {
     constexpr bool POW2 = (C & (C - 1)) == 0;
     constexpr int LOG2C = ilog2_c(C);
     const int32_t rs = a.rowptr[v], re = a.rowptr[v + 1];
@@ -340,29 +337,22 @@ __device__ __forceinline__ int32_t wave_tab_fill(const MoveArgs &a, int32_t v, i
     int32_t cc[MAXE];
 #pragma unroll
     for (int j = 0; j < MAXE; ++j) cc[j] = (u[j] >= 0 && u[j] != vs) ? a.ccomm[u[j]] : -1;
-    double own = 0.0;
 #pragma unroll
     for (int j = 0; j < MAXE; ++j) {
         if (cc[j] < 0) continue;
-        if (cc[j] == cur) {
-            own += (double)wv[j];
-            continue;
-        }
         if constexpr (POW2)
             lds_insert(K, W, 32 - LOG2C, C - 1, cc[j], (double)wv[j]);
         else
             lds_insert_n(K, W, (uint32_t)C, cc[j], (double)wv[j]);
     }
-    kc = lanes_sum<64>(own);
     return cur;
 }
 
-// Decision of v (in community cur, kc = its weight to cur) from its filled
-// table, which holds the other communities; the whole wave calls it.
+// Decision of v (in community cur) from its filled table; the whole wave
+// calls it.
 template <int C>
 __device__ __forceinline__ void wave_tab_decide(const MoveArgs &a, int32_t v, int32_t cur,
-                                                double kc, const int32_t *K, const double *W,
-                                                int lane) {
+                                                const int32_t *K, const double *W, int lane) {
     constexpr int S = C / 64;  // slots per lane
     const float k32 = (float)a.k[v];
     const float2 md = vertex_mden(a, v);
@@ -378,6 +368,11 @@ __device__ __forceinline__ void wave_tab_decide(const MoveArgs &a, int32_t v, in
         kk[j] = K[lane + 64 * j];
         vv[j] = W[lane + 64 * j];
     }
+    double kc = 0.0;
+#pragma unroll
+    for (int j = 0; j < S; ++j)
+        if (kk[j] == cur) kc = vv[j];
+    kc = lanes_sum<64>(kc);
     uint64_t best = pack_best(0.0f, cur);
     double bestK = 0.0;
     if (a.bounds) {
@@ -441,14 +436,13 @@ __global__ __launch_bounds__(256) void k_move_wave(MoveArgs a, const int32_t *__
     wave_tab_clear<C>(K, W, lane);
     __syncthreads();
     int32_t v = 0, cur = 0;
-    double kc = 0.0;
     if (active) {
         v = list[slot];
-        cur = wave_tab_fill<C, MAXE>(a, v, K, W, lane, kc);
+        cur = wave_tab_fill<C, MAXE>(a, v, K, W, lane);
     }
     __syncthreads();
     if (!active) return;  // only wave-level communication below
-    wave_tab_decide<C>(a, v, cur, kc, K, W, lane);
+    wave_tab_decide<C>(a, v, cur, K, W, lane);
 }
 
 // Small-table attempt for the rows of the one-workgroup tiers (degrees above
@@ -625,7 +619,10 @@ __global__ __launch_bounds__(256, 6) void k_move_hinted(MoveArgs a, const int32_
 // A row whose inserts all landed is decided here (done[vi] = 1, hint[vi] =
 // its count); the chunked tier takes the others.  Rows outside a pruned
 // sweep's frontier are skipped, as the chunked tier skips them.
-template <int T, int TIER>
+// HINTS (tiers 0-3): rows decided by the small-table attempt (done) are
+// skipped and every row evaluated records its distinct communities (hint);
+// the instance without them is the plain tier.
+template <int T, int TIER, bool HINTS = false>
 __global__ __launch_bounds__(T, 8) void k_move_wg(MoveArgs a, const int32_t *__restrict__ list,
                                                   int32_t n, int cap, int pct, int32_t *hint,
                                                   int32_t *done, int32_t limit) {
@@ -644,9 +641,9 @@ __global__ __launch_bounds__(T, 8) void k_move_wg(MoveArgs a, const int32_t *__r
         if constexpr (TIER == 4) {
             // the chunked tier takes it, or the small-table attempt decided it
             if (hint[vi] > kHeavyTryLimit || done[vi]) continue;
-        } else {
+        } else if constexpr (HINTS) {
             // decided by the small-table attempt (k_move_hinted) this sweep
-            if (done && done[vi]) continue;
+            if (done[vi]) continue;
         }
         const int32_t v = list[vi];
         if constexpr (TIER == 4) {
@@ -672,9 +669,9 @@ __global__ __launch_bounds__(T, 8) void k_move_wg(MoveArgs a, const int32_t *__r
         __syncthreads();
         const int32_t cur = a.comm[v];
         const int32_t vs = col_self(a, v);
-        // entries of v's own community are summed in registers (the key most
-        // entries of a settled row name: in the table its slot would
-        // serialise the lanes' atomics), the others go to the table
+        // TIER 4 (rows of settled levels): entries of v's own community are
+        // summed in registers (the key most entries of such a row name: in
+        // the table its slot would serialise the lanes' atomics)
         double own = 0.0;
         if constexpr (TIER == 4) {
             volatile int32_t *fill = &fill_s, *stop = &stop_s;
@@ -730,21 +727,18 @@ __global__ __launch_bounds__(T, 8) void k_move_wg(MoveArgs a, const int32_t *__r
                 for (int j = 0; j < 4; ++j)
                     cc[j] = (u[j] >= 0 && u[j] != vs) ? a.ccomm[u[j]] : -1;
 #pragma unroll
-                for (int j = 0; j < 4; ++j) {
-                    if (cc[j] == cur)
-                        own += (double)wv[j];
-                    else if (cc[j] >= 0)
-                        lds_insert_n(K, W, (uint32_t)Cv, cc[j], (double)wv[j]);
-                }
+                for (int j = 0; j < 4; ++j)
+                    if (cc[j] >= 0) lds_insert_n(K, W, (uint32_t)Cv, cc[j], (double)wv[j]);
             }
-            own = lanes_sum<64>(own);
-            if (lane == 0) red_kc[wid] = own;
             __syncthreads();
         }
-        // weight to the own community, summed over the waves in a fixed order
+        // weight to the own community: TIER 4 summed it over the waves (in a
+        // fixed order); the others read it from the table below
         double kc = 0.0;
+        if constexpr (TIER == 4) {
 #pragma unroll
-        for (int w2 = 0; w2 < T / 64; ++w2) kc += red_kc[w2];
+            for (int w2 = 0; w2 < T / 64; ++w2) kc += red_kc[w2];
+        }
         const float k32 = (float)a.k[v];
         const float2 md = vertex_mden(a, v);
         const float vM32 = md.x, vden = md.y;
@@ -755,11 +749,12 @@ __global__ __launch_bounds__(T, 8) void k_move_wg(MoveArgs a, const int32_t *__r
             // Each wave evaluates the heaviest candidate among its own slots
             // first; its other slots are read only if their bound can still
             // change the decision.
+            if constexpr (TIER != 4) kc = lds_value_n(K, W, (uint32_t)Cv, cur);
             double km = -1.0;
             int32_t ks = -1, nd = 0;
             for (int i = threadIdx.x; i < Cv; i += T) {
                 const int32_t kx = K[i];
-                nd += kx >= 0 ? 1 : 0;
+                if constexpr (HINTS) nd += kx >= 0 ? 1 : 0;
                 if (kx >= 0 && kx != cur && W[i] > km) {
                     km = W[i];
                     ks = i;
@@ -798,11 +793,11 @@ __global__ __launch_bounds__(T, 8) void k_move_wg(MoveArgs a, const int32_t *__r
                                    vM32, vden);
             }
             lanes_max_pair<64>(best, bestK);
-            nd = wave_sum_i32(nd);
+            if constexpr (HINTS) nd = wave_sum_i32(nd);
             if (lane == 0) {
                 red_best[wid] = best;
                 red_K[wid] = bestK;
-                red_n[wid] = nd;
+                if constexpr (HINTS) red_n[wid] = nd;
             }
             __syncthreads();
             if (threadIdx.x == 0) {
@@ -817,7 +812,7 @@ __global__ __launch_bounds__(T, 8) void k_move_wg(MoveArgs a, const int32_t *__r
                 if constexpr (TIER == 4) {
                     done[vi] = 1;
                     hint[vi] = fill_s;
-                } else if (hint) {
+                } else if constexpr (HINTS) {
                     int32_t c = 0;
                     for (int w2 = 0; w2 < T / 64; ++w2) c += red_n[w2];
                     hint[vi] = c;
@@ -827,7 +822,7 @@ __global__ __launch_bounds__(T, 8) void k_move_wg(MoveArgs a, const int32_t *__r
             continue;
         }
         uint64_t best = pack_best(0.0f, cur);
-        double bestK = 0.0;
+        double bestK = 0.0, kcs = 0.0;  // kcs: the own community's slot (TIER < 4)
         int32_t nd = 0;
         for (int i0 = threadIdx.x; i0 < Cv; i0 += 4 * T) {
             int32_t kk[4];
@@ -840,21 +835,27 @@ __global__ __launch_bounds__(T, 8) void k_move_wg(MoveArgs a, const int32_t *__r
                 vv[j] = i < Cv ? W[i] : 0.0;
             }
 #pragma unroll
-            for (int j = 0; j < 4; ++j) rr[j] = kk[j] >= 0 ? a.crec[kk[j]] : 0ull;
+            for (int j = 0; j < 4; ++j) rr[j] = (kk[j] >= 0 && kk[j] != cur) ? a.crec[kk[j]] : 0ull;
 #pragma unroll
             for (int j = 0; j < 4; ++j) {
                 if (kk[j] < 0) continue;
-                ++nd;
+                if constexpr (HINTS) ++nd;
+                if (kk[j] == cur) {  // TIER < 4 only: TIER 4 keeps it out of the table
+                    kcs = vv[j];
+                    continue;
+                }
                 consider_k(best, bestK, kk[j], vv[j], rr[j], cur, size_cur, k32, scur, vM32,
                            vden);
             }
         }
         lanes_max_pair<64>(best, bestK);
-        nd = wave_sum_i32(nd);
+        if constexpr (HINTS) nd = wave_sum_i32(nd);
+        if constexpr (TIER != 4) kcs = lanes_sum<64>(kcs);
         if (lane == 0) {
             red_best[wid] = best;
             red_K[wid] = bestK;
-            red_n[wid] = nd;
+            if constexpr (HINTS) red_n[wid] = nd;
+            if constexpr (TIER != 4) red_kc[wid] = kcs;
         }
         __syncthreads();
         if (threadIdx.x == 0) {
@@ -866,11 +867,15 @@ __global__ __launch_bounds__(T, 8) void k_move_wg(MoveArgs a, const int32_t *__r
                     bK = red_K[w2];
                 }
             }
+            if constexpr (TIER != 4) {
+                kc = 0.0;
+                for (int w2 = 0; w2 < T / 64; ++w2) kc += red_kc[w2];
+            }
             write_decision(a, v, cur, b, bK, kc);
             if constexpr (TIER == 4) {
                 done[vi] = 1;
                 hint[vi] = fill_s;
-            } else if (hint) {
+            } else if constexpr (HINTS) {
                 int32_t c = 0;
                 for (int w2 = 0; w2 < T / 64; ++w2) c += red_n[w2];
                 hint[vi] = c;
@@ -2151,28 +2156,38 @@ void launch_move_wave(int cap, int inst, const MoveArgs &a, const int32_t *list,
 namespace {
 // One workgroup tier at `lanes` lanes per row (instances per tier, so that
 // each tier is its own kernel in traces).
+template <int TIER, bool HINTS>
+void launch_wg_tier_h(int lanes, unsigned g, size_t lds, hipStream_t s, const MoveArgs &a,
+                      const int32_t *list, int32_t n, int slots, int pct, int32_t *hint,
+                      int32_t *done) {
+    switch (lanes) {
+    case 128:
+        hipLaunchKernelGGL((k_move_wg<128, TIER, HINTS>), dim3(g), dim3(128), lds, s, a, list, n,
+                           slots, pct, hint, done, 0);
+        break;
+    case 256:
+        hipLaunchKernelGGL((k_move_wg<256, TIER, HINTS>), dim3(g), dim3(256), lds, s, a, list, n,
+                           slots, pct, hint, done, 0);
+        break;
+    case 512:
+        hipLaunchKernelGGL((k_move_wg<512, TIER, HINTS>), dim3(g), dim3(512), lds, s, a, list, n,
+                           slots, pct, hint, done, 0);
+        break;
+    default:
+        hipLaunchKernelGGL((k_move_wg<1024, TIER, HINTS>), dim3(g), dim3(1024), lds, s, a, list, n,
+                           slots, pct, hint, done, 0);
+        break;
+    }
+}
+
 template <int TIER>
 void launch_wg_tier(int lanes, unsigned g, size_t lds, hipStream_t s, const MoveArgs &a,
                     const int32_t *list, int32_t n, int slots, int pct, int32_t *hint,
                     int32_t *done) {
-    switch (lanes) {
-    case 128:
-        hipLaunchKernelGGL((k_move_wg<128, TIER>), dim3(g), dim3(128), lds, s, a, list, n, slots,
-                           pct, hint, done, 0);
-        break;
-    case 256:
-        hipLaunchKernelGGL((k_move_wg<256, TIER>), dim3(g), dim3(256), lds, s, a, list, n, slots,
-                           pct, hint, done, 0);
-        break;
-    case 512:
-        hipLaunchKernelGGL((k_move_wg<512, TIER>), dim3(g), dim3(512), lds, s, a, list, n, slots,
-                           pct, hint, done, 0);
-        break;
-    default:
-        hipLaunchKernelGGL((k_move_wg<1024, TIER>), dim3(g), dim3(1024), lds, s, a, list, n, slots,
-                           pct, hint, done, 0);
-        break;
-    }
+    if (hint && done)
+        launch_wg_tier_h<TIER, true>(lanes, g, lds, s, a, list, n, slots, pct, hint, done);
+    else
+        launch_wg_tier_h<TIER, false>(lanes, g, lds, s, a, list, n, slots, pct, nullptr, nullptr);
 }
 }  // namespace
 
@@ -2373,6 +2388,14 @@ void set_tier_lds_limits(int bytes) {
     (void)hipFuncSetAttribute(reinterpret_cast<const void *>(&k_move_wg<512, TIER>),
                               hipFuncAttributeMaxDynamicSharedMemorySize, bytes);
     (void)hipFuncSetAttribute(reinterpret_cast<const void *>(&k_move_wg<1024, TIER>),
+                              hipFuncAttributeMaxDynamicSharedMemorySize, bytes);
+    (void)hipFuncSetAttribute(reinterpret_cast<const void *>(&k_move_wg<128, TIER, true>),
+                              hipFuncAttributeMaxDynamicSharedMemorySize, bytes);
+    (void)hipFuncSetAttribute(reinterpret_cast<const void *>(&k_move_wg<256, TIER, true>),
+                              hipFuncAttributeMaxDynamicSharedMemorySize, bytes);
+    (void)hipFuncSetAttribute(reinterpret_cast<const void *>(&k_move_wg<512, TIER, true>),
+                              hipFuncAttributeMaxDynamicSharedMemorySize, bytes);
+    (void)hipFuncSetAttribute(reinterpret_cast<const void *>(&k_move_wg<1024, TIER, true>),
                               hipFuncAttributeMaxDynamicSharedMemorySize, bytes);
 }
 }  // namespace
