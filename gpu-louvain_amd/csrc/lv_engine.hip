@@ -523,6 +523,9 @@ constexpr int32_t kXcdMinVertices = 1 << 21;  // XCD-local row passes from this 
 // candidate bounds from this level size; the small-table attempt skips input
 // levels of this size
 constexpr int32_t kBoundsMinVertices = 1 << 20;
+// Neighbour moves are counted from the rows' side when the movers' row
+// entries reach E / kPullDen (from the movers' side below that).
+constexpr int64_t kPullDen = 16;
 // A phase with a frontier sweeps only full after this many sweeps (bounds a
 // phase whose pruned and full sweeps keep undoing each other).
 constexpr int64_t kMaxPrunedSweeps = 4096;
@@ -619,7 +622,8 @@ struct lv_graph {
         LV_HIP(hipMemcpyAsync(dst, h, src.size(), hipMemcpyHostToDevice, stream));
         return LV_OK;
     }
-    uint32_t *bits = nullptr;    // two mover bitmaps (alternating group steps)
+    uint32_t *bits = nullptr;    // two mover bitmaps (alternating group steps), then the
+                                 // marks of the neighbour-move count
     int32_t bit_words = 0;
     uint64_t step = 0;           // group steps applied with marks
     bool own_valid = false;      // d_q[1] holds the weight inside communities
@@ -652,6 +656,7 @@ struct lv_graph {
     // its last evaluation (INT32_MAX: none yet) and whether the attempt
     // decided it this sweep
     bool hinted_on = true;
+    int32_t nmv_pull = 0;  // lv_options.nmv_pull
     DevVec<int32_t> ghint, hdone_w;
     bool heavy_try = false;  // LDS attempt of the chunked tier's rows
     int32_t heavy_limit = lv::kHeavyTryLimit;  // ... given up at this many communities
@@ -1042,6 +1047,7 @@ struct lv_graph {
                         lv::move_sub_shape(l, bound / l)) ? l : kDefSub[b - 1];
         }
         hinted_on = opt.hinted >= 0;
+        nmv_pull = opt.nmv_pull;
         heavy_try = opt.heavy_lds >= 0 && 12 * kHeavyTrySlots <= max_lds;
         heavy_limit = opt.heavy_lds > 0 ? std::min(opt.heavy_lds, lv::kHeavyTryLimit)
                                         : lv::kHeavyTryLimit;
@@ -1115,7 +1121,7 @@ struct lv_graph {
         LV_TRY(dev_alloc(&fd_off, 64));
         LV_TRY(dev_alloc(&mlp, lv::kParams));
         bit_words = (int32_t)((Vp + 31) / 32);
-        LV_TRY(dev_alloc(&bits, 2 * (int64_t)bit_words));
+        LV_TRY(dev_alloc(&bits, 3 * (int64_t)bit_words));
         LV_TRY(dev_alloc(&keys8, Vp));
         LV_TRY(dev_alloc(&order, Vp));
         LV_TRY(dev_alloc(&d_off, 64));
@@ -1910,11 +1916,22 @@ struct lv_graph {
         }
         if (track) {
             lv::fill_i32(nmv, 0, V, s);
-            // movers' neighbours lie in their own graph: the active graphs'
-            // list covers every count that changes
-            const bool gl = G > 0 && glist_ok;
-            lv::launch_neighbor_moves(rowptr(), col(), gl ? glist : order,
-                                      gl ? glist_n : h_off[lv::kNumBuckets], hist, nmv, s);
+            // Many movers (judged by the last sweep known on the host): every
+            // row counts its moved neighbours; few: the movers add one at
+            // each neighbour.  The counts are the same.
+            const bool pull =
+                nmv_pull > 0 ||
+                (nmv_pull == 0 && (last_moved_edges < 0 || last_moved_edges * kPullDen >= E));
+            if (pull) {
+                lv::launch_neighbor_count(rowptr(), col(), hist, V, lv::kNmvMaxDegree, order, st,
+                                          ins.p, bits + 2 * (int64_t)bit_words, nmv, s);
+            } else {
+                // movers' neighbours lie in their own graph: the active
+                // graphs' list covers every count that changes
+                const bool gl = G > 0 && glist_ok;
+                lv::launch_neighbor_moves(rowptr(), col(), gl ? glist : order,
+                                          gl ? glist_n : h_off[lv::kNumBuckets], hist, nmv, s);
+            }
             LV_LAUNCHED("neighbour moves");
         }
         return LV_OK;

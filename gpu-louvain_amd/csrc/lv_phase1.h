@@ -242,6 +242,15 @@ void launch_modularity_rows(const int32_t *rowptr, const int32_t *col, const flo
 void launch_modularity_finish(const SegTable &st, double M, double *partials, double *q,
                               hipStream_t s);
 
+// nmv[v] = entries (v, u) of v's row whose u moved in the last sweep (hist
+// bit 0) and has degree 1..max_degree, for the rows of st (its segments and
+// chunks: nmv must be zero before for the chunked rows, and for vertices in
+// no row).  marks: (V + 31) / 32 words of scratch.
+void launch_neighbor_count(const int32_t *rowptr, const int32_t *col, const uint8_t *hist,
+                           int32_t V, int32_t max_degree, const int32_t *order,
+                           const SegTable &st, const Chunk *chunks, uint32_t *marks,
+                           int32_t *nmv, hipStream_t s);
+
 // XCD-local row passes over the heavy rows (degree > 32, bucket-ordered
 // list): split[i*9 .. i*9+9) = the row of list[i] cut at the column-block
 // boundaries x*ceil(V/8); *unsorted |= 1 when a listed row is not sorted

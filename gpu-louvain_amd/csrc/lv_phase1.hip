@@ -1866,6 +1866,90 @@ __global__ __launch_bounds__(kRowThreads) void k_modularity(
     }
 }
 
+// Neighbour moves counted from the rows' side: marks = the vertices that
+// moved in the last sweep (hist bit 0) with degree 1..max_degree, one bit
+// each; then every row counts its entries whose column is marked.  The same
+// counts as the movers adding one per entry of their rows (the CSR is
+// symmetric), without atomics on the neighbours: a sweep in which many
+// vertices moved pushes to the same hubs from many rows at once.
+__global__ void k_mark_movers(const int32_t *__restrict__ rowptr,
+                              const uint8_t *__restrict__ hist, int32_t V, int32_t max_degree,
+                              int32_t nwords, uint32_t *marks) {
+    const int32_t v = blockIdx.x * blockDim.x + threadIdx.x;
+    bool m = false;
+    if (v < V && (hist[v] & 1u)) {
+        const int32_t d = rowptr[v + 1] - rowptr[v];
+        m = d > 0 && d <= max_degree;
+    }
+    const uint64_t b = __ballot(m);
+    const int lane = (int)threadIdx.x & 63;
+    const int32_t w0 = (v - lane) >> 5;  // the wave's first word (v - lane: multiple of 64)
+    if (lane == 0 && w0 < nwords) marks[w0] = (uint32_t)b;
+    if (lane == 32 && w0 + 1 < nwords) marks[w0 + 1] = (uint32_t)(b >> 32);
+}
+
+// Marked columns in a row slice [lo, hi) (stride st), four entries a step.
+__device__ __forceinline__ int32_t marked_in(const int32_t *__restrict__ col,
+                                             const uint32_t *__restrict__ marks, int64_t lo,
+                                             int64_t hi, int st) {
+    int32_t c = 0;
+    int64_t e = lo;
+    for (; e + 3 * st < hi; e += 4 * st) {
+        const int32_t u0 = col[e], u1 = col[e + st], u2 = col[e + 2 * st], u3 = col[e + 3 * st];
+        const uint32_t m0 = marks[u0 >> 5], m1 = marks[u1 >> 5], m2 = marks[u2 >> 5],
+                       m3 = marks[u3 >> 5];
+        c += (int32_t)((m0 >> (u0 & 31)) & 1u) + (int32_t)((m1 >> (u1 & 31)) & 1u) +
+             (int32_t)((m2 >> (u2 & 31)) & 1u) + (int32_t)((m3 >> (u3 & 31)) & 1u);
+    }
+    for (; e < hi; e += st) c += test_bit(marks, col[e]) ? 1 : 0;
+    return c;
+}
+
+__device__ __forceinline__ int32_t block_sum_i32(int32_t x, int32_t *lds) {
+    x = wave_sum_i32(x);
+    if ((threadIdx.x & 63) == 0) lds[threadIdx.x >> 6] = x;
+    __syncthreads();
+    int32_t t = 0;
+    for (int i = 0; i < (int)(blockDim.x >> 6); ++i) t += lds[i];
+    return t;
+}
+
+// nmv[v] = marked entries of v's row, rows laid out as in the modularity
+// pass (st); the chunks of a heavy row add their counts (nmv zero before).
+__global__ __launch_bounds__(kRowThreads) void k_neighbor_count(
+    const int32_t *__restrict__ rowptr, const int32_t *__restrict__ col,
+    const int32_t *__restrict__ order, SegTable st, const Chunk *__restrict__ chunks,
+    const uint32_t *__restrict__ marks, int32_t *nmv) {
+    __shared__ int32_t red[kRowThreads / 64];
+    const int b = blockIdx.x;
+    if (b >= st.chunk_block_start) {
+        const Chunk ch = chunks[b - st.chunk_block_start];
+        const int32_t c = block_sum_i32(
+            marked_in(col, marks, ch.lo + threadIdx.x, ch.hi, kRowThreads), red);
+        if (threadIdx.x == 0 && c != 0) atomicAdd(&nmv[ch.v], c);
+        return;
+    }
+    const Seg sg = st.s[find_seg(st, b)];
+    const int lg = sg.lanes_log2;
+    if (lg <= 6) {
+        const int X = 1 << lg;
+        const int32_t idx = (b - sg.block_start) * (kRowThreads >> lg) + (threadIdx.x >> lg);
+        const int lane = threadIdx.x & (X - 1);
+        int32_t v = -1, c = 0;
+        if (idx < sg.count) {
+            v = order[sg.list_off + idx];
+            c = marked_in(col, marks, rowptr[v] + lane, rowptr[v + 1], X);
+        }
+        for (int off = X >> 1; off > 0; off >>= 1) c += __shfl_xor(c, off, 64);
+        if (lane == 0 && v >= 0) nmv[v] = c;
+    } else {
+        const int32_t v = order[sg.list_off + (b - sg.block_start)];
+        const int32_t c = block_sum_i32(
+            marked_in(col, marks, rowptr[v] + threadIdx.x, rowptr[v + 1], kRowThreads), red);
+        if (threadIdx.x == 0) nmv[v] = c;
+    }
+}
+
 // ---------------------------------------------------- XCD-local row passes
 //
 // Workgroups are dispatched to the 8 XCDs round-robin (workgroup b runs on
@@ -2311,6 +2395,18 @@ void launch_modularity_rows(const int32_t *rowptr, const int32_t *col, const flo
                             double *partials, hipStream_t s) {
     hipLaunchKernelGGL(k_modularity, dim3(st.total_blocks), dim3(kRowThreads), 0, s, rowptr, col,
                        w, comm, sigma, V, order, st, chunks, partials);
+}
+
+void launch_neighbor_count(const int32_t *rowptr, const int32_t *col, const uint8_t *hist,
+                           int32_t V, int32_t max_degree, const int32_t *order,
+                           const SegTable &st, const Chunk *chunks, uint32_t *marks,
+                           int32_t *nmv, hipStream_t s) {
+    if (V <= 0 || st.sq_block_start <= 0) return;
+    hipLaunchKernelGGL(k_mark_movers, dim3(blocks_for(V, 256)), dim3(256), 0, s, rowptr, hist, V,
+                       max_degree, (V + 31) / 32, marks);
+    // every block below sq_block_start: the rows' segments, then the chunks
+    hipLaunchKernelGGL(k_neighbor_count, dim3(st.sq_block_start), dim3(kRowThreads), 0, s, rowptr,
+                       col, order, st, chunks, marks, nmv);
 }
 
 void launch_modularity_finish(const SegTable &st, double M, double *partials, double *q,

@@ -92,6 +92,7 @@ class Options(ctypes.Structure):
         ("sub_lanes", ctypes.c_int32 * 3),
         ("col_order", ctypes.c_int32),
         ("hinted", ctypes.c_int32),
+        ("nmv_pull", ctypes.c_int32),
     ]
 
 

@@ -105,6 +105,11 @@ typedef struct {
                                 are first tried one wave per row in a 512-slot
                                 table (plain sweeps; same decisions): 0 = on,
                                 -1 = off                                        */
+    int32_t nmv_pull;         /* the neighbour-move counts of tracked sweeps:
+                                0 = from the rows' side when the last sweep's
+                                movers hold >= E/16 row entries, else from the
+                                movers' side; 1 = always the rows' side, -1 =
+                                always the movers' (same counts)               */
 } lv_options;
 
 /* Run statistics (lv_run).  Times are device times from HIP events. */

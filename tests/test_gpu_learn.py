@@ -133,6 +133,27 @@ def test_features_and_observations_match_oracle(lvlib, oracle, full_frontier, ca
             assert np.abs(got - want).max() <= 2e-6, f"sweep {s}: max feature error {np.abs(got - want).max()}"
 
 
+@pytest.mark.parametrize("side", [1, -1])
+@pytest.mark.parametrize("case", ["planted", "rmat14_chunked"])
+def test_neighbor_moves_both_sides_match_oracle(lvlib, full_frontier, case, side):
+    # nmv counted from the rows' side (1: every row counts its moved
+    # neighbours, heavy rows by chunks) and from the movers' side (-1)
+    print(f"nmv case {case} side {side}", flush=True)
+    if case == "planted":
+        g, opt = lvlib.gen_planted(4000, 16, 0.02, 0.001, seed=24), {}
+    else:
+        g, opt = lvlib.gen_rmat(14, 16, seed=25), {"heavy_wg_max": 320, "chunk_edges": 64,
+                                                   "part_target": 64}
+    with lvlib.Graph(g, nmv_pull=side, **opt) as G:
+        for s in range(4):
+            before = G.communities().copy()
+            G.sweep_pruned(full_frontier, track=True)
+            moved = before != G.communities()
+            st = G.move_state()
+            want = learn.neighbor_moves(g.rowptr, g.col, moved)
+            assert np.array_equal(st["nmv"], want), f"sweep {s}: nmv"
+
+
 @pytest.mark.parametrize("case", ["planted", "rmat13_unit", "rmat14_chunked"])
 def test_move_margins_match_oracle(lvlib, oracle, case):
     print(f"margins case {case}", flush=True)
