@@ -1416,11 +1416,9 @@ struct lv_graph {
             t.total_blocks = gb;
         }
 
-        // column numbering of the move kernels: on large single-graph levels
-        // whose vertices all have entries (all of them are in `order`)
+        // column numbering of the move kernels (and of the modularity row pass)
         col_shadow = false;
-        if (G == 0 && opt.col_order >= 0 && V > 0 && h_off[lv::kNumBuckets] == V &&
-            (opt.col_order > 0 || V >= kColOrderMinVertices)) {
+        if (col_order_wanted()) {
             LV_TRY(c_pos.reserve(V));
             LV_TRY(c_col.reserve(E));
             LV_TRY(c_comm.reserve(V));
@@ -1472,8 +1470,19 @@ struct lv_graph {
         return sync();
     }
 
+    // Whether the level's move kernels read bucket-ordered columns: large
+    // single-graph levels whose vertices all have entries (all of them are
+    // in `order`).  Needs h_off.
+    bool col_order_wanted() const {
+        return G == 0 && opt.col_order >= 0 && V > 0 && h_off[lv::kNumBuckets] == V &&
+               (opt.col_order > 0 || V >= kColOrderMinVertices);
+    }
+
     // Split table and segment tables of the XCD-local modularity pass; off
-    // when disabled, when the level has no heavy rows or when one is unsorted.
+    // when disabled, when the level has no heavy rows or when one is unsorted,
+    // and by default on levels with bucket-ordered columns: the plain row
+    // pass reads those (their hub neighbours share lines) and was faster
+    // than the column-block split at RMAT-22 (861 vs 1068 us a pass).
     lv_status setup_xcd() {
         xcd_ok = false;
         st_light = lv::SegTable{};
@@ -1484,7 +1493,7 @@ struct lv_graph {
         // worth it only when comm (4V bytes) is well beyond one XCD's 4 MB
         // L2; below that the plain row pass already hits L2 often
         if (opt.xcd_rows < 0 || xh_n <= 0 || V < 8) return LV_OK;
-        if (opt.xcd_rows == 0 && V < kXcdMinVertices) return LV_OK;
+        if (opt.xcd_rows == 0 && (V < kXcdMinVertices || col_order_wanted())) return LV_OK;
         hipStream_t s = stream;
         LV_TRY(xsplit.reserve(9 * (int64_t)xh_n));
         int32_t *flag = a_scalars + 8;
@@ -1695,7 +1704,8 @@ struct lv_graph {
             hipEvent_t t0 = tbegin();
             hipEvent_t tr = tbegin();
             lv::launch_modularity_rows(rowptr(), col(), w(), comm, sigma, V, order, st_light, ins.p,
-                                       qpart.p, stream);
+                                       qpart.p, stream, col_shadow ? c_col.p : nullptr,
+                                       c_comm.p);
             tend(tr, -1, 12.0 * e_light + 16.0 * (double)V, LV_KD_MODULARITY_ROWS);
             hipEvent_t tx = tbegin();
             lv::launch_modularity_xcd(col(), w(), comm, order + xh_lo, xst, xsplit.p, qpart.p,
@@ -1713,7 +1723,7 @@ struct lv_graph {
         hipEvent_t t0 = tbegin();
         hipEvent_t tr = tbegin();
         lv::launch_modularity_rows(rowptr(), col(), w(), comm, sigma, V, order, st, ins.p, qpart.p,
-                                   stream);
+                                   stream, col_shadow ? c_col.p : nullptr, c_comm.p);
         tend(tr, -1, 12.0 * (double)E + 16.0 * (double)V, LV_KD_MODULARITY_ROWS);
         lv::launch_modularity_finish(st, M, qpart.p, d_q, stream);
         tend(t0, LV_K_MODULARITY, 12.0 * (double)E + 16.0 * (double)V);

@@ -234,11 +234,14 @@ void launch_modularity(const int32_t *rowptr, const int32_t *col, const float *w
                        double *q, hipStream_t s);
 
 // The two halves of launch_modularity: the row/square pass (per-block
-// partials) and the fixed-order reduction into q.
+// partials) and the fixed-order reduction into q.  With ccol, the rows'
+// columns are read as ccol (a renumbering of col) and their communities as
+// ccomm (comm in that numbering).
 void launch_modularity_rows(const int32_t *rowptr, const int32_t *col, const float *w,
                             const int32_t *comm, const double *sigma, int32_t V,
                             const int32_t *order, const SegTable &st, const Chunk *chunks,
-                            double *partials, hipStream_t s);
+                            double *partials, hipStream_t s, const int32_t *ccol = nullptr,
+                            const int32_t *ccomm = nullptr);
 void launch_modularity_finish(const SegTable &st, double M, double *partials, double *q,
                               hipStream_t s);
 

@@ -1821,11 +1821,15 @@ __global__ void k_level_state(int32_t V, const double *__restrict__ k, int32_t *
 // modularity_optimisation.cu:365-401): one launch produces per-block
 // partials of sum_v (weight to own community) and sum_c S_c^2; a one-block
 // pass adds them in a fixed order (run-to-run deterministic).
+// The row entries name their columns through gcol and gcomm (the move
+// kernels' renumbered columns, or col and comm themselves); comm[v] is the
+// row's own community.
 __global__ __launch_bounds__(kRowThreads) void k_modularity(
-    const int32_t *__restrict__ rowptr, const int32_t *__restrict__ col,
+    const int32_t *__restrict__ rowptr, const int32_t *__restrict__ gcol,
     const float *__restrict__ w, const int32_t *__restrict__ comm,
-    const double *__restrict__ sigma, int32_t V, const int32_t *__restrict__ order, SegTable st,
-    const Chunk *__restrict__ chunks, double *partials) {
+    const int32_t *__restrict__ gcomm, const double *__restrict__ sigma, int32_t V,
+    const int32_t *__restrict__ order, SegTable st, const Chunk *__restrict__ chunks,
+    double *partials) {
     __shared__ double lds[kRowThreads / 64];
     const int b = blockIdx.x;
     double own = 0.0, sq = 0.0;
@@ -1840,7 +1844,7 @@ __global__ __launch_bounds__(kRowThreads) void k_modularity(
         }
     } else if (b >= st.chunk_block_start) {
         const Chunk ch = chunks[b - st.chunk_block_start];
-        own = own_weight(col, w, comm, comm[ch.v], ch.lo + threadIdx.x, ch.hi, kRowThreads);
+        own = own_weight(gcol, w, gcomm, comm[ch.v], ch.lo + threadIdx.x, ch.hi, kRowThreads);
     } else {
         const Seg sg = st.s[find_seg(st, b)];
         const int lg = sg.lanes_log2;
@@ -1850,11 +1854,11 @@ __global__ __launch_bounds__(kRowThreads) void k_modularity(
             const int lane = threadIdx.x & (X - 1);
             if (idx < sg.count) {
                 const int32_t v = order[sg.list_off + idx];
-                own = own_weight(col, w, comm, comm[v], rowptr[v] + lane, rowptr[v + 1], X);
+                own = own_weight(gcol, w, gcomm, comm[v], rowptr[v] + lane, rowptr[v + 1], X);
             }
         } else {
             const int32_t v = order[sg.list_off + (b - sg.block_start)];
-            own = own_weight(col, w, comm, comm[v], rowptr[v] + threadIdx.x, rowptr[v + 1],
+            own = own_weight(gcol, w, gcomm, comm[v], rowptr[v] + threadIdx.x, rowptr[v + 1],
                              kRowThreads);
         }
     }
@@ -2392,9 +2396,11 @@ void launch_sweep_delta(const int32_t *rowptr, const int32_t *col, const float *
 void launch_modularity_rows(const int32_t *rowptr, const int32_t *col, const float *w,
                             const int32_t *comm, const double *sigma, int32_t V,
                             const int32_t *order, const SegTable &st, const Chunk *chunks,
-                            double *partials, hipStream_t s) {
-    hipLaunchKernelGGL(k_modularity, dim3(st.total_blocks), dim3(kRowThreads), 0, s, rowptr, col,
-                       w, comm, sigma, V, order, st, chunks, partials);
+                            double *partials, hipStream_t s, const int32_t *ccol,
+                            const int32_t *ccomm) {
+    hipLaunchKernelGGL(k_modularity, dim3(st.total_blocks), dim3(kRowThreads), 0, s, rowptr,
+                       ccol ? ccol : col, w, comm, ccol ? ccomm : comm, sigma, V, order, st,
+                       chunks, partials);
 }
 
 void launch_neighbor_count(const int32_t *rowptr, const int32_t *col, const uint8_t *hist,

@@ -52,8 +52,10 @@ typedef struct {
                                 movers' rows hold at most E/q_incremental entries
                                 (0 = 2, -1 = never)                                    */
     int32_t xcd_rows;        /* XCD-local passes over heavy rows when rows are sorted:
-                                0 = modularity pass on levels with V >= 2^21, 1 = it
-                                on every level, 2 = it and the chunked tier, -1 = off */
+                                0 = modularity pass on levels with V >= 2^21 whose
+                                move kernels read original columns (col_order off),
+                                1 = it on every level, 2 = it and the chunked tier,
+                                -1 = off                                           */
     int32_t agg_mode;        /* aggregation: 0 = radix sort of (row, column) community
                                 keys, 1 = per-community hash merges                 */
     int32_t compact;         /* 0: every level is stored without its vertices of

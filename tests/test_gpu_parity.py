@@ -301,6 +301,28 @@ def test_xcd_modularity_pass_matches_plain_pass(lvlib, oracle, scale, seed):
         assert abs(A.modularity() - B.modularity()) == 0.0
 
 
+@pytest.mark.parametrize("case", ["rmat14_chunked", "self_loops"])
+def test_row_pass_through_bucket_ordered_columns(lvlib, oracle, case):
+    """The modularity row pass reading the move kernels' bucket-ordered
+    columns (col_order) gives the same Q bits as over the original columns;
+    every sweep ends with a full pass (q_incremental -1)."""
+    if case == "rmat14_chunked":
+        g, opt = lvlib.gen_rmat(14, 16, seed=63), {"heavy_wg_max": 320, "chunk_edges": 64,
+                                                   "part_target": 64}
+    else:
+        g, opt = _csr(lvlib, *graphs.random_graph(400, 3000, 7, True, 40)), {}
+    print(f"row pass case {case}", flush=True)
+    og = oracle.Graph.from_csr(g)
+    common = dict(q_incremental=-1, xcd_rows=-1, **opt)
+    with lvlib.Graph(g, col_order=1, **common) as A, lvlib.Graph(g, col_order=-1, **common) as B:
+        for s in range(4):
+            qa, ma = A.sweep()
+            qb, mb = B.sweep()
+            assert ma == mb and qa == qb, f"sweep {s}"
+            assert np.array_equal(A.communities(), B.communities())
+            assert abs(qa - oracle.modularity(og, A.communities())) <= 1e-9
+
+
 @pytest.mark.parametrize("kind", ["rmat", "self_loops"])
 def test_singleton_modularity_equals_full_pass(lvlib, oracle, kind):
     """Q of a singleton state (after a level's set-up or a reset) is computed
