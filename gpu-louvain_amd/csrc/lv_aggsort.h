@@ -31,11 +31,14 @@ bool radix_sort_pairs(uint64_t *keys, float *vals, uint64_t *tk, float *tv, int6
 
 // keys[e] = newid[comm[v]] << 32 | newid[comm[col[e]]], vals[e] = w[e] for
 // every CSR entry e of row v (rows walked by the level's segment table);
-// cn (V entries) receives newid[comm[v]] first.
+// cn (V entries) receives newid[comm[v]] first.  With ccol (col renumbered
+// by cpos: ccol[e] = cpos[col[e]]) the columns are read through ccol and
+// ccn (V entries, cn in that order), whose gathers share lines more often.
 void launch_agg_pairs(const int32_t *rowptr, const int32_t *col, const float *w,
                       const int32_t *comm, const int32_t *newid, int32_t V, const int32_t *order,
                       const SegTable &st, const Chunk *chunks, int32_t *cn, uint64_t *keys,
-                      float *vals, hipStream_t s);
+                      float *vals, hipStream_t s, const int32_t *ccol = nullptr,
+                      const int32_t *cpos = nullptr, int32_t *ccn = nullptr);
 
 // From sorted keys: head[i] = 1 where a key run starts.
 void launch_run_heads(const uint64_t *keys, int64_t n, int32_t *head, hipStream_t s);

@@ -142,10 +142,17 @@ __global__ __launch_bounds__(kSortThreads) void k_radix_scatter(
 
 // cn[v] = newid[comm[v]]: the new id of each vertex's community, so that a
 // row entry costs one gather.
+// cn[v] = newid[comm[v]]; with cpos also ccn[cpos[v]] = cn[v] (the same in
+// the renumbered column order).
 __global__ void k_comm_newid(const int32_t *__restrict__ comm, const int32_t *__restrict__ newid,
-                             int32_t V, int32_t *cn) {
+                             int32_t V, int32_t *cn, const int32_t *__restrict__ cpos,
+                             int32_t *ccn) {
     const int32_t v = blockIdx.x * blockDim.x + threadIdx.x;
-    if (v < V) cn[v] = newid[comm[v]];
+    if (v < V) {
+        const int32_t x = newid[comm[v]];
+        cn[v] = x;
+        if (cpos) ccn[cpos[v]] = x;
+    }
 }
 
 __device__ __forceinline__ void emit_row(const int32_t *__restrict__ col,
@@ -160,16 +167,18 @@ __device__ __forceinline__ void emit_row(const int32_t *__restrict__ col,
     }
 }
 
+// Row entries name their columns through gcol and gcn: the renumbered
+// columns and cn in that order, or col and cn; the row's own id is cn[v].
 __global__ __launch_bounds__(kRowThreads) void k_agg_pairs(
-    const int32_t *__restrict__ rowptr, const int32_t *__restrict__ col,
+    const int32_t *__restrict__ rowptr, const int32_t *__restrict__ gcol,
     const float *__restrict__ w, const int32_t *__restrict__ cn,
-    const int32_t *__restrict__ order, SegTable st, const Chunk *__restrict__ chunks,
-    uint64_t *keys, float *vals) {
+    const int32_t *__restrict__ gcn, const int32_t *__restrict__ order, SegTable st,
+    const Chunk *__restrict__ chunks, uint64_t *keys, float *vals) {
     const int b = blockIdx.x;
     if (b >= st.chunk_block_start) {
         const Chunk ch = chunks[b - st.chunk_block_start];
         const uint64_t src = (uint32_t)cn[ch.v];
-        emit_row(col, w, cn, src, ch.lo + threadIdx.x, ch.hi, kRowThreads, keys, vals);
+        emit_row(gcol, w, gcn, src, ch.lo + threadIdx.x, ch.hi, kRowThreads, keys, vals);
         return;
     }
     int si = 0;
@@ -181,13 +190,13 @@ __global__ __launch_bounds__(kRowThreads) void k_agg_pairs(
         if (idx < sg.count) {
             const int32_t v = order[sg.list_off + idx];
             const uint64_t src = (uint32_t)cn[v];
-            emit_row(col, w, cn, src, rowptr[v] + (threadIdx.x & ((1 << lg) - 1)),
+            emit_row(gcol, w, gcn, src, rowptr[v] + (threadIdx.x & ((1 << lg) - 1)),
                      rowptr[v + 1], 1 << lg, keys, vals);
         }
     } else {
         const int32_t v = order[sg.list_off + (b - sg.block_start)];
         const uint64_t src = (uint32_t)cn[v];
-        emit_row(col, w, cn, src, rowptr[v] + threadIdx.x, rowptr[v + 1], kRowThreads, keys,
+        emit_row(gcol, w, gcn, src, rowptr[v] + threadIdx.x, rowptr[v + 1], kRowThreads, keys,
                  vals);
     }
 }
@@ -276,12 +285,14 @@ bool radix_sort_pairs(uint64_t *keys, float *vals, uint64_t *tk, float *tv, int6
 void launch_agg_pairs(const int32_t *rowptr, const int32_t *col, const float *w,
                       const int32_t *comm, const int32_t *newid, int32_t V, const int32_t *order,
                       const SegTable &st, const Chunk *chunks, int32_t *cn, uint64_t *keys,
-                      float *vals, hipStream_t s) {
+                      float *vals, hipStream_t s, const int32_t *ccol, const int32_t *cpos,
+                      int32_t *ccn) {
     const int32_t nb = st.chunk_block_start + st.nchunks;
     if (nb <= 0 || V <= 0) return;
-    hipLaunchKernelGGL(k_comm_newid, dim3(blocks_for(V, 256)), dim3(256), 0, s, comm, newid, V, cn);
-    hipLaunchKernelGGL(k_agg_pairs, dim3(nb), dim3(kRowThreads), 0, s, rowptr, col, w, cn, order,
-                       st, chunks, keys, vals);
+    hipLaunchKernelGGL(k_comm_newid, dim3(blocks_for(V, 256)), dim3(256), 0, s, comm, newid, V, cn,
+                       ccol ? cpos : nullptr, ccn);
+    hipLaunchKernelGGL(k_agg_pairs, dim3(nb), dim3(kRowThreads), 0, s, rowptr, ccol ? ccol : col,
+                       w, cn, ccol ? ccn : cn, order, st, chunks, keys, vals);
 }
 
 void launch_run_heads(const uint64_t *keys, int64_t n, int32_t *head, hipStream_t s) {

@@ -695,6 +695,7 @@ struct lv_graph {
     // in `order`, the rows' columns in that numbering and comm copied into it
     bool col_shadow = false;
     DevVec<int32_t> c_pos, c_col, c_comm;
+    DevVec<int32_t> c_cn;  // the aggregation's new ids in that column order
 
     // row-reduction segments
     lv::SegTable st{};
@@ -2719,8 +2720,11 @@ struct lv_graph {
             LV_TRY(spos.reserve(E));
             LV_TRY(sacc.reserve(E));
             LV_TRY(sscratch.reserve(std::max(lv::radix_scratch_elems(E), lv::scan_scratch_elems(E) + 8)));
+            // the level's bucket-ordered columns, when it has them
+            if (col_shadow) LV_TRY(c_cn.reserve(V));
             lv::launch_agg_pairs(rowptr(), col(), w(), comm, a_newid, V, order, st, ins.p, a_mstart,
-                                 skA.p, svA.p, s);
+                                 skA.p, svA.p, s, col_shadow ? c_col.p : nullptr, c_pos.p,
+                                 c_cn.p);
             // (row id, column id) as one key of 2*bits bits, column id low:
             // 8-bit digits over it (stable LSD)
             int bits = 1;
@@ -2902,6 +2906,7 @@ struct lv_graph {
         c_pos.release();
         c_col.release();
         c_comm.release();
+        c_cn.release();
         item_best.release();
         item_K.release();
         qpart.release();
