@@ -307,18 +307,19 @@ __global__ __launch_bounds__(256) void k_move_sub(MoveArgs a, const int32_t *__r
 
 // One wave's LDS hash table of C slots (keys K, FP64 sums W; C a multiple
 // of 64): a power of two hashes by shift, other sizes by multiply-shift.
-template <int C>
+// L: lanes per row (64, or 32 for two rows per wave); lane < L.
+template <int C, int L = 64>
 __device__ __forceinline__ void wave_tab_clear(int32_t *K, double *W, int lane) {
 #pragma unroll
-    for (int j = 0; j < C / 64; ++j) {
-        K[lane + 64 * j] = -1;
-        W[lane + 64 * j] = 0.0;
+    for (int j = 0; j < C / L; ++j) {
+        K[lane + L * j] = -1;
+        W[lane + L * j] = 0.0;
     }
 }
 
-// Inserts vertex v's row (MAXE entries per lane, deg <= 64 * MAXE) into the
-// wave's table; returns v's community.
-template <int C, int MAXE>
+// Inserts vertex v's row (MAXE entries per lane, deg <= L * MAXE) into the
+// row's table; returns v's community.
+template <int C, int MAXE, int L = 64>
 __device__ __forceinline__ int32_t wave_tab_fill(const MoveArgs &a, int32_t v, int32_t *K,
                                                  double *W, int lane) {
     constexpr bool POW2 = (C & (C - 1)) == 0;
@@ -330,7 +331,7 @@ __device__ __forceinline__ int32_t wave_tab_fill(const MoveArgs &a, int32_t v, i
     float wv[MAXE];
 #pragma unroll
     for (int j = 0; j < MAXE; ++j) {
-        const int32_t e = rs + lane + 64 * j;
+        const int32_t e = rs + lane + L * j;
         u[j] = e < re ? a.ccol[e] : -1;
         wv[j] = e < re ? a.w[e] : 0.0f;
     }
@@ -348,12 +349,12 @@ __device__ __forceinline__ int32_t wave_tab_fill(const MoveArgs &a, int32_t v, i
     return cur;
 }
 
-// Decision of v (in community cur) from its filled table; the whole wave
-// calls it.
-template <int C>
+// Decision of v (in community cur) from its filled table; the L lanes of
+// the row call it together.
+template <int C, int L = 64>
 __device__ __forceinline__ void wave_tab_decide(const MoveArgs &a, int32_t v, int32_t cur,
                                                 const int32_t *K, const double *W, int lane) {
-    constexpr int S = C / 64;  // slots per lane
+    constexpr int S = C / L;  // slots per lane
     const float k32 = (float)a.k[v];
     const float2 md = vertex_mden(a, v);
     const float vM32 = md.x, vden = md.y;
@@ -365,14 +366,14 @@ __device__ __forceinline__ void wave_tab_decide(const MoveArgs &a, int32_t v, in
     uint64_t rr[S];
 #pragma unroll
     for (int j = 0; j < S; ++j) {
-        kk[j] = K[lane + 64 * j];
-        vv[j] = W[lane + 64 * j];
+        kk[j] = K[lane + L * j];
+        vv[j] = W[lane + L * j];
     }
     double kc = 0.0;
 #pragma unroll
     for (int j = 0; j < S; ++j)
         if (kk[j] == cur) kc = vv[j];
-    kc = lanes_sum<64>(kc);
+    kc = lanes_sum<L>(kc);
     uint64_t best = pack_best(0.0f, cur);
     double bestK = 0.0;
     if (a.bounds) {
@@ -386,11 +387,11 @@ __device__ __forceinline__ void wave_tab_decide(const MoveArgs &a, int32_t v, in
                 km = vv[j];
                 js = j;
             }
-        const int src = lanes_argmax<64>(km, lane);
+        const int src = lanes_argmax<L>(km, lane);
         if (lane == src && km >= 0.0)
             consider_k(best, bestK, kk[js], vv[js], a.crec[kk[js]], cur, size_cur, k32, scur, vM32,
                        vden);
-        lanes_max_pair<64>(best, bestK);  // src holds the best, the others the start key
+        lanes_max_pair<L>(best, bestK);  // src holds the best, the others the start key
         const float t = (float)kc / vM32;
         bool ev[S];
 #pragma unroll
@@ -411,7 +412,7 @@ __device__ __forceinline__ void wave_tab_decide(const MoveArgs &a, int32_t v, in
             consider_k(best, bestK, kk[j], vv[j], rr[j], cur, size_cur, k32, scur, vM32, vden);
         }
     }
-    lanes_max_pair<64>(best, bestK);
+    lanes_max_pair<L>(best, bestK);
     if (lane == 0) write_decision(a, v, cur, best, bestK, kc);
 }
 
@@ -419,30 +420,38 @@ __device__ __forceinline__ void wave_tab_decide(const MoveArgs &a, int32_t v, in
 // for degrees <= 319: load <= 0.71 and 7 blocks per CU.  Each lane holds MAXE
 // row entries: 256-slot tables take degrees <= 128 (2 per lane), or up to
 // 255 (4 per lane) in instance G = 2, for a group-5 split above 128.
-template <int C, int G>
+// L = 32: two rows per wave, one per half, so that a CU holds twice the
+// rows in flight (instance G = 3: degrees <= 84 in 128-slot tables, G = 4:
+// degrees <= 128 in 192-slot tables).
+template <int C, int G, int L = 64>
 __global__ __launch_bounds__(256) void k_move_wave(MoveArgs a, const int32_t *__restrict__ list,
                                                    int32_t n) {
     if (gated_off(a.gate)) return;
-    static_assert(C % 64 == 0, "whole slots per lane");
-    constexpr int MAXE = C == 256 ? (G == 2 ? 4 : 2) : 5;  // edges per lane
-    static_assert(64 * MAXE >= (C == 256 ? (G == 2 ? 255 : 128) : 319), "a row fits the lanes");
-    __shared__ int32_t keys_s[4 * C];
-    __shared__ double vals_s[4 * C];
-    const int lane = threadIdx.x & 63, wid = threadIdx.x >> 6;
-    const int32_t slot = blockIdx.x * 4 + wid;
+    static_assert(C % L == 0, "whole slots per lane");
+    constexpr int R = 64 / L;  // rows per wave
+    constexpr int MAXE =
+        L == 32 ? (G == 3 ? 3 : 4) : (C == 256 ? (G == 2 ? 4 : 2) : 5);  // edges per lane
+    static_assert(L * MAXE >= (L == 32 ? (G == 3 ? 84 : 128)
+                                       : C == 256 ? (G == 2 ? 255 : 128) : 319),
+                  "a row fits the lanes");
+    __shared__ int32_t keys_s[4 * R * C];
+    __shared__ double vals_s[4 * R * C];
+    const int lane = threadIdx.x & (L - 1);
+    const int seg = threadIdx.x / L;  // row of the block
+    const int32_t slot = blockIdx.x * (4 * R) + seg;
     const bool active = slot < n;
-    int32_t *K = keys_s + wid * C;
-    double *W = vals_s + wid * C;
-    wave_tab_clear<C>(K, W, lane);
+    int32_t *K = keys_s + seg * C;
+    double *W = vals_s + seg * C;
+    wave_tab_clear<C, L>(K, W, lane);
     __syncthreads();
     int32_t v = 0, cur = 0;
     if (active) {
         v = list[slot];
-        cur = wave_tab_fill<C, MAXE>(a, v, K, W, lane);
+        cur = wave_tab_fill<C, MAXE, L>(a, v, K, W, lane);
     }
     __syncthreads();
-    if (!active) return;  // only wave-level communication below
-    wave_tab_decide<C>(a, v, cur, K, W, lane);
+    if (!active) return;  // only communication inside the row's lanes below
+    wave_tab_decide<C, L>(a, v, cur, K, W, lane);
 }
 
 // Small-table attempt for the rows of the one-workgroup tiers (degrees above
@@ -2231,7 +2240,13 @@ void launch_move_wave(int cap, int inst, const MoveArgs &a, const int32_t *list,
                       hipStream_t s) {
     if (n <= 0) return;
     const unsigned g = blocks_for(n, 4);
-    if (cap > 256)
+    if (cap == 128)
+        hipLaunchKernelGGL((k_move_wave<128, 3, 32>), dim3(blocks_for(n, 8)), dim3(256), 0, s, a,
+                           list, n);
+    else if (cap == 192)
+        hipLaunchKernelGGL((k_move_wave<192, 4, 32>), dim3(blocks_for(n, 8)), dim3(256), 0, s, a,
+                           list, n);
+    else if (cap > 256)
         hipLaunchKernelGGL((k_move_wave<448, 0>), dim3(g), dim3(256), 0, s, a, list, n);
     else if (inst == 0)
         hipLaunchKernelGGL((k_move_wave<256, 0>), dim3(g), dim3(256), 0, s, a, list, n);
