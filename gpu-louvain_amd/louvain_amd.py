@@ -48,7 +48,7 @@ DETAIL_NAMES = ["move_b0", "move_b1", "move_b2", "move_b3", "move_b4", "move_b5"
                 "chunk_aggregate", "chunk_merge", "chunk_final", "modularity_rows", "frontier",
                 "modularity_xcd", "move_b8", "move_b9", "move_b10", "heavy_lds", "hinted"]
 DETAIL_KERNELS = ["k_move_sub<4, 1>", "k_move_sub<4, 2>", "k_move_sub<4, 4>", "k_move_sub<4, 8>",
-                  "k_move_wave<128, 3, 32>", "k_move_wave<192, 4, 32>", "k_move_wave<448, 0>",
+                  "k_move_wave<128, 3, 32>", "k_move_wave<192, 4, 32>", "k_move_wave<448, 0, 64>",
                   "k_move_wg<128, 0, false>",
                   "k_chunk_aggregate", "k_chunk_merge", "k_heavy_final", "k_modularity", "k_mlp<false>",
                   "k_modularity_xcd", "k_move_wg<256, 1, false>", "k_move_wg<256, 2, false>",
