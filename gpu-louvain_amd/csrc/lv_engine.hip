@@ -1621,7 +1621,7 @@ struct lv_graph {
                 // degrees <= 128 (bucket 4, and bucket 5 unless its bound is
                 // raised): two rows per wave
                 lv::launch_move_wave(b == 4 ? 128 : b == 5 ? (bb.hi[5] > 128 ? 256 : 192) : 448,
-                                     b == 5 && bb.hi[5] > 128 ? 2 : b - 4, a, list, n, s);
+                                     a, list, n, s);
                 tend(t0, LV_K_MOVE_WAVE, bytes, LV_KD_MOVE_BUCKET0 + b);
                 break;
             case 7: case 8: case 9: case 10: {

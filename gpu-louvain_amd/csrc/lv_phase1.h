@@ -133,11 +133,9 @@ bool move_sub_shape(int X, int E);
 void launch_move_sub(int X, int E, const MoveArgs &a, const int32_t *list, int32_t n,
                      hipStream_t s);
 // cap: LDS slots per row: 128 (degrees <= 84) or 192 (degrees <= 128) with
-// two rows per wave, 256 or 448 with one row per wave; inst names the
-// 256-slot instance: 0 for degrees <= 84, 1 for degrees <= 128, 2 for
-// degrees <= 255.
-void launch_move_wave(int cap, int inst, const MoveArgs &a, const int32_t *list, int32_t n,
-                      hipStream_t s);
+// two rows per wave, 256 (degrees <= 255) or 448 (degrees <= 319) with one
+// row per wave.
+void launch_move_wave(int cap, const MoveArgs &a, const int32_t *list, int32_t n, hipStream_t s);
 // tier 0..3: the workgroup buckets in degree order (separate kernel
 // instances), lanes (128, 256, 512 or 1024) per vertex.
 // slots: LDS table entries allocated per workgroup (12 bytes each, more than

@@ -416,10 +416,10 @@ __device__ __forceinline__ void wave_tab_decide(const MoveArgs &a, int32_t v, in
     if (lane == 0) write_decision(a, v, cur, best, bestK, kc);
 }
 
-// C: LDS slots per wave; G names the instance (one per bucket).  448 slots
-// for degrees <= 319: load <= 0.71 and 7 blocks per CU.  Each lane holds MAXE
-// row entries: 256-slot tables take degrees <= 128 (2 per lane), or up to
-// 255 (4 per lane) in instance G = 2, for a group-5 split above 128.
+// C: LDS slots per row; G names the instance.  448 slots for degrees <= 319:
+// load <= 0.71 and 7 blocks per CU.  Each lane holds MAXE row entries:
+// 256-slot tables (instance G = 2, 4 per lane) take degrees up to 255, for a
+// group-5 split above 128.
 // L = 32: two rows per wave, one per half, so that a CU holds twice the
 // rows in flight (instance G = 3: degrees <= 84 in 128-slot tables, G = 4:
 // degrees <= 128 in 192-slot tables).
@@ -429,10 +429,8 @@ __global__ __launch_bounds__(256) void k_move_wave(MoveArgs a, const int32_t *__
     if (gated_off(a.gate)) return;
     static_assert(C % L == 0, "whole slots per lane");
     constexpr int R = 64 / L;  // rows per wave
-    constexpr int MAXE =
-        L == 32 ? (G == 3 ? 3 : 4) : (C == 256 ? (G == 2 ? 4 : 2) : 5);  // edges per lane
-    static_assert(L * MAXE >= (L == 32 ? (G == 3 ? 84 : 128)
-                                       : C == 256 ? (G == 2 ? 255 : 128) : 319),
+    constexpr int MAXE = L == 32 ? (G == 3 ? 3 : 4) : (C == 256 ? 4 : 5);  // edges per lane
+    static_assert(L * MAXE >= (L == 32 ? (G == 3 ? 84 : 128) : C == 256 ? 255 : 319),
                   "a row fits the lanes");
     __shared__ int32_t keys_s[4 * R * C];
     __shared__ double vals_s[4 * R * C];
@@ -2236,8 +2234,7 @@ void launch_move_sub(int X, int E, const MoveArgs &a, const int32_t *list, int32
 #undef LV_SUB
 }
 
-void launch_move_wave(int cap, int inst, const MoveArgs &a, const int32_t *list, int32_t n,
-                      hipStream_t s) {
+void launch_move_wave(int cap, const MoveArgs &a, const int32_t *list, int32_t n, hipStream_t s) {
     if (n <= 0) return;
     const unsigned g = blocks_for(n, 4);
     if (cap == 128)
@@ -2246,14 +2243,10 @@ void launch_move_wave(int cap, int inst, const MoveArgs &a, const int32_t *list,
     else if (cap == 192)
         hipLaunchKernelGGL((k_move_wave<192, 4, 32>), dim3(blocks_for(n, 8)), dim3(256), 0, s, a,
                            list, n);
-    else if (cap > 256)
-        hipLaunchKernelGGL((k_move_wave<448, 0>), dim3(g), dim3(256), 0, s, a, list, n);
-    else if (inst == 0)
-        hipLaunchKernelGGL((k_move_wave<256, 0>), dim3(g), dim3(256), 0, s, a, list, n);
-    else if (inst == 1)
-        hipLaunchKernelGGL((k_move_wave<256, 1>), dim3(g), dim3(256), 0, s, a, list, n);
-    else
+    else if (cap == 256)
         hipLaunchKernelGGL((k_move_wave<256, 2>), dim3(g), dim3(256), 0, s, a, list, n);
+    else
+        hipLaunchKernelGGL((k_move_wave<448, 0>), dim3(g), dim3(256), 0, s, a, list, n);
 }
 
 namespace {
