@@ -970,6 +970,10 @@ struct lv_graph {
         int clg = 6;  // table_log2's floor
         while ((1 << clg) < 2 * chunk_edges && clg < lv::kChunkCapLog2) ++clg;
         t.chunk_cap_log2 = clg;
+        // input levels: a chunk's communities barely repeat there (on
+        // aggregated levels rows that stay this diverse are rare, and merging
+        // a chunk's duplicates keeps stage B's work down)
+        t.direct = cur_buf < 0 ? 1 : 0;
         return t;
     }
 

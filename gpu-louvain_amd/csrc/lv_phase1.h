@@ -70,6 +70,8 @@ struct ChunkedTier {
     int32_t n_chunk_order, n_item_order;
     int32_t range_blk;    // > 0: partitions by community-id range (see tier_part)
     int32_t chunk_cap_log2;  // stage A table slots (log2): twice the chunk size
+    int32_t direct;          // stage A may write rows of mostly distinct
+                             // communities without merging a chunk's duplicates
     // LDS attempt (launch_heavy_try), per heavy vertex: decided in LDS this
     // sweep step; distinct communities of its last evaluation (the attempt
     // is made when they fit); their count summed by stage B.

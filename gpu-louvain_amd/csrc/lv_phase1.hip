@@ -910,75 +910,36 @@ __device__ __forceinline__ uint32_t tier_part(const ChunkedTier &t, int32_t c, u
     return part_of(c, P);
 }
 
+// Stage A merges a chunk's duplicate communities in LDS unless the row's
+// last evaluation saw at least kDirectNum / kDirectDen distinct communities
+// per entry.
+constexpr int64_t kDirectNum = 6, kDirectDen = 10;
+static_assert(kChunkEdges <= 4 * 256, "a direct chunk holds four entries per lane");
+
 // smallest power of two >= 2n (load <= 1/2), between 2^6 and 2^cap_lg
 __device__ __forceinline__ int table_log2(int64_t n, int cap_lg) {
     const int lg = n > 1 ? 64 - __clzll((unsigned long long)(2 * n - 1)) : 1;
     return min(cap_lg, max(6, lg));
 }
 
-__global__ __launch_bounds__(256) void k_chunk_aggregate(MoveArgs a, ChunkedTier t) {
-    if (gated_off(a.gate)) return;
-    // dynamic LDS sized by the chunk size: vals[C], keys[C], pcnt[kMaxParts]
-    extern __shared__ double agg_lds[];
-    const int C = 1 << t.chunk_cap_log2;
-    double *vals = agg_lds;
-    int32_t *keys = reinterpret_cast<int32_t *>(agg_lds + C);
-    int32_t *pcnt = keys + C;
-    const int32_t ci = t.chunk_order ? t.chunk_order[blockIdx.x] : (int32_t)blockIdx.x;
-    if (ci < 0) return;  // padding of the XCD dispatch order
-    const Chunk ch = t.chunks[ci];
-    if (a.active && !a.active[ch.v]) return;  // not in this sweep's frontier
-    if (t.done[ch.hv]) return;                // decided in LDS
-    const int P = ch.P;
-    const int cl = table_log2(ch.hi - ch.lo, t.chunk_cap_log2);
-    const int Cv = 1 << cl;
-    for (int i = threadIdx.x; i < Cv; i += 256) {
-        keys[i] = -1;
-        vals[i] = 0.0;
-    }
-    for (int p = threadIdx.x; p < P; p += 256) pcnt[p] = 0;
+// Adds a chunk's own-community sum to its vertex's K_cur (one FP64 atomic
+// per chunk); every lane calls it.
+__device__ __forceinline__ void chunk_own_sum(const ChunkedTier &t, const Chunk &ch, double own) {
+    __shared__ double own_s[4];
+    own = lanes_sum<64>(own);
+    if ((threadIdx.x & 63) == 0) own_s[threadIdx.x >> 6] = own;
     __syncthreads();
-    const int32_t vs = col_self(a, ch.v);
-    const int32_t cur = a.comm[ch.v];
-    // entries of the vertex's own community are summed in registers and
-    // added to kcur (one FP64 atomic per chunk), the others go to the table
-    double own = 0.0;
-    // four entries per lane at a time: their loads, then their gathers, are
-    // in flight together ahead of the LDS inserts
-    for (int64_t e0 = ch.lo + threadIdx.x; e0 < ch.hi; e0 += 4 * 256) {
-        int32_t u[4];
-        float wf[4];
-#pragma unroll
-        for (int j = 0; j < 4; ++j) {
-            const int64_t e = e0 + j * 256;
-            u[j] = e < ch.hi ? __builtin_nontemporal_load(&a.ccol[e]) : -1;
-            wf[j] = e < ch.hi ? __builtin_nontemporal_load(&a.w[e]) : 0.0f;
-        }
-        int32_t cc[4];
-#pragma unroll
-        for (int j = 0; j < 4; ++j) cc[j] = (u[j] >= 0 && u[j] != vs) ? a.ccomm[u[j]] : -1;
-#pragma unroll
-        for (int j = 0; j < 4; ++j) {
-            if (cc[j] == cur)
-                own += (double)wf[j];
-            else if (cc[j] >= 0)
-                lds_insert(keys, vals, 32 - cl, (uint32_t)Cv - 1, cc[j], (double)wf[j]);
-        }
+    if (threadIdx.x == 0) {
+        const double x = own_s[0] + own_s[1] + own_s[2] + own_s[3];
+        if (x != 0.0) atomicAdd(&t.kcur[ch.hv], x);
     }
-    {
-        __shared__ double own_s[4];
-        own = lanes_sum<64>(own);
-        if ((threadIdx.x & 63) == 0) own_s[threadIdx.x >> 6] = own;
-        __syncthreads();
-        if (threadIdx.x == 0) {
-            const double x = own_s[0] + own_s[1] + own_s[2] + own_s[3];
-            if (x != 0.0) atomicAdd(&t.kcur[ch.hv], x);
-        }
-    }
-    for (int s = threadIdx.x; s < Cv; s += 256) {
-        const int32_t k = keys[s];
-        if (k >= 0) atomicAdd(&pcnt[tier_part(t, k, (uint32_t)P)], 1);
-    }
+}
+
+// From the chunk's pair counts per partition (pcnt): its partition offsets
+// (poff), the vertex's partition totals (ptot), and pcnt turned into each
+// partition's first output slot.  Every lane calls it.
+__device__ __forceinline__ void chunk_part_offsets(const ChunkedTier &t, const Chunk &ch, int P,
+                                                   int32_t *pcnt) {
     __syncthreads();
     if (threadIdx.x < 64) {  // wave 0: exclusive scan of the partition counts
         const int lane = threadIdx.x;
@@ -1002,6 +963,102 @@ __global__ __launch_bounds__(256) void k_chunk_aggregate(MoveArgs a, ChunkedTier
         if (lane == 0) t.poff[ch.pbase + P] = carry;
     }
     __syncthreads();
+}
+
+__global__ __launch_bounds__(256) void k_chunk_aggregate(MoveArgs a, ChunkedTier t) {
+    if (gated_off(a.gate)) return;
+    // dynamic LDS sized by the chunk size: vals[C], keys[C], pcnt[kMaxParts]
+    extern __shared__ double agg_lds[];
+    const int C = 1 << t.chunk_cap_log2;
+    double *vals = agg_lds;
+    int32_t *keys = reinterpret_cast<int32_t *>(agg_lds + C);
+    int32_t *pcnt = keys + C;
+    const int32_t ci = t.chunk_order ? t.chunk_order[blockIdx.x] : (int32_t)blockIdx.x;
+    if (ci < 0) return;  // padding of the XCD dispatch order
+    const Chunk ch = t.chunks[ci];
+    if (a.active && !a.active[ch.v]) return;  // not in this sweep's frontier
+    if (t.done[ch.hv]) return;                // decided in LDS
+    const int P = ch.P;
+    const int32_t vs = col_self(a, ch.v);
+    const int32_t cur = a.comm[ch.v];
+    // entries of the vertex's own community are summed in registers and
+    // added to kcur (one FP64 atomic per chunk), the others go to the table
+    double own = 0.0;
+    for (int p = threadIdx.x; p < P; p += 256) pcnt[p] = 0;
+    // On input levels (t.direct), a row whose last evaluation saw distinct
+    // communities for most of its entries (every row before its first)
+    // gains little from merging a chunk's duplicates here: its entries go to
+    // their partitions as they are, and stage B merges them all the same.
+    // A chunk holds at most kChunkEdges = 4 x 256 entries: four per lane.
+    const int64_t rdeg = a.rowptr[ch.v + 1] - a.rowptr[ch.v];
+    if (t.direct && (int64_t)t.hint[ch.hv] * kDirectDen >= rdeg * kDirectNum) {  // block-uniform
+        int32_t cc[4], pp[4], pos[4];
+        float wf[4];
+#pragma unroll
+        for (int j = 0; j < 4; ++j) {
+            const int64_t e = ch.lo + threadIdx.x + j * 256;
+            const int32_t u = e < ch.hi ? __builtin_nontemporal_load(&a.ccol[e]) : -1;
+            wf[j] = e < ch.hi ? __builtin_nontemporal_load(&a.w[e]) : 0.0f;
+            cc[j] = (u >= 0 && u != vs) ? a.ccomm[u] : -1;
+        }
+        __syncthreads();  // pcnt cleared
+#pragma unroll
+        for (int j = 0; j < 4; ++j) {
+            pp[j] = -1;
+            pos[j] = 0;
+            if (cc[j] == cur) {
+                own += (double)wf[j];
+            } else if (cc[j] >= 0) {
+                pp[j] = (int32_t)tier_part(t, cc[j], (uint32_t)P);
+                pos[j] = atomicAdd(&pcnt[pp[j]], 1);
+            }
+        }
+        chunk_own_sum(t, ch, own);  // (its barrier also completes the counts)
+        chunk_part_offsets(t, ch, P, pcnt);
+#pragma unroll
+        for (int j = 0; j < 4; ++j) {
+            if (pp[j] < 0) continue;
+            const int64_t o = ch.out + pcnt[pp[j]] + pos[j];
+            t.out_key[o] = cc[j];
+            t.out_val[o] = (double)wf[j];
+        }
+        return;
+    }
+    const int cl = table_log2(ch.hi - ch.lo, t.chunk_cap_log2);
+    const int Cv = 1 << cl;
+    for (int i = threadIdx.x; i < Cv; i += 256) {
+        keys[i] = -1;
+        vals[i] = 0.0;
+    }
+    __syncthreads();
+    // four entries per lane at a time: their loads, then their gathers, are
+    // in flight together ahead of the LDS inserts
+    for (int64_t e0 = ch.lo + threadIdx.x; e0 < ch.hi; e0 += 4 * 256) {
+        int32_t u[4];
+        float wf[4];
+#pragma unroll
+        for (int j = 0; j < 4; ++j) {
+            const int64_t e = e0 + j * 256;
+            u[j] = e < ch.hi ? __builtin_nontemporal_load(&a.ccol[e]) : -1;
+            wf[j] = e < ch.hi ? __builtin_nontemporal_load(&a.w[e]) : 0.0f;
+        }
+        int32_t cc[4];
+#pragma unroll
+        for (int j = 0; j < 4; ++j) cc[j] = (u[j] >= 0 && u[j] != vs) ? a.ccomm[u[j]] : -1;
+#pragma unroll
+        for (int j = 0; j < 4; ++j) {
+            if (cc[j] == cur)
+                own += (double)wf[j];
+            else if (cc[j] >= 0)
+                lds_insert(keys, vals, 32 - cl, (uint32_t)Cv - 1, cc[j], (double)wf[j]);
+        }
+    }
+    chunk_own_sum(t, ch, own);
+    for (int s = threadIdx.x; s < Cv; s += 256) {
+        const int32_t k = keys[s];
+        if (k >= 0) atomicAdd(&pcnt[tier_part(t, k, (uint32_t)P)], 1);
+    }
+    chunk_part_offsets(t, ch, P, pcnt);
     for (int s = threadIdx.x; s < Cv; s += 256) {
         const int32_t k = keys[s];
         if (k < 0) continue;
